@@ -1,0 +1,282 @@
+#!/usr/bin/env python3
+"""Benchmark of the GPU BVH path on the 2.3M-triangle workload.
+
+Workload (one step = one frame): 1024x1024 heightfield (6-octave value noise,
+amplitude 0.2) + 200,000 random small triangles = 2,297,152 triangles;
+full LBVH build (7 stages) + child-pair packing, 1920x1080 pinhole primary
+rays traced, one cosine-weighted diffuse bounce per hit generated on the GPU
+and traced (the incoherent ray set).  All inputs are resident in HBM before
+the timed region; the pristine triangle array is restored by a device copy at
+the start of each step (the build reorders it in place, as the reference).
+
+`value` is the plain closest-hit Mrays/s on the bounce rays (kernel time from
+HIP events on the stream the kernels run on).  Primary-ray Mrays/s and LBVH
+build Mtri/s ride along in the same line.  The CPU baseline is the oracle's
+restatement of the reference CPU bbvh plugin (spatial-median tree, child-is
+tracer, OpenMP over rows) on the same bounce rays, timed on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "closest-hit Mrays/s (plain vs hit-predicted) at 2M tris; LBVH build Mtri/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--grid", type=int, default=1024)
+    p.add_argument("--random", type=int, default=200_000)
+    p.add_argument("--width", type=int, default=1920)
+    p.add_argument("--height", type=int, default=1080)
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    p.add_argument("--cpu-rays", type=int, default=0,
+                   help="bounce rays in the CPU sample (0 = all of them)")
+    return p.parse_args()
+
+
+def cpu_baseline(tris_h, bo, bd, bt, n_bounce, w, max_rays):
+    """The oracle restatement of the reference CPU plugin on this host."""
+    import oracle
+
+    threads = len(os.sched_getaffinity(0))
+    if os.environ.get("OMP_NUM_THREADS"):
+        threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
+    t0 = time.perf_counter()
+    mb = oracle.median_build(tris_h)
+    build_s = time.perf_counter() - t0
+    rows = n_bounce // w
+    if max_rays:
+        rows = min(rows, max(1, max_rays // w))
+    n = rows * w
+    hits, ms, _ = oracle.trace_cis(mb["tris"], mb["nodes"], bo[: 3 * n], bd[: 3 * n], bt[:n], w,
+                                   rows, threads=threads)
+    cpu = platform.processor() or platform.machine()
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {
+        "value": n / ms / 1e3, "unit": "Mrays/s", "cores": threads, "kind": "port",
+        "sample": (f"{n} of the {n_bounce} bounce rays ({rows} rows of {w}) traced by the "
+                   f"oracle's child-is tracer on its spatial-median tree (built single-threaded "
+                   f"in {build_s:.1f} s, not timed), OpenMP schedule(dynamic,32), {cpu}"),
+        "median_build_s": build_s,
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU sharding is "
+                                   "designed (DESIGN.md) but not run: use --gpus 1"}))
+        sys.exit(2)
+
+    import rta_amd
+    from rta_amd import scenes
+    from rta_amd.abi import NODE_DTYPE, call, ptr
+    from rta_amd.device import DeviceBuffer, Event, Stream
+
+    if rta_amd.device_count() < 1:
+        raise SystemExit("no HIP device visible")
+    call("rta_hip_set_device", 0)
+    dev_name = rta_amd.device_name()
+    log(f"device {dev_name}")
+
+    t0 = time.perf_counter()
+    tris_h = scenes.config2_scene(grid=args.grid, n_random=args.random)
+    n = len(tris_h)
+    cam = scenes.config2_camera(args.width, args.height)
+    R = cam.w * cam.h
+    log(f"scene {n} triangles, {R} primary rays ({time.perf_counter() - t0:.1f} s)")
+
+    s = Stream()
+    sp = ptr(s)
+    pristine = DeviceBuffer.from_array(tris_h)
+    tris = DeviceBuffer(104 * n)
+    nodes = DeviceBuffer(32 * (2 * n - 1))
+    idx = DeviceBuffer(4 * n)
+    status = DeviceBuffer(4)
+    sz = C.c_size_t(0)
+    call("rta_lbvh_workspace_bytes", n, C.byref(sz))
+    ws = DeviceBuffer(sz.value)
+    ws_bytes = C.c_size_t(sz.value)
+    pairs = DeviceBuffer(64 * (2 * n - 1))
+    tri4 = DeviceBuffer(48 * n)
+    prim = rta_amd.RayBuffers(R)
+    bnc = rta_amd.RayBuffers(R)
+    hits_p = DeviceBuffer(16 * R)
+    hits_b = DeviceBuffer(16 * R)
+    src = DeviceBuffer(4 * R)
+    count = DeviceBuffer(4)
+    call("rta_bounce_work_bytes", R, C.byref(sz))
+    bwork = DeviceBuffer(sz.value)
+    bwork_bytes = C.c_size_t(sz.value)
+    err = DeviceBuffer(4)
+    err.fill_u32(0)
+    f3 = lambda v: (C.c_float * 3)(*[float(x) for x in v])
+    vd, pos, up = f3(cam.view_dir), f3(cam.pos), f3(cam.up)
+    fmax = float(rta_amd.FLT_MAX)
+
+    # the root's link words are fixed by the scene (always an inner node
+    # here); read them once after a first build
+    def build_and_pack():
+        call("rta_hip_memcpy_dtod", ptr(tris), ptr(pristine), C.c_size_t(104 * n), sp)
+        ev[0].record(s)
+        call("rta_lbvh_build", ptr(tris), n, ptr(nodes), ptr(idx), ptr(ws), ws_bytes, ptr(status),
+             None, sp)
+        ev[1].record(s)
+        call("rta_bvh_pack", ptr(pairs), ptr(tri4), ptr(nodes), 2 * n - 1, ptr(tris), n, sp)
+        ev[2].record(s)
+
+    def frame(root):
+        build_and_pack()
+        call("rta_setup_shirley_rays", ptr(prim.gpu_direction), ptr(prim.gpu_origin),
+             ptr(prim.gpu_maxt), cam.fovy, cam.aspect, cam.w, cam.h, vd, pos, up, fmax, sp)
+        call("rta_reset_intersections", ptr(hits_p), R, 1, sp)
+        ev[3].record(s)
+        call("rta_trace_cis_packed", ptr(tri4), ptr(pairs), root[0], root[1], ptr(prim.gpu_origin),
+             ptr(prim.gpu_direction), ptr(prim.gpu_maxt), R, None, ptr(hits_p), ptr(err), sp)
+        ev[4].record(s)
+        call("rta_diffuse_bounce", ptr(bnc.gpu_origin), ptr(bnc.gpu_direction), ptr(bnc.gpu_maxt),
+             ptr(src), ptr(count), ptr(hits_p), ptr(tris), ptr(prim.gpu_origin),
+             ptr(prim.gpu_direction), R, C.c_uint64(scenes.BOUNCE_SEED), ptr(bwork), bwork_bytes,
+             sp)
+        call("rta_reset_intersections", ptr(hits_b), R, 1, sp)
+        ev[5].record(s)
+        call("rta_trace_cis_packed", ptr(tri4), ptr(pairs), root[0], root[1], ptr(bnc.gpu_origin),
+             ptr(bnc.gpu_direction), ptr(bnc.gpu_maxt), R, ptr(count), ptr(hits_b), ptr(err), sp)
+        ev[6].record(s)
+
+    ev = [Event() for _ in range(7)]
+    build_and_pack()
+    s.synchronize()
+    r0 = nodes.download(NODE_DTYPE, 1)[0]
+    root = (int(r0["type_left_elems"]), int(r0["right_tris"]))
+
+    log(f"warmup {args.warmup} steps")
+    for _ in range(args.warmup):
+        frame(root)
+    s.synchronize()
+
+    seg = {"build": 0.0, "pack": 0.0, "primary": 0.0, "bounce_gen": 0.0, "bounce": 0.0}
+    log(f"timing {args.steps} steps")
+    call("rta_hip_device_synchronize")
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        frame(root)
+        ev[6].synchronize()
+        seg["build"] += ev[0].elapsed_ms(ev[1])
+        seg["pack"] += ev[1].elapsed_ms(ev[2])
+        seg["primary"] += ev[3].elapsed_ms(ev[4])
+        seg["bounce_gen"] += ev[4].elapsed_ms(ev[5])
+        seg["bounce"] += ev[5].elapsed_ms(ev[6])
+    call("rta_hip_device_synchronize")
+    wall_ms = (time.perf_counter() - t_start) * 1e3 / args.steps
+    K = args.steps
+    avg = {k: v / K for k, v in seg.items()}
+
+    e = int(err.download(np.uint32, 1)[0])
+    if e:
+        raise SystemExit(f"traversal reported {e} stack/guard errors")
+    n_bounce = int(count.download(np.uint32, 1)[0])
+    st = int(status.download(np.uint32, 1)[0])
+
+    # per-stage build breakdown (one extra, synchronising build)
+    times = rta_amd.abi.RtaTimes()
+    call("rta_hip_memcpy_dtod", ptr(tris), ptr(pristine), C.c_size_t(104 * n), sp)
+    call("rta_lbvh_build", ptr(tris), n, ptr(nodes), ptr(idx), ptr(ws), ws_bytes, ptr(status),
+         C.byref(times), sp)
+    stage_ms = times.as_dict()
+
+    # per-ray inner-node visits V and triangle tests T on the bounce rays,
+    # from the instrumented plain kernel (equal to the oracle's counts, see
+    # tests/test_gpu_parity.py::test_trace_config1_bit_exact)
+    stats = DeviceBuffer(8 * max(n_bounce, 1))
+    call("rta_reset_intersections", ptr(hits_b), R, 1, sp)
+    call("rta_trace_cis", ptr(tris), n, ptr(nodes), ptr(bnc.gpu_origin), ptr(bnc.gpu_direction),
+         ptr(bnc.gpu_maxt), n_bounce, 1, ptr(hits_b), ptr(stats), ptr(err), sp)
+    s.synchronize()
+    vt = stats.download(np.uint32, 2 * n_bounce).reshape(-1, 2).astype(np.float64)
+    V, T = float(vt[:, 0].mean()), float(vt[:, 1].mean())
+    b_ray = 28 + 16 + 64 * V + 36 * T
+    achieved = b_ray * n_bounce / (avg["bounce"] * 1e-3) / 1e9
+
+    bounce_mrays = n_bounce / (avg["bounce"] * 1e3)
+    primary_mrays = R / (avg["primary"] * 1e3)
+    build_mtri = n / (avg["build"] * 1e3)
+    log(f"primary {primary_mrays:.1f} Mrays/s, bounce {bounce_mrays:.1f} Mrays/s, "
+        f"build {build_mtri:.1f} Mtri/s ({avg['build']:.3f} ms), V={V:.1f} T={T:.2f}")
+
+    cpu = None
+    if not args.no_cpu:
+        log("CPU baseline (oracle restatement of the reference CPU plugin)")
+        o, d, t = bnc.download(n_bounce)
+        cpu = cpu_baseline(tris_h, o, d, t, n_bounce, cam.w, args.cpu_rays)
+        log(f"CPU {cpu['value']:.3f} Mrays/s on {cpu['cores']} threads")
+
+    out = {
+        "metric": METRIC,
+        "value": round(bounce_mrays, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall_ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": ("2,297,152-triangle heightfield+random scene, LBVH build + 1920x1080 "
+                         "primary rays + one diffuse bounce per hit; value = plain closest-hit "
+                         "Mrays/s on the bounce rays"),
+            "n_tris": n, "primary_rays": R, "bounce_rays": n_bounce,
+            "traversal": "child-is, packed 64-B child-pair nodes, LDS stack",
+        },
+        "primary_mrays_s": round(primary_mrays, 3),
+        "bounce_mrays_s": round(bounce_mrays, 3),
+        "build_mtri_s": round(build_mtri, 3),
+        "build_ms": round(avg["build"], 4),
+        "pack_ms": round(avg["pack"], 4),
+        "bounce_gen_ms": round(avg["bounce_gen"], 4),
+        "primary_trace_ms": round(avg["primary"], 4),
+        "bounce_trace_ms": round(avg["bounce"], 4),
+        "build_stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "lbvh_status": st,
+        "roofline": {
+            "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "trace_cis_packed_kernel (bounce rays)",
+            "bytes_per_ray": round(b_ray, 1), "V": round(V, 3), "T": round(T, 3),
+        },
+        "cpu_baseline": cpu,
+        "device": dev_name,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

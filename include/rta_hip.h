@@ -1,0 +1,249 @@
+/*
+ * rta_hip.h -- C-ABI of the bbvh-hip GPU BVH path for MI355X (gfx950).
+ *
+ * This is the drop-in boundary below the bbvh-hip plugin: plain pointers,
+ * sizes and an explicit stream, no C++ or framework types.  Every entry
+ * point returns an int status: 0 on success, a negative RTA_E_* code for
+ * argument / state errors detected on the host, or a positive HIP runtime
+ * error code.  rta_hip_last_error() gives a human readable message for the
+ * last failure on the calling thread.  Nothing in this library exits the
+ * process (the reference prints and calls exit(), librta/cuda.h:70-78).
+ *
+ * Device pointers are plain addresses from rta_hip_malloc (or any hipMalloc
+ * allocation of the same process).  Launches are asynchronous on the given
+ * stream; NULL means the legacy default stream.
+ *
+ * Data layouts are byte-compatible with the reference:
+ *   rta_triangle  104 B  librta/basic_types.h:67-74, librta/cuda-vec.h:30-37
+ *   rta_aabb       24 B  librta/cuda-vec.h:40-43
+ *   rta_node       32 B  bbvh/bbvh-node.h:5-43 (== bbvh_node_float4 byte
+ *                        image, bbvh-cuda/bbvh-cuda-node.h:37-91)
+ *   rta_hit        16 B  librta/basic_types.h:81-95
+ * Rays are three arrays: origin[3R], direction[3R] (packed float3) and
+ * max_t[R] (librta/cuda.h:90-107).
+ */
+#ifndef RTA_HIP_H
+#define RTA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ types */
+
+typedef struct rta_float3 { float x, y, z; } rta_float3;
+typedef struct rta_float2 { float x, y; } rta_float2;
+
+typedef struct __attribute__((aligned(8))) rta_triangle {
+	rta_float3 a, b, c;
+	rta_float3 na, nb, nc;
+	rta_float2 ta, tb, tc;
+	int32_t material_index;
+} rta_triangle;
+
+typedef struct rta_aabb { rta_float3 min, max; } rta_aabb;
+
+/* type_left_elems: bit0 = inner, >>1 = left child (inner) or triangle count
+ * (leaf); right_tris: right child (inner) or first triangle (leaf). */
+typedef struct rta_node {
+	uint32_t type_left_elems;
+	uint32_t right_tris;
+	rta_aabb box;
+} rta_node;
+
+typedef struct rta_hit { float t, beta, gamma; uint32_t ref; } rta_hit;
+
+/* Derived traversal layout: one 64-byte record per inner node holding both
+ * children's link words and boxes (SoA within the record), so an inner node
+ * visit is one aligned 64-byte fetch.  Built losslessly from rta_node[]. */
+typedef struct __attribute__((aligned(64))) rta_node_pair {
+	float lo_x[2], lo_y[2], lo_z[2];
+	float hi_x[2], hi_y[2], hi_z[2];
+	uint32_t link[2];   /* child's type_left_elems word                 */
+	uint32_t index[2];  /* child's right_tris word (leaf: first tri)    */
+} rta_node_pair;
+
+/* Per-stage build times in milliseconds (hipEvent based), in the stage
+ * order of bbvh-cuda/lbvh.h:173-198. */
+typedef struct rta_lbvh_times {
+	float init_triangles, voxel_bounds, morton, sort, tree, boxes, permute;
+	float total;
+} rta_lbvh_times;
+
+typedef void *rta_stream;
+typedef void *rta_event;
+
+enum {
+	RTA_OK = 0,
+	RTA_E_INVALID = -1,     /* bad argument (NULL pointer, n <= 0, ...)  */
+	RTA_E_WORKSPACE = -2,   /* workspace smaller than required           */
+	RTA_E_NOT_READY = -3,   /* library not initialised / no device       */
+	RTA_E_STACK = -4        /* traversal stack overflow reported by GPU  */
+};
+
+/* Build status bits written by rta_lbvh_build into *status (device word). */
+enum {
+	RTA_LBVH_CODE_ZERO = 1u /* some Morton code is 0: the reference's
+	                           bottom-up pass is racy there
+	                           (bbvh-cuda/lbvh-kernels.cu:288-291)        */
+};
+
+/* ---------------------------------------------------------------- runtime */
+
+const char *rta_hip_last_error(void);
+int rta_hip_device_count(int *count);
+int rta_hip_set_device(int device);
+int rta_hip_device_name(char *buf, int buflen);
+int rta_hip_malloc(void **ptr, size_t bytes);
+int rta_hip_free(void *ptr);
+int rta_hip_memcpy_htod(void *dst, const void *src, size_t bytes);
+int rta_hip_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int rta_hip_memcpy_dtod(void *dst, const void *src, size_t bytes, rta_stream s);
+/* Fill with a kernel (not hipMemset), safe inside captured graphs. */
+int rta_hip_fill_u32(uint32_t *dst, uint32_t value, size_t count, rta_stream s);
+int rta_hip_stream_create(rta_stream *s);
+int rta_hip_stream_destroy(rta_stream s);
+int rta_hip_stream_synchronize(rta_stream s);
+int rta_hip_device_synchronize(void);
+int rta_hip_event_create(rta_event *e);
+int rta_hip_event_destroy(rta_event e);
+int rta_hip_event_record(rta_event e, rta_stream s);
+int rta_hip_event_synchronize(rta_event e);
+int rta_hip_event_elapsed_ms(float *ms, rta_event start, rta_event stop);
+/* Plain coalesced copy kernel of `bytes` (multiple of 16), used to calibrate
+ * HBM traffic counters against a kernel of known size. */
+int rta_hip_copy_kernel(void *dst, const void *src, size_t bytes, rta_stream s);
+
+/* ---------------------------------------------- LBVH stages (Karras 2012) */
+
+/* initTriangles, bbvh-cuda/lbvh-kernels.cu:344-359,482-493:
+ * scratch[i] = src[i]; boxes[i] = AABB(tri i); centers[i] = ((a+b)+c)/3.
+ * Also writes one partial scene box per 256 triangles into `partials`
+ * (ceil(n/256) entries) for rta_lbvh_voxel_bounds. */
+int rta_lbvh_init_triangles(rta_triangle *scratch, const rta_triangle *src,
+                            rta_aabb *boxes, rta_float3 *centers,
+                            rta_aabb *partials, int n, rta_stream s);
+
+/* calculateVoxelBounds, bbvh-cuda/lbvh-kernels.cu:518-535: min/max over the
+ * per-block partials (initial box +-1e11f).  The result stays on device. */
+int rta_lbvh_voxel_bounds(rta_aabb *bounds, const rta_aabb *partials,
+                          int n, rta_stream s);
+
+/* calculateMortonCodes2 + k::morton_codes2, lbvh-kernels.cu:169-183,405-421:
+ * d = 1024/(max-min) per axis, q = (uint)((c-min)*d) & 0x3ff, 30-bit
+ * interleave (x in bit 0); indices[i] = i.  ORs RTA_LBVH_CODE_ZERO into
+ * *status (device word, may be NULL) when a code is 0. */
+int rta_lbvh_morton_codes(uint32_t *codes, uint32_t *indices,
+                          const rta_float3 *centers, const rta_aabb *bounds,
+                          uint32_t *status, int n, rta_stream s);
+
+/* sortMortonCodes, lbvh-kernels.cu:537-555: stable ascending sort of the
+ * 30-bit codes carrying indices (hand-written LDS radix sort). */
+int rta_lbvh_sort_work_bytes(int n, size_t *bytes);
+int rta_lbvh_sort(uint32_t *codes, uint32_t *indices, int n,
+                  void *work, size_t work_bytes, rta_stream s);
+
+/* buildTreeKarras + k::build_tree_karras, lbvh-kernels.cu:186-256,424-446:
+ * inner nodes 0..n-2, leaves n-1..2n-2; writes links and parents[child];
+ * also clears visit[0..n-2] for the bottom-up pass. */
+int rta_lbvh_build_tree(rta_node *nodes, uint32_t *parents, uint32_t *visit,
+                        const uint32_t *codes, int n, rta_stream s);
+
+/* buildBoundingBoxesKarras2 + k::build_bounding_boxes_karras2,
+ * lbvh-kernels.cu:258-341,448-478: bottom-up box union with an atomic visit
+ * counter; a parent whose two children are leaves with equal representative
+ * codes collapses into one multi-triangle leaf.  `visit` must be zero
+ * (rta_lbvh_build_tree clears it). */
+int rta_lbvh_build_boxes(rta_node *nodes, uint32_t *visit,
+                         const rta_aabb *boxes, const uint32_t *codes,
+                         const uint32_t *indices, const uint32_t *parents,
+                         int n, rta_stream s);
+
+/* invert_triangle_permutation_cuda, lbvh-kernels.cu:557-578:
+ * out[i] = scratch[indices[i]]. */
+int rta_lbvh_permute(rta_triangle *out, const rta_triangle *scratch,
+                     const uint32_t *indices, int n, rta_stream s);
+
+/* lbvh_constructor::buildCUDA, bbvh-cuda/lbvh.h:153-211: all seven stages.
+ * tris (device, n) is reordered in place into Morton order (it is the
+ * acceleration structure's triangle array afterwards); nodes (2n-1) and
+ * indices (n, sorted position -> input index) are outputs.  `status` is an
+ * optional device word (RTA_LBVH_* bits).  times may be NULL; when given,
+ * the call synchronises the stream to read the stage events. */
+int rta_lbvh_workspace_bytes(int n, size_t *bytes);
+int rta_lbvh_build(rta_triangle *tris, int n, rta_node *nodes,
+                   uint32_t *indices, void *workspace, size_t workspace_bytes,
+                   uint32_t *status, rta_lbvh_times *times, rta_stream s);
+
+/* --------------------------------------------------------------- tracing */
+
+/* reset_intersections, librta/cuda-kernels.cu:24-44: t = FLT_MAX, beta = 1,
+ * gamma = 2, ref untouched, for w*h hits. */
+int rta_reset_intersections(rta_hit *hits, int w, int h, rta_stream s);
+
+/* setup_shirley_rays, librta/cuda-kernels.cu:50-112 (pinhole camera of
+ * librta/raytrav.h:308-351).  The camera basis and tan(fovy/2) are formed on
+ * the host once, the per-pixel arithmetic on the GPU is the host
+ * generator's, so the rays are bit-identical to rta_host_shirley_rays. */
+int rta_setup_shirley_rays(float *dirs, float *origins, float *max_t,
+                           float fovy, float aspect, int w, int h,
+                           const float *view_dir, const float *pos,
+                           const float *up, float maxt, rta_stream s);
+
+/* trace_cis (2f4 CIS) + launcher, bbvh-cuda/kernels.cu:777-855,1217-1226:
+ * closest hit over the 32-byte node array, child-is, near child first
+ * (left wins ties), strict t < closest.t and t <= max_t.  Rays 0..w*h-1;
+ * hits must hold the reset sentinel (rta_reset_intersections): a miss keeps
+ * it.  stats (optional, 2 words per ray) receives the number of inner-node
+ * visits and triangle tests of each ray.  err (optional device word) counts
+ * stack overflows (RTA_E_STACK on the host side when non-zero). */
+int rta_trace_cis(const rta_triangle *tris, int n_tris, const rta_node *nodes,
+                  const float *ray_orig, const float *ray_dir,
+                  const float *max_t, int w, int h, rta_hit *hits,
+                  uint32_t *stats, uint32_t *err, rta_stream s);
+
+/* Derived layout for the fast path: pairs[i] for inner node i (n_nodes/2
+ * records) and the vertex-only triangle array tri4 (3 float4 per triangle:
+ * a, a-b, a-c with the same float values intersect_tri_opt forms). */
+int rta_bvh_pack(rta_node_pair *pairs, float *tri4, const rta_node *nodes,
+                 int n_nodes, const rta_triangle *tris, int n_tris,
+                 rta_stream s);
+
+/* Same semantics and results as rta_trace_cis over the packed layout.
+ * root_link/root_index: the root node's two link words (host copies of
+ * nodes[0].type_left_elems / right_tris).  n_rays bounds the launch; when
+ * ray_count (device word) is given, only rays below min(n_rays, *ray_count)
+ * are traced, so a ray count produced on the GPU (rta_diffuse_bounce) needs
+ * no host round trip. */
+int rta_trace_cis_packed(const float *tri4, const rta_node_pair *pairs,
+                         uint32_t root_link, uint32_t root_index,
+                         const float *ray_orig, const float *ray_dir,
+                         const float *max_t, int n_rays,
+                         const uint32_t *ray_count, rta_hit *hits,
+                         uint32_t *err, rta_stream s);
+
+/* -------------------------------------------------------------- bouncing */
+
+/* One cosine-weighted diffuse bounce per hit (the incoherent ray set).
+ * For every ray i with hits[i].t != FLT_MAX, in ray order, writes a bounce
+ * ray: origin = o + t*d + 1e-4*n, n the unit geometric normal of triangle
+ * hits[i].ref facing the incoming ray, direction = normalize(n + s) with s
+ * uniform on the unit sphere (Marsaglia 1972, counter-based RNG keyed on
+ * (seed, i)), max_t = FLT_MAX.  src_index[k] = i.  *count (device word) =
+ * number of bounce rays.  work: rta_bounce_work_bytes(n_rays). */
+int rta_bounce_work_bytes(int n_rays, size_t *bytes);
+int rta_diffuse_bounce(float *out_orig, float *out_dir, float *out_maxt,
+                       uint32_t *src_index, uint32_t *count,
+                       const rta_hit *hits, const rta_triangle *tris,
+                       const float *ray_orig, const float *ray_dir,
+                       int n_rays, uint64_t seed, void *work,
+                       size_t work_bytes, rta_stream s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RTA_HIP_H */

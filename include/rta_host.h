@@ -1,0 +1,53 @@
+/*
+ * rta_host.h -- host-side helpers of the bbvh-hip path (plain C ABI):
+ * seeded synthetic scene generators and the host pinhole ray generator.
+ *
+ * All generators are deterministic functions of their arguments (counter
+ * based hashing, no global RNG state), so every run and every process sees
+ * byte-identical scenes and rays.
+ */
+#ifndef RTA_HOST_H
+#define RTA_HOST_H
+
+#include <stdint.h>
+#include "rta_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Icosphere: regular icosahedron subdivided `subdiv` times (shared
+ * vertices), projected to `radius` around `center`.  20*4^subdiv triangles;
+ * normals are the normalised positions, texcoords 0, material -1.
+ * Returns the triangle count; `out` may be NULL to query it.  `jitter` > 0
+ * displaces every shared vertex radially by jitter*U(-1,1) (seeded). */
+int rta_scene_icosphere(int subdiv, float radius, const float *center, float jitter,
+                        uint64_t seed, rta_triangle *out);
+
+/* Heightfield over [x0,x1]x[y0,y1]: nx*ny quads of two triangles each (fixed
+ * diagonal), z = z0 + amplitude * fBm(x, y) where fBm is `octaves` octaves
+ * of seeded value noise (base frequency `base_freq` lattice cells per unit,
+ * lacunarity, gain), normalised by the octave weight sum (|fBm| <= 1).
+ * Returns 2*nx*ny; `out` may be NULL. */
+int rta_scene_heightfield(int nx, int ny, float x0, float y0, float x1, float y1, float z0,
+                          uint64_t seed, int octaves, float amplitude, float lacunarity,
+                          float gain, float base_freq, rta_triangle *out);
+
+/* `count` small triangles: centres uniform in [lo,hi]^3, each vertex at
+ * centre + len_k * dir_k with len_k uniform in [edge_min, edge_max] and dir_k
+ * uniform on the sphere.  Returns count; `out` may be NULL. */
+int rta_scene_random_triangles(int count, uint64_t seed, float lo, float hi, float edge_min,
+                               float edge_max, rta_triangle *out);
+
+/* Host version of the pinhole generator (librta/raytrav.h:300-351): the
+ * same camera basis and per-pixel arithmetic as rta_setup_shirley_rays, so
+ * host and device rays are bit-identical.  Ray id = y*w + x. */
+int rta_host_shirley_rays(float *dirs, float *origins, float *max_t, float fovy, float aspect,
+                          int w, int h, const float *view_dir, const float *pos, const float *up,
+                          float maxt);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RTA_HOST_H */

@@ -1,0 +1,487 @@
+// Karras LBVH build for gfx950: the seven stages of the reference builder
+// (bbvh-cuda/lbvh.h:153-211, bbvh-cuda/lbvh-kernels.cu) as hand-written HIP
+// kernels.  Results are bit-identical to the reference's semantics; where
+// the reference is racy the behaviour chosen here is deterministic and
+// documented at the kernel.
+#include "common.hpp"
+
+#include <cstdio>
+
+namespace rta_hip {
+
+constexpr int kInitBlock = 256;
+constexpr int kTriWords = sizeof(rta_triangle) / 8;   // 13 x 8 bytes
+constexpr float kBoundsInit = 100000000000.0f;        // lbvh-kernels.cu:519
+
+// ---------------------------------------------------------- wave helpers
+
+__device__ inline float wave_min(float v) {
+	for (int off = kWave / 2; off > 0; off >>= 1) v = sel_min(v, __shfl_xor(v, off));
+	return v;
+}
+__device__ inline float wave_max(float v) {
+	for (int off = kWave / 2; off > 0; off >>= 1) v = sel_max(v, __shfl_xor(v, off));
+	return v;
+}
+
+// ------------------------------------------------ stage 1: initTriangles
+
+// One block handles 256 consecutive triangles (26,624 contiguous bytes):
+// a coalesced 8-byte-word copy src -> scratch staged through LDS, then each
+// thread forms its triangle's box (lbvh-kernels.cu:141-153) and centroid
+// ((a+b)+c)/3.0f (lbvh-kernels.cu:60-63) from LDS, and the block reduces
+// its boxes into one partial scene box.
+__global__ __launch_bounds__(kInitBlock) void init_triangles_kernel(
+    rta_triangle *__restrict__ scratch, const rta_triangle *__restrict__ src,
+    rta_aabb *__restrict__ boxes, rta_float3 *__restrict__ centers,
+    rta_aabb *__restrict__ partials, int n) {
+	__shared__ uint2 words[kInitBlock * kTriWords];
+	__shared__ float red[6][kInitBlock / kWave];
+
+	const int tid = threadIdx.x;
+	const long long base = (long long)blockIdx.x * kInitBlock;
+	const int count = (int)((n - base) < kInitBlock ? (n - base) : kInitBlock);
+	const int nwords = count * kTriWords;
+	const uint2 *gsrc = reinterpret_cast<const uint2 *>(src + base);
+	uint2 *gdst = reinterpret_cast<uint2 *>(scratch + base);
+	for (int w = tid; w < nwords; w += kInitBlock) {
+		uint2 v = gsrc[w];
+		words[w] = v;
+		gdst[w] = v;
+	}
+	__syncthreads();
+
+	float mn[3] = {kBoundsInit, kBoundsInit, kBoundsInit};
+	float mx[3] = {-kBoundsInit, -kBoundsInit, -kBoundsInit};
+	if (tid < count) {
+		const float *f = reinterpret_cast<const float *>(&words[tid * kTriWords]);
+		float lo[3], hi[3], ce[3];
+		for (int c = 0; c < 3; ++c) {
+			float va = f[c], vb = f[3 + c], vc = f[6 + c];
+			lo[c] = sel_min(sel_min(va, vb), vc);
+			hi[c] = sel_max(sel_max(va, vb), vc);
+			ce[c] = ((va + vb) + vc) / 3.0f;
+			mn[c] = lo[c];
+			mx[c] = hi[c];
+		}
+		rta_aabb b;
+		b.min = {lo[0], lo[1], lo[2]};
+		b.max = {hi[0], hi[1], hi[2]};
+		boxes[base + tid] = b;
+		centers[base + tid] = {ce[0], ce[1], ce[2]};
+	}
+	// block reduction of the boxes (min/max are exact; the order only
+	// matters for the sign of a zero, which cannot change a Morton code)
+	const int lane = tid % kWave, wv = tid / kWave;
+	for (int c = 0; c < 3; ++c) {
+		float a = wave_min(mn[c]);
+		float b = wave_max(mx[c]);
+		if (lane == 0) { red[c][wv] = a; red[3 + c][wv] = b; }
+	}
+	__syncthreads();
+	if (tid == 0) {
+		float r[6];
+		for (int c = 0; c < 3; ++c) { r[c] = kBoundsInit; r[3 + c] = -kBoundsInit; }
+		for (int w = 0; w < kInitBlock / kWave; ++w)
+			for (int c = 0; c < 3; ++c) {
+				r[c] = sel_min(r[c], red[c][w]);
+				r[3 + c] = sel_max(r[3 + c], red[3 + c][w]);
+			}
+		rta_aabb p;
+		p.min = {r[0], r[1], r[2]};
+		p.max = {r[3], r[4], r[5]};
+		partials[blockIdx.x] = p;
+	}
+}
+
+// ----------------------------------------------- stage 2: voxel bounds
+
+constexpr int kBoundsBlock = 1024;
+
+__global__ __launch_bounds__(kBoundsBlock) void voxel_bounds_kernel(
+    rta_aabb *__restrict__ bounds, const rta_aabb *__restrict__ partials, int np) {
+	__shared__ float red[6][kBoundsBlock / kWave];
+	float mn[3] = {kBoundsInit, kBoundsInit, kBoundsInit};
+	float mx[3] = {-kBoundsInit, -kBoundsInit, -kBoundsInit};
+	for (int i = threadIdx.x; i < np; i += kBoundsBlock) {
+		rta_aabb p = partials[i];
+		mn[0] = sel_min(mn[0], p.min.x); mn[1] = sel_min(mn[1], p.min.y); mn[2] = sel_min(mn[2], p.min.z);
+		mx[0] = sel_max(mx[0], p.max.x); mx[1] = sel_max(mx[1], p.max.y); mx[2] = sel_max(mx[2], p.max.z);
+	}
+	const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
+	for (int c = 0; c < 3; ++c) {
+		float a = wave_min(mn[c]);
+		float b = wave_max(mx[c]);
+		if (lane == 0) { red[c][wv] = a; red[3 + c][wv] = b; }
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		float r[6];
+		for (int c = 0; c < 3; ++c) { r[c] = kBoundsInit; r[3 + c] = -kBoundsInit; }
+		for (int w = 0; w < kBoundsBlock / kWave; ++w)
+			for (int c = 0; c < 3; ++c) {
+				r[c] = sel_min(r[c], red[c][w]);
+				r[3 + c] = sel_max(r[3 + c], red[3 + c][w]);
+			}
+		rta_aabb b;
+		b.min = {r[0], r[1], r[2]};
+		b.max = {r[3], r[4], r[5]};
+		*bounds = b;
+	}
+}
+
+// ------------------------------------------------ stage 3: Morton codes
+
+// d = GRIDSIZE/(max-min) is formed per thread with the same correctly
+// rounded division the reference does once on the host
+// (lbvh-kernels.cu:414-417), so every thread sees the identical d.
+__global__ __launch_bounds__(256) void morton_kernel(
+    uint32_t *__restrict__ codes, uint32_t *__restrict__ indices,
+    const rta_float3 *__restrict__ centers, const rta_aabb *__restrict__ bounds,
+    uint32_t *__restrict__ status, int n) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const rta_aabb vb = *bounds;
+	const float dx = 1024.0f / (vb.max.x - vb.min.x);
+	const float dy = 1024.0f / (vb.max.y - vb.min.y);
+	const float dz = 1024.0f / (vb.max.z - vb.min.z);
+	const rta_float3 c = centers[i];
+	const uint32_t x = f2u_rz_sat((c.x - vb.min.x) * dx);
+	const uint32_t y = f2u_rz_sat((c.y - vb.min.y) * dy);
+	const uint32_t z = f2u_rz_sat((c.z - vb.min.z) * dz);
+	const uint32_t code = morton_encode_10bit(x, y, z);
+	codes[i] = code;
+	indices[i] = (uint32_t)i;
+	if (code == 0u && status) atomicOr(status, (uint32_t)RTA_LBVH_CODE_ZERO);
+}
+
+// ----------------------------------------- stage 5: Karras hierarchy
+
+// Common prefix length of the augmented keys (code, position):
+// clz(ci^cj), or 32+clz(i^j) for equal codes, -1 outside [0,n)
+// (__cpl, lbvh-kernels.cu:84-92).
+__device__ inline int cpl(int i, int j, const uint32_t *__restrict__ codes, int n) {
+	if (j < 0 || j >= n) return -1;
+	const uint32_t ci = codes[i], cj = codes[j];
+	if (ci == cj) return __clz((int)((uint32_t)i ^ (uint32_t)j)) + 32;
+	return __clz((int)(ci ^ cj));
+}
+
+// Thread per node (k::build_tree_karras, lbvh-kernels.cu:186-256).  The
+// direction/range/split searches are the reference's, including its
+// ceil-halving split search that always runs in +1 direction from the
+// lower end of the range.  Inner nodes also clear their visit counter.
+__global__ __launch_bounds__(256) void karras_kernel(
+    rta_node *__restrict__ nodes, uint32_t *__restrict__ parents,
+    uint32_t *__restrict__ visit, const uint32_t *__restrict__ codes, int n) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= 2 * n - 1) return;
+	uint2 *links = reinterpret_cast<uint2 *>(&nodes[i]);
+	if (i >= n - 1) {
+		// leaf: tris = i-(n-1), elems = 1, leaf bit clear
+		*links = make_uint2(1u << 1, (uint32_t)(i - (n - 1)));
+		return;
+	}
+	int d = (cpl(i, i + 1, codes, n) - cpl(i, i - 1, codes, n)) >= 0 ? 1 : -1;
+	const int cpl_min = cpl(i, i - d, codes, n);
+	int l_max = 2;
+	while (cpl(i, i + l_max * d, codes, n) > cpl_min) l_max *= 2;
+	int l = 0;
+	for (int t = l_max / 2; t >= 1; t /= 2)
+		if (cpl(i, i + (l + t) * d, codes, n) > cpl_min) l += t;
+	const int j = i + l * d;
+	const int from = j < i ? j : i;
+	const int to = j < i ? i : j;
+
+	const int cpl_node = cpl(from, to, codes, n);
+	int s = 0;
+	for (int t = (l / 2 + l % 2); t >= 1; t = (t / 2 + t % 2)) {
+		if (cpl(from, from + (s + t), codes, n) > cpl_node) s += t;
+		if (t <= 1) break;
+	}
+	const int mid = from + s;
+	int left = mid, right = mid + 1;
+	if (from == mid) left += n - 1;
+	if (to == mid + 1) right += n - 1;
+	*links = make_uint2(((uint32_t)left << 1) | 1u, (uint32_t)right);
+	parents[left] = (uint32_t)i;
+	parents[right] = (uint32_t)i;
+	visit[i] = 0u;
+}
+
+// ----------------------------------------- stage 6: bottom-up boxes
+
+__device__ inline rta_aabb load_box(const rta_node *node) {
+	const float *f = reinterpret_cast<const float *>(node) + 2;
+	rta_aabb b;
+	b.min = {f[0], f[1], f[2]};
+	b.max = {f[3], f[4], f[5]};
+	return b;
+}
+__device__ inline void store_box(rta_node *node, const rta_aabb &b) {
+	float *f = reinterpret_cast<float *>(node) + 2;
+	f[0] = b.min.x; f[1] = b.min.y; f[2] = b.min.z;
+	f[3] = b.max.x; f[4] = b.max.y; f[5] = b.max.z;
+}
+
+// Thread per leaf, walking to the root (k::build_bounding_boxes_karras2,
+// lbvh-kernels.cu:258-341).  Deterministic form of the reference:
+//  * a separate visit counter decides first/second arrival (the reference
+//    sums representative codes into the flag, so a first arrival with code
+//    0 also stops the second thread, lbvh-kernels.cu:288-291);
+//  * acquire/release atomics make the first arrival's subtree visible;
+//  * the union is formed as MIN(left, right) / MAX(left, right) with the
+//    reference's `(a<b)?a:b`, fixing the sign of zero that the reference
+//    leaves to arrival order;
+//  * merge rule: the second arrival's representative code equals the first
+//    arrival's (code == old in the reference) and both children are leaves
+//    -> the parent becomes a leaf with tris = min, elems = sum
+//    (lbvh-kernels.cu:302-330).
+__global__ __launch_bounds__(256) void bottom_up_kernel(
+    rta_node *__restrict__ nodes, uint32_t *__restrict__ visit,
+    const rta_aabb *__restrict__ boxes, const uint32_t *__restrict__ codes,
+    const uint32_t *__restrict__ indices, const uint32_t *__restrict__ parents, int n) {
+	const int k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= n) return;
+	const uint32_t leaf0 = (uint32_t)(n - 1);
+	uint32_t cur = leaf0 + (uint32_t)k;
+	rta_aabb box = boxes[indices[k]];
+	store_box(&nodes[cur], box);
+
+	// A Karras tree over 62-bit augmented keys is at most 63 levels deep;
+	// the cap turns a corrupt parent array into a stop instead of a hang.
+	for (int level = 0; cur != 0u && level < 128; ++level) {
+		const uint32_t code = cur >= leaf0 ? codes[cur - leaf0] : codes[cur];
+		const uint32_t parent = parents[cur];
+		if (parent >= leaf0) return;
+		const uint32_t old = __hip_atomic_fetch_add(&visit[parent], 1u, __ATOMIC_ACQ_REL,
+		                                            __HIP_MEMORY_SCOPE_AGENT);
+		if (old == 0u) return;  // first arrival: the sibling thread continues
+
+		const uint2 pl = *reinterpret_cast<const uint2 *>(&nodes[parent]);
+		const uint32_t left = pl.x >> 1, right = pl.y;
+		const bool cur_is_left = (left == cur);
+		const uint32_t sib = cur_is_left ? right : left;
+		const rta_aabb sbox = load_box(&nodes[sib]);
+		const rta_aabb &lb = cur_is_left ? box : sbox;
+		const rta_aabb &rb = cur_is_left ? sbox : box;
+		rta_aabb u;
+		u.min.x = sel_min(lb.min.x, rb.min.x);
+		u.min.y = sel_min(lb.min.y, rb.min.y);
+		u.min.z = sel_min(lb.min.z, rb.min.z);
+		u.max.x = sel_max(lb.max.x, rb.max.x);
+		u.max.y = sel_max(lb.max.y, rb.max.y);
+		u.max.z = sel_max(lb.max.z, rb.max.z);
+		box = u;
+		store_box(&nodes[parent], box);
+
+		const uint32_t sib_code = sib >= leaf0 ? codes[sib - leaf0] : codes[sib];
+		if (code == sib_code) {
+			const uint2 ll = *reinterpret_cast<const uint2 *>(&nodes[left]);
+			const uint2 rl = *reinterpret_cast<const uint2 *>(&nodes[right]);
+			if (!link_inner(ll.x) && !link_inner(rl.x)) {
+				const uint32_t tris = ll.y < rl.y ? ll.y : rl.y;
+				const uint32_t elems = (rl.x >> 1) + (ll.x >> 1);
+				*reinterpret_cast<uint2 *>(&nodes[parent]) = make_uint2(elems << 1, tris);
+			}
+		}
+		cur = parent;
+	}
+}
+
+// -------------------------------------------------- stage 7: permute
+
+// out[i] = scratch[indices[i]] (invert_triangle_permutation_cuda,
+// lbvh-kernels.cu:557-578).  Each thread moves one 104-byte triangle as
+// thirteen 8-byte words.
+__global__ __launch_bounds__(256) void permute_kernel(
+    rta_triangle *__restrict__ out, const rta_triangle *__restrict__ scratch,
+    const uint32_t *__restrict__ indices, int n) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const uint2 *s = reinterpret_cast<const uint2 *>(scratch + indices[i]);
+	uint2 *d = reinterpret_cast<uint2 *>(out + i);
+	uint2 w[kTriWords];
+#pragma unroll
+	for (int k = 0; k < kTriWords; ++k) w[k] = s[k];
+#pragma unroll
+	for (int k = 0; k < kTriWords; ++k) d[k] = w[k];
+}
+
+// --------------------------------------------------------- workspace
+
+static inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+struct LbvhWorkspace {
+	rta_triangle *scratch;
+	rta_aabb *boxes;
+	rta_float3 *centers;
+	rta_aabb *partials;
+	rta_aabb *bounds;
+	uint32_t *codes;
+	uint32_t *parents;
+	uint32_t *visit;
+	void *sort_work;
+	size_t sort_bytes;
+	size_t total;
+};
+
+static LbvhWorkspace carve(void *base, int n) {
+	LbvhWorkspace w{};
+	size_t off = 0;
+	char *p = reinterpret_cast<char *>(base);
+	auto take = [&](size_t bytes) {
+		char *r = p ? p + off : nullptr;
+		off += align256(bytes);
+		return r;
+	};
+	const size_t nb = (size_t)n;
+	const size_t nparts = (nb + kInitBlock - 1) / kInitBlock;
+	w.scratch = (rta_triangle *)take(nb * sizeof(rta_triangle));
+	w.boxes = (rta_aabb *)take(nb * sizeof(rta_aabb));
+	w.centers = (rta_float3 *)take(nb * sizeof(rta_float3));
+	w.partials = (rta_aabb *)take(nparts * sizeof(rta_aabb));
+	w.bounds = (rta_aabb *)take(sizeof(rta_aabb));
+	w.codes = (uint32_t *)take(nb * 4);
+	w.parents = (uint32_t *)take((2 * nb - 1) * 4);
+	w.visit = (uint32_t *)take(nb * 4);
+	size_t sb = 0;
+	rta_lbvh_sort_work_bytes(n, &sb);
+	w.sort_work = take(sb);
+	w.sort_bytes = sb;
+	w.total = off;
+	return w;
+}
+
+}  // namespace rta_hip
+
+using namespace rta_hip;
+
+extern "C" {
+
+int rta_lbvh_init_triangles(rta_triangle *scratch, const rta_triangle *src, rta_aabb *boxes,
+                            rta_float3 *centers, rta_aabb *partials, int n, rta_stream s) {
+	if (n <= 0) return invalid("rta_lbvh_init_triangles: n must be > 0");
+	if (!scratch || !src || !boxes || !centers || !partials)
+		return invalid("rta_lbvh_init_triangles: NULL pointer");
+	hipLaunchKernelGGL(init_triangles_kernel, dim3(grid_for(n, kInitBlock)), dim3(kInitBlock), 0,
+	                   as_stream(s), scratch, src, boxes, centers, partials, n);
+	RTA_LAUNCH_CHECK("init_triangles_kernel");
+	return RTA_OK;
+}
+
+int rta_lbvh_voxel_bounds(rta_aabb *bounds, const rta_aabb *partials, int n, rta_stream s) {
+	if (n <= 0) return invalid("rta_lbvh_voxel_bounds: n must be > 0");
+	if (!bounds || !partials) return invalid("rta_lbvh_voxel_bounds: NULL pointer");
+	const int np = (int)grid_for(n, kInitBlock);
+	hipLaunchKernelGGL(voxel_bounds_kernel, dim3(1), dim3(kBoundsBlock), 0, as_stream(s), bounds,
+	                   partials, np);
+	RTA_LAUNCH_CHECK("voxel_bounds_kernel");
+	return RTA_OK;
+}
+
+int rta_lbvh_morton_codes(uint32_t *codes, uint32_t *indices, const rta_float3 *centers,
+                          const rta_aabb *bounds, uint32_t *status, int n, rta_stream s) {
+	if (n <= 0) return invalid("rta_lbvh_morton_codes: n must be > 0");
+	if (!codes || !indices || !centers || !bounds)
+		return invalid("rta_lbvh_morton_codes: NULL pointer");
+	hipLaunchKernelGGL(morton_kernel, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(s), codes,
+	                   indices, centers, bounds, status, n);
+	RTA_LAUNCH_CHECK("morton_kernel");
+	return RTA_OK;
+}
+
+int rta_lbvh_build_tree(rta_node *nodes, uint32_t *parents, uint32_t *visit,
+                        const uint32_t *codes, int n, rta_stream s) {
+	if (n <= 0) return invalid("rta_lbvh_build_tree: n must be > 0");
+	if (!nodes || !parents || !codes || (n > 1 && !visit))
+		return invalid("rta_lbvh_build_tree: NULL pointer");
+	hipLaunchKernelGGL(karras_kernel, dim3(grid_for(2LL * n - 1, 256)), dim3(256), 0,
+	                   as_stream(s), nodes, parents, visit, codes, n);
+	RTA_LAUNCH_CHECK("karras_kernel");
+	return RTA_OK;
+}
+
+int rta_lbvh_build_boxes(rta_node *nodes, uint32_t *visit, const rta_aabb *boxes,
+                         const uint32_t *codes, const uint32_t *indices,
+                         const uint32_t *parents, int n, rta_stream s) {
+	if (n <= 0) return invalid("rta_lbvh_build_boxes: n must be > 0");
+	if (!nodes || !boxes || !codes || !indices || !parents || (n > 1 && !visit))
+		return invalid("rta_lbvh_build_boxes: NULL pointer");
+	hipLaunchKernelGGL(bottom_up_kernel, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(s),
+	                   nodes, visit, boxes, codes, indices, parents, n);
+	RTA_LAUNCH_CHECK("bottom_up_kernel");
+	return RTA_OK;
+}
+
+int rta_lbvh_permute(rta_triangle *out, const rta_triangle *scratch, const uint32_t *indices,
+                     int n, rta_stream s) {
+	if (n <= 0) return invalid("rta_lbvh_permute: n must be > 0");
+	if (!out || !scratch || !indices) return invalid("rta_lbvh_permute: NULL pointer");
+	hipLaunchKernelGGL(permute_kernel, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(s), out,
+	                   scratch, indices, n);
+	RTA_LAUNCH_CHECK("permute_kernel");
+	return RTA_OK;
+}
+
+int rta_lbvh_workspace_bytes(int n, size_t *bytes) {
+	if (n <= 0 || !bytes) return invalid("rta_lbvh_workspace_bytes: bad arguments");
+	*bytes = carve(nullptr, n).total;
+	return RTA_OK;
+}
+
+int rta_lbvh_build(rta_triangle *tris, int n, rta_node *nodes, uint32_t *indices, void *workspace,
+                   size_t workspace_bytes, uint32_t *status, rta_lbvh_times *times,
+                   rta_stream s) {
+	// n = 0 is rejected, as the reference cannot build it (bbvh-cuda/lbvh.h:47)
+	if (n <= 0) return invalid("rta_lbvh_build: n must be > 0");
+	if (!tris || !nodes || !indices || !workspace) return invalid("rta_lbvh_build: NULL pointer");
+	LbvhWorkspace w = carve(workspace, n);
+	if (workspace_bytes < w.total) {
+		set_error("rta_lbvh_build: workspace too small");
+		return RTA_E_WORKSPACE;
+	}
+	hipEvent_t ev[8] = {};
+	if (times) {
+		for (int i = 0; i < 8; ++i) RTA_HIP_TRY(hipEventCreate(&ev[i]));
+		RTA_HIP_TRY(hipEventRecord(ev[0], as_stream(s)));
+	}
+	int rc = RTA_OK;
+	auto mark = [&](int k) {
+		if (times && rc == RTA_OK) {
+			hipError_t e = hipEventRecord(ev[k], as_stream(s));
+			if (e != hipSuccess) rc = hip_status(e, "hipEventRecord");
+		}
+	};
+	if (status) rc = rta_hip_fill_u32(status, 0u, 1, s);
+	if (rc == RTA_OK) rc = rta_lbvh_init_triangles(w.scratch, tris, w.boxes, w.centers, w.partials, n, s);
+	mark(1);
+	if (rc == RTA_OK) rc = rta_lbvh_voxel_bounds(w.bounds, w.partials, n, s);
+	mark(2);
+	if (rc == RTA_OK) rc = rta_lbvh_morton_codes(w.codes, indices, w.centers, w.bounds, status, n, s);
+	mark(3);
+	if (rc == RTA_OK) rc = rta_lbvh_sort(w.codes, indices, n, w.sort_work, w.sort_bytes, s);
+	mark(4);
+	if (rc == RTA_OK) rc = rta_lbvh_build_tree(nodes, w.parents, w.visit, w.codes, n, s);
+	mark(5);
+	if (rc == RTA_OK) rc = rta_lbvh_build_boxes(nodes, w.visit, w.boxes, w.codes, indices, w.parents, n, s);
+	mark(6);
+	if (rc == RTA_OK) rc = rta_lbvh_permute(tris, w.scratch, indices, n, s);
+	mark(7);
+	if (times) {
+		if (rc == RTA_OK) {
+			hipError_t e = hipEventSynchronize(ev[7]);
+			if (e != hipSuccess) rc = hip_status(e, "hipEventSynchronize");
+		}
+		if (rc == RTA_OK) {
+			float *slots[7] = {&times->init_triangles, &times->voxel_bounds, &times->morton,
+			                   &times->sort, &times->tree, &times->boxes, &times->permute};
+			for (int k = 0; k < 7; ++k) (void)hipEventElapsedTime(slots[k], ev[k], ev[k + 1]);
+			(void)hipEventElapsedTime(&times->total, ev[0], ev[7]);
+		}
+		for (int i = 0; i < 8; ++i) (void)hipEventDestroy(ev[i]);
+	}
+	return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,512 @@
+// Closest-hit traversal and the GPU glue around it: intersection reset,
+// pinhole ray generation, child-pair packing and the diffuse bouncer.
+//
+// Traversal semantics are those of the reference's 2f4 CIS kernel
+// (bbvh-cuda/kernels.cu:777-855), which for max_t = FLT_MAX equal the CPU
+// bbvh child-is tracer (bbvh/bbvh.h:407-455): at an inner node both child
+// boxes are tested, a child is kept iff it is hit with dist < closest.t and
+// dist <= max_t, the nearer child (left on ties) is entered and the other
+// pushed; leaves test their triangles in index order and accept
+// t < closest.t && t <= max_t.  Box and triangle tests are the exact ones
+// of librta/intersect.h (see common.hpp).
+#include "common.hpp"
+
+namespace rta_hip {
+
+constexpr int kTraceBlock = 128;
+constexpr int kStack = 32;   // the reference's stack depth (kernels.cu:787)
+// Upper bound on loop iterations of one ray: a malformed tree ends in an
+// error count instead of a hang.  A valid tree needs at most
+// 2*nodes iterations; 2^26 covers every tree this library can hold.
+constexpr uint32_t kMaxSteps = 1u << 26;
+
+// ------------------------------------------------------------- reset
+
+__global__ void reset_kernel(rta_hit *hits, int n) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	hits[i].t = FLT_MAX;
+	hits[i].beta = 1.0f;
+	hits[i].gamma = 2.0f;
+}
+
+// ---------------------------------------------------------- ray gen
+
+struct CamBasis {
+	float ux, uy, uz, vx, vy, vz, wx, wy, wz;
+	float px, py, pz;
+	float width, height;
+};
+
+__global__ void shirley_kernel(float *dirs, float *orgs, float *maxts, int w, int h, CamBasis cb,
+                               float maxt) {
+	const int x = blockIdx.x * blockDim.x + threadIdx.x;
+	const int y = blockIdx.y * blockDim.y + threadIdx.y;
+	if (x >= w || y >= h) return;
+	const size_t id = (size_t)y * w + x;
+	orgs[3 * id + 0] = cb.px;
+	orgs[3 * id + 1] = cb.py;
+	orgs[3 * id + 2] = cb.pz;
+	maxts[id] = maxt;
+	float u_s = ((float)x + 0.5f) / (float)w * 2.0f - 1.0f;
+	float v_s = ((float)y + 0.5f) / (float)h * 2.0f - 1.0f;
+	u_s = cb.width * u_s;
+	v_s = cb.height * v_s;
+	float dx = cb.ux * u_s, dy = cb.uy * u_s, dz = cb.uz * u_s;
+	dx = dx + cb.vx * v_s;
+	dy = dy + cb.vy * v_s;
+	dz = dz + cb.vz * v_s;
+	dx = dx + cb.wx;
+	dy = dy + cb.wy;
+	dz = dz + cb.wz;
+	const float len = sqrtf(dx * dx + dy * dy + dz * dz);
+	dirs[3 * id + 0] = dx / len;
+	dirs[3 * id + 1] = dy / len;
+	dirs[3 * id + 2] = dz / len;
+}
+
+// -------------------------------------------- plain CIS over rta_node[]
+
+// The per-lane stack lives in LDS, entry-major so that a wave's pushes and
+// pops at different depths spread over the banks.
+template <bool STATS>
+__global__ __launch_bounds__(kTraceBlock) void trace_cis_kernel(
+    const rta_triangle *__restrict__ tris, uint32_t n_tris, const rta_node *__restrict__ nodes,
+    uint32_t n_nodes, const float *__restrict__ ro, const float *__restrict__ rd,
+    const float *__restrict__ rmax, int n_rays, rta_hit *__restrict__ hits,
+    uint32_t *__restrict__ stats, uint32_t *__restrict__ err) {
+	__shared__ uint32_t stack[kStack][kTraceBlock];
+	const int lt = threadIdx.x;
+	const int tid = blockIdx.x * kTraceBlock + lt;
+	if (tid >= n_rays) return;
+
+	const float ox = ro[3 * tid + 0], oy = ro[3 * tid + 1], oz = ro[3 * tid + 2];
+	const float dx = rd[3 * tid + 0], dy = rd[3 * tid + 1], dz = rd[3 * tid + 2];
+	const float tmax = rmax[tid];
+	rta_hit closest = hits[tid];
+	closest.t = FLT_MAX;
+	uint32_t n_inner = 0, n_tested = 0;
+
+	const float4 *nv = reinterpret_cast<const float4 *>(nodes);
+	float4 curr = nv[0];
+	int sp = -1;
+	for (uint32_t guard = 0;; ++guard) {
+		const uint32_t cl = __float_as_uint(curr.x);
+		const uint32_t cr = __float_as_uint(curr.y);
+		if (guard >= kMaxSteps) { atomicAdd(err, 1u); break; }
+		if (link_inner(cl)) {
+			if (STATS) ++n_inner;
+			const uint32_t L = cl >> 1, R = cr;
+			if (L >= n_nodes || R >= n_nodes) { atomicAdd(err, 1u); break; }
+			const float4 l0 = nv[2 * L], l1 = nv[2 * L + 1];
+			const float4 r0 = nv[2 * R], r1 = nv[2 * R + 1];
+			float dl, dr;
+			bool hl = box_test(l0.z, l0.w, l1.x, l1.y, l1.z, l1.w, ox, oy, oz, dx, dy, dz, dl);
+			bool hr = box_test(r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, ox, oy, oz, dx, dy, dz, dr);
+			hl = hl && dl < closest.t && dl <= tmax;
+			hr = hr && dr < closest.t && dr <= tmax;
+			if (hl && hr) {
+				if (sp + 1 >= kStack) { atomicAdd(err, 1u); break; }
+				if (dl <= dr) { stack[++sp][lt] = R; curr = l0; }
+				else          { stack[++sp][lt] = L; curr = r0; }
+				continue;
+			}
+			if (hl) { curr = l0; continue; }
+			if (hr) { curr = r0; continue; }
+		} else {
+			const uint32_t elems = cl >> 1, off = cr;
+			if (off >= n_tris || elems > n_tris - off) { atomicAdd(err, 1u); break; }
+			for (uint32_t i = 0; i < elems; ++i) {
+				if (STATS) ++n_tested;
+				const float *tf = reinterpret_cast<const float *>(tris + off + i);
+				const float ax = tf[0], ay = tf[1], az = tf[2];
+				float t, be, ga;
+				if (tri_test(ax, ay, az, ax - tf[3], ay - tf[4], az - tf[5], ax - tf[6],
+				             ay - tf[7], az - tf[8], ox, oy, oz, dx, dy, dz, t, be, ga)) {
+					if (t < closest.t && t <= tmax) {
+						closest.t = t;
+						closest.beta = be;
+						closest.gamma = ga;
+						closest.ref = off + i;
+					}
+				}
+			}
+		}
+		if (sp < 0) break;
+		curr = nv[2 * stack[sp--][lt]];
+	}
+	hits[tid] = closest;
+	if (STATS) {
+		stats[2 * tid + 0] = n_inner;
+		stats[2 * tid + 1] = n_tested;
+	}
+}
+
+// ------------------------------------------------ child-pair packing
+
+// word for a child as seen from its parent's pair record:
+//   inner child c -> (c << 1) | 1        (index of its own pair record)
+//   leaf child    -> (first_tri << 1)    (the leaf's last triangle carries
+//                                         a flag in the tri4 array)
+__device__ inline uint32_t pair_word(const rta_node &c, uint32_t id) {
+	return link_inner(c.type_left_elems) ? ((id << 1) | 1u) : (c.right_tris << 1);
+}
+
+__global__ void pack_tris_kernel(float4 *__restrict__ tri4, const rta_triangle *__restrict__ tris,
+                                 int n) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const float *f = reinterpret_cast<const float *>(tris + i);
+	const float ax = f[0], ay = f[1], az = f[2];
+	tri4[3 * i + 0] = make_float4(ax, ay, az, 0.0f);
+	tri4[3 * i + 1] = make_float4(ax - f[3], ay - f[4], az - f[5], 0.0f);
+	tri4[3 * i + 2] = make_float4(ax - f[6], ay - f[7], az - f[8], 0.0f);
+}
+
+__device__ inline void mark_leaf_end(float4 *tri4, const rta_node &leaf) {
+	const uint32_t last = leaf.right_tris + (leaf.type_left_elems >> 1) - 1u;
+	tri4[3 * last].w = __uint_as_float(1u);
+}
+
+__global__ void pack_pairs_kernel(rta_node_pair *__restrict__ pairs, float4 *__restrict__ tri4,
+                                  const rta_node *__restrict__ nodes, int n_nodes) {
+	const int x = blockIdx.x * blockDim.x + threadIdx.x;
+	if (x >= n_nodes) return;
+	const rta_node nx = nodes[x];
+	if (!link_inner(nx.type_left_elems)) {
+		if (x == 0) mark_leaf_end(tri4, nx);  // a leaf root has no parent record
+		return;
+	}
+	const uint32_t L = nx.type_left_elems >> 1, R = nx.right_tris;
+	const rta_node nl = nodes[L], nr = nodes[R];
+	rta_node_pair p;
+	p.lo_x[0] = nl.box.min.x; p.lo_x[1] = nr.box.min.x;
+	p.lo_y[0] = nl.box.min.y; p.lo_y[1] = nr.box.min.y;
+	p.lo_z[0] = nl.box.min.z; p.lo_z[1] = nr.box.min.z;
+	p.hi_x[0] = nl.box.max.x; p.hi_x[1] = nr.box.max.x;
+	p.hi_y[0] = nl.box.max.y; p.hi_y[1] = nr.box.max.y;
+	p.hi_z[0] = nl.box.max.z; p.hi_z[1] = nr.box.max.z;
+	p.link[0] = pair_word(nl, L);
+	p.link[1] = pair_word(nr, R);
+	p.index[0] = L;
+	p.index[1] = R;
+	pairs[x] = p;
+	if (!link_inner(nl.type_left_elems)) mark_leaf_end(tri4, nl);
+	if (!link_inner(nr.type_left_elems)) mark_leaf_end(tri4, nr);
+}
+
+// -------------------------------------------- packed CIS (fast path)
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void trace_cis_packed_kernel(
+    const float4 *__restrict__ tri4, const float4 *__restrict__ pairs, uint32_t root_word,
+    const float *__restrict__ ro, const float *__restrict__ rd, const float *__restrict__ rmax,
+    int n_rays, const uint32_t *__restrict__ ray_count, rta_hit *__restrict__ hits,
+    uint32_t *__restrict__ err) {
+	__shared__ uint32_t stack[kStack][BLOCK];
+	const int lt = threadIdx.x;
+	const int tid = blockIdx.x * BLOCK + lt;
+	if (tid >= n_rays) return;
+	if (ray_count && (uint32_t)tid >= *ray_count) return;
+
+	const float ox = ro[3 * tid + 0], oy = ro[3 * tid + 1], oz = ro[3 * tid + 2];
+	const float dx = rd[3 * tid + 0], dy = rd[3 * tid + 1], dz = rd[3 * tid + 2];
+	const float tmax = rmax[tid];
+	rta_hit closest = hits[tid];
+	closest.t = FLT_MAX;
+
+	uint32_t curr = root_word;
+	int sp = -1;
+	for (uint32_t guard = 0;; ++guard) {
+		if (guard >= kMaxSteps) { atomicAdd(err, 1u); break; }
+		if (curr & 1u) {
+			const float4 *p = pairs + 4 * (size_t)(curr >> 1);
+			const float4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
+			float dl, dr;
+			bool hl = box_test(q0.x, q0.z, q1.x, q1.z, q2.x, q2.z, ox, oy, oz, dx, dy, dz, dl);
+			bool hr = box_test(q0.y, q0.w, q1.y, q1.w, q2.y, q2.w, ox, oy, oz, dx, dy, dz, dr);
+			hl = hl && dl < closest.t && dl <= tmax;
+			hr = hr && dr < closest.t && dr <= tmax;
+			const uint32_t wl = __float_as_uint(q3.x), wr = __float_as_uint(q3.y);
+			if (hl && hr) {
+				if (sp + 1 >= kStack) { atomicAdd(err, 1u); break; }
+				if (dl <= dr) { stack[++sp][lt] = wr; curr = wl; }
+				else          { stack[++sp][lt] = wl; curr = wr; }
+				continue;
+			}
+			if (hl) { curr = wl; continue; }
+			if (hr) { curr = wr; continue; }
+		} else {
+			uint32_t i = curr >> 1;
+			while (true) {
+				const float4 v0 = tri4[3 * (size_t)i + 0];
+				const float4 v1 = tri4[3 * (size_t)i + 1];
+				const float4 v2 = tri4[3 * (size_t)i + 2];
+				float t, be, ga;
+				if (tri_test(v0.x, v0.y, v0.z, v1.x, v1.y, v1.z, v2.x, v2.y, v2.z, ox, oy, oz, dx,
+				             dy, dz, t, be, ga)) {
+					if (t < closest.t && t <= tmax) {
+						closest.t = t;
+						closest.beta = be;
+						closest.gamma = ga;
+						closest.ref = i;
+					}
+				}
+				if (__float_as_uint(v0.w) != 0u) break;
+				++i;
+			}
+		}
+		if (sp < 0) break;
+		curr = stack[sp--][lt];
+	}
+	hits[tid] = closest;
+}
+
+// ------------------------------------------------------- bouncing
+
+constexpr int kBounceBlock = 256;
+
+__global__ __launch_bounds__(kBounceBlock) void bounce_count_kernel(const rta_hit *__restrict__ hits,
+                                                                    int n, uint32_t *__restrict__ block_counts) {
+	__shared__ uint32_t wsum[kBounceBlock / kWave];
+	const int i = blockIdx.x * kBounceBlock + threadIdx.x;
+	const bool hit = i < n && hits[i].t != FLT_MAX;
+	const uint64_t b = __ballot(hit);
+	if (__lane_id() == 0) wsum[threadIdx.x / kWave] = (uint32_t)__popcll(b);
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		uint32_t s = 0;
+		for (int w = 0; w < kBounceBlock / kWave; ++w) s += wsum[w];
+		block_counts[blockIdx.x] = s;
+	}
+}
+
+// single block: exclusive scan of block counts, total into *count
+__global__ __launch_bounds__(1024) void bounce_scan_kernel(uint32_t *__restrict__ block_counts, int nb,
+                                                           uint32_t *__restrict__ count) {
+	__shared__ uint32_t wsum[1024 / kWave];
+	__shared__ uint32_t carry_s;
+	if (threadIdx.x == 0) carry_s = 0;
+	__syncthreads();
+	for (int base = 0; base < nb; base += 1024) {
+		const int idx = base + threadIdx.x;
+		const uint32_t v = idx < nb ? block_counts[idx] : 0u;
+		uint32_t x = v;
+		const int lane = __lane_id();
+		for (int off = 1; off < kWave; off <<= 1) {
+			uint32_t y = __shfl_up(x, off);
+			if (lane >= off) x += y;
+		}
+		if (lane == kWave - 1) wsum[threadIdx.x / kWave] = x;
+		__syncthreads();
+		uint32_t before = 0, all = 0;
+		for (int w = 0; w < 1024 / kWave; ++w) {
+			const uint32_t s = wsum[w];
+			if (w < (int)(threadIdx.x / kWave)) before += s;
+			all += s;
+		}
+		const uint32_t carry = carry_s;
+		if (idx < nb) block_counts[idx] = carry + before + x - v;
+		__syncthreads();
+		if (threadIdx.x == 0) carry_s = carry + all;
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) *count = carry_s;
+}
+
+__device__ inline void bounce_ray(const rta_hit &h, const rta_triangle *tris, float ox, float oy,
+                                  float oz, float dx, float dy, float dz, uint64_t seed, uint32_t i,
+                                  float *o_out, float *d_out) {
+	const float *f = reinterpret_cast<const float *>(tris + h.ref);
+	const float e1x = f[3] - f[0], e1y = f[4] - f[1], e1z = f[5] - f[2];
+	const float e2x = f[6] - f[0], e2y = f[7] - f[1], e2z = f[8] - f[2];
+	float nx = e1y * e2z - e1z * e2y;
+	float ny = e1z * e2x - e1x * e2z;
+	float nz = e1x * e2y - e1y * e2x;
+	const float nl = sqrtf(nx * nx + ny * ny + nz * nz);
+	nx = nx / nl; ny = ny / nl; nz = nz / nl;
+	if (nx * dx + ny * dy + nz * dz > 0.0f) { nx = -nx; ny = -ny; nz = -nz; }
+	const float px = ox + h.t * dx, py = oy + h.t * dy, pz = oz + h.t * dz;
+	o_out[0] = px + 1e-4f * nx;
+	o_out[1] = py + 1e-4f * ny;
+	o_out[2] = pz + 1e-4f * nz;
+	// uniform point on the unit sphere, Marsaglia (1972)
+	const uint64_t base = mix64(seed ^ mix64((uint64_t)i));
+	float x1 = 0.0f, x2 = 0.0f, s = 0.0f;
+	for (uint32_t it = 0; it < 64; ++it) {
+		const uint64_t r = mix64(base + it);
+		const float u1 = (float)(uint32_t)(r >> 40) * (1.0f / 16777216.0f);
+		const float u2 = (float)(uint32_t)((r >> 16) & 0xffffffu) * (1.0f / 16777216.0f);
+		const float a = 2.0f * u1 - 1.0f, b = 2.0f * u2 - 1.0f;
+		const float q = a * a + b * b;
+		if (q < 1.0f) { x1 = a; x2 = b; s = q; break; }
+	}
+	const float sq = sqrtf(1.0f - s);
+	const float sx = (2.0f * x1) * sq, sy = (2.0f * x2) * sq, sz = 1.0f - 2.0f * s;
+	float vx = nx + sx, vy = ny + sy, vz = nz + sz;
+	const float vl = sqrtf(vx * vx + vy * vy + vz * vz);
+	if (vl > 0.0f) { vx = vx / vl; vy = vy / vl; vz = vz / vl; }
+	else { vx = nx; vy = ny; vz = nz; }
+	d_out[0] = vx; d_out[1] = vy; d_out[2] = vz;
+}
+
+__global__ __launch_bounds__(kBounceBlock) void bounce_emit_kernel(
+    float *__restrict__ out_o, float *__restrict__ out_d, float *__restrict__ out_t,
+    uint32_t *__restrict__ src_index, const uint32_t *__restrict__ block_offsets,
+    const rta_hit *__restrict__ hits, const rta_triangle *__restrict__ tris,
+    const float *__restrict__ ro, const float *__restrict__ rd, int n, uint64_t seed) {
+	__shared__ uint32_t wsum[kBounceBlock / kWave];
+	const int i = blockIdx.x * kBounceBlock + threadIdx.x;
+	rta_hit h;
+	bool hit = false;
+	if (i < n) { h = hits[i]; hit = h.t != FLT_MAX; }
+	const uint64_t b = __ballot(hit);
+	const int lane = __lane_id(), wv = threadIdx.x / kWave;
+	if (lane == 0) wsum[wv] = (uint32_t)__popcll(b);
+	__syncthreads();
+	if (!hit) return;
+	uint32_t before = 0;
+	for (int w = 0; w < wv; ++w) before += wsum[w];
+	const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+	const uint32_t k = block_offsets[blockIdx.x] + before + (uint32_t)__popcll(b & lt);
+	float o[3], d[3];
+	bounce_ray(h, tris, ro[3 * i], ro[3 * i + 1], ro[3 * i + 2], rd[3 * i], rd[3 * i + 1],
+	           rd[3 * i + 2], seed, (uint32_t)i, o, d);
+	out_o[3 * (size_t)k + 0] = o[0]; out_o[3 * (size_t)k + 1] = o[1]; out_o[3 * (size_t)k + 2] = o[2];
+	out_d[3 * (size_t)k + 0] = d[0]; out_d[3 * (size_t)k + 1] = d[1]; out_d[3 * (size_t)k + 2] = d[2];
+	out_t[k] = FLT_MAX;
+	src_index[k] = (uint32_t)i;
+}
+
+}  // namespace rta_hip
+
+using namespace rta_hip;
+
+extern "C" {
+
+int rta_reset_intersections(rta_hit *hits, int w, int h, rta_stream s) {
+	if (w <= 0 || h <= 0 || !hits) return invalid("rta_reset_intersections: bad arguments");
+	const long long n = (long long)w * h;
+	hipLaunchKernelGGL(reset_kernel, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(s), hits, (int)n);
+	RTA_LAUNCH_CHECK("reset_kernel");
+	return RTA_OK;
+}
+
+// camera basis exactly as cam_ray_generator_shirley::generate_ray_dir
+// (librta/raytrav.h:308-333): W = dir/|dir|, U = (up x W)/|up x W|, V = W x U
+static CamBasis make_basis(float fovy, float aspect, const float *vd, const float *pos,
+                           const float *up) {
+	CamBasis cb;
+	float fh = (float)((double)fovy / 2.0);
+	cb.height = tanf((float)(M_PI * (double)fh / 180.0f));
+	cb.width = aspect * cb.height;
+	const float wl = sqrtf(vd[0] * vd[0] + vd[1] * vd[1] + vd[2] * vd[2]);
+	cb.wx = vd[0] / wl; cb.wy = vd[1] / wl; cb.wz = vd[2] / wl;
+	float tx = up[1] * cb.wz - up[2] * cb.wy;
+	float ty = up[2] * cb.wx - up[0] * cb.wz;
+	float tz = up[0] * cb.wy - up[1] * cb.wx;
+	const float tl = sqrtf(tx * tx + ty * ty + tz * tz);
+	cb.ux = tx / tl; cb.uy = ty / tl; cb.uz = tz / tl;
+	cb.vx = cb.wy * cb.uz - cb.wz * cb.uy;
+	cb.vy = cb.wz * cb.ux - cb.wx * cb.uz;
+	cb.vz = cb.wx * cb.uy - cb.wy * cb.ux;
+	cb.px = pos[0]; cb.py = pos[1]; cb.pz = pos[2];
+	return cb;
+}
+
+int rta_setup_shirley_rays(float *dirs, float *origins, float *max_t, float fovy, float aspect,
+                           int w, int h, const float *view_dir, const float *pos, const float *up,
+                           float maxt, rta_stream s) {
+	if (!dirs || !origins || !max_t || !view_dir || !pos || !up || w <= 0 || h <= 0)
+		return invalid("rta_setup_shirley_rays: bad arguments");
+	const CamBasis cb = make_basis(fovy, aspect, view_dir, pos, up);
+	dim3 threads(16, 16);
+	dim3 blocks(grid_for(w, 16), grid_for(h, 16));
+	hipLaunchKernelGGL(shirley_kernel, blocks, threads, 0, as_stream(s), dirs, origins, max_t, w, h,
+	                   cb, maxt);
+	RTA_LAUNCH_CHECK("shirley_kernel");
+	return RTA_OK;
+}
+
+int rta_trace_cis(const rta_triangle *tris, int n_tris, const rta_node *nodes,
+                  const float *ray_orig, const float *ray_dir, const float *max_t, int w, int h,
+                  rta_hit *hits, uint32_t *stats, uint32_t *err, rta_stream s) {
+	if (!tris || n_tris <= 0 || !nodes || !ray_orig || !ray_dir || !max_t || !hits || !err ||
+	    w <= 0 || h <= 0)
+		return invalid("rta_trace_cis: bad arguments");
+	const long long n = (long long)w * h;
+	if (n > 0x7fffffffLL) return invalid("rta_trace_cis: too many rays");
+	if (stats)
+		hipLaunchKernelGGL(trace_cis_kernel<true>, dim3(grid_for(n, kTraceBlock)), dim3(kTraceBlock),
+		                   0, as_stream(s), tris, (uint32_t)n_tris, nodes, (uint32_t)(2 * n_tris - 1),
+		                   ray_orig, ray_dir, max_t, (int)n, hits, stats, err);
+	else
+		hipLaunchKernelGGL(trace_cis_kernel<false>, dim3(grid_for(n, kTraceBlock)), dim3(kTraceBlock),
+		                   0, as_stream(s), tris, (uint32_t)n_tris, nodes, (uint32_t)(2 * n_tris - 1),
+		                   ray_orig, ray_dir, max_t, (int)n, hits, stats, err);
+	RTA_LAUNCH_CHECK("trace_cis_kernel");
+	return RTA_OK;
+}
+
+int rta_bvh_pack(rta_node_pair *pairs, float *tri4, const rta_node *nodes, int n_nodes,
+                 const rta_triangle *tris, int n_tris, rta_stream s) {
+	if (!pairs || !tri4 || !nodes || !tris || n_nodes <= 0 || n_tris <= 0)
+		return invalid("rta_bvh_pack: bad arguments");
+	hipLaunchKernelGGL(pack_tris_kernel, dim3(grid_for(n_tris, 256)), dim3(256), 0, as_stream(s),
+	                   (float4 *)tri4, tris, n_tris);
+	RTA_LAUNCH_CHECK("pack_tris_kernel");
+	hipLaunchKernelGGL(pack_pairs_kernel, dim3(grid_for(n_nodes, 256)), dim3(256), 0, as_stream(s),
+	                   pairs, (float4 *)tri4, nodes, n_nodes);
+	RTA_LAUNCH_CHECK("pack_pairs_kernel");
+	return RTA_OK;
+}
+
+int rta_trace_cis_packed(const float *tri4, const rta_node_pair *pairs, uint32_t root_link,
+                         uint32_t root_index, const float *ray_orig, const float *ray_dir,
+                         const float *max_t, int n_rays, const uint32_t *ray_count,
+                         rta_hit *hits, uint32_t *err, rta_stream s) {
+	if (!tri4 || !pairs || !ray_orig || !ray_dir || !max_t || !hits || !err || n_rays <= 0)
+		return invalid("rta_trace_cis_packed: bad arguments");
+	// the root as its parent's pair record would describe it
+	const uint32_t root_word = link_inner(root_link) ? 1u : (root_index << 1);
+	hipLaunchKernelGGL(trace_cis_packed_kernel<kTraceBlock>, dim3(grid_for(n_rays, kTraceBlock)),
+	                   dim3(kTraceBlock), 0, as_stream(s), (const float4 *)tri4,
+	                   (const float4 *)pairs, root_word, ray_orig, ray_dir, max_t, n_rays, ray_count,
+	                   hits, err);
+	RTA_LAUNCH_CHECK("trace_cis_packed_kernel");
+	return RTA_OK;
+}
+
+int rta_bounce_work_bytes(int n_rays, size_t *bytes) {
+	if (n_rays <= 0 || !bytes) return invalid("rta_bounce_work_bytes: bad arguments");
+	*bytes = (size_t)grid_for(n_rays, kBounceBlock) * 4;
+	return RTA_OK;
+}
+
+int rta_diffuse_bounce(float *out_orig, float *out_dir, float *out_maxt, uint32_t *src_index,
+                       uint32_t *count, const rta_hit *hits, const rta_triangle *tris,
+                       const float *ray_orig, const float *ray_dir, int n_rays, uint64_t seed,
+                       void *work, size_t work_bytes, rta_stream s) {
+	if (!out_orig || !out_dir || !out_maxt || !src_index || !count || !hits || !tris ||
+	    !ray_orig || !ray_dir || !work || n_rays <= 0)
+		return invalid("rta_diffuse_bounce: bad arguments");
+	size_t need = 0;
+	rta_bounce_work_bytes(n_rays, &need);
+	if (work_bytes < need) {
+		set_error("rta_diffuse_bounce: work buffer too small");
+		return RTA_E_WORKSPACE;
+	}
+	const unsigned nb = grid_for(n_rays, kBounceBlock);
+	uint32_t *bc = reinterpret_cast<uint32_t *>(work);
+	hipStream_t st = as_stream(s);
+	hipLaunchKernelGGL(bounce_count_kernel, dim3(nb), dim3(kBounceBlock), 0, st, hits, n_rays, bc);
+	RTA_LAUNCH_CHECK("bounce_count_kernel");
+	hipLaunchKernelGGL(bounce_scan_kernel, dim3(1), dim3(1024), 0, st, bc, (int)nb, count);
+	RTA_LAUNCH_CHECK("bounce_scan_kernel");
+	hipLaunchKernelGGL(bounce_emit_kernel, dim3(nb), dim3(kBounceBlock), 0, st, out_orig, out_dir,
+	                   out_maxt, src_index, bc, hits, tris, ray_orig, ray_dir, n_rays, seed);
+	RTA_LAUNCH_CHECK("bounce_emit_kernel");
+	return RTA_OK;
+}
+
+}  // extern "C"

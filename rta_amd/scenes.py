@@ -1,0 +1,110 @@
+"""Seeded synthetic scenes, cameras and the workload configurations.
+
+Scene geometry comes from the C++ generators in librta_host.so (see
+include/rta_host.h); everything is a deterministic function of the seeds, so
+the GPU path and the CPU oracle always see byte-identical inputs.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from .abi import FLT_MAX, TRI_DTYPE, host, ptr
+
+
+def _f3(v):
+    return (C.c_float * 3)(*[float(x) for x in v])
+
+
+def icosphere(subdiv: int, radius: float = 1.0, center=(0.0, 0.0, 0.0), jitter: float = 0.0,
+              seed: int = 0) -> np.ndarray:
+    lib = host()
+    n = lib.rta_scene_icosphere(subdiv, radius, _f3(center), jitter, seed, None)
+    out = np.zeros(n, dtype=TRI_DTYPE)
+    lib.rta_scene_icosphere(subdiv, radius, _f3(center), jitter, seed, ptr(out))
+    return out
+
+
+def heightfield(nx: int, ny: int, x0=0.0, y0=0.0, x1=1.0, y1=1.0, z0=0.0, seed: int = 1,
+                octaves: int = 6, amplitude: float = 0.2, lacunarity: float = 2.0,
+                gain: float = 0.5, base_freq: float = 4.0) -> np.ndarray:
+    lib = host()
+    args = (nx, ny, x0, y0, x1, y1, z0, seed, octaves, amplitude, lacunarity, gain, base_freq)
+    n = lib.rta_scene_heightfield(*args, None)
+    if n < 0:
+        raise ValueError("bad heightfield arguments")
+    out = np.zeros(n, dtype=TRI_DTYPE)
+    lib.rta_scene_heightfield(*args, ptr(out))
+    return out
+
+
+def random_triangles(count: int, seed: int = 2, lo=0.0, hi=1.0, edge_min=0.001,
+                     edge_max=0.01) -> np.ndarray:
+    out = np.zeros(count, dtype=TRI_DTYPE)
+    host().rta_scene_random_triangles(count, seed, lo, hi, edge_min, edge_max, ptr(out))
+    return out
+
+
+@dataclass
+class Camera:
+    pos: tuple
+    target: tuple
+    up: tuple
+    fovy: float
+    w: int
+    h: int
+
+    @property
+    def view_dir(self):
+        return tuple(float(t) - float(p) for t, p in zip(self.target, self.pos))
+
+    @property
+    def aspect(self) -> float:
+        return float(np.float32(self.w) / np.float32(self.h))
+
+    def host_rays(self, maxt: float = float(FLT_MAX)):
+        """Pixel-centre pinhole rays on the host: (origins[3R], dirs[3R], max_t[R])."""
+        n = self.w * self.h
+        o = np.empty(3 * n, np.float32)
+        d = np.empty(3 * n, np.float32)
+        t = np.empty(n, np.float32)
+        rc = host().rta_host_shirley_rays(ptr(d), ptr(o), ptr(t), self.fovy, self.aspect, self.w,
+                                          self.h, _f3(self.view_dir), _f3(self.pos), _f3(self.up),
+                                          maxt)
+        if rc != 0:
+            raise ValueError("bad camera")
+        return o, d, t
+
+
+# ------------------------------------------------------- configurations
+
+def config1_scene() -> np.ndarray:
+    """Icosphere of radius 1 subdivided 5 times: 20,480 triangles."""
+    return icosphere(5, 1.0)
+
+
+def config1_camera(w: int = 512, h: int = 512) -> Camera:
+    """Pinhole at distance 3 looking at the sphere, 60 degree vertical FOV."""
+    return Camera(pos=(0.0, 0.0, 3.0), target=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fovy=60.0,
+                  w=w, h=h)
+
+
+def config2_scene(grid: int = 1024, n_random: int = 200_000, seed: int = 7) -> np.ndarray:
+    """1024x1024 heightfield over [0,1]^2 (6-octave value noise, amplitude 0.2,
+    lacunarity 2, gain 0.5) plus 200,000 small random triangles in the unit
+    cube: 2,297,152 triangles at full size."""
+    hf = heightfield(grid, grid, seed=seed)
+    rnd = random_triangles(n_random, seed=seed + 1000)
+    return np.concatenate([hf, rnd])
+
+
+def config2_camera(w: int = 1920, h: int = 1080) -> Camera:
+    """Above the heightfield looking down at ~50 degrees; ~96% of the
+    pixel rays hit geometry."""
+    return Camera(pos=(0.5, 0.05, 0.6), target=(0.5, 0.55, 0.0), up=(0.0, 0.0, 1.0),
+                  fovy=35.0, w=w, h=h)
+
+
+BOUNCE_SEED = 0x5EED_B0B0

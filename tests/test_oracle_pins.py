@@ -1,0 +1,389 @@
+"""Pin the CPU oracle (oracle/oracle.cpp) against checks that do not restate
+it: known answers, closed forms, exact rationals, brute force and
+invariants.  The reference ships no tests, golden vectors or fixtures for
+this path, so these are the pins ("Parity")."""
+from __future__ import annotations
+
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import (exact_cramer, expected_lbvh_nodes, morton_bitloop, radix_tree, round_f32,
+                     tri_boxes)
+from rta_amd import scenes
+
+F32MAX = np.float32(np.finfo(np.float32).max)
+
+
+# ------------------------------------------------------------ Morton
+
+def test_morton_known_answers():
+    assert oracle.morton_encode(0, 0, 0) == 0
+    assert oracle.morton_encode(1, 0, 0) == 1
+    assert oracle.morton_encode(0, 1, 0) == 2
+    assert oracle.morton_encode(0, 0, 1) == 4
+    assert oracle.morton_encode(1023, 1023, 1023) == 0x3FFFFFFF
+    assert oracle.morton_encode(2, 0, 0) == 8
+    # only the low 10 bits count: coordinate 1024 wraps to 0
+    # (lbvh-kernels.cu:101-103)
+    assert oracle.morton_encode(1024, 1024, 1024) == 0
+    assert oracle.morton_encode(1025, 0, 0) == 1
+
+
+def test_morton_matches_bit_loop():
+    rng = np.random.default_rng(1)
+    for x, y, z in rng.integers(0, 1 << 12, size=(2000, 3)):
+        assert oracle.morton_encode(int(x), int(y), int(z)) == morton_bitloop(
+            int(x) & 1023, int(y) & 1023, int(z) & 1023)
+
+
+# ------------------------------------------------- triangle test (exact)
+
+def _tri(a, b, c):
+    t = np.zeros(1, oracle.TRI_DTYPE)
+    t["a"], t["b"], t["c"] = a, b, c
+    return t
+
+
+def test_triangle_exact_rational_cases():
+    cases = [
+        # (a, b, c, o, d): all intermediates are small dyadics, so the only
+        # roundings are the three final divisions
+        ((0, 0, 0), (1, 0, 0), (0, 1, 0), (0.25, 0.25, 1), (0, 0, -1)),
+        ((0, 0, 0), (2, 0, 0), (0, 2, 0), (0.5, 0.25, 3), (0, 0, -1)),
+        ((1, 1, 1), (4, 1, 1), (1, 4, 1), (2, 2, 5), (0, 0, -1)),
+        ((0, 0, 0), (3, 0, 0), (0, 3, 0), (1, 1, 2), (0.5, 0.25, -1)),
+        ((0, 0, 0), (0, 3, 0), (0, 0, 3), (5, 1, 1), (-1, 0, 0)),
+        ((-1, -1, 2), (3, -1, 2), (-1, 3, 2), (0, 0, 0), (0.125, 0.25, 1)),
+    ]
+    for a, b, c, o, d in cases:
+        hit, t, beta, gamma = oracle.intersect_tri(_tri(a, b, c), o, d)
+        ex = exact_cramer(a, b, c, o, d)
+        t_e, b_e, g_e = ex
+        expect_hit = t_e > 0 and b_e > 0 and g_e > 0 and b_e + g_e <= 1
+        assert hit == expect_hit, (a, b, c, o, d)
+        if hit:
+            assert t == round_f32(t_e)
+            assert beta == round_f32(b_e)
+            assert gamma == round_f32(g_e)
+
+
+def test_triangle_acceptance_boundaries():
+    tri = _tri((0, 0, 0), (1, 0, 0), (0, 1, 0))
+    d = (0, 0, -1)
+    # on the hypotenuse: beta+gamma == 1 is accepted (<= 1)
+    assert oracle.intersect_tri(tri, (0.5, 0.5, 1), d)[0]
+    # on an edge where beta == 0 or gamma == 0: rejected (strict > 0)
+    assert not oracle.intersect_tri(tri, (0.5, 0.0, 1), d)[0]
+    assert not oracle.intersect_tri(tri, (0.0, 0.5, 1), d)[0]
+    # behind the origin (t < 0) and exactly at it (t == 0): rejected
+    assert not oracle.intersect_tri(tri, (0.25, 0.25, -1), d)[0]
+    assert not oracle.intersect_tri(tri, (0.25, 0.25, 0), d)[0]
+    # parallel ray: M == 0 -> NaN/inf quotients -> no hit
+    assert not oracle.intersect_tri(tri, (0.25, 0.25, 1), (1, 0, 0))[0]
+    # beta is the weight of vertex b, gamma of vertex c
+    hit, t, beta, gamma = oracle.intersect_tri(tri, (0.75, 0.125, 1), d)
+    assert hit and beta == np.float32(0.75) and gamma == np.float32(0.125) and t == 1
+
+
+def test_triangle_random_vs_float64():
+    rng = np.random.default_rng(5)
+    n_hit = 0
+    for _ in range(3000):
+        a, b, c = rng.normal(size=(3, 3)).astype(np.float32)
+        o = rng.normal(size=3).astype(np.float32) * 3
+        tgt = (a + rng.uniform(0, 1) * (b - a) * 0.9 + rng.uniform(0, 1) * (c - a) * 0.5)
+        d = (tgt - o + rng.normal(size=3) * 0.3).astype(np.float32)
+        m = np.stack([a - b, a - c, d], axis=1).astype(np.float64)
+        if abs(np.linalg.det(m)) < 1e-3:
+            continue
+        beta, gamma, t = np.linalg.solve(m, (a - o).astype(np.float64))
+        margin = 1e-3
+        clear_hit = t > margin and beta > margin and gamma > margin and beta + gamma < 1 - margin
+        clear_miss = t < -margin or beta < -margin or gamma < -margin or beta + gamma > 1 + margin
+        hit, tt, bb, gg = oracle.intersect_tri(_tri(a, b, c), o, d)
+        if clear_hit:
+            n_hit += 1
+            assert hit
+            assert abs(tt - t) <= 1e-4 * max(1.0, abs(t))
+            assert abs(bb - beta) <= 1e-4 and abs(gg - gamma) <= 1e-4
+        elif clear_miss:
+            assert not hit
+    assert n_hit > 300
+
+
+# ----------------------------------------------------- box test (exact)
+
+@pytest.mark.parametrize("o,d,hit,dist", [
+    ((0.5, 0.5, -1), (0, 0, 1), True, 1.0),        # d = 0 slabs, origin inside them
+    ((2, 0.5, -1), (0, 0, 1), False, None),        # d.x = 0, origin outside the x slab
+    ((0.5, 0.5, 0.5), (0, 0, 1), True, -0.5),      # origin inside: dist = t_near < 0
+    ((0.5, 0.5, 2), (0, 0, 1), False, None),       # box behind the ray (t_far < 0)
+    ((-1, -1, -1), (1, 1, 1), True, 1.0),          # diagonal
+    ((1, 0.5, -1), (0, 0, 1), True, 1.0),          # origin on the slab boundary
+    ((0.5, 0.5, 3), (0, 0, -1), True, 2.0),        # negative direction swaps t1/t2
+    ((-1, 0.5, 0.5), (1, 2, 0), False, None),      # slabs do not overlap
+    ((0.5, 0.5, -1), (-0.0, 0, 1), True, 1.0),     # -0 direction counts as zero
+])
+def test_box_cases(o, d, hit, dist):
+    h, dd = oracle.intersect_aabb((0, 0, 0), (1, 1, 1), o, d)
+    assert h == hit
+    if hit:
+        assert dd == np.float32(dist)
+
+
+def test_box_dist_is_correctly_rounded_quotient():
+    # t = (min - o) / d with a single rounding
+    h, dd = oracle.intersect_aabb((1, -10, -10), (2, 10, 10), (0, 0, 0), (3, 0.5, 0.25))
+    assert h and dd == round_f32(Fraction(1) / Fraction(3))
+
+
+# -------------------------------------------------------------- LBVH
+
+def _clustered_scene(n, cells, seed):
+    """Triangles concentrated in a few Morton cells: many duplicate codes."""
+    rng = np.random.default_rng(seed)
+    t = np.zeros(n, oracle.TRI_DTYPE)
+    centres = rng.uniform(0, 1, size=(cells, 3)).astype(np.float32)
+    pick = rng.integers(0, cells, size=n)
+    base = centres[pick]
+    for k in ("a", "b", "c"):
+        t[k] = base + rng.uniform(-1e-5, 1e-5, size=(n, 3)).astype(np.float32)
+    # two far corner triangles fix the bounds
+    t[0]["a"], t[0]["b"], t[0]["c"] = (0, 0, 0), (1e-4, 0, 0), (0, 1e-4, 1e-4)
+    t[1]["a"], t[1]["b"], t[1]["c"] = (1, 1, 1), (0.99, 1, 1), (1, 0.99, 0.99)
+    return t
+
+
+def _check_lbvh_against_pins(tris):
+    n = len(tris)
+    out = oracle.lbvh_build(tris)
+    # Morton stage: recompute codes independently (float32 numpy ops are
+    # the same IEEE operations)
+    bmin, bmax = tri_boxes(tris)
+    lo = bmin.min(axis=0)
+    hi = bmax.max(axis=0)
+    assert np.array_equal(out["bounds"]["min"][0], lo) and np.array_equal(out["bounds"]["max"][0], hi)
+    ctr = ((tris["a"] + tris["b"]) + tris["c"]) / np.float32(3.0)
+    dd = np.float32(1024.0) / (hi - lo)
+    q = ((ctr - lo) * dd)
+    qi = np.where(q > 0, np.trunc(q), 0).astype(np.int64) & 1023
+    codes = np.array([morton_bitloop(int(x), int(y), int(z)) for x, y, z in qi], np.uint32)
+    assert np.array_equal(out["codes_input"], codes)
+    # sort stage: unique stable order
+    order = np.argsort(codes, kind="stable")
+    assert np.array_equal(out["indices"], order)
+    assert np.array_equal(out["codes_sorted"], codes[order])
+    # permutation
+    assert np.array_equal(out["tris"].view(np.uint8).reshape(n, -1),
+                          tris[order].view(np.uint8).reshape(n, -1))
+    # topology, boxes and collapse from the range characterisation
+    tle, rt, emin, emax, parents = expected_lbvh_nodes(codes[order], order, bmin, bmax)
+    nodes = out["nodes"]
+    assert np.array_equal(nodes["type_left_elems"], tle)
+    assert np.array_equal(nodes["right_tris"], rt)
+    assert np.array_equal(nodes["min"], emin)      # == treats -0 and +0 alike
+    assert np.array_equal(nodes["max"], emax)
+    assert np.array_equal(out["parents"][1:], parents[1:])
+    return out
+
+
+def test_lbvh_small_icosphere():
+    _check_lbvh_against_pins(scenes.icosphere(2))
+
+
+def test_lbvh_duplicate_codes_collapse():
+    tris = _clustered_scene(3000, 40, seed=3)
+    out = _check_lbvh_against_pins(tris)
+    # every maximal run of equal codes (code != 0) became one reachable leaf
+    nodes = out["nodes"]
+    leaves = []
+    stack = [0]
+    while stack:
+        x = stack.pop()
+        if nodes["type_left_elems"][x] & 1:
+            stack += [nodes["type_left_elems"][x] >> 1, nodes["right_tris"][x]]
+        else:
+            leaves.append((int(nodes["right_tris"][x]), int(nodes["type_left_elems"][x] >> 1)))
+    leaves.sort()
+    cs = out["codes_sorted"]
+    runs = np.flatnonzero(np.diff(cs)) + 1
+    starts = np.concatenate([[0], runs])
+    ends = np.concatenate([runs, [len(cs)]])
+    assert leaves == [(int(s), int(e - s)) for s, e in zip(starts, ends)]
+
+
+def test_lbvh_icosphere_config1():
+    _check_lbvh_against_pins(scenes.config1_scene())
+
+
+def test_lbvh_edge_sizes():
+    one = scenes.icosphere(0)[:1]
+    out = oracle.lbvh_build(one)
+    assert out["nodes"]["type_left_elems"][0] == 2 and out["nodes"]["right_tris"][0] == 0
+    two = scenes.icosphere(0)[:2]
+    _check_lbvh_against_pins(two)
+    with pytest.raises(ValueError):
+        oracle.lbvh_build(np.zeros(0, oracle.TRI_DTYPE))
+
+
+def test_lbvh_code_zero_status():
+    tris = _clustered_scene(500, 10, seed=9)
+    # the corner triangle's centroid quantises to (0,0,0) -> code 0
+    out = oracle.lbvh_build(tris)
+    assert out["status"] & 1
+    _check_lbvh_against_pins(tris)
+
+
+def test_radix_tree_helper_on_known_tree():
+    # codes 1,2,4,5,19,24,25,30 -> Karras 2012 figure 3 shape (root splits 4|4)
+    codes = np.array([1, 2, 4, 5, 19, 24, 25, 30], np.uint32)
+    tree = radix_tree(codes)
+    assert tree[0][:2] == (0, 7)
+    assert tree[0][2] == 3 and tree[0][3] == 4
+    assert len(tree) == 7
+
+
+# --------------------------------------------------- median build
+
+def test_median_build_invariants():
+    tris = scenes.icosphere(3)
+    mb = oracle.median_build(tris)
+    nodes, out_tris, order = mb["nodes"], mb["tris"], mb["order"]
+    n = len(tris)
+    assert sorted(order.tolist()) == list(range(n))
+    # nodes are in pre-order, leaves hold <= 4 triangles and partition the array
+    covered = np.zeros(n, int)
+
+    def rec(x):
+        nd = nodes[x]
+        if nd["type_left_elems"] & 1:
+            l, r = int(nd["type_left_elems"] >> 1), int(nd["right_tris"])
+            assert l == x + 1 and r > l
+            lo1, hi1 = rec(l)
+            lo2, hi2 = rec(r)
+            lo, hi = np.minimum(lo1, lo2), np.maximum(hi1, hi2)
+        else:
+            e, f = int(nd["type_left_elems"] >> 1), int(nd["right_tris"])
+            assert 1 <= e <= 4
+            covered[f:f + e] += 1
+            v = np.concatenate([out_tris["a"][f:f + e], out_tris["b"][f:f + e],
+                                out_tris["c"][f:f + e]])
+            lo, hi = v.min(axis=0), v.max(axis=0)
+        assert np.array_equal(nd["min"], lo) and np.array_equal(nd["max"], hi)
+        return lo, hi
+
+    rec(0)
+    assert (covered == 1).all()
+
+
+# ----------------------------------------------------------- traversal
+
+def _rays_at(tris, n, seed):
+    rng = np.random.default_rng(seed)
+    o = rng.uniform(-2, 2, size=(n, 3)).astype(np.float32)
+    tgt = tris["a"][rng.integers(0, len(tris), n)] + rng.normal(scale=0.05, size=(n, 3))
+    d = (tgt - o).astype(np.float32)
+    d /= np.linalg.norm(d, axis=1, keepdims=True).astype(np.float32)
+    return o.reshape(-1), d.reshape(-1), np.full(n, F32MAX, np.float32)
+
+
+def test_cis_matches_brute_force():
+    tris = scenes.icosphere(3, jitter=0.05, seed=4)
+    out = oracle.lbvh_build(tris)
+    o, d, t = _rays_at(tris, 4000, seed=11)
+    hits, _, _ = oracle.trace_cis(out["tris"], out["nodes"], o, d, t, len(t))
+    brute = oracle.trace_brute(out["tris"], o, d)
+    hit_b = brute["t"] != F32MAX
+    hit_c = hits["t"] != F32MAX
+    assert hit_b.sum() > 1500
+    # the BVH finds the same closest t; a miss of the BVH on a brute-force
+    # hit can only come from a box test grazing the box boundary
+    same = hit_b == hit_c
+    assert same.mean() > 0.999
+    both = hit_b & hit_c
+    assert np.array_equal(hits["t"][both], brute["t"][both])
+    same_ref = hits["ref"][both] == brute["ref"][both]
+    # a different triangle with the identical t is an exact tie on a shared edge
+    assert same_ref.mean() > 0.99
+
+
+def test_cis_lbvh_vs_median_tree():
+    tris = scenes.icosphere(4)
+    cam = scenes.config1_camera(96, 96)
+    o, d, t = cam.host_rays()
+    lb = oracle.lbvh_build(tris)
+    mb = oracle.median_build(tris)
+    h1, _, _ = oracle.trace_cis(lb["tris"], lb["nodes"], o, d, t, cam.w, cam.h)
+    h2, _, _ = oracle.trace_cis(mb["tris"], mb["nodes"], o, d, t, cam.w, cam.h)
+    hit = h1["t"] != F32MAX
+    assert np.array_equal(hit, h2["t"] != F32MAX)
+    assert np.array_equal(h1["t"][hit], h2["t"][hit])
+    id1 = lb["indices"][h1["ref"][hit]]
+    id2 = mb["order"][h2["ref"][hit]]
+    assert (id1 == id2).mean() > 0.999
+
+
+def test_dis_equals_cis_hits():
+    tris = scenes.icosphere(3)
+    out = oracle.lbvh_build(tris)
+    cam = scenes.config1_camera(64, 64)
+    o, d, t = cam.host_rays()
+    h1, _, _ = oracle.trace_cis(out["tris"], out["nodes"], o, d, t, 64, 64)
+    h2, _ = oracle.trace_dis(out["tris"], out["nodes"], o, d, t, 64, 64)
+    assert np.array_equal(h1["t"], h2["t"])
+
+
+def test_cis_stats_counts():
+    tris = scenes.icosphere(3)
+    out = oracle.lbvh_build(tris)
+    o, d, t = _rays_at(tris, 500, seed=2)
+    hits, _, st = oracle.trace_cis(out["tris"], out["nodes"], o, d, t, 500, stats=True)
+    st = st.reshape(-1, 2)
+    assert (st[:, 0] >= 1).all()               # the root is always visited
+    assert (st[hits["t"] != F32MAX, 1] >= 1).all()
+
+
+def test_cis_finite_max_t():
+    tris = scenes.icosphere(3)
+    out = oracle.lbvh_build(tris)
+    cam = scenes.config1_camera(32, 32)
+    o, d, t = cam.host_rays()
+    tm = np.full_like(t, 2.5)                  # the sphere front is at t ~ 2
+    h, _, _ = oracle.trace_cis(out["tris"], out["nodes"], o, d, tm, 32, 32)
+    full, _, _ = oracle.trace_cis(out["tris"], out["nodes"], o, d, t, 32, 32)
+    near = full["t"] <= np.float32(2.5)
+    assert near.sum() > 100 and (~near).sum() > 100
+    assert np.array_equal(h["t"][near], full["t"][near])
+    assert np.array_equal(h["ref"][near], full["ref"][near])
+    # the CPU tracer reports t = max_t when nothing lies within max_t
+    # (closest.t starts at max_t, bbvh.h:412)
+    assert (h["t"][~near] == np.float32(2.5)).all()
+
+
+# ----------------------------------------------------------- bounce
+
+def test_bounce_properties():
+    tris = scenes.icosphere(3)
+    out = oracle.lbvh_build(tris)
+    cam = scenes.config1_camera(64, 64)
+    o, d, t = cam.host_rays()
+    h, _, _ = oracle.trace_cis(out["tris"], out["nodes"], o, d, t, 64, 64)
+    bo, bd, bt, src = oracle.diffuse_bounce(h, out["tris"], o, d, seed=3)
+    hit = np.flatnonzero(h["t"] != F32MAX)
+    assert np.array_equal(src, hit.astype(np.uint32))
+    bd = bd.reshape(-1, 3).astype(np.float64)
+    assert np.allclose(np.linalg.norm(bd, axis=1), 1.0, atol=1e-6)
+    # directions leave the surface on the viewer's side; cosine-weighted:
+    # E[cos theta] = 2/3
+    tr = out["tris"][h["ref"][hit]]
+    nrm = np.cross(tr["b"] - tr["a"], tr["c"] - tr["a"]).astype(np.float64)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    dirs = d.reshape(-1, 3)[hit]
+    nrm *= -np.sign((nrm * dirs).sum(axis=1))[:, None]
+    cos = (nrm * bd).sum(axis=1)
+    assert (cos > -1e-6).all()
+    assert abs(cos.mean() - 2.0 / 3.0) < 0.02
+    assert (bt == F32MAX).all()
