@@ -146,18 +146,24 @@ int rta_lbvh_sort_work_bytes(int n, size_t *bytes);
 int rta_lbvh_sort(uint32_t *codes, uint32_t *indices, int n,
                   void *work, size_t work_bytes, rta_stream s);
 
+/* Scratch shared by the two calls below (visit counters, node ranges,
+ * frontier lists): rta_lbvh_boxes_work_bytes(n) bytes.  The reference
+ * allocates its counters inside the timed call and never frees them
+ * (lbvh-kernels.cu:451-453); here the caller owns them. */
+int rta_lbvh_boxes_work_bytes(int n, size_t *bytes);
+
 /* buildTreeKarras + k::build_tree_karras, lbvh-kernels.cu:186-256,424-446:
  * inner nodes 0..n-2, leaves n-1..2n-2; writes links and parents[child];
- * also clears visit[0..n-2] for the bottom-up pass. */
-int rta_lbvh_build_tree(rta_node *nodes, uint32_t *parents, uint32_t *visit,
+ * also initialises `work` for rta_lbvh_build_boxes. */
+int rta_lbvh_build_tree(rta_node *nodes, uint32_t *parents, void *work,
                         const uint32_t *codes, int n, rta_stream s);
 
 /* buildBoundingBoxesKarras2 + k::build_bounding_boxes_karras2,
- * lbvh-kernels.cu:258-341,448-478: bottom-up box union with an atomic visit
- * counter; a parent whose two children are leaves with equal representative
- * codes collapses into one multi-triangle leaf.  `visit` must be zero
- * (rta_lbvh_build_tree clears it). */
-int rta_lbvh_build_boxes(rta_node *nodes, uint32_t *visit,
+ * lbvh-kernels.cu:258-341,448-478: bottom-up box union with a visit
+ * counter per inner node; a parent whose two children are leaves with
+ * equal representative codes collapses into one multi-triangle leaf.
+ * `work` must come from rta_lbvh_build_tree of the same codes. */
+int rta_lbvh_build_boxes(rta_node *nodes, void *work,
                          const rta_aabb *boxes, const uint32_t *codes,
                          const uint32_t *indices, const uint32_t *parents,
                          int n, rta_stream s);

@@ -90,6 +90,7 @@ HIP_SIGNATURES = {
     "rta_lbvh_morton_codes": [_vp, _vp, _vp, _vp, _vp, _i, _vp],
     "rta_lbvh_sort_work_bytes": [_i, _psz],
     "rta_lbvh_sort": [_vp, _vp, _i, _vp, _sz, _vp],
+    "rta_lbvh_boxes_work_bytes": [_i, _psz],
     "rta_lbvh_build_tree": [_vp, _vp, _vp, _vp, _i, _vp],
     "rta_lbvh_build_boxes": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _vp],
     "rta_lbvh_permute": [_vp, _vp, _vp, _i, _vp],

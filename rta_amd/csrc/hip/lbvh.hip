@@ -170,10 +170,12 @@ __device__ inline int cpl(int i, int j, const uint32_t *__restrict__ codes, int 
 // Thread per node (k::build_tree_karras, lbvh-kernels.cu:186-256).  The
 // direction/range/split searches are the reference's, including its
 // ceil-halving split search that always runs in +1 direction from the
-// lower end of the range.  Inner nodes also clear their visit counter.
+// lower end of the range.  Inner nodes also clear their visit counter and
+// record their sorted range [from, to] for the bottom-up pass.
 __global__ __launch_bounds__(256) void karras_kernel(
     rta_node *__restrict__ nodes, uint32_t *__restrict__ parents,
-    uint32_t *__restrict__ visit, const uint32_t *__restrict__ codes, int n) {
+    uint32_t *__restrict__ visit, uint2 *__restrict__ ranges,
+    const uint32_t *__restrict__ codes, int n) {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= 2 * n - 1) return;
 	uint2 *links = reinterpret_cast<uint2 *>(&nodes[i]);
@@ -207,9 +209,160 @@ __global__ __launch_bounds__(256) void karras_kernel(
 	parents[left] = (uint32_t)i;
 	parents[right] = (uint32_t)i;
 	visit[i] = 0u;
+	ranges[i] = make_uint2((uint32_t)from, (uint32_t)to);
 }
 
 // ----------------------------------------- stage 6: bottom-up boxes
+
+// Bottom-up boxes (k::build_bounding_boxes_karras2, lbvh-kernels.cu:258-341):
+// every leaf climbs towards the root; at each inner node the first arriving
+// child stops and the second forms the union and continues.  Deterministic
+// form of the reference:
+//  * a separate visit counter decides first/second arrival (the reference
+//    sums representative codes into the flag, so a first arrival with code
+//    0 also stops the second thread, lbvh-kernels.cu:288-291);
+//  * the union is MIN(left, right) / MAX(left, right) with the reference's
+//    `(a<b)?a:b`, fixing the sign of zero that the reference leaves to
+//    arrival order;
+//  * merge rule: both children are leaves and their representative codes
+//    (code of the leaf, or codes[i] for inner node i, :261) are equal
+//    (`code == old` in the reference) -> the parent becomes one leaf with
+//    tris = min, elems = sum (lbvh-kernels.cu:302-330).
+//
+// Hierarchical schedule.  A subtree covers a contiguous range of sorted
+// positions, so all communication inside a chunk of consecutive leaves
+// stays in one workgroup:
+//  level 1: a 1024-thread block owns 1024 leaves; counters, boxes and leaf
+//           state live in LDS; a climb stops when the parent's range leaves
+//           the chunk and the child is recorded in the block's frontier
+//           slots (at most 2 per tree level, so <= 126 for depth <= 63);
+//  level k: one block merges the frontiers of kMergeFan level-(k-1) chunks
+//           with workgroup-scope atomics on the global visit counters,
+//           until one block covers all leaves and reaches the root.
+// No step needs device-scope cache maintenance; results do not depend on
+// the schedule.
+
+constexpr int kChunk = 1024;
+constexpr int kSlots = 128;
+constexpr int kMergeFan = 64;
+constexpr uint32_t kStateNone = 0u;           // lelems: node not completed
+constexpr uint32_t kStateInner = 0xffffffffu; // lelems: completed inner node
+
+__device__ inline rta_aabb union_lr(const rta_aabb &lb, const rta_aabb &rb) {
+	rta_aabb u;
+	u.min.x = sel_min(lb.min.x, rb.min.x);
+	u.min.y = sel_min(lb.min.y, rb.min.y);
+	u.min.z = sel_min(lb.min.z, rb.min.z);
+	u.max.x = sel_max(lb.max.x, rb.max.x);
+	u.max.y = sel_max(lb.max.y, rb.max.y);
+	u.max.z = sel_max(lb.max.z, rb.max.z);
+	return u;
+}
+
+__device__ inline uint32_t rep_code(const uint32_t *codes, uint32_t node, uint32_t leaf0) {
+	return node >= leaf0 ? codes[node - leaf0] : codes[node];
+}
+
+__global__ __launch_bounds__(kChunk) void bottom_up_chunk_kernel(
+    rta_node *__restrict__ nodes, const rta_aabb *__restrict__ boxes,
+    const uint32_t *__restrict__ codes, const uint32_t *__restrict__ indices,
+    const uint32_t *__restrict__ parents, const uint2 *__restrict__ ranges,
+    uint32_t *__restrict__ slots, uint32_t *__restrict__ counts, uint32_t *__restrict__ status,
+    int n) {
+	__shared__ float lb_leaf[6][kChunk];
+	__shared__ float lb_int[6][kChunk];
+	__shared__ uint32_t lvisit[kChunk];
+	__shared__ uint32_t ltris[kChunk];
+	__shared__ uint32_t lelems[kChunk];
+	__shared__ uint32_t front[kSlots];
+	__shared__ uint32_t nfront;
+
+	const int t = threadIdx.x;
+	const uint32_t base = blockIdx.x * kChunk;
+	const uint32_t leaf0 = (uint32_t)(n - 1);
+	const uint32_t end = min(base + (uint32_t)kChunk, (uint32_t)n);
+	lvisit[t] = 0;
+	lelems[t] = kStateNone;
+	if (t == 0) nfront = 0;
+	const uint32_t k = base + t;
+	rta_aabb box;
+	if (k < (uint32_t)n) {
+		box = boxes[indices[k]];
+		lb_leaf[0][t] = box.min.x; lb_leaf[1][t] = box.min.y; lb_leaf[2][t] = box.min.z;
+		lb_leaf[3][t] = box.max.x; lb_leaf[4][t] = box.max.y; lb_leaf[5][t] = box.max.z;
+	}
+	__syncthreads();
+
+	if (k < (uint32_t)n) {
+		uint32_t cur = leaf0 + k;
+		for (int level = 0; cur != 0u && level < 128; ++level) {
+			const uint32_t parent = parents[cur];
+			const uint2 rg = ranges[parent];
+			if (rg.x < base || rg.y >= end) {
+				const uint32_t slot = atomicAdd(&nfront, 1u);
+				if (slot < (uint32_t)kSlots) front[slot] = cur;
+				else atomicOr(status, 2u);
+				break;
+			}
+			const uint32_t p = parent - base;
+			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+			const uint32_t old = atomicAdd(&lvisit[p], 1u);
+			if (old == 0u) break;
+			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+
+			const uint2 pl = *reinterpret_cast<const uint2 *>(&nodes[parent]);
+			const uint32_t left = pl.x >> 1, right = pl.y;
+			const bool cur_is_left = (left == cur);
+			const uint32_t sib = cur_is_left ? right : left;
+			rta_aabb sbox;
+			if (sib >= leaf0) {
+				const uint32_t q = sib - leaf0 - base;
+				sbox.min = {lb_leaf[0][q], lb_leaf[1][q], lb_leaf[2][q]};
+				sbox.max = {lb_leaf[3][q], lb_leaf[4][q], lb_leaf[5][q]};
+			} else {
+				const uint32_t q = sib - base;
+				sbox.min = {lb_int[0][q], lb_int[1][q], lb_int[2][q]};
+				sbox.max = {lb_int[3][q], lb_int[4][q], lb_int[5][q]};
+			}
+			box = cur_is_left ? union_lr(box, sbox) : union_lr(sbox, box);
+			lb_int[0][p] = box.min.x; lb_int[1][p] = box.min.y; lb_int[2][p] = box.min.z;
+			lb_int[3][p] = box.max.x; lb_int[4][p] = box.max.y; lb_int[5][p] = box.max.z;
+
+			// children state: original leaf, collapsed inner node, or inner
+			uint32_t state = kStateInner, ptris = 0;
+			if (rep_code(codes, cur, leaf0) == rep_code(codes, sib, leaf0)) {
+				uint32_t le, lt, re, rt;
+				bool l_leaf, r_leaf;
+				if (left >= leaf0) { l_leaf = true; le = 1; lt = left - leaf0; }
+				else { le = lelems[left - base]; lt = ltris[left - base]; l_leaf = le != kStateInner; }
+				if (right >= leaf0) { r_leaf = true; re = 1; rt = right - leaf0; }
+				else { re = lelems[right - base]; rt = ltris[right - base]; r_leaf = re != kStateInner; }
+				if (l_leaf && r_leaf) { state = re + le; ptris = lt < rt ? lt : rt; }
+			}
+			ltris[p] = ptris;
+			lelems[p] = state;
+			cur = parent;
+		}
+	}
+	__syncthreads();
+
+	// write-out: leaf boxes, completed inner nodes, frontier
+	if (k < (uint32_t)n) {
+		float *f = reinterpret_cast<float *>(&nodes[leaf0 + k]) + 2;
+		f[0] = lb_leaf[0][t]; f[1] = lb_leaf[1][t]; f[2] = lb_leaf[2][t];
+		f[3] = lb_leaf[3][t]; f[4] = lb_leaf[4][t]; f[5] = lb_leaf[5][t];
+	}
+	if (k < leaf0 && lelems[t] != kStateNone) {
+		float *f = reinterpret_cast<float *>(&nodes[k]) + 2;
+		f[0] = lb_int[0][t]; f[1] = lb_int[1][t]; f[2] = lb_int[2][t];
+		f[3] = lb_int[3][t]; f[4] = lb_int[4][t]; f[5] = lb_int[5][t];
+		if (lelems[t] != kStateInner)
+			*reinterpret_cast<uint2 *>(&nodes[k]) = make_uint2(lelems[t] << 1, ltris[t]);
+	}
+	const uint32_t nf = min(nfront, (uint32_t)kSlots);
+	if ((uint32_t)t < nf) slots[(size_t)blockIdx.x * kSlots + t] = front[t];
+	if (t == 0) counts[blockIdx.x] = nf;
+}
 
 __device__ inline rta_aabb load_box(const rta_node *node) {
 	const float *f = reinterpret_cast<const float *>(node) + 2;
@@ -224,69 +377,106 @@ __device__ inline void store_box(rta_node *node, const rta_aabb &b) {
 	f[3] = b.max.x; f[4] = b.max.y; f[5] = b.max.z;
 }
 
-// Thread per leaf, walking to the root (k::build_bounding_boxes_karras2,
-// lbvh-kernels.cu:258-341).  Deterministic form of the reference:
-//  * a separate visit counter decides first/second arrival (the reference
-//    sums representative codes into the flag, so a first arrival with code
-//    0 also stops the second thread, lbvh-kernels.cu:288-291);
-//  * acquire/release atomics make the first arrival's subtree visible;
-//  * the union is formed as MIN(left, right) / MAX(left, right) with the
-//    reference's `(a<b)?a:b`, fixing the sign of zero that the reference
-//    leaves to arrival order;
-//  * merge rule: the second arrival's representative code equals the first
-//    arrival's (code == old in the reference) and both children are leaves
-//    -> the parent becomes a leaf with tris = min, elems = sum
-//    (lbvh-kernels.cu:302-330).
-__global__ __launch_bounds__(256) void bottom_up_kernel(
+// One block merges the frontier slots of up to kMergeFan chunks of the
+// previous level; its own chunk is `chunk` leaves wide.
+__global__ __launch_bounds__(1024) void bottom_up_merge_kernel(
     rta_node *__restrict__ nodes, uint32_t *__restrict__ visit,
-    const rta_aabb *__restrict__ boxes, const uint32_t *__restrict__ codes,
-    const uint32_t *__restrict__ indices, const uint32_t *__restrict__ parents, int n) {
-	const int k = blockIdx.x * blockDim.x + threadIdx.x;
-	if (k >= n) return;
-	const uint32_t leaf0 = (uint32_t)(n - 1);
-	uint32_t cur = leaf0 + (uint32_t)k;
-	rta_aabb box = boxes[indices[k]];
-	store_box(&nodes[cur], box);
-
-	// A Karras tree over 62-bit augmented keys is at most 63 levels deep;
-	// the cap turns a corrupt parent array into a stop instead of a hang.
-	for (int level = 0; cur != 0u && level < 128; ++level) {
-		const uint32_t code = cur >= leaf0 ? codes[cur - leaf0] : codes[cur];
-		const uint32_t parent = parents[cur];
-		if (parent >= leaf0) return;
-		const uint32_t old = __hip_atomic_fetch_add(&visit[parent], 1u, __ATOMIC_ACQ_REL,
-		                                            __HIP_MEMORY_SCOPE_AGENT);
-		if (old == 0u) return;  // first arrival: the sibling thread continues
-
-		const uint2 pl = *reinterpret_cast<const uint2 *>(&nodes[parent]);
-		const uint32_t left = pl.x >> 1, right = pl.y;
-		const bool cur_is_left = (left == cur);
-		const uint32_t sib = cur_is_left ? right : left;
-		const rta_aabb sbox = load_box(&nodes[sib]);
-		const rta_aabb &lb = cur_is_left ? box : sbox;
-		const rta_aabb &rb = cur_is_left ? sbox : box;
-		rta_aabb u;
-		u.min.x = sel_min(lb.min.x, rb.min.x);
-		u.min.y = sel_min(lb.min.y, rb.min.y);
-		u.min.z = sel_min(lb.min.z, rb.min.z);
-		u.max.x = sel_max(lb.max.x, rb.max.x);
-		u.max.y = sel_max(lb.max.y, rb.max.y);
-		u.max.z = sel_max(lb.max.z, rb.max.z);
-		box = u;
-		store_box(&nodes[parent], box);
-
-		const uint32_t sib_code = sib >= leaf0 ? codes[sib - leaf0] : codes[sib];
-		if (code == sib_code) {
-			const uint2 ll = *reinterpret_cast<const uint2 *>(&nodes[left]);
-			const uint2 rl = *reinterpret_cast<const uint2 *>(&nodes[right]);
-			if (!link_inner(ll.x) && !link_inner(rl.x)) {
-				const uint32_t tris = ll.y < rl.y ? ll.y : rl.y;
-				const uint32_t elems = (rl.x >> 1) + (ll.x >> 1);
-				*reinterpret_cast<uint2 *>(&nodes[parent]) = make_uint2(elems << 1, tris);
-			}
+    const uint32_t *__restrict__ codes, const uint32_t *__restrict__ parents,
+    const uint2 *__restrict__ ranges, const uint32_t *__restrict__ in_slots,
+    const uint32_t *__restrict__ in_counts, int n_in, uint32_t *__restrict__ out_slots,
+    uint32_t *__restrict__ out_counts, uint32_t *__restrict__ status, long long chunk, int n) {
+	__shared__ uint32_t prefix[kMergeFan + 1];
+	__shared__ uint32_t front[kSlots];
+	__shared__ uint32_t nfront;
+	const int t = threadIdx.x;
+	const int g0 = blockIdx.x * kMergeFan;
+	const int ng = min(kMergeFan, n_in - g0);
+	if (t == 0) {
+		uint32_t acc = 0;
+		prefix[0] = 0;
+		for (int g = 0; g < ng; ++g) {
+			acc += in_counts[g0 + g];
+			prefix[g + 1] = acc;
 		}
-		cur = parent;
+		nfront = 0;
 	}
+	__syncthreads();
+	const uint32_t leaf0 = (uint32_t)(n - 1);
+	const uint32_t lo = (uint32_t)(blockIdx.x * chunk);
+	const uint32_t hi = (uint32_t)min((long long)n, (long long)(blockIdx.x + 1) * chunk);
+	const uint32_t total = prefix[ng];
+	for (uint32_t f = t; f < total; f += blockDim.x) {
+		int g = 0;
+		while (prefix[g + 1] <= f) ++g;
+		uint32_t cur = in_slots[(size_t)(g0 + g) * kSlots + (f - prefix[g])];
+		rta_aabb box = load_box(&nodes[cur]);
+		for (int level = 0; cur != 0u && level < 128; ++level) {
+			const uint32_t parent = parents[cur];
+			const uint2 rg = ranges[parent];
+			if (rg.x < lo || rg.y >= hi) {
+				const uint32_t slot = atomicAdd(&nfront, 1u);
+				if (slot < (uint32_t)kSlots) front[slot] = cur;
+				else atomicOr(status, 2u);
+				break;
+			}
+			const uint32_t old = __hip_atomic_fetch_add(&visit[parent], 1u, __ATOMIC_ACQ_REL,
+			                                            __HIP_MEMORY_SCOPE_WORKGROUP);
+			if (old == 0u) break;
+			const uint2 pl = *reinterpret_cast<const uint2 *>(&nodes[parent]);
+			const uint32_t left = pl.x >> 1, right = pl.y;
+			const bool cur_is_left = (left == cur);
+			const uint32_t sib = cur_is_left ? right : left;
+			const rta_aabb sbox = load_box(&nodes[sib]);
+			box = cur_is_left ? union_lr(box, sbox) : union_lr(sbox, box);
+			store_box(&nodes[parent], box);
+			if (rep_code(codes, cur, leaf0) == rep_code(codes, sib, leaf0)) {
+				const uint2 ll = *reinterpret_cast<const uint2 *>(&nodes[left]);
+				const uint2 rl = *reinterpret_cast<const uint2 *>(&nodes[right]);
+				if (!link_inner(ll.x) && !link_inner(rl.x)) {
+					const uint32_t tris = ll.y < rl.y ? ll.y : rl.y;
+					const uint32_t elems = (rl.x >> 1) + (ll.x >> 1);
+					*reinterpret_cast<uint2 *>(&nodes[parent]) = make_uint2(elems << 1, tris);
+				}
+			}
+			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+			cur = parent;
+		}
+	}
+	__syncthreads();
+	const uint32_t nf = min(nfront, (uint32_t)kSlots);
+	if (out_slots && (uint32_t)t < nf) out_slots[(size_t)blockIdx.x * kSlots + t] = front[t];
+	if (out_counts && t == 0) out_counts[blockIdx.x] = nf;
+}
+
+struct BoxesWork {
+	uint32_t *visit;
+	uint2 *ranges;
+	uint32_t *slots[2];
+	uint32_t *counts[2];
+	uint32_t *status;
+	size_t total;
+};
+
+static BoxesWork carve_boxes(void *base, int n) {
+	BoxesWork w{};
+	size_t off = 0;
+	char *p = reinterpret_cast<char *>(base);
+	auto take = [&](size_t bytes) {
+		char *r = p ? p + off : nullptr;
+		off += (bytes + 255) & ~(size_t)255;
+		return r;
+	};
+	const size_t nb = (size_t)n;
+	const size_t chunks = (nb + kChunk - 1) / kChunk;
+	w.visit = (uint32_t *)take(nb * 4);
+	w.ranges = (uint2 *)take(nb * 8);
+	for (int i = 0; i < 2; ++i) {
+		w.slots[i] = (uint32_t *)take(chunks * kSlots * 4);
+		w.counts[i] = (uint32_t *)take(chunks * 4);
+	}
+	w.status = (uint32_t *)take(4);
+	w.total = off;
+	return w;
 }
 
 // -------------------------------------------------- stage 7: permute
@@ -320,7 +510,7 @@ struct LbvhWorkspace {
 	rta_aabb *bounds;
 	uint32_t *codes;
 	uint32_t *parents;
-	uint32_t *visit;
+	void *boxes_work;
 	void *sort_work;
 	size_t sort_bytes;
 	size_t total;
@@ -344,7 +534,7 @@ static LbvhWorkspace carve(void *base, int n) {
 	w.bounds = (rta_aabb *)take(sizeof(rta_aabb));
 	w.codes = (uint32_t *)take(nb * 4);
 	w.parents = (uint32_t *)take((2 * nb - 1) * 4);
-	w.visit = (uint32_t *)take(nb * 4);
+	w.boxes_work = take(carve_boxes(nullptr, n).total);
 	size_t sb = 0;
 	rta_lbvh_sort_work_bytes(n, &sb);
 	w.sort_work = take(sb);
@@ -391,26 +581,48 @@ int rta_lbvh_morton_codes(uint32_t *codes, uint32_t *indices, const rta_float3 *
 	return RTA_OK;
 }
 
-int rta_lbvh_build_tree(rta_node *nodes, uint32_t *parents, uint32_t *visit,
-                        const uint32_t *codes, int n, rta_stream s) {
+int rta_lbvh_boxes_work_bytes(int n, size_t *bytes) {
+	if (n <= 0 || !bytes) return invalid("rta_lbvh_boxes_work_bytes: bad arguments");
+	*bytes = carve_boxes(nullptr, n).total;
+	return RTA_OK;
+}
+
+int rta_lbvh_build_tree(rta_node *nodes, uint32_t *parents, void *work, const uint32_t *codes,
+                        int n, rta_stream s) {
 	if (n <= 0) return invalid("rta_lbvh_build_tree: n must be > 0");
-	if (!nodes || !parents || !codes || (n > 1 && !visit))
-		return invalid("rta_lbvh_build_tree: NULL pointer");
+	if (!nodes || !parents || !codes || !work) return invalid("rta_lbvh_build_tree: NULL pointer");
+	BoxesWork w = carve_boxes(work, n);
 	hipLaunchKernelGGL(karras_kernel, dim3(grid_for(2LL * n - 1, 256)), dim3(256), 0,
-	                   as_stream(s), nodes, parents, visit, codes, n);
+	                   as_stream(s), nodes, parents, w.visit, w.ranges, codes, n);
 	RTA_LAUNCH_CHECK("karras_kernel");
 	return RTA_OK;
 }
 
-int rta_lbvh_build_boxes(rta_node *nodes, uint32_t *visit, const rta_aabb *boxes,
+int rta_lbvh_build_boxes(rta_node *nodes, void *work, const rta_aabb *boxes,
                          const uint32_t *codes, const uint32_t *indices,
                          const uint32_t *parents, int n, rta_stream s) {
 	if (n <= 0) return invalid("rta_lbvh_build_boxes: n must be > 0");
-	if (!nodes || !boxes || !codes || !indices || !parents || (n > 1 && !visit))
+	if (!nodes || !boxes || !codes || !indices || !parents || !work)
 		return invalid("rta_lbvh_build_boxes: NULL pointer");
-	hipLaunchKernelGGL(bottom_up_kernel, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(s),
-	                   nodes, visit, boxes, codes, indices, parents, n);
-	RTA_LAUNCH_CHECK("bottom_up_kernel");
+	BoxesWork w = carve_boxes(work, n);
+	hipStream_t st = as_stream(s);
+	int groups = (int)grid_for(n, kChunk);
+	hipLaunchKernelGGL(bottom_up_chunk_kernel, dim3(groups), dim3(kChunk), 0, st, nodes, boxes,
+	                   codes, indices, parents, w.ranges, w.slots[0], w.counts[0], w.status, n);
+	RTA_LAUNCH_CHECK("bottom_up_chunk_kernel");
+	long long chunk = kChunk;
+	int in = 0;
+	while (groups > 1) {
+		const int blocks = (groups + kMergeFan - 1) / kMergeFan;
+		chunk *= kMergeFan;
+		hipLaunchKernelGGL(bottom_up_merge_kernel, dim3(blocks), dim3(1024), 0, st, nodes,
+		                   w.visit, codes, parents, w.ranges, w.slots[in], w.counts[in], groups,
+		                   blocks > 1 ? w.slots[1 - in] : nullptr,
+		                   blocks > 1 ? w.counts[1 - in] : nullptr, w.status, chunk, n);
+		RTA_LAUNCH_CHECK("bottom_up_merge_kernel");
+		groups = blocks;
+		in = 1 - in;
+	}
 	return RTA_OK;
 }
 
@@ -462,9 +674,9 @@ int rta_lbvh_build(rta_triangle *tris, int n, rta_node *nodes, uint32_t *indices
 	mark(3);
 	if (rc == RTA_OK) rc = rta_lbvh_sort(w.codes, indices, n, w.sort_work, w.sort_bytes, s);
 	mark(4);
-	if (rc == RTA_OK) rc = rta_lbvh_build_tree(nodes, w.parents, w.visit, w.codes, n, s);
+	if (rc == RTA_OK) rc = rta_lbvh_build_tree(nodes, w.parents, w.boxes_work, w.codes, n, s);
 	mark(5);
-	if (rc == RTA_OK) rc = rta_lbvh_build_boxes(nodes, w.visit, w.boxes, w.codes, indices, w.parents, n, s);
+	if (rc == RTA_OK) rc = rta_lbvh_build_boxes(nodes, w.boxes_work, w.boxes, w.codes, indices, w.parents, n, s);
 	mark(6);
 	if (rc == RTA_OK) rc = rta_lbvh_permute(tris, w.scratch, indices, n, s);
 	mark(7);

@@ -71,8 +71,9 @@ def gpu_stages(tris):
     status.fill_u32(0)
     nodes = DeviceBuffer(32 * (2 * n - 1))
     parents = DeviceBuffer(4 * (2 * n - 1))
-    visit = DeviceBuffer(4 * max(n - 1, 1))
     sz = C.c_size_t(0)
+    call("rta_lbvh_boxes_work_bytes", n, C.byref(sz))
+    visit = DeviceBuffer(sz.value)
     call("rta_lbvh_sort_work_bytes", n, C.byref(sz))
     work = DeviceBuffer(sz.value)
     out = {}
