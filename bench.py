@@ -124,6 +124,7 @@ def main():
     ws_bytes = C.c_size_t(sz.value)
     pairs = DeviceBuffer(64 * (2 * n - 1))
     tri4 = DeviceBuffer(48 * n)
+    pflags = DeviceBuffer(4)
     prim = rta_amd.RayBuffers(R)
     bnc = rta_amd.RayBuffers(R)
     hits_p = DeviceBuffer(16 * R)
@@ -147,7 +148,8 @@ def main():
         call("rta_lbvh_build", ptr(tris), n, ptr(nodes), ptr(idx), ptr(ws), ws_bytes, ptr(status),
              None, sp)
         ev[1].record(s)
-        call("rta_bvh_pack", ptr(pairs), ptr(tri4), ptr(nodes), 2 * n - 1, ptr(tris), n, sp)
+        call("rta_bvh_pack", ptr(pairs), ptr(tri4), ptr(pflags), ptr(nodes), 2 * n - 1, ptr(tris),
+             n, sp)
         ev[2].record(s)
 
     def frame(root):
@@ -156,7 +158,7 @@ def main():
              ptr(prim.gpu_maxt), cam.fovy, cam.aspect, cam.w, cam.h, vd, pos, up, fmax, sp)
         call("rta_reset_intersections", ptr(hits_p), R, 1, sp)
         ev[3].record(s)
-        call("rta_trace_cis_packed", ptr(tri4), ptr(pairs), root[0], root[1], ptr(prim.gpu_origin),
+        call("rta_trace_cis_packed", ptr(tri4), ptr(pairs), ptr(pflags), root[0], root[1], ptr(prim.gpu_origin),
              ptr(prim.gpu_direction), ptr(prim.gpu_maxt), R, None, ptr(hits_p), ptr(err), sp)
         ev[4].record(s)
         call("rta_diffuse_bounce", ptr(bnc.gpu_origin), ptr(bnc.gpu_direction), ptr(bnc.gpu_maxt),
@@ -165,7 +167,7 @@ def main():
              sp)
         call("rta_reset_intersections", ptr(hits_b), R, 1, sp)
         ev[5].record(s)
-        call("rta_trace_cis_packed", ptr(tri4), ptr(pairs), root[0], root[1], ptr(bnc.gpu_origin),
+        call("rta_trace_cis_packed", ptr(tri4), ptr(pairs), ptr(pflags), root[0], root[1], ptr(bnc.gpu_origin),
              ptr(bnc.gpu_direction), ptr(bnc.gpu_maxt), R, ptr(count), ptr(hits_b), ptr(err), sp)
         ev[6].record(s)
 

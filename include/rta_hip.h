@@ -117,6 +117,14 @@ int rta_hip_event_elapsed_ms(float *ms, rta_event start, rta_event stop);
  * HBM traffic counters against a kernel of known size. */
 int rta_hip_copy_kernel(void *dst, const void *src, size_t bytes, rta_stream s);
 
+/* Self-test of the division-free slab arithmetic on the current device:
+ * compares the FMA-corrected quotient used by rta_trace_cis_packed with the
+ * GPU's IEEE division on n seeded operand pairs of the admitted ranges.
+ * *mismatches gets the count; example (2 floats, may be NULL) one failing
+ * (a, b).  No counterpart in the reference (which divides directly). */
+int rta_hip_check_division(uint64_t n, uint64_t seed, uint64_t *mismatches,
+                           float *example);
+
 /* ---------------------------------------------- LBVH stages (Karras 2012) */
 
 /* initTriangles, bbvh-cuda/lbvh-kernels.cu:344-359,482-493:
@@ -211,20 +219,25 @@ int rta_trace_cis(const rta_triangle *tris, int n_tris, const rta_node *nodes,
                   const float *max_t, int w, int h, rta_hit *hits,
                   uint32_t *stats, uint32_t *err, rta_stream s);
 
-/* Derived layout for the fast path: pairs[i] for inner node i (n_nodes/2
- * records) and the vertex-only triangle array tri4 (3 float4 per triangle:
- * a, a-b, a-c with the same float values intersect_tri_opt forms). */
-int rta_bvh_pack(rta_node_pair *pairs, float *tri4, const rta_node *nodes,
-                 int n_nodes, const rta_triangle *tris, int n_tris,
-                 rta_stream s);
+/* Derived layout for the fast path: pairs[i] for inner node i (n_nodes
+ * records, only inner nodes' written) and the vertex-only triangle array
+ * tri4 (3 float4 per triangle: a, a-b, a-c with the same float values
+ * intersect_tri_opt forms; the w of a leaf's last triangle is nonzero).
+ * flags (device word): bit 0 set when some box coordinate is outside the
+ * range where the division-free slab test is exact (then the traversal
+ * uses IEEE divisions for every ray). */
+int rta_bvh_pack(rta_node_pair *pairs, float *tri4, uint32_t *flags,
+                 const rta_node *nodes, int n_nodes,
+                 const rta_triangle *tris, int n_tris, rta_stream s);
 
 /* Same semantics and results as rta_trace_cis over the packed layout.
- * root_link/root_index: the root node's two link words (host copies of
- * nodes[0].type_left_elems / right_tris).  n_rays bounds the launch; when
- * ray_count (device word) is given, only rays below min(n_rays, *ray_count)
- * are traced, so a ray count produced on the GPU (rta_diffuse_bounce) needs
- * no host round trip. */
+ * flags: the word rta_bvh_pack wrote.  root_link/root_index: the root
+ * node's two link words (host copies of nodes[0].type_left_elems /
+ * right_tris).  n_rays bounds the launch; when ray_count (device word) is
+ * given, only rays below min(n_rays, *ray_count) are traced, so a ray count
+ * produced on the GPU (rta_diffuse_bounce) needs no host round trip. */
 int rta_trace_cis_packed(const float *tri4, const rta_node_pair *pairs,
+                         const uint32_t *flags,
                          uint32_t root_link, uint32_t root_index,
                          const float *ray_orig, const float *ray_dir,
                          const float *max_t, int n_rays,

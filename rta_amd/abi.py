@@ -85,6 +85,7 @@ HIP_SIGNATURES = {
     "rta_hip_event_synchronize": [_vp],
     "rta_hip_event_elapsed_ms": [_pf, _vp, _vp],
     "rta_hip_copy_kernel": [_vp, _vp, _sz, _vp],
+    "rta_hip_check_division": [_u64, _u64, C.POINTER(C.c_uint64), _vp],
     "rta_lbvh_init_triangles": [_vp, _vp, _vp, _vp, _vp, _i, _vp],
     "rta_lbvh_voxel_bounds": [_vp, _vp, _i, _vp],
     "rta_lbvh_morton_codes": [_vp, _vp, _vp, _vp, _vp, _i, _vp],
@@ -99,8 +100,8 @@ HIP_SIGNATURES = {
     "rta_reset_intersections": [_vp, _i, _i, _vp],
     "rta_setup_shirley_rays": [_vp, _vp, _vp, _f, _f, _i, _i, _vp, _vp, _vp, _f, _vp],
     "rta_trace_cis": [_vp, _i, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp, _vp, _vp],
-    "rta_bvh_pack": [_vp, _vp, _vp, _i, _vp, _i, _vp],
-    "rta_trace_cis_packed": [_vp, _vp, _u32, _u32, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp],
+    "rta_bvh_pack": [_vp, _vp, _vp, _vp, _i, _vp, _i, _vp],
+    "rta_trace_cis_packed": [_vp, _vp, _vp, _u32, _u32, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp],
     "rta_bounce_work_bytes": [_i, _psz],
     "rta_diffuse_bounce": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _u64, _vp, _sz, _vp],
 }

@@ -73,8 +73,9 @@ class BinaryLbvh:
 class PackedBvh:
     """Child-pair records (64 B per inner node) and vertex-only triangles."""
 
-    def __init__(self, pairs: DeviceBuffer, tri4: DeviceBuffer, root_link: int, root_index: int):
-        self.pairs, self.tri4 = pairs, tri4
+    def __init__(self, pairs: DeviceBuffer, tri4: DeviceBuffer, flags: DeviceBuffer,
+                 root_link: int, root_index: int):
+        self.pairs, self.tri4, self.flags = pairs, tri4, flags
         self.root_link, self.root_index = root_link, root_index
 
     @classmethod
@@ -82,10 +83,15 @@ class PackedBvh:
                    stream=None) -> "PackedBvh":
         pairs = DeviceBuffer(64 * n_nodes)
         tri4 = DeviceBuffer(48 * n_tris)
-        call("rta_bvh_pack", ptr(pairs), ptr(tri4), ptr(node_data), n_nodes, ptr(tris), n_tris,
-             ptr(stream))
+        flags = DeviceBuffer(4)
+        call("rta_bvh_pack", ptr(pairs), ptr(tri4), ptr(flags), ptr(node_data), n_nodes, ptr(tris),
+             n_tris, ptr(stream))
         root = node_data.download(NODE_DTYPE, 1)[0]
-        return cls(pairs, tri4, int(root["type_left_elems"]), int(root["right_tris"]))
+        return cls(pairs, tri4, flags, int(root["type_left_elems"]), int(root["right_tris"]))
+
+    def exact_division_path(self) -> bool:
+        """True when the boxes admit the division-free slab test."""
+        return (int(self.flags.download(np.uint32, 1)[0]) & 1) == 0
 
 
 class LbvhConstructor:
@@ -232,7 +238,8 @@ class CisTracer:
                  ptr(stats), ptr(self._err), ptr(self.stream))
         else:
             pk = self.bvh.packed(self.stream)
-            call("rta_trace_cis_packed", ptr(pk.tri4), ptr(pk.pairs), pk.root_link, pk.root_index,
+            call("rta_trace_cis_packed", ptr(pk.tri4), ptr(pk.pairs), ptr(pk.flags), pk.root_link,
+                 pk.root_index,
                  ptr(rg.gpu_origin), ptr(rg.gpu_direction), ptr(rg.gpu_maxt), n, ptr(ray_count),
                  ptr(hits), ptr(self._err), ptr(self.stream))
 

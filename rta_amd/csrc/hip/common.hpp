@@ -12,6 +12,7 @@
 #include <string>
 
 #include "../../../include/rta_hip.h"
+#include "exact.h"
 
 namespace rta_hip {
 
@@ -110,6 +111,54 @@ __device__ inline bool box_test(float lo_x, float lo_y, float lo_z,
 	return ok;
 }
 
+// The same slab test with every quotient formed by div_cr from the ray's
+// reciprocal direction (yx = RN(1/dx), ...).  Identical (hit, dist) to
+// box_test whenever div_cr is exact for all six quotients: see
+// slab_operands_fast_ok / ray_fast_ok.
+__device__ inline bool box_test_fast(float lo_x, float lo_y, float lo_z,
+                                     float hi_x, float hi_y, float hi_z,
+                                     float ox, float oy, float oz,
+                                     float dx, float dy, float dz,
+                                     float yx, float yy, float yz, float &dist) {
+	float tn = -FLT_MAX, tf = FLT_MAX;
+	bool ok = true;
+#define RTA_SLAB(LO, HI, O, D, Y)                               \
+	{                                                           \
+		if (D == 0.0f) {                                        \
+			if (O < LO || O > HI) ok = false;                   \
+		} else {                                                \
+			float t1 = div_cr(LO - O, D, Y);                    \
+			float t2 = div_cr(HI - O, D, Y);                    \
+			if (t1 > t2) { float tmp = t1; t1 = t2; t2 = tmp; } \
+			if (t1 > tn) tn = t1;                               \
+			if (t2 < tf) tf = t2;                               \
+			if (tn > tf) ok = false;                            \
+			if (tf < 0.0f) ok = false;                          \
+		}                                                       \
+	}
+	RTA_SLAB(lo_x, hi_x, ox, dx, yx)
+	RTA_SLAB(lo_y, hi_y, oy, dy, yy)
+	RTA_SLAB(lo_z, hi_z, oz, dz, yz)
+#undef RTA_SLAB
+	dist = tn;
+	return ok;
+}
+
+// A coordinate is "fast-safe" if it is 0 or 2^-73 <= |x| <= 2^59.  For two
+// such values lo - o is 0 or a multiple of 2^-96 of magnitude <= 2^60, so
+// every slab numerator is admitted by dividend_fast_ok.
+__device__ __host__ inline bool coord_fast_ok(float x) {
+	const float fx = fabsf(x);
+	return x == 0.0f || (fx >= 0x1p-73f && fx <= 0x1p59f);
+}
+
+// A ray may use box_test_fast (given fast-safe boxes) when its origin is
+// fast-safe and every direction component is 0 or in [2^-60, 2^28].
+__device__ inline bool ray_fast_ok(float ox, float oy, float oz, float dx, float dy, float dz) {
+	return coord_fast_ok(ox) && coord_fast_ok(oy) && coord_fast_ok(oz) && divisor_fast_ok(dx) &&
+	       divisor_fast_ok(dy) && divisor_fast_ok(dz);
+}
+
 // intersect_tri_opt, librta/intersect.h:8-53, operation for operation.
 // (ea_x, ea_y, ea_z) = a - b and (ec_x, ...) = a - c per component, which
 // is what the reference forms first (its `a,b,c` and `d,e,f`).
@@ -134,6 +183,15 @@ __device__ inline bool tri_test(float ax, float ay, float az,
 	common3 = b * l - k * c;
 	float gamma = i * common1 + h * common2 + g * common3;
 	float tt = -(f * common1 + e * common2 + d * common3);
+
+	// A quotient num/M can only be > 0 when num is nonzero and has M's sign
+	// (a zero or underflowed quotient is not > 0 either).  Rejecting here
+	// skips three divisions without changing any outcome; NaNs and M == 0
+	// pass on to the divisions, which reject them as the reference does.
+	const uint32_t sm = __float_as_uint(M);
+	if (beta == 0.0f || gamma == 0.0f || tt == 0.0f ||
+	    ((sm ^ __float_as_uint(beta)) | (sm ^ __float_as_uint(gamma)) | (sm ^ __float_as_uint(tt))) >> 31)
+		return false;
 
 	beta /= M;
 	gamma /= M;

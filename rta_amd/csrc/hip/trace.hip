@@ -153,8 +153,9 @@ __device__ inline uint32_t pair_word(const rta_node &c, uint32_t id) {
 }
 
 __global__ void pack_tris_kernel(float4 *__restrict__ tri4, const rta_triangle *__restrict__ tris,
-                                 int n) {
+                                 uint32_t *__restrict__ flags, int n) {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i == 0) *flags = 0u;
 	if (i >= n) return;
 	const float *f = reinterpret_cast<const float *>(tris + i);
 	const float ax = f[0], ay = f[1], az = f[2];
@@ -168,8 +169,17 @@ __device__ inline void mark_leaf_end(float4 *tri4, const rta_node &leaf) {
 	tri4[3 * last].w = __uint_as_float(1u);
 }
 
+__device__ inline bool box_fast_ok(const rta_aabb &b) {
+	return coord_fast_ok(b.min.x) && coord_fast_ok(b.min.y) && coord_fast_ok(b.min.z) &&
+	       coord_fast_ok(b.max.x) && coord_fast_ok(b.max.y) && coord_fast_ok(b.max.z);
+}
+
+// One pair record per inner node; flags bit 0 is set when some tested box
+// has a coordinate outside the fast-safe range (then every ray uses the
+// IEEE division path).
 __global__ void pack_pairs_kernel(rta_node_pair *__restrict__ pairs, float4 *__restrict__ tri4,
-                                  const rta_node *__restrict__ nodes, int n_nodes) {
+                                  uint32_t *__restrict__ flags, const rta_node *__restrict__ nodes,
+                                  int n_nodes) {
 	const int x = blockIdx.x * blockDim.x + threadIdx.x;
 	if (x >= n_nodes) return;
 	const rta_node nx = nodes[x];
@@ -193,19 +203,28 @@ __global__ void pack_pairs_kernel(rta_node_pair *__restrict__ pairs, float4 *__r
 	pairs[x] = p;
 	if (!link_inner(nl.type_left_elems)) mark_leaf_end(tri4, nl);
 	if (!link_inner(nr.type_left_elems)) mark_leaf_end(tri4, nr);
+	if (!box_fast_ok(nl.box) || !box_fast_ok(nr.box)) atomicOr(flags, 1u);
 }
 
 // -------------------------------------------- packed CIS (fast path)
 
+// Blocks are handed out to the 8 XCDs round-robin; mapping block b to ray
+// tile (b % 8) * per + b / 8 gives each XCD one contiguous eighth of the
+// rays, so neighbouring (coherent) rays share an L2.
+constexpr int kXcds = 8;
+
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void trace_cis_packed_kernel(
-    const float4 *__restrict__ tri4, const float4 *__restrict__ pairs, uint32_t root_word,
-    const float *__restrict__ ro, const float *__restrict__ rd, const float *__restrict__ rmax,
-    int n_rays, const uint32_t *__restrict__ ray_count, rta_hit *__restrict__ hits,
+    const float4 *__restrict__ tri4, const float4 *__restrict__ pairs,
+    const uint32_t *__restrict__ flags, uint32_t root_word, const float *__restrict__ ro,
+    const float *__restrict__ rd, const float *__restrict__ rmax, int n_rays,
+    const uint32_t *__restrict__ ray_count, rta_hit *__restrict__ hits,
     uint32_t *__restrict__ err) {
 	__shared__ uint32_t stack[kStack][BLOCK];
 	const int lt = threadIdx.x;
-	const int tid = blockIdx.x * BLOCK + lt;
+	const int per = gridDim.x / kXcds;
+	const int tile = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+	const int tid = tile * BLOCK + lt;
 	if (tid >= n_rays) return;
 	if (ray_count && (uint32_t)tid >= *ray_count) return;
 
@@ -214,6 +233,8 @@ __global__ __launch_bounds__(BLOCK) void trace_cis_packed_kernel(
 	const float tmax = rmax[tid];
 	rta_hit closest = hits[tid];
 	closest.t = FLT_MAX;
+	const bool fast = ((*flags & 1u) == 0u) && ray_fast_ok(ox, oy, oz, dx, dy, dz);
+	const float yx = 1.0f / dx, yy = 1.0f / dy, yz = 1.0f / dz;
 
 	uint32_t curr = root_word;
 	int sp = -1;
@@ -223,8 +244,16 @@ __global__ __launch_bounds__(BLOCK) void trace_cis_packed_kernel(
 			const float4 *p = pairs + 4 * (size_t)(curr >> 1);
 			const float4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
 			float dl, dr;
-			bool hl = box_test(q0.x, q0.z, q1.x, q1.z, q2.x, q2.z, ox, oy, oz, dx, dy, dz, dl);
-			bool hr = box_test(q0.y, q0.w, q1.y, q1.w, q2.y, q2.w, ox, oy, oz, dx, dy, dz, dr);
+			bool hl, hr;
+			if (fast) {
+				hl = box_test_fast(q0.x, q0.z, q1.x, q1.z, q2.x, q2.z, ox, oy, oz, dx, dy, dz, yx,
+				                   yy, yz, dl);
+				hr = box_test_fast(q0.y, q0.w, q1.y, q1.w, q2.y, q2.w, ox, oy, oz, dx, dy, dz, yx,
+				                   yy, yz, dr);
+			} else {
+				hl = box_test(q0.x, q0.z, q1.x, q1.z, q2.x, q2.z, ox, oy, oz, dx, dy, dz, dl);
+				hr = box_test(q0.y, q0.w, q1.y, q1.w, q2.y, q2.w, ox, oy, oz, dx, dy, dz, dr);
+			}
 			hl = hl && dl < closest.t && dl <= tmax;
 			hr = hr && dr < closest.t && dr <= tmax;
 			const uint32_t wl = __float_as_uint(q3.x), wr = __float_as_uint(q3.y);
@@ -448,31 +477,33 @@ int rta_trace_cis(const rta_triangle *tris, int n_tris, const rta_node *nodes,
 	return RTA_OK;
 }
 
-int rta_bvh_pack(rta_node_pair *pairs, float *tri4, const rta_node *nodes, int n_nodes,
-                 const rta_triangle *tris, int n_tris, rta_stream s) {
-	if (!pairs || !tri4 || !nodes || !tris || n_nodes <= 0 || n_tris <= 0)
+int rta_bvh_pack(rta_node_pair *pairs, float *tri4, uint32_t *flags, const rta_node *nodes,
+                 int n_nodes, const rta_triangle *tris, int n_tris, rta_stream s) {
+	if (!pairs || !tri4 || !flags || !nodes || !tris || n_nodes <= 0 || n_tris <= 0)
 		return invalid("rta_bvh_pack: bad arguments");
 	hipLaunchKernelGGL(pack_tris_kernel, dim3(grid_for(n_tris, 256)), dim3(256), 0, as_stream(s),
-	                   (float4 *)tri4, tris, n_tris);
+	                   (float4 *)tri4, tris, flags, n_tris);
 	RTA_LAUNCH_CHECK("pack_tris_kernel");
 	hipLaunchKernelGGL(pack_pairs_kernel, dim3(grid_for(n_nodes, 256)), dim3(256), 0, as_stream(s),
-	                   pairs, (float4 *)tri4, nodes, n_nodes);
+	                   pairs, (float4 *)tri4, flags, nodes, n_nodes);
 	RTA_LAUNCH_CHECK("pack_pairs_kernel");
 	return RTA_OK;
 }
 
-int rta_trace_cis_packed(const float *tri4, const rta_node_pair *pairs, uint32_t root_link,
-                         uint32_t root_index, const float *ray_orig, const float *ray_dir,
-                         const float *max_t, int n_rays, const uint32_t *ray_count,
-                         rta_hit *hits, uint32_t *err, rta_stream s) {
-	if (!tri4 || !pairs || !ray_orig || !ray_dir || !max_t || !hits || !err || n_rays <= 0)
+int rta_trace_cis_packed(const float *tri4, const rta_node_pair *pairs, const uint32_t *flags,
+                         uint32_t root_link, uint32_t root_index, const float *ray_orig,
+                         const float *ray_dir, const float *max_t, int n_rays,
+                         const uint32_t *ray_count, rta_hit *hits, uint32_t *err, rta_stream s) {
+	if (!tri4 || !pairs || !flags || !ray_orig || !ray_dir || !max_t || !hits || !err ||
+	    n_rays <= 0)
 		return invalid("rta_trace_cis_packed: bad arguments");
 	// the root as its parent's pair record would describe it
 	const uint32_t root_word = link_inner(root_link) ? 1u : (root_index << 1);
-	hipLaunchKernelGGL(trace_cis_packed_kernel<kTraceBlock>, dim3(grid_for(n_rays, kTraceBlock)),
-	                   dim3(kTraceBlock), 0, as_stream(s), (const float4 *)tri4,
-	                   (const float4 *)pairs, root_word, ray_orig, ray_dir, max_t, n_rays, ray_count,
-	                   hits, err);
+	const unsigned tiles = grid_for(n_rays, kTraceBlock);
+	const unsigned blocks = (tiles + kXcds - 1) / kXcds * kXcds;
+	hipLaunchKernelGGL(trace_cis_packed_kernel<kTraceBlock>, dim3(blocks), dim3(kTraceBlock), 0,
+	                   as_stream(s), (const float4 *)tri4, (const float4 *)pairs, flags, root_word,
+	                   ray_orig, ray_dir, max_t, n_rays, ray_count, hits, err);
 	RTA_LAUNCH_CHECK("trace_cis_packed_kernel");
 	return RTA_OK;
 }
