@@ -47,6 +47,10 @@ def parse():
     p.add_argument("--random", type=int, default=200_000)
     p.add_argument("--width", type=int, default=1920)
     p.add_argument("--height", type=int, default=1080)
+    p.add_argument("--trav", choices=("persistent", "packed"), default="persistent")
+    p.add_argument("--block", type=int, default=64)
+    p.add_argument("--refill", type=int, default=8)
+    p.add_argument("--steps-per-refill", dest="steps_per_refill", type=int, default=32)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-rays", type=int, default=0,
                    help="bounce rays in the CPU sample (0 = all of them)")
@@ -95,7 +99,7 @@ def main():
 
     import rta_amd
     from rta_amd import scenes
-    from rta_amd.abi import NODE_DTYPE, call, ptr
+    from rta_amd.abi import NODE_DTYPE, TraceConfig, call, ptr
     from rta_amd.device import DeviceBuffer, Event, Stream
 
     if rta_amd.device_count() < 1:
@@ -152,14 +156,26 @@ def main():
              n, sp)
         ev[2].record(s)
 
+    queue = DeviceBuffer(4)
+    tcfg = TraceConfig(args.block, args.refill, args.steps_per_refill, 0)
+
+    def trace(rays, ray_count, hits, root):
+        if args.trav == "packed":
+            call("rta_trace_cis_packed", ptr(tri4), ptr(pairs), ptr(pflags), root[0], root[1],
+                 ptr(rays.gpu_origin), ptr(rays.gpu_direction), ptr(rays.gpu_maxt), R,
+                 ptr(ray_count), ptr(hits), ptr(err), sp)
+        else:
+            call("rta_trace_cis_persistent", ptr(tri4), ptr(pairs), ptr(pflags), root[0], root[1],
+                 ptr(rays.gpu_origin), ptr(rays.gpu_direction), ptr(rays.gpu_maxt), R,
+                 ptr(ray_count), ptr(hits), ptr(err), ptr(queue), C.byref(tcfg), sp)
+
     def frame(root):
         build_and_pack()
         call("rta_setup_shirley_rays", ptr(prim.gpu_direction), ptr(prim.gpu_origin),
              ptr(prim.gpu_maxt), cam.fovy, cam.aspect, cam.w, cam.h, vd, pos, up, fmax, sp)
         call("rta_reset_intersections", ptr(hits_p), R, 1, sp)
         ev[3].record(s)
-        call("rta_trace_cis_packed", ptr(tri4), ptr(pairs), ptr(pflags), root[0], root[1], ptr(prim.gpu_origin),
-             ptr(prim.gpu_direction), ptr(prim.gpu_maxt), R, None, ptr(hits_p), ptr(err), sp)
+        trace(prim, None, hits_p, root)
         ev[4].record(s)
         call("rta_diffuse_bounce", ptr(bnc.gpu_origin), ptr(bnc.gpu_direction), ptr(bnc.gpu_maxt),
              ptr(src), ptr(count), ptr(hits_p), ptr(tris), ptr(prim.gpu_origin),
@@ -167,8 +183,7 @@ def main():
              sp)
         call("rta_reset_intersections", ptr(hits_b), R, 1, sp)
         ev[5].record(s)
-        call("rta_trace_cis_packed", ptr(tri4), ptr(pairs), ptr(pflags), root[0], root[1], ptr(bnc.gpu_origin),
-             ptr(bnc.gpu_direction), ptr(bnc.gpu_maxt), R, ptr(count), ptr(hits_b), ptr(err), sp)
+        trace(bnc, count, hits_b, root)
         ev[6].record(s)
 
     ev = [Event() for _ in range(7)]
@@ -256,7 +271,10 @@ def main():
                          "primary rays + one diffuse bounce per hit; value = plain closest-hit "
                          "Mrays/s on the bounce rays"),
             "n_tris": n, "primary_rays": R, "bounce_rays": n_bounce,
-            "traversal": "child-is, packed 64-B child-pair nodes, LDS stack",
+            "traversal": (f"child-is, packed 64-B child-pair nodes, LDS stack, {args.trav}"
+                          + (f" (block {args.block}, refill {args.refill}, "
+                             f"steps {args.steps_per_refill})" if args.trav == "persistent"
+                             else "")),
         },
         "primary_mrays_s": round(primary_mrays, 3),
         "bounce_mrays_s": round(bounce_mrays, 3),
@@ -271,7 +289,7 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "trace_cis_packed_kernel (bounce rays)",
+            "kernel": f"trace_cis_{args.trav}_kernel (bounce rays)",
             "bytes_per_ray": round(b_ray, 1), "V": round(V, 3), "T": round(T, 3),
         },
         "cpu_baseline": cpu,

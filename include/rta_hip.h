@@ -244,6 +244,27 @@ int rta_trace_cis_packed(const float *tri4, const rta_node_pair *pairs,
                          const uint32_t *ray_count, rta_hit *hits,
                          uint32_t *err, rta_stream s);
 
+/* Launch shape of the persistent tracer: block size (64, 128 or 256
+ * threads), refill_min (a wave fetches new rays once at least this many of
+ * its 64 lanes are idle, 1..64), steps (traversal steps between refill
+ * checks) and blocks_per_cu (0 = as many as fit). */
+typedef struct rta_trace_config {
+	int block, refill_min, steps, blocks_per_cu;
+} rta_trace_config;
+
+/* Persistent form of rta_trace_cis_packed: a machine-filling grid of waves
+ * that refill idle lanes from a queue counter (`queue`, one device word of
+ * scratch, cleared by this call with a kernel).  Same results, except that
+ * a miss only stores t = FLT_MAX and leaves beta, gamma and ref as they
+ * were (the reset sentinel).  cfg may be NULL (defaults 64/8/32/0). */
+int rta_trace_cis_persistent(const float *tri4, const rta_node_pair *pairs,
+                             const uint32_t *flags, uint32_t root_link,
+                             uint32_t root_index, const float *ray_orig,
+                             const float *ray_dir, const float *max_t,
+                             int n_rays, const uint32_t *ray_count,
+                             rta_hit *hits, uint32_t *err, uint32_t *queue,
+                             const rta_trace_config *cfg, rta_stream s);
+
 /* -------------------------------------------------------------- bouncing */
 
 /* One cosine-weighted diffuse bounce per hit (the incoherent ray set).

@@ -52,6 +52,15 @@ class RtaTimes(C.Structure):
         return {f: float(getattr(self, f)) for f in TIMES_FIELDS}
 
 
+class TraceConfig(C.Structure):
+    """rta_trace_config: launch shape of the persistent tracer."""
+    _fields_ = [("block", C.c_int), ("refill_min", C.c_int), ("steps", C.c_int),
+                ("blocks_per_cu", C.c_int)]
+
+    def as_tuple(self):
+        return (self.block, self.refill_min, self.steps, self.blocks_per_cu)
+
+
 class RtaError(RuntimeError):
     """A non-zero status from the C-ABI (negative: argument/state, positive: HIP)."""
 
@@ -102,6 +111,8 @@ HIP_SIGNATURES = {
     "rta_trace_cis": [_vp, _i, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp, _vp, _vp],
     "rta_bvh_pack": [_vp, _vp, _vp, _vp, _i, _vp, _i, _vp],
     "rta_trace_cis_packed": [_vp, _vp, _vp, _u32, _u32, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp],
+    "rta_trace_cis_persistent": [_vp, _vp, _vp, _u32, _u32, _vp, _vp, _vp, _i, _vp, _vp, _vp,
+                                 _vp, _vp, _vp],
     "rta_bounce_work_bytes": [_i, _psz],
     "rta_diffuse_bounce": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _u64, _vp, _sz, _vp],
 }

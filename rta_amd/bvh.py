@@ -22,7 +22,7 @@ import time
 import numpy as np
 
 from .abi import (FLT_MAX, HIT_DTYPE, NODE_DTYPE, RTA_E_STACK, TRI_DTYPE, RtaError, RtaTimes,
-                  call, ptr)
+                  TraceConfig, call, ptr)
 from .device import DeviceBuffer
 
 
@@ -203,17 +203,22 @@ class GpuBouncer:
 class CisTracer:
     """Closest-hit child-is tracer.  layout 'plain' traverses the 32-byte
     node array directly (the reference kernel's data), 'packed' the derived
-    child-pair layout; both give identical results."""
+    child-pair layout, 'persistent' the packed layout with machine-filling
+    waves that refill idle lanes; all give identical results."""
+
+    LAYOUTS = ("plain", "packed", "persistent")
 
     def __init__(self, raygen: RayBuffers, bvh: BinaryLbvh, bouncer: GpuBouncer,
-                 layout: str = "packed", stream=None):
-        if layout not in ("plain", "packed"):
-            raise ValueError("layout must be 'plain' or 'packed'")
+                 layout: str = "persistent", stream=None, config: TraceConfig | None = None):
+        if layout not in self.LAYOUTS:
+            raise ValueError(f"layout must be one of {self.LAYOUTS}")
         self.raygen, self.bvh, self.bouncer, self.layout = raygen, bvh, bouncer, layout
         self.stream = stream
+        self.config = config
         self.timings: list[float] = []
         self._err = DeviceBuffer(4)
         self._err.fill_u32(0)
+        self._queue = DeviceBuffer(4)
 
     def identification(self) -> str:
         return f"bbvh-hip cis tracer ({self.layout})"
@@ -236,12 +241,19 @@ class CisTracer:
             call("rta_trace_cis", ptr(self.bvh.triangle_data), self.bvh.n, ptr(self.bvh.node_data),
                  ptr(rg.gpu_origin), ptr(rg.gpu_direction), ptr(rg.gpu_maxt), n, 1, ptr(hits),
                  ptr(stats), ptr(self._err), ptr(self.stream))
-        else:
+        elif self.layout == "packed":
             pk = self.bvh.packed(self.stream)
             call("rta_trace_cis_packed", ptr(pk.tri4), ptr(pk.pairs), ptr(pk.flags), pk.root_link,
                  pk.root_index,
                  ptr(rg.gpu_origin), ptr(rg.gpu_direction), ptr(rg.gpu_maxt), n, ptr(ray_count),
                  ptr(hits), ptr(self._err), ptr(self.stream))
+        else:
+            pk = self.bvh.packed(self.stream)
+            cfg = C.byref(self.config) if self.config is not None else None
+            call("rta_trace_cis_persistent", ptr(pk.tri4), ptr(pk.pairs), ptr(pk.flags),
+                 pk.root_link, pk.root_index, ptr(rg.gpu_origin), ptr(rg.gpu_direction),
+                 ptr(rg.gpu_maxt), n, ptr(ray_count), ptr(hits), ptr(self._err), ptr(self._queue),
+                 cfg, ptr(self.stream))
 
     def check_errors(self) -> None:
         err = int(self._err.download(np.uint32, 1)[0])

@@ -1,0 +1,114 @@
+// bbvh-hip plugin entry points: the four C symbols of the plugin contract
+// (bbvh-cuda/main.cpp:347-513 plays this role for the CUDA plugin).
+//
+// Options (after `--` on the host's command line):
+//   -v             verbose (per-stage build times)
+//   -b lbvh        GPU LBVH build (the only builder of this plugin)
+//   -t cis         child-is traversal (the only traversal of this plugin)
+//   -l cpu|2f4     accepted for compatibility; both byte images are one
+//   --trav plain|packed|persistent   traversal kernel (default persistent)
+//   --block N --refill N --steps N   persistent launch shape
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <iostream>
+#include <string>
+
+#include "../include/rta/librta.h"
+#include "bbvh_hip.h"
+
+using namespace rta;
+
+namespace {
+
+struct Cmdline {
+	bool verbose = false;
+	hip::trace_layout layout = hip::trace_layout::persistent;
+	rta_trace_config config{64, 8, 32, 0};
+};
+Cmdline cmdline;
+char plugin_name[] = "bbvh-hip";
+
+int usage(const char *msg, const char *arg) {
+	std::cerr << "bbvh-hip: " << msg << (arg ? arg : "") << std::endl;
+	std::cerr << "usage: bbvh-hip [-v] [-b lbvh] [-t cis] [-l cpu|2f4] "
+	             "[--trav plain|packed|persistent] [--block N] [--refill N] [--steps N]"
+	          << std::endl;
+	return 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+char *description() { return (char *)"bbvh-hip: binary LBVH built and traced on MI355X (gfx950)"; }
+
+void initialize() { hip::plugin_with_hip_loaded = true; }
+
+int parse_cmdline(int argc, char **argv) {
+	if (argc > 0) argv[0] = plugin_name;
+	for (int i = 1; i < argc; ++i) {
+		std::string a = argv[i];
+		auto value = [&](const char *what) -> const char * {
+			if (i + 1 >= argc) return nullptr;
+			(void)what;
+			return argv[++i];
+		};
+		if (a == "-v" || a == "--verbose") cmdline.verbose = true;
+		else if (a == "-b" || a == "--bvh") {
+			const char *v = value("bvh");
+			if (!v || std::string(v) != "lbvh")
+				return usage("only the GPU LBVH builder is available: -b lbvh, got ", v);
+		} else if (a == "-t" || a == "--trav") {
+			const char *v = value("trav");
+			if (!v) return usage("missing value for ", a.c_str());
+			std::string s = v;
+			if (s == "cis" || s == "persistent") cmdline.layout = hip::trace_layout::persistent;
+			else if (s == "packed") cmdline.layout = hip::trace_layout::packed;
+			else if (s == "plain") cmdline.layout = hip::trace_layout::plain;
+			else return usage("unknown traversal ", v);
+		} else if (a == "-l" || a == "--layout") {
+			const char *v = value("layout");
+			if (!v || (std::string(v) != "cpu" && std::string(v) != "2f4"))
+				return usage("layout must be cpu or 2f4, got ", v);
+		} else if (a == "--block" || a == "--refill" || a == "--steps") {
+			const char *v = value("n");
+			if (!v) return usage("missing value for ", a.c_str());
+			const int x = atoi(v);
+			if (a == "--block") cmdline.config.block = x;
+			else if (a == "--refill") cmdline.config.refill_min = x;
+			else cmdline.config.steps = x;
+		} else if (a == "-s" || a == "--shadow" || a == "-A" || a == "--ailabox") {
+			return usage("not provided by this plugin: ", a.c_str());
+		} else {
+			return usage("unknown option ", a.c_str());
+		}
+	}
+	return 0;
+}
+
+rt_set create_rt_set(basic_flat_triangle_list<simple_triangle> &triangle_lists, int w, int h) {
+	typedef simple_aabb box_t;
+	typedef simple_triangle tri_t;
+	// the build runs on device triangles (bbvh-cuda/main.cpp:466-470)
+	basic_flat_triangle_list<tri_t> ftl;
+	void *p = 0;
+	hip::check(rta_hip_malloc(&p, sizeof(tri_t) * (size_t)triangle_lists.triangles), "triangles");
+	hip::check(rta_hip_memcpy_htod(p, triangle_lists.triangle,
+	                               sizeof(tri_t) * (size_t)triangle_lists.triangles),
+	           "upload triangles");
+	ftl.triangle = (tri_t *)p;
+	ftl.triangles = triangle_lists.triangles;
+	auto *ctor = new hip::lbvh_constructor<box_t, tri_t>;
+	ctor->verbose = cmdline.verbose;
+	hip::binary_lbvh<box_t, tri_t> *bvh = ctor->build(&ftl);
+	rt_set set;
+	set.as = bvh;
+	set.ctor = ctor;
+	set.rt = new hip::bbvh_gpu_cis_tracer<box_t, tri_t>(0, bvh, 0, cmdline.layout, cmdline.config);
+	set.rgen = new hip::cam_ray_generator_shirley(w, h);
+	return set;
+}
+
+}  // extern "C"

@@ -1,0 +1,157 @@
+// The ray tracing interfaces a plugin implements and a host drives (the
+// roles of librta/raytrav.h:43-437 in the reference): acceleration
+// structure, its constructor, bouncer, ray generator and ray tracer, with
+// basic_raytracer::trace() running the generate / (prepare, trace, bounce)*
+// loop whose trace_rays() timings the host reports.
+#pragma once
+
+#include <math.h>
+
+#include <string>
+#include <vector>
+
+#include "basic_types.h"
+
+namespace rta {
+
+class acceleration_structure {
+public:
+	virtual ~acceleration_structure() {}
+	virtual std::string identification() = 0;
+	virtual void dump_acceleration_structure(const std::string &) {}
+	virtual void dump_primitives(const std::string &) {}
+};
+
+template <typename box_t_, typename tri_t_>
+class basic_acceleration_structure : public acceleration_structure {
+public:
+	typedef box_t_ box_t;
+	typedef tri_t_ tri_t;
+	//! array indexable with triangle_intersection::ref (device memory for GPU structures)
+	virtual tri_t *triangle_ptr() = 0;
+	virtual int triangle_count() = 0;
+	//! host-accessible copy of triangle_ptr(); may allocate (then release with
+	//! free_canonical_triangles)
+	virtual tri_t *canonical_triangle_ptr() { return triangle_ptr(); }
+	virtual void free_canonical_triangles(tri_t *) {}
+};
+
+class acceleration_structure_constructor {
+public:
+	virtual ~acceleration_structure_constructor() {}
+	virtual std::string identification() = 0;
+};
+
+template <typename box_t_, typename tri_t_>
+class basic_acceleration_structure_constructor : public acceleration_structure_constructor {
+public:
+	typedef box_t_ box_t;
+	typedef tri_t_ tri_t;
+	typedef typename tri_t::input_flat_triangle_list_t flat_triangle_list_t;
+	virtual basic_acceleration_structure<box_t, tri_t> *build(flat_triangle_list_t *tris) = 0;
+	//! whether build() expects triangles in host memory
+	virtual bool expects_host_triangles() { return true; }
+};
+
+class bouncer {
+public:
+	virtual ~bouncer() {}
+	virtual void bounce() = 0;
+	virtual bool trace_further_bounces() = 0;
+	virtual void new_pass() {}
+	virtual std::string identification() = 0;
+};
+
+//! Host ray storage: origin and direction per pixel plus max_t.
+class ray_generator {
+public:
+	uint res_w, res_h;
+	std::vector<vec3f> origins, directions;
+	std::vector<float> ray_max_t;
+	ray_generator(uint res_x, uint res_y)
+	    : res_w(res_x), res_h(res_y), origins((size_t)res_x * res_y), directions((size_t)res_x * res_y),
+	      ray_max_t((size_t)res_x * res_y) {}
+	virtual ~ray_generator() {}
+	virtual void generate_rays() = 0;
+	uint res_x() const { return res_w; }
+	uint res_y() const { return res_h; }
+	vec3f *origin(uint x, uint y) { return &origins[(size_t)y * res_w + x]; }
+	vec3f *direction(uint x, uint y) { return &directions[(size_t)y * res_w + x]; }
+	float &max_t(uint x, uint y) { return ray_max_t[(size_t)y * res_w + x]; }
+	virtual std::string identification() = 0;
+};
+
+//! Pinhole camera after Shirley (librta/raytrav.h:300-351).
+class cam_ray_generator_shirley : public ray_generator {
+protected:
+	float aspect, fovy;
+	vec3f position, dir, up;
+
+public:
+	cam_ray_generator_shirley(uint res_x, uint res_y)
+	    : ray_generator(res_x, res_y), aspect(float(res_x) / float(res_y)), fovy(45.0f),
+	      position{0, 0, 0}, dir{0, 0, -1}, up{0, 1, 0} {}
+	void setup(const vec3f *new_position, const vec3f *new_dir, const vec3f *new_up,
+	           float new_fovy) {
+		position = *new_position;
+		dir = *new_dir;
+		up = *new_up;
+		fovy = new_fovy;
+	}
+	void generate_rays() override;  // host version, in librta
+	std::string identification() override { return "pinhole ray generator (Shirley)"; }
+	float fov() const { return fovy; }
+	float aspect_ratio() const { return aspect; }
+	const vec3f &camera_position() const { return position; }
+	const vec3f &camera_direction() const { return dir; }
+	const vec3f &camera_up() const { return up; }
+};
+
+class raytracer {
+public:
+	virtual ~raytracer() {}
+	virtual void trace() = 0;
+	virtual std::string identification() = 0;
+	virtual raytracer *copy() = 0;
+	//! whether the tracer honours the ray generator's max_t
+	virtual bool supports_max_t() = 0;
+};
+
+template <typename box_t_, typename tri_t_> class basic_raytracer : public raytracer {
+public:
+	typedef box_t_ box_t;
+	typedef tri_t_ tri_t;
+
+protected:
+	rta::ray_generator *raygen;
+	rta::bouncer *bouncer;
+	basic_acceleration_structure<box_t, tri_t> *accel_struct;
+	virtual float trace_rays() = 0;
+
+public:
+	std::vector<float> timings;  //!< trace_rays() ms of each bounce of the last trace()
+	basic_raytracer(rta::ray_generator *rg, rta::bouncer *b, basic_acceleration_structure<box_t, tri_t> *as)
+	    : raygen(rg), bouncer(b), accel_struct(as) {}
+	//! untimed preparation before each trace_rays (e.g. resetting hits)
+	virtual void prepare_trace() {}
+	void trace() override {
+		raygen->generate_rays();
+		bouncer->new_pass();
+		timings.clear();
+		do {
+			prepare_trace();
+			float ms = trace_rays();
+			timings.push_back(ms);
+			bouncer->bounce();
+		} while (bouncer->trace_further_bounces());
+	}
+	virtual void ray_generator(rta::ray_generator *rg) { raygen = rg; }
+	virtual void ray_bouncer(rta::bouncer *b) { bouncer = b; }
+	virtual void acceleration_structure(basic_acceleration_structure<box_t, tri_t> *as) {
+		accel_struct = as;
+	}
+	basic_acceleration_structure<box_t, tri_t> *acceleration_structure() { return accel_struct; }
+	basic_raytracer *copy() override = 0;
+};
+
+}  // namespace rta

@@ -291,6 +291,137 @@ __global__ __launch_bounds__(BLOCK) void trace_cis_packed_kernel(
 	hits[tid] = closest;
 }
 
+// ---------------------------- persistent packed CIS with ray refill
+
+__global__ void fill_queue_kernel(uint32_t *queue) {
+	if (threadIdx.x == 0) *queue = 0u;
+}
+
+// Persistent waves: each lane traces one ray at a time; when at least
+// `refill_min` lanes of a wave are idle the wave takes that many new rays
+// (consecutive ids, so they are coherent) from a global queue counter with
+// one atomic.  A wave never waits for another, and it leaves the loop once
+// the queue is exhausted and all of its lanes are idle.  Results equal
+// trace_cis_packed_kernel's: each ray runs the identical traversal; a miss
+// stores only t = FLT_MAX over the reset sentinel (the reference writes
+// the sentinel back with t = FLT_MAX, kernels.cu:793-794,853).
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void trace_cis_persistent_kernel(
+    const float4 *__restrict__ tri4, const float4 *__restrict__ pairs,
+    const uint32_t *__restrict__ flags, uint32_t root_word, const float *__restrict__ ro,
+    const float *__restrict__ rd, const float *__restrict__ rmax, int n_rays,
+    const uint32_t *__restrict__ ray_count, rta_hit *__restrict__ hits,
+    uint32_t *__restrict__ err, uint32_t *__restrict__ queue, int refill_min, int steps) {
+	__shared__ uint32_t stack[kStack][BLOCK];
+	const int lt = threadIdx.x;
+	const int lane = __lane_id();
+	const uint64_t lower = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+	uint32_t n = (uint32_t)n_rays;
+	if (ray_count) n = min(n, *ray_count);
+	const bool scene_fast = (*flags & 1u) == 0u;
+
+	bool active = false, exhausted = false;
+	uint32_t ray = 0, curr = 0;
+	int sp = -1;
+	uint32_t guard = 0;
+	float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, yx = 0, yy = 0, yz = 0, tmax = 0;
+	float ct = FLT_MAX, cb = 0, cg = 0;
+	uint32_t cref = 0;
+	bool fast = false;
+
+	while (true) {
+		const uint64_t idle = __ballot(!active);
+		const int nidle = __popcll(idle);
+		if (!exhausted && nidle >= refill_min) {
+			const int leader = __ffsll((unsigned long long)idle) - 1;
+			uint32_t base = 0;
+			if (lane == leader) base = atomicAdd(queue, (uint32_t)nidle);
+			base = __shfl(base, leader);
+			if (base + (uint32_t)nidle >= n) exhausted = true;
+			if (!active) {
+				const uint32_t my = base + (uint32_t)__popcll(idle & lower);
+				if (my < n) {
+					ray = my;
+					ox = ro[3 * (size_t)ray + 0]; oy = ro[3 * (size_t)ray + 1]; oz = ro[3 * (size_t)ray + 2];
+					dx = rd[3 * (size_t)ray + 0]; dy = rd[3 * (size_t)ray + 1]; dz = rd[3 * (size_t)ray + 2];
+					tmax = rmax[ray];
+					fast = scene_fast && ray_fast_ok(ox, oy, oz, dx, dy, dz);
+					yx = 1.0f / dx; yy = 1.0f / dy; yz = 1.0f / dz;
+					ct = FLT_MAX;
+					curr = root_word;
+					sp = -1;
+					guard = 0;
+					active = true;
+				}
+			}
+		}
+		// with refill_min <= 64 an all-idle wave has just refilled, so an
+		// all-idle wave here means the queue is exhausted
+		if (__ballot(active) == 0ull) break;
+		for (int k = 0; k < steps && active; ++k) {
+			bool done = false;
+			if (++guard >= kMaxSteps) { atomicAdd(err, 1u); done = true; }
+			else if (curr & 1u) {
+				const float4 *p = pairs + 4 * (size_t)(curr >> 1);
+				const float4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
+				float dl, dr;
+				bool hl, hr;
+				if (fast) {
+					hl = box_test_fast(q0.x, q0.z, q1.x, q1.z, q2.x, q2.z, ox, oy, oz, dx, dy, dz,
+					                   yx, yy, yz, dl);
+					hr = box_test_fast(q0.y, q0.w, q1.y, q1.w, q2.y, q2.w, ox, oy, oz, dx, dy, dz,
+					                   yx, yy, yz, dr);
+				} else {
+					hl = box_test(q0.x, q0.z, q1.x, q1.z, q2.x, q2.z, ox, oy, oz, dx, dy, dz, dl);
+					hr = box_test(q0.y, q0.w, q1.y, q1.w, q2.y, q2.w, ox, oy, oz, dx, dy, dz, dr);
+				}
+				hl = hl && dl < ct && dl <= tmax;
+				hr = hr && dr < ct && dr <= tmax;
+				const uint32_t wl = __float_as_uint(q3.x), wr = __float_as_uint(q3.y);
+				if (hl && hr) {
+					if (sp + 1 >= kStack) { atomicAdd(err, 1u); done = true; }
+					else if (dl <= dr) { stack[++sp][lt] = wr; curr = wl; }
+					else { stack[++sp][lt] = wl; curr = wr; }
+				} else if (hl) {
+					curr = wl;
+				} else if (hr) {
+					curr = wr;
+				} else if (sp >= 0) {
+					curr = stack[sp--][lt];
+				} else {
+					done = true;
+				}
+			} else {
+				uint32_t i = curr >> 1;
+				while (true) {
+					const float4 v0 = tri4[3 * (size_t)i + 0];
+					const float4 v1 = tri4[3 * (size_t)i + 1];
+					const float4 v2 = tri4[3 * (size_t)i + 2];
+					float t, be, ga;
+					if (tri_test(v0.x, v0.y, v0.z, v1.x, v1.y, v1.z, v2.x, v2.y, v2.z, ox, oy, oz,
+					             dx, dy, dz, t, be, ga)) {
+						if (t < ct && t <= tmax) { ct = t; cb = be; cg = ga; cref = i; }
+					}
+					if (__float_as_uint(v0.w) != 0u) break;
+					++i;
+				}
+				if (sp >= 0) curr = stack[sp--][lt];
+				else done = true;
+			}
+			if (done) {
+				if (ct != FLT_MAX) {
+					rta_hit hres;
+					hres.t = ct; hres.beta = cb; hres.gamma = cg; hres.ref = cref;
+					hits[ray] = hres;
+				} else {
+					hits[ray].t = FLT_MAX;
+				}
+				active = false;
+			}
+		}
+	}
+}
+
 // ------------------------------------------------------- bouncing
 
 constexpr int kBounceBlock = 256;
@@ -409,6 +540,38 @@ __global__ __launch_bounds__(kBounceBlock) void bounce_emit_kernel(
 
 }  // namespace rta_hip
 
+namespace rta_hip {
+
+template <int BLOCK>
+int launch_persistent(const float *tri4, const rta_node_pair *pairs, const uint32_t *flags,
+                      uint32_t root_word, const float *ro, const float *rd, const float *rmax,
+                      int n_rays, const uint32_t *ray_count, rta_hit *hits, uint32_t *err,
+                      uint32_t *queue, int refill_min, int steps, int blocks_per_cu,
+                      hipStream_t st) {
+	static int cus = 0, occ = 0;
+	if (cus == 0) {
+		int dev = 0;
+		RTA_HIP_TRY(hipGetDevice(&dev));
+		RTA_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+		RTA_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+		    &occ, trace_cis_persistent_kernel<BLOCK>, BLOCK, 0));
+		if (occ < 1) occ = 1;
+	}
+	const int bpc = blocks_per_cu > 0 ? blocks_per_cu : occ;
+	const long long want = (long long)cus * bpc;
+	const long long need = ((long long)n_rays + BLOCK - 1) / BLOCK;
+	const unsigned blocks = (unsigned)(want < need ? want : need);
+	hipLaunchKernelGGL(fill_queue_kernel, dim3(1), dim3(64), 0, st, queue);
+	RTA_LAUNCH_CHECK("fill_queue_kernel");
+	hipLaunchKernelGGL(trace_cis_persistent_kernel<BLOCK>, dim3(blocks), dim3(BLOCK), 0, st,
+	                   (const float4 *)tri4, (const float4 *)pairs, flags, root_word, ro, rd, rmax,
+	                   n_rays, ray_count, hits, err, queue, refill_min, steps);
+	RTA_LAUNCH_CHECK("trace_cis_persistent_kernel");
+	return RTA_OK;
+}
+
+}  // namespace rta_hip
+
 using namespace rta_hip;
 
 extern "C" {
@@ -506,6 +669,38 @@ int rta_trace_cis_packed(const float *tri4, const rta_node_pair *pairs, const ui
 	                   ray_orig, ray_dir, max_t, n_rays, ray_count, hits, err);
 	RTA_LAUNCH_CHECK("trace_cis_packed_kernel");
 	return RTA_OK;
+}
+
+int rta_trace_cis_persistent(const float *tri4, const rta_node_pair *pairs,
+                             const uint32_t *flags, uint32_t root_link, uint32_t root_index,
+                             const float *ray_orig, const float *ray_dir, const float *max_t,
+                             int n_rays, const uint32_t *ray_count, rta_hit *hits, uint32_t *err,
+                             uint32_t *queue, const rta_trace_config *cfg, rta_stream s) {
+	if (!tri4 || !pairs || !flags || !ray_orig || !ray_dir || !max_t || !hits || !err ||
+	    !queue || n_rays <= 0)
+		return invalid("rta_trace_cis_persistent: bad arguments");
+	rta_trace_config c = {64, 8, 32, 0};
+	if (cfg) c = *cfg;
+	if (c.refill_min < 1 || c.refill_min > 64 || c.steps < 1 || c.blocks_per_cu < 0)
+		return invalid("rta_trace_cis_persistent: bad config");
+	const uint32_t root_word = link_inner(root_link) ? 1u : (root_index << 1);
+	hipStream_t st = as_stream(s);
+	switch (c.block) {
+	case 64:
+		return launch_persistent<64>(tri4, pairs, flags, root_word, ray_orig, ray_dir, max_t, n_rays,
+		                             ray_count, hits, err, queue, c.refill_min, c.steps,
+		                             c.blocks_per_cu, st);
+	case 128:
+		return launch_persistent<128>(tri4, pairs, flags, root_word, ray_orig, ray_dir, max_t,
+		                              n_rays, ray_count, hits, err, queue, c.refill_min, c.steps,
+		                              c.blocks_per_cu, st);
+	case 256:
+		return launch_persistent<256>(tri4, pairs, flags, root_word, ray_orig, ray_dir, max_t,
+		                              n_rays, ray_count, hits, err, queue, c.refill_min, c.steps,
+		                              c.blocks_per_cu, st);
+	default:
+		return invalid("rta_trace_cis_persistent: block must be 64, 128 or 256");
+	}
 }
 
 int rta_bounce_work_bytes(int n_rays, size_t *bytes) {

@@ -18,7 +18,7 @@ import pytest
 import oracle
 import rta_amd
 from rta_amd import scenes
-from rta_amd.abi import AABB_DTYPE, HIT_DTYPE, NODE_DTYPE, TRI_DTYPE, call, ptr
+from rta_amd.abi import AABB_DTYPE, HIT_DTYPE, NODE_DTYPE, TRI_DTYPE, TraceConfig, call, ptr
 from rta_amd.device import DeviceBuffer
 
 pytestmark = pytest.mark.gpu
@@ -209,10 +209,13 @@ def test_reset_sentinel(gpu):
     assert (h["ref"] == 0xDEADBEEF).all()   # ref untouched (cuda-kernels.cu:26-28)
 
 
-def trace_gpu(bvh, o, d, t, layout, stats=False):
+LAYOUTS = ["plain", "packed", "persistent"]
+
+
+def trace_gpu(bvh, o, d, t, layout, stats=False, config=None):
     rays = rta_amd.RayBuffers.from_host(o, d, t)
     bouncer = rta_amd.GpuBouncer(len(t))
-    tracer = rta_amd.CisTracer(rays, bvh, bouncer, layout=layout)
+    tracer = rta_amd.CisTracer(rays, bvh, bouncer, layout=layout, config=config)
     tracer.prepare_trace()
     st = DeviceBuffer(8 * len(t)) if stats else None
     tracer.launch(stats=st)
@@ -233,7 +236,7 @@ def assert_hits_equal(got, ref):
     assert (got["beta"][~hit] == 1).all() and (got["gamma"][~hit] == 2).all()
 
 
-@pytest.mark.parametrize("layout", ["plain", "packed"])
+@pytest.mark.parametrize("layout", LAYOUTS)
 def test_trace_config1_bit_exact(gpu, layout):
     say(f"trace config1 layout={layout}")
     tris = scenes.config1_scene()
@@ -250,7 +253,7 @@ def test_trace_config1_bit_exact(gpu, layout):
         assert np.array_equal(st, ref_st)
 
 
-@pytest.mark.parametrize("layout", ["plain", "packed"])
+@pytest.mark.parametrize("layout", LAYOUTS)
 def test_trace_incoherent_and_duplicates(gpu, layout):
     say(f"trace incoherent layout={layout}")
     tris = np.concatenate([clustered(4000, 23, seed=8), scenes.icosphere(3, jitter=0.05, seed=2)])
@@ -291,10 +294,29 @@ def test_trace_median_tree(gpu):
     nodes = np.zeros(2 * len(tris) - 1, NODE_DTYPE)
     nodes[: len(mb["nodes"])] = mb["nodes"]
     bvh.node_data = DeviceBuffer.from_array(nodes)
-    for layout in ("plain", "packed"):
+    for layout in LAYOUTS:
         got, _ = trace_gpu(bvh, o, d, t, layout)
         assert_hits_equal(got, ref)
         bvh.invalidate()
+
+
+@pytest.mark.parametrize("shape", [(64, 1, 1), (64, 64, 1), (128, 16, 7), (256, 33, 100),
+                                   (64, 8, 32)])
+def test_persistent_launch_shapes(gpu, shape):
+    """Every launch shape of the persistent tracer gives the oracle's hits."""
+    say(f"persistent shape={shape}")
+    tris = scenes.icosphere(4, jitter=0.05, seed=3)
+    ref_b = oracle.lbvh_build(tris)
+    rng = np.random.default_rng(9)
+    n = 30_001
+    o = rng.uniform(-1.2, 1.2, size=(n, 3)).astype(np.float32).reshape(-1)
+    d = rng.normal(size=(n, 3)).astype(np.float32)
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32).reshape(-1)
+    t = np.full(n, F32MAX, np.float32)
+    ref, _, _ = oracle.trace_cis(ref_b["tris"], ref_b["nodes"], o, d, t, n)
+    bvh = rta_amd.LbvhConstructor().build(DeviceBuffer.from_array(tris), len(tris))
+    got, _ = trace_gpu(bvh, o, d, t, "persistent", config=TraceConfig(*shape, 0))
+    assert_hits_equal(got, ref)
 
 
 def test_bounce_bit_exact(gpu):
