@@ -52,6 +52,8 @@ def parse():
     p.add_argument("--refill", type=int, default=8)
     p.add_argument("--steps-per-refill", dest="steps_per_refill", type=int, default=32)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="only run a 1 GiB device copy kernel (to scale HBM traffic counters)")
     p.add_argument("--cpu-rays", type=int, default=0,
                    help="bounce rays in the CPU sample (0 = all of them)")
     return p.parse_args()
@@ -93,8 +95,9 @@ def cpu_baseline(tris_h, bo, bd, bt, n_bounce, w, max_rays):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU sharding is "
-                                   "designed (DESIGN.md) but not run: use --gpus 1"}))
+        print(json.dumps({"error": "bench.py measures one GPU; the traversal shards by rays "
+                                   "(one replica of the tree per GPU) but no multi-GPU run "
+                                   "is provided: use --gpus 1"}))
         sys.exit(2)
 
     import rta_amd
@@ -107,6 +110,25 @@ def main():
     call("rta_hip_set_device", 0)
     dev_name = rta_amd.device_name()
     log(f"device {dev_name}")
+
+    if args.copy_calibration:
+        # a plain copy of known size: 1 GiB read + 1 GiB written per dispatch
+        nbytes = 1 << 30
+        src_b, dst_b = DeviceBuffer(nbytes), DeviceBuffer(nbytes)
+        st = Stream()
+        a, b = Event(), Event()
+        call("rta_hip_copy_kernel", ptr(dst_b), ptr(src_b), C.c_size_t(nbytes), ptr(st))
+        times = []
+        for _ in range(3):
+            a.record(st)
+            call("rta_hip_copy_kernel", ptr(dst_b), ptr(src_b), C.c_size_t(nbytes), ptr(st))
+            b.record(st)
+            b.synchronize()
+            times.append(a.elapsed_ms(b))
+        ms = min(times)
+        print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes,
+                          "ms": ms, "GB_s": 2 * nbytes / ms / 1e6}), flush=True)
+        return
 
     t0 = time.perf_counter()
     tris_h = scenes.config2_scene(grid=args.grid, n_random=args.random)
@@ -246,6 +268,21 @@ def main():
     log(f"primary {primary_mrays:.1f} Mrays/s, bounce {bounce_mrays:.1f} Mrays/s, "
         f"build {build_mtri:.1f} Mtri/s ({avg['build']:.3f} ms), V={V:.1f} T={T:.2f}")
 
+    # HBM traffic per bounce launch from the committed rocprofv3 --pmc
+    # measurement of this same workload (FETCH_SIZE and WRITE_SIZE, each
+    # scaled by what the counter reads for a copy kernel of known size)
+    traffic, traffic_src = None, None
+    pmc_file = os.path.join(ROOT, "profiles", "trace_persistent_heightfield_2M_pmc.json")
+    default_workload = (args.trav == "persistent" and args.grid == 1024 and
+                        args.random == 200_000 and (args.width, args.height) == (1920, 1080))
+    if default_workload and os.path.exists(pmc_file):
+        with open(pmc_file) as f:
+            pmc = json.load(f)
+        rec = pmc["dispatches"].get("bounce rays")
+        if rec:
+            traffic = round(rec["fetch_bytes"] + rec["write_bytes"])
+            traffic_src = os.path.relpath(pmc_file, ROOT)
+
     cpu = None
     if not args.no_cpu:
         log("CPU baseline (oracle restatement of the reference CPU plugin)")
@@ -288,7 +325,8 @@ def main():
         "lbvh_status": st,
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "traffic_source": traffic_src,
             "kernel": f"trace_cis_{args.trav}_kernel (bounce rays)",
             "bytes_per_ray": round(b_ray, 1), "V": round(V, 3), "T": round(T, 3),
         },

@@ -3,8 +3,8 @@ ray generator and the bouncer against the CPU oracle, bit for bit.
 
 Bar: integer/index outputs (codes, permutation, links, parents, hit ids)
 identical; float outputs (boxes, centroids, t, beta, gamma, rays) identical
-as IEEE bit patterns (the north star allows 1e-6 relative for t and the
-barycentrics; these kernels meet it with zero difference), except that the
+as IEEE bit patterns (a 1e-6 relative tolerance on t and the barycentrics
+would be acceptable; these kernels meet it with zero difference), except that the
 sign of a zero inside a box is compared by value (-0 == +0, which the
 reference leaves to thread arrival order)."""
 from __future__ import annotations

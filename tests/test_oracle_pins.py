@@ -1,7 +1,7 @@
 """Pin the CPU oracle (oracle/oracle.cpp) against checks that do not restate
 it: known answers, closed forms, exact rationals, brute force and
 invariants.  The reference ships no tests, golden vectors or fixtures for
-this path, so these are the pins ("Parity")."""
+this path, so these are the pins."""
 from __future__ import annotations
 
 from fractions import Fraction
