@@ -49,8 +49,9 @@ def parse():
     p.add_argument("--height", type=int, default=1080)
     p.add_argument("--trav", choices=("persistent", "packed"), default="persistent")
     p.add_argument("--block", type=int, default=64)
-    p.add_argument("--refill", type=int, default=8)
-    p.add_argument("--steps-per-refill", dest="steps_per_refill", type=int, default=32)
+    p.add_argument("--refill", type=int, default=16)
+    p.add_argument("--steps-per-refill", dest="steps_per_refill", type=int, default=16)
+    p.add_argument("--leaf-min", dest="leaf_min", type=int, default=6)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run a 1 GiB device copy kernel (to scale HBM traffic counters)")
@@ -179,7 +180,7 @@ def main():
         ev[2].record(s)
 
     queue = DeviceBuffer(4)
-    tcfg = TraceConfig(args.block, args.refill, args.steps_per_refill, 0)
+    tcfg = TraceConfig(args.block, args.refill, args.steps_per_refill, 0, args.leaf_min)
 
     def trace(rays, ray_count, hits, root):
         if args.trav == "packed":
@@ -310,8 +311,8 @@ def main():
             "n_tris": n, "primary_rays": R, "bounce_rays": n_bounce,
             "traversal": (f"child-is, packed 64-B child-pair nodes, LDS stack, {args.trav}"
                           + (f" (block {args.block}, refill {args.refill}, "
-                             f"steps {args.steps_per_refill})" if args.trav == "persistent"
-                             else "")),
+                             f"steps {args.steps_per_refill}, leaf_min {args.leaf_min})"
+                             if args.trav == "persistent" else "")),
         },
         "primary_mrays_s": round(primary_mrays, 3),
         "bounce_mrays_s": round(bounce_mrays, 3),

@@ -125,6 +125,16 @@ int rta_hip_copy_kernel(void *dst, const void *src, size_t bytes, rta_stream s);
 int rta_hip_check_division(uint64_t n, uint64_t seed, uint64_t *mismatches,
                            float *example);
 
+/* Enumeration of the one-correction quotient used by the traversal kernels:
+ * for every divisor significand B = 1 + mb*2^-23 with mb in
+ * [mb_begin, mb_begin + mb_count) and every dividend significand A in [1, 2),
+ * compares it with the GPU's IEEE quotient A/B.  mb_begin + mb_count <= 2^23
+ * (the 64 calls of 2^17 cover all 2^46 pairs, about a second each).
+ * example (2 uint32 bit patterns, may be NULL) gets one failing (A, B).
+ * Synchronous.  No counterpart in the reference. */
+int rta_hip_check_division_pairs(uint32_t mb_begin, uint32_t mb_count, uint64_t *mismatches,
+                                 uint32_t *example);
+
 /* ---------------------------------------------- LBVH stages (Karras 2012) */
 
 /* initTriangles, bbvh-cuda/lbvh-kernels.cu:344-359,482-493:
@@ -247,16 +257,19 @@ int rta_trace_cis_packed(const float *tri4, const rta_node_pair *pairs,
 /* Launch shape of the persistent tracer: block size (64, 128 or 256
  * threads), refill_min (a wave fetches new rays once at least this many of
  * its 64 lanes are idle, 1..64), steps (traversal steps between refill
- * checks) and blocks_per_cu (0 = as many as fit). */
+ * checks), blocks_per_cu (0 = as many as fit) and leaf_min (a wave tests
+ * leaf triangles once at least this many of its lanes wait at a leaf, or no
+ * lane is at an inner node; 1..64, 0 = default 6).  None of these changes
+ * a result. */
 typedef struct rta_trace_config {
-	int block, refill_min, steps, blocks_per_cu;
+	int block, refill_min, steps, blocks_per_cu, leaf_min;
 } rta_trace_config;
 
 /* Persistent form of rta_trace_cis_packed: a machine-filling grid of waves
  * that refill idle lanes from a queue counter (`queue`, one device word of
  * scratch, cleared by this call with a kernel).  Same results, except that
  * a miss only stores t = FLT_MAX and leaves beta, gamma and ref as they
- * were (the reset sentinel).  cfg may be NULL (defaults 64/8/32/0). */
+ * were (the reset sentinel).  cfg may be NULL (defaults 64/16/16/0/6). */
 int rta_trace_cis_persistent(const float *tri4, const rta_node_pair *pairs,
                              const uint32_t *flags, uint32_t root_link,
                              uint32_t root_index, const float *ray_orig,

@@ -55,10 +55,10 @@ class RtaTimes(C.Structure):
 class TraceConfig(C.Structure):
     """rta_trace_config: launch shape of the persistent tracer."""
     _fields_ = [("block", C.c_int), ("refill_min", C.c_int), ("steps", C.c_int),
-                ("blocks_per_cu", C.c_int)]
+                ("blocks_per_cu", C.c_int), ("leaf_min", C.c_int)]
 
     def as_tuple(self):
-        return (self.block, self.refill_min, self.steps, self.blocks_per_cu)
+        return (self.block, self.refill_min, self.steps, self.blocks_per_cu, self.leaf_min)
 
 
 class RtaError(RuntimeError):
@@ -95,6 +95,7 @@ HIP_SIGNATURES = {
     "rta_hip_event_elapsed_ms": [_pf, _vp, _vp],
     "rta_hip_copy_kernel": [_vp, _vp, _sz, _vp],
     "rta_hip_check_division": [_u64, _u64, C.POINTER(C.c_uint64), _vp],
+    "rta_hip_check_division_pairs": [_u32, _u32, C.POINTER(C.c_uint64), _vp],
     "rta_lbvh_init_triangles": [_vp, _vp, _vp, _vp, _vp, _i, _vp],
     "rta_lbvh_voxel_bounds": [_vp, _vp, _i, _vp],
     "rta_lbvh_morton_codes": [_vp, _vp, _vp, _vp, _vp, _i, _vp],

@@ -199,7 +199,7 @@ public:
 
 	bbvh_gpu_cis_tracer(rta::ray_generator *gen, binary_lbvh<box_t, tri_t> *bvh, rta::bouncer *b,
 	                    trace_layout layout = trace_layout::persistent,
-	                    rta_trace_config cfg = rta_trace_config{64, 8, 32, 0})
+	                    rta_trace_config cfg = rta_trace_config{64, 16, 16, 0, 0})
 	    : gpu_raytracer<box_t, tri_t>(gen, b, bvh), bvh(bvh), layout(layout), config(cfg) {
 		void *p = 0;
 		check(rta_hip_malloc(&p, 8), "error words");

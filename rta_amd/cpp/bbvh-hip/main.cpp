@@ -7,7 +7,7 @@
 //   -t cis         child-is traversal (the only traversal of this plugin)
 //   -l cpu|2f4     accepted for compatibility; both byte images are one
 //   --trav plain|packed|persistent   traversal kernel (default persistent)
-//   --block N --refill N --steps N   persistent launch shape
+//   --block N --refill N --steps N --leaf-min N   persistent launch shape
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -25,7 +25,7 @@ namespace {
 struct Cmdline {
 	bool verbose = false;
 	hip::trace_layout layout = hip::trace_layout::persistent;
-	rta_trace_config config{64, 8, 32, 0};
+	rta_trace_config config{64, 16, 16, 0, 0};
 };
 Cmdline cmdline;
 char plugin_name[] = "bbvh-hip";
@@ -33,7 +33,7 @@ char plugin_name[] = "bbvh-hip";
 int usage(const char *msg, const char *arg) {
 	std::cerr << "bbvh-hip: " << msg << (arg ? arg : "") << std::endl;
 	std::cerr << "usage: bbvh-hip [-v] [-b lbvh] [-t cis] [-l cpu|2f4] "
-	             "[--trav plain|packed|persistent] [--block N] [--refill N] [--steps N]"
+	             "[--trav plain|packed|persistent] [--block N] [--refill N] [--steps N] [--leaf-min N]"
 	          << std::endl;
 	return 1;
 }
@@ -72,12 +72,13 @@ int parse_cmdline(int argc, char **argv) {
 			const char *v = value("layout");
 			if (!v || (std::string(v) != "cpu" && std::string(v) != "2f4"))
 				return usage("layout must be cpu or 2f4, got ", v);
-		} else if (a == "--block" || a == "--refill" || a == "--steps") {
+		} else if (a == "--block" || a == "--refill" || a == "--steps" || a == "--leaf-min") {
 			const char *v = value("n");
 			if (!v) return usage("missing value for ", a.c_str());
 			const int x = atoi(v);
 			if (a == "--block") cmdline.config.block = x;
 			else if (a == "--refill") cmdline.config.refill_min = x;
+			else if (a == "--leaf-min") cmdline.config.leaf_min = x;
 			else cmdline.config.steps = x;
 		} else if (a == "-s" || a == "--shadow" || a == "-A" || a == "--ailabox") {
 			return usage("not provided by this plugin: ", a.c_str());

@@ -111,37 +111,32 @@ __device__ inline bool box_test(float lo_x, float lo_y, float lo_z,
 	return ok;
 }
 
-// The same slab test with every quotient formed by div_cr from the ray's
-// reciprocal direction (yx = RN(1/dx), ...).  Identical (hit, dist) to
-// box_test whenever div_cr is exact for all six quotients: see
-// slab_operands_fast_ok / ray_fast_ok.
-__device__ inline bool box_test_fast(float lo_x, float lo_y, float lo_z,
-                                     float hi_x, float hi_y, float hi_z,
-                                     float ox, float oy, float oz,
-                                     float dx, float dy, float dz,
-                                     float yx, float yy, float yz, float &dist) {
-	float tn = -FLT_MAX, tf = FLT_MAX;
-	bool ok = true;
-#define RTA_SLAB(LO, HI, O, D, Y)                               \
-	{                                                           \
-		if (D == 0.0f) {                                        \
-			if (O < LO || O > HI) ok = false;                   \
-		} else {                                                \
-			float t1 = div_cr(LO - O, D, Y);                    \
-			float t2 = div_cr(HI - O, D, Y);                    \
-			if (t1 > t2) { float tmp = t1; t1 = t2; t2 = tmp; } \
-			if (t1 > tn) tn = t1;                               \
-			if (t2 < tf) tf = t2;                               \
-			if (tn > tf) ok = false;                            \
-			if (tf < 0.0f) ok = false;                          \
-		}                                                       \
-	}
-	RTA_SLAB(lo_x, hi_x, ox, dx, yx)
-	RTA_SLAB(lo_y, hi_y, oy, dy, yy)
-	RTA_SLAB(lo_z, hi_z, oz, dz, yz)
-#undef RTA_SLAB
+// The slab test for a ray none of whose direction components is zero, with
+// every quotient formed by div_cr1 from the ray's reciprocal direction
+// (yx = RN(1/dx), ...), without branches.  With no zero component all three
+// axes take the reference's division branch, and:
+//  - `if (t1 > t2) swap` leaves (t1, t2) = (min, max) of the two quotients
+//    (equal values are interchangeable), so near = fminf, far = fmaxf;
+//  - `if (t1 > t_near) t_near = t1` over the axes, from -FLT_MAX, is the
+//    maximum of the finite near values (fmaxf may pick the other zero of
+//    -0/+0, which no later comparison can tell apart), likewise t_far;
+//  - t_near only grows and t_far only shrinks, so some axis has
+//    t_near > t_far or t_far < 0 after its update exactly when the final
+//    values do.
+// Hence (hit, dist) equal box_test's whenever every quotient is exact:
+// coord_fast_ok boxes and origins, ray_nz_ok directions.
+__device__ inline bool box_test_nz(float lo_x, float lo_y, float lo_z,
+                                   float hi_x, float hi_y, float hi_z,
+                                   float ox, float oy, float oz,
+                                   float dx, float dy, float dz,
+                                   float yx, float yy, float yz, float &dist) {
+	const float ax = div_cr1(lo_x - ox, dx, yx), bx = div_cr1(hi_x - ox, dx, yx);
+	const float ay = div_cr1(lo_y - oy, dy, yy), by = div_cr1(hi_y - oy, dy, yy);
+	const float az = div_cr1(lo_z - oz, dz, yz), bz = div_cr1(hi_z - oz, dz, yz);
+	const float tn = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fminf(az, bz));
+	const float tf = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
 	dist = tn;
-	return ok;
+	return tn <= tf && tf >= 0.0f;
 }
 
 // A coordinate is "fast-safe" if it is 0 or 2^-73 <= |x| <= 2^59.  For two
@@ -152,11 +147,14 @@ __device__ __host__ inline bool coord_fast_ok(float x) {
 	return x == 0.0f || (fx >= 0x1p-73f && fx <= 0x1p59f);
 }
 
-// A ray may use box_test_fast (given fast-safe boxes) when its origin is
-// fast-safe and every direction component is 0 or in [2^-60, 2^28].
-__device__ inline bool ray_fast_ok(float ox, float oy, float oz, float dx, float dy, float dz) {
-	return coord_fast_ok(ox) && coord_fast_ok(oy) && coord_fast_ok(oz) && divisor_fast_ok(dx) &&
-	       divisor_fast_ok(dy) && divisor_fast_ok(dz);
+// A ray may use box_test_nz (given fast-safe boxes) when its origin is
+// fast-safe and every direction component is nonzero with magnitude in
+// [2^-60, 2^28].  Other rays (axis-parallel ones among them) take the IEEE
+// box_test.
+__device__ inline bool ray_nz_ok(float ox, float oy, float oz, float dx, float dy, float dz) {
+	return coord_fast_ok(ox) && coord_fast_ok(oy) && coord_fast_ok(oz) && dx != 0.0f &&
+	       dy != 0.0f && dz != 0.0f && divisor_fast_ok(dx) && divisor_fast_ok(dy) &&
+	       divisor_fast_ok(dz);
 }
 
 // intersect_tri_opt, librta/intersect.h:8-53, operation for operation.

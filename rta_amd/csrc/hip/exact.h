@@ -40,6 +40,24 @@ RTA_HD inline float div_cr(float a, float b, float y) {
 	return fmaf(r1, y, q1);
 }
 
+// div_cr1(a, b, y): RN(a/b) with one correction step (one multiply, two
+// FMAs), for the same operand ranges as div_cr:
+//   q0 = RN(a*y),  r0 = RN(a - b*q0),  q1 = RN(q0 + r0*y).
+// No theorem covers this shorter sequence (q0 is not always faithful), so it
+// rests on enumeration: for every pair of significands A, B in [1, 2) (2^46
+// pairs) q1 == RN(A/B) (rta_hip_check_division_pairs, all 64 chunks).  The
+// enumeration extends to every exponent pair in range because each step
+// scales exactly by powers of two there: y, q0 and q1 stay normal (quotients
+// in [2^-124, 2^120]), and r0 is either exact (the exact a - b*q0 is a
+// multiple of 2^(e_a - 47) >= 2^-143, so it is representable even when
+// subnormal) or it needs more than 24 bits, in which case
+// |a - b*q0| >= 2^(e_a - 23) >= 2^-119 is normal and rounds as at unit scale.
+RTA_HD inline float div_cr1(float a, float b, float y) {
+	const float q0 = a * y;
+	const float r0 = fmaf(-b, q0, a);
+	return fmaf(r0, y, q0);
+}
+
 // |b| of a divisor usable with div_cr (zero is handled by the caller)
 RTA_HD inline bool divisor_fast_ok(float b) {
 	const float fb = fabsf(b);

@@ -213,6 +213,22 @@ __global__ void pack_pairs_kernel(rta_node_pair *__restrict__ pairs, float4 *__r
 // rays, so neighbouring (coherent) rays share an L2.
 constexpr int kXcds = 8;
 
+// Both child boxes of a pair record (q0..q2: lo_x, lo_y, lo_z, hi_x, hi_y,
+// hi_z of the left and right child, interleaved).  `fast` rays take the
+// branch-free exact-division test, the others the IEEE one.
+__device__ inline void pair_boxes(const float4 &q0, const float4 &q1, const float4 &q2, bool fast,
+                                  float ox, float oy, float oz, float dx, float dy, float dz,
+                                  float yx, float yy, float yz, bool &hl, float &dl, bool &hr,
+                                  float &dr) {
+	if (fast) {
+		hl = box_test_nz(q0.x, q0.z, q1.x, q1.z, q2.x, q2.z, ox, oy, oz, dx, dy, dz, yx, yy, yz, dl);
+		hr = box_test_nz(q0.y, q0.w, q1.y, q1.w, q2.y, q2.w, ox, oy, oz, dx, dy, dz, yx, yy, yz, dr);
+	} else {
+		hl = box_test(q0.x, q0.z, q1.x, q1.z, q2.x, q2.z, ox, oy, oz, dx, dy, dz, dl);
+		hr = box_test(q0.y, q0.w, q1.y, q1.w, q2.y, q2.w, ox, oy, oz, dx, dy, dz, dr);
+	}
+}
+
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void trace_cis_packed_kernel(
     const float4 *__restrict__ tri4, const float4 *__restrict__ pairs,
@@ -233,7 +249,7 @@ __global__ __launch_bounds__(BLOCK) void trace_cis_packed_kernel(
 	const float tmax = rmax[tid];
 	rta_hit closest = hits[tid];
 	closest.t = FLT_MAX;
-	const bool fast = ((*flags & 1u) == 0u) && ray_fast_ok(ox, oy, oz, dx, dy, dz);
+	const bool fast = ((*flags & 1u) == 0u) && ray_nz_ok(ox, oy, oz, dx, dy, dz);
 	const float yx = 1.0f / dx, yy = 1.0f / dy, yz = 1.0f / dz;
 
 	uint32_t curr = root_word;
@@ -245,15 +261,7 @@ __global__ __launch_bounds__(BLOCK) void trace_cis_packed_kernel(
 			const float4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
 			float dl, dr;
 			bool hl, hr;
-			if (fast) {
-				hl = box_test_fast(q0.x, q0.z, q1.x, q1.z, q2.x, q2.z, ox, oy, oz, dx, dy, dz, yx,
-				                   yy, yz, dl);
-				hr = box_test_fast(q0.y, q0.w, q1.y, q1.w, q2.y, q2.w, ox, oy, oz, dx, dy, dz, yx,
-				                   yy, yz, dr);
-			} else {
-				hl = box_test(q0.x, q0.z, q1.x, q1.z, q2.x, q2.z, ox, oy, oz, dx, dy, dz, dl);
-				hr = box_test(q0.y, q0.w, q1.y, q1.w, q2.y, q2.w, ox, oy, oz, dx, dy, dz, dr);
-			}
+			pair_boxes(q0, q1, q2, fast, ox, oy, oz, dx, dy, dz, yx, yy, yz, hl, dl, hr, dr);
 			hl = hl && dl < closest.t && dl <= tmax;
 			hr = hr && dr < closest.t && dr <= tmax;
 			const uint32_t wl = __float_as_uint(q3.x), wr = __float_as_uint(q3.y);
@@ -301,17 +309,25 @@ __global__ void fill_queue_kernel(uint32_t *queue) {
 // `refill_min` lanes of a wave are idle the wave takes that many new rays
 // (consecutive ids, so they are coherent) from a global queue counter with
 // one atomic.  A wave never waits for another, and it leaves the loop once
-// the queue is exhausted and all of its lanes are idle.  Results equal
-// trace_cis_packed_kernel's: each ray runs the identical traversal; a miss
-// stores only t = FLT_MAX over the reset sentinel (the reference writes
-// the sentinel back with t = FLT_MAX, kernels.cu:793-794,853).
+// the queue is exhausted and all of its lanes are idle.
+//
+// Each traversal step of the wave is either an inner-node step (every lane
+// standing at an inner node visits it) or a leaf step (every lane standing
+// at a leaf tests its triangles and pops): leaf steps run only once at least
+// `leaf_min` lanes wait at a leaf or no lane stands at an inner node, so the
+// wave does not pay for both code paths in most steps.  Waiting changes
+// when a lane advances, never what it does next, so every ray runs the
+// identical traversal and results equal trace_cis_packed_kernel's; a miss
+// stores only t = FLT_MAX over the reset sentinel (the reference writes the
+// sentinel back with t = FLT_MAX, kernels.cu:793-794,853).
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void trace_cis_persistent_kernel(
     const float4 *__restrict__ tri4, const float4 *__restrict__ pairs,
     const uint32_t *__restrict__ flags, uint32_t root_word, const float *__restrict__ ro,
     const float *__restrict__ rd, const float *__restrict__ rmax, int n_rays,
     const uint32_t *__restrict__ ray_count, rta_hit *__restrict__ hits,
-    uint32_t *__restrict__ err, uint32_t *__restrict__ queue, int refill_min, int steps) {
+    uint32_t *__restrict__ err, uint32_t *__restrict__ queue, int refill_min, int steps,
+    int leaf_min) {
 	__shared__ uint32_t stack[kStack][BLOCK];
 	const int lt = threadIdx.x;
 	const int lane = __lane_id();
@@ -345,7 +361,7 @@ __global__ __launch_bounds__(BLOCK) void trace_cis_persistent_kernel(
 					ox = ro[3 * (size_t)ray + 0]; oy = ro[3 * (size_t)ray + 1]; oz = ro[3 * (size_t)ray + 2];
 					dx = rd[3 * (size_t)ray + 0]; dy = rd[3 * (size_t)ray + 1]; dz = rd[3 * (size_t)ray + 2];
 					tmax = rmax[ray];
-					fast = scene_fast && ray_fast_ok(ox, oy, oz, dx, dy, dz);
+					fast = scene_fast && ray_nz_ok(ox, oy, oz, dx, dy, dz);
 					yx = 1.0f / dx; yy = 1.0f / dy; yz = 1.0f / dz;
 					ct = FLT_MAX;
 					curr = root_word;
@@ -358,34 +374,33 @@ __global__ __launch_bounds__(BLOCK) void trace_cis_persistent_kernel(
 		// with refill_min <= 64 an all-idle wave has just refilled, so an
 		// all-idle wave here means the queue is exhausted
 		if (__ballot(active) == 0ull) break;
-		for (int k = 0; k < steps && active; ++k) {
+		for (int k = 0; k < steps; ++k) {
+			const bool at_inner = active && (curr & 1u) != 0u;
+			const bool at_leaf = active && (curr & 1u) == 0u;
+			const uint64_t inner_lanes = __ballot(at_inner);
+			const uint64_t leaf_lanes = __ballot(at_leaf);
+			if ((inner_lanes | leaf_lanes) == 0ull) break;
+			// wave-uniform choice of this step's kind
+			const bool leaf_step = inner_lanes == 0ull || __popcll(leaf_lanes) >= leaf_min;
+			if (!(leaf_step ? at_leaf : at_inner)) continue;
 			bool done = false;
 			if (++guard >= kMaxSteps) { atomicAdd(err, 1u); done = true; }
-			else if (curr & 1u) {
+			else if (!leaf_step) {
 				const float4 *p = pairs + 4 * (size_t)(curr >> 1);
 				const float4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
 				float dl, dr;
 				bool hl, hr;
-				if (fast) {
-					hl = box_test_fast(q0.x, q0.z, q1.x, q1.z, q2.x, q2.z, ox, oy, oz, dx, dy, dz,
-					                   yx, yy, yz, dl);
-					hr = box_test_fast(q0.y, q0.w, q1.y, q1.w, q2.y, q2.w, ox, oy, oz, dx, dy, dz,
-					                   yx, yy, yz, dr);
-				} else {
-					hl = box_test(q0.x, q0.z, q1.x, q1.z, q2.x, q2.z, ox, oy, oz, dx, dy, dz, dl);
-					hr = box_test(q0.y, q0.w, q1.y, q1.w, q2.y, q2.w, ox, oy, oz, dx, dy, dz, dr);
-				}
+				pair_boxes(q0, q1, q2, fast, ox, oy, oz, dx, dy, dz, yx, yy, yz, hl, dl, hr, dr);
 				hl = hl && dl < ct && dl <= tmax;
 				hr = hr && dr < ct && dr <= tmax;
 				const uint32_t wl = __float_as_uint(q3.x), wr = __float_as_uint(q3.y);
 				if (hl && hr) {
+					// nearer child first, the left one on a tie (kernels.cu:822-829)
+					const bool left_first = dl <= dr;
 					if (sp + 1 >= kStack) { atomicAdd(err, 1u); done = true; }
-					else if (dl <= dr) { stack[++sp][lt] = wr; curr = wl; }
-					else { stack[++sp][lt] = wl; curr = wr; }
-				} else if (hl) {
-					curr = wl;
-				} else if (hr) {
-					curr = wr;
+					else { stack[++sp][lt] = left_first ? wr : wl; curr = left_first ? wl : wr; }
+				} else if (hl || hr) {
+					curr = hl ? wl : wr;
 				} else if (sp >= 0) {
 					curr = stack[sp--][lt];
 				} else {
@@ -547,7 +562,7 @@ int launch_persistent(const float *tri4, const rta_node_pair *pairs, const uint3
                       uint32_t root_word, const float *ro, const float *rd, const float *rmax,
                       int n_rays, const uint32_t *ray_count, rta_hit *hits, uint32_t *err,
                       uint32_t *queue, int refill_min, int steps, int blocks_per_cu,
-                      hipStream_t st) {
+                      int leaf_min, hipStream_t st) {
 	static int cus = 0, occ = 0;
 	if (cus == 0) {
 		int dev = 0;
@@ -565,7 +580,7 @@ int launch_persistent(const float *tri4, const rta_node_pair *pairs, const uint3
 	RTA_LAUNCH_CHECK("fill_queue_kernel");
 	hipLaunchKernelGGL(trace_cis_persistent_kernel<BLOCK>, dim3(blocks), dim3(BLOCK), 0, st,
 	                   (const float4 *)tri4, (const float4 *)pairs, flags, root_word, ro, rd, rmax,
-	                   n_rays, ray_count, hits, err, queue, refill_min, steps);
+	                   n_rays, ray_count, hits, err, queue, refill_min, steps, leaf_min);
 	RTA_LAUNCH_CHECK("trace_cis_persistent_kernel");
 	return RTA_OK;
 }
@@ -679,9 +694,11 @@ int rta_trace_cis_persistent(const float *tri4, const rta_node_pair *pairs,
 	if (!tri4 || !pairs || !flags || !ray_orig || !ray_dir || !max_t || !hits || !err ||
 	    !queue || n_rays <= 0)
 		return invalid("rta_trace_cis_persistent: bad arguments");
-	rta_trace_config c = {64, 8, 32, 0};
+	rta_trace_config c = {64, 16, 16, 0, 0};
 	if (cfg) c = *cfg;
-	if (c.refill_min < 1 || c.refill_min > 64 || c.steps < 1 || c.blocks_per_cu < 0)
+	if (c.leaf_min == 0) c.leaf_min = 6;
+	if (c.refill_min < 1 || c.refill_min > 64 || c.steps < 1 || c.blocks_per_cu < 0 ||
+	    c.leaf_min < 1 || c.leaf_min > 64)
 		return invalid("rta_trace_cis_persistent: bad config");
 	const uint32_t root_word = link_inner(root_link) ? 1u : (root_index << 1);
 	hipStream_t st = as_stream(s);
@@ -689,15 +706,15 @@ int rta_trace_cis_persistent(const float *tri4, const rta_node_pair *pairs,
 	case 64:
 		return launch_persistent<64>(tri4, pairs, flags, root_word, ray_orig, ray_dir, max_t, n_rays,
 		                             ray_count, hits, err, queue, c.refill_min, c.steps,
-		                             c.blocks_per_cu, st);
+		                             c.blocks_per_cu, c.leaf_min, st);
 	case 128:
 		return launch_persistent<128>(tri4, pairs, flags, root_word, ray_orig, ray_dir, max_t,
 		                              n_rays, ray_count, hits, err, queue, c.refill_min, c.steps,
-		                              c.blocks_per_cu, st);
+		                              c.blocks_per_cu, c.leaf_min, st);
 	case 256:
 		return launch_persistent<256>(tri4, pairs, flags, root_word, ray_orig, ray_dir, max_t,
 		                              n_rays, ray_count, hits, err, queue, c.refill_min, c.steps,
-		                              c.blocks_per_cu, st);
+		                              c.blocks_per_cu, c.leaf_min, st);
 	default:
 		return invalid("rta_trace_cis_persistent: block must be 64, 128 or 256");
 	}

@@ -53,3 +53,36 @@ def test_div_cr_matches_ieee_on_gpu(gpu):
     rta_amd.call("rta_hip_check_division", C.c_uint64(1 << 31), C.c_uint64(12345), C.byref(bad),
                  ex)
     assert bad.value == 0, f"mismatch example a={ex[0]!r} b={ex[1]!r}"
+
+
+def _pairs(mb_begin, mb_count):
+    bad = C.c_uint64(0)
+    ex = (C.c_uint32 * 2)()
+    rta_amd.call("rta_hip_check_division_pairs", mb_begin, mb_count, C.byref(bad), ex)
+    return bad.value, (ex[0], ex[1])
+
+
+@pytest.mark.gpu
+def test_div_cr1_significand_pairs_on_gpu(gpu):
+    """div_cr1 (one correction step) against IEEE division for every dividend
+    significand and three blocks of 2048 divisor significands (low, middle,
+    high end of [1, 2)).  RTA_DIVISION_ENUM_ALL=1 runs all 2^23 divisor
+    significands (2^46 pairs), which is what exact.h relies on."""
+    if os.environ.get("RTA_DIVISION_ENUM_ALL") == "1":
+        chunks = [(k << 17, 1 << 17) for k in range(64)]
+    else:
+        chunks = [(0, 2048), ((1 << 22) - 1024, 2048), ((1 << 23) - 2048, 2048)]
+    total = 0
+    for mb0, cnt in chunks:
+        bad, ex = _pairs(mb0, cnt)
+        print(f"divisor significands [{mb0}, {mb0 + cnt}): {bad} mismatches", flush=True)
+        assert bad == 0, f"mismatch A=0x{ex[0]:08x} B=0x{ex[1]:08x}"
+        total += cnt
+    assert total >= 3 * 2048
+
+
+@pytest.mark.gpu
+def test_division_pairs_rejects_bad_range(gpu):
+    bad = C.c_uint64(0)
+    with pytest.raises(rta_amd.RtaError):
+        rta_amd.call("rta_hip_check_division_pairs", (1 << 23) - 4, 8, C.byref(bad), None)

@@ -263,6 +263,9 @@ def test_trace_incoherent_and_duplicates(gpu, layout):
     o = rng.uniform(-1.5, 1.5, size=(n, 3)).astype(np.float32)
     d = rng.normal(size=(n, 3)).astype(np.float32)
     d[: n // 10, 0] = 0.0                     # axis-parallel rays: d == 0 slabs
+    d[n // 10::7, 1] = 0.0                    # and single ones inside waves of others
+    d[n // 10::13, 2] = 0.0
+    o[n // 10::11, 0] = np.float32(1e-30)     # origins outside the exact-division range
     d /= np.linalg.norm(d, axis=1, keepdims=True).astype(np.float32)
     o, d = o.reshape(-1), d.reshape(-1)
     t = np.full(n, F32MAX, np.float32)
@@ -300,8 +303,8 @@ def test_trace_median_tree(gpu):
         bvh.invalidate()
 
 
-@pytest.mark.parametrize("shape", [(64, 1, 1), (64, 64, 1), (128, 16, 7), (256, 33, 100),
-                                   (64, 8, 32)])
+@pytest.mark.parametrize("shape", [(64, 1, 1, 0, 1), (64, 64, 1, 0, 64), (128, 16, 7, 0, 3),
+                                   (256, 33, 100, 0, 40), (64, 8, 32, 0, 0), (64, 8, 32, 1, 16)])
 def test_persistent_launch_shapes(gpu, shape):
     """Every launch shape of the persistent tracer gives the oracle's hits."""
     say(f"persistent shape={shape}")
@@ -315,7 +318,7 @@ def test_persistent_launch_shapes(gpu, shape):
     t = np.full(n, F32MAX, np.float32)
     ref, _, _ = oracle.trace_cis(ref_b["tris"], ref_b["nodes"], o, d, t, n)
     bvh = rta_amd.LbvhConstructor().build(DeviceBuffer.from_array(tris), len(tris))
-    got, _ = trace_gpu(bvh, o, d, t, "persistent", config=TraceConfig(*shape, 0))
+    got, _ = trace_gpu(bvh, o, d, t, "persistent", config=TraceConfig(*shape))
     assert_hits_equal(got, ref)
 
 

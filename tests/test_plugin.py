@@ -42,7 +42,8 @@ def test_plugin_option_parsing():
         return lib.parse_cmdline(len(args) + 1, argv)
 
     assert parse("-b", "lbvh", "-t", "cis", "-l", "2f4", "-v") == 0
-    assert parse("--trav", "packed", "--block", "64", "--refill", "8", "--steps", "4") == 0
+    assert parse("--trav", "packed", "--block", "64", "--refill", "8", "--steps", "4",
+                 "--leaf-min", "12") == 0
     assert parse("-b", "median") != 0        # no CPU-built tree in this plugin
     assert parse("-s") != 0                  # any-hit tracer not provided
     assert parse("--bogus") != 0
