@@ -132,9 +132,9 @@ def main():
         return
 
     t0 = time.perf_counter()
-    tris_h = scenes.config2_scene(grid=args.grid, n_random=args.random)
+    tris_h = scenes.terrain_scene(grid=args.grid, n_random=args.random)
     n = len(tris_h)
-    cam = scenes.config2_camera(args.width, args.height)
+    cam = scenes.terrain_camera(args.width, args.height)
     R = cam.w * cam.h
     log(f"scene {n} triangles, {R} primary rays ({time.perf_counter() - t0:.1f} s)")
 
@@ -252,7 +252,7 @@ def main():
 
     # per-ray inner-node visits V and triangle tests T on the bounce rays,
     # from the instrumented plain kernel (equal to the oracle's counts, see
-    # tests/test_gpu_parity.py::test_trace_config1_bit_exact)
+    # tests/test_gpu_parity.py::test_trace_sphere_bit_exact)
     stats = DeviceBuffer(8 * max(n_bounce, 1))
     call("rta_reset_intersections", ptr(hits_b), R, 1, sp)
     call("rta_trace_cis", ptr(tris), n, ptr(nodes), ptr(bnc.gpu_origin), ptr(bnc.gpu_direction),

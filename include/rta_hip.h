@@ -75,6 +75,7 @@ typedef struct rta_lbvh_times {
 
 typedef void *rta_stream;
 typedef void *rta_event;
+typedef void *rta_graph;
 
 enum {
 	RTA_OK = 0,
@@ -104,6 +105,8 @@ int rta_hip_memcpy_dtoh(void *dst, const void *src, size_t bytes);
 int rta_hip_memcpy_dtod(void *dst, const void *src, size_t bytes, rta_stream s);
 /* Fill with a kernel (not hipMemset), safe inside captured graphs. */
 int rta_hip_fill_u32(uint32_t *dst, uint32_t value, size_t count, rta_stream s);
+/* *word += delta on the device (one thread), e.g. a frame counter. */
+int rta_hip_add_u32(uint32_t *word, uint32_t delta, rta_stream s);
 int rta_hip_stream_create(rta_stream *s);
 int rta_hip_stream_destroy(rta_stream s);
 int rta_hip_stream_synchronize(rta_stream s);
@@ -113,6 +116,18 @@ int rta_hip_event_destroy(rta_event e);
 int rta_hip_event_record(rta_event e, rta_stream s);
 int rta_hip_event_synchronize(rta_event e);
 int rta_hip_event_elapsed_ms(float *ms, rta_event start, rta_event stop);
+/* Stream capture into a replayable graph (hipStreamBeginCapture, thread-local
+ * mode): every entry point of this library that takes a stream enqueues
+ * only kernels and copies on it, never a memset or a host synchronisation
+ * (the build's optional stage timers excepted), so a whole frame can be
+ * captured once and replayed with rta_hip_graph_launch.  s must be a
+ * created stream (not NULL).  Buffers are baked into the graph: replays
+ * reuse the same addresses and sizes.  No counterpart in the reference
+ * (whose launches are synchronous). */
+int rta_hip_graph_begin(rta_stream s);
+int rta_hip_graph_end(rta_stream s, rta_graph *g);
+int rta_hip_graph_launch(rta_graph g, rta_stream s);
+int rta_hip_graph_destroy(rta_graph g);
 /* Plain coalesced copy kernel of `bytes` (multiple of 16), used to calibrate
  * HBM traffic counters against a kernel of known size. */
 int rta_hip_copy_kernel(void *dst, const void *src, size_t bytes, rta_stream s);
@@ -278,6 +293,28 @@ int rta_trace_cis_persistent(const float *tri4, const rta_node_pair *pairs,
                              rta_hit *hits, uint32_t *err, uint32_t *queue,
                              const rta_trace_config *cfg, rta_stream s);
 
+/* trace_shadow_cis, bbvh-cuda/kernels.cu:857-939 (the `-s` tracer,
+ * bbvh-cuda/main.cpp:93): the closest-hit traversal up to the first
+ * triangle a leaf accepts (t < closest.t && t <= max_t, closest.t starting
+ * at FLT_MAX); that triangle's record is stored and the ray ends.  A miss
+ * stores t = FLT_MAX over the reset sentinel.  Same arguments as the
+ * closest-hit forms. */
+int rta_trace_shadow_cis_packed(const float *tri4, const rta_node_pair *pairs,
+                                const uint32_t *flags, uint32_t root_link,
+                                uint32_t root_index, const float *ray_orig,
+                                const float *ray_dir, const float *max_t,
+                                int n_rays, const uint32_t *ray_count,
+                                rta_hit *hits, uint32_t *err, rta_stream s);
+int rta_trace_shadow_cis_persistent(const float *tri4,
+                                    const rta_node_pair *pairs,
+                                    const uint32_t *flags, uint32_t root_link,
+                                    uint32_t root_index, const float *ray_orig,
+                                    const float *ray_dir, const float *max_t,
+                                    int n_rays, const uint32_t *ray_count,
+                                    rta_hit *hits, uint32_t *err,
+                                    uint32_t *queue,
+                                    const rta_trace_config *cfg, rta_stream s);
+
 /* -------------------------------------------------------------- bouncing */
 
 /* One cosine-weighted diffuse bounce per hit (the incoherent ray set).
@@ -294,6 +331,30 @@ int rta_diffuse_bounce(float *out_orig, float *out_dir, float *out_maxt,
                        const float *ray_orig, const float *ray_dir,
                        int n_rays, uint64_t seed, void *work,
                        size_t work_bytes, rta_stream s);
+
+/* Per-frame geometry of a dynamic indexed mesh (rta_scene_instances): writes
+ * n_tris triangle records from verts (3 floats per vertex) and idx (3 vertex
+ * ids per triangle), every vertex moved by sigma times a unit-variance
+ * Irwin-Hall(4) variate per axis, keyed on (seed, frame, vertex id), so
+ * shared vertices move together.  Normals and texcoords 0, material -1.
+ * frame_word (device word, may be NULL) replaces `frame` when given, so a
+ * captured frame can advance it with rta_hip_add_u32.  No counterpart in
+ * the reference. */
+int rta_jitter_triangles(rta_triangle *out, const float *verts, const uint32_t *idx,
+                         int n_tris, float sigma, uint64_t seed, uint32_t frame,
+                         const uint32_t *frame_word, rta_stream s);
+
+/* One shadow ray per hit towards a point light (light: host float[3]):
+ * origin = o + t*d + 1e-4*n as for the bounce, direction = the unit vector
+ * from that origin to the light, max_t = the distance to it (a light at the
+ * origin itself gives direction n and max_t 0).  Compaction, src_index,
+ * *count and work as for rta_diffuse_bounce.  No counterpart in the
+ * reference (whose bouncers trace no further rays, librta/cuda.h:232-234). */
+int rta_shadow_rays(float *out_orig, float *out_dir, float *out_maxt,
+                    uint32_t *src_index, uint32_t *count, const rta_hit *hits,
+                    const rta_triangle *tris, const float *ray_orig,
+                    const float *ray_dir, int n_rays, const float *light,
+                    void *work, size_t work_bytes, rta_stream s);
 
 #ifdef __cplusplus
 }

@@ -39,6 +39,19 @@ int rta_scene_heightfield(int nx, int ny, float x0, float y0, float x1, float y1
 int rta_scene_random_triangles(int count, uint64_t seed, float lo, float hi, float edge_min,
                                float edge_max, rta_triangle *out);
 
+/* The large dynamic scene as an indexed mesh: n_inst instances on a square
+ * grid of unit cells (instance k in cell (k % side, k / side), side =
+ * ceil(sqrt(n_inst))), each, by a seeded coin, either a value-noise
+ * heightfield patch of g x g quads (6 octaves, amplitude 0.2) or a geodesic
+ * sphere (octahedron subdivided until it has 2*g*g triangles, radius
+ * 0.3..0.4, resting 0.1 above the ground).  Every instance has 2*g*g
+ * triangles (g a power of two >= 2; g = 256 and 64 instances give
+ * 8,388,608).  verts: 3 floats per vertex; idx: 3 vertex ids per triangle,
+ * instance by instance.  Returns the triangle count and sets *n_verts; with
+ * verts or idx NULL nothing else is written. */
+int rta_scene_instances(int n_inst, int g, uint64_t seed, int *n_verts, float *verts,
+                        uint32_t *idx);
+
 /* Host version of the pinhole generator (librta/raytrav.h:300-351): the
  * same camera basis and per-pixel arithmetic as rta_setup_shirley_rays, so
  * host and device rays are bit-identical.  Ray id = y*w + x. */

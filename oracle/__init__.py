@@ -26,6 +26,7 @@ AABB_DTYPE = np.dtype([("min", "<f4", 3), ("max", "<f4", 3)])
 NODE_DTYPE = np.dtype([("type_left_elems", "<u4"), ("right_tris", "<u4"),
                        ("min", "<f4", 3), ("max", "<f4", 3)])
 HIT_DTYPE = np.dtype([("t", "<f4"), ("beta", "<f4"), ("gamma", "<f4"), ("ref", "<u4")])
+F32MAX = np.float32(np.finfo(np.float32).max)
 
 _lib = None
 
@@ -47,10 +48,13 @@ def lib() -> C.CDLL:
             "oracle_morton_encode": (u32, [u32, u32, u32]),
             "oracle_lbvh_build": (i, [vp, i, vp, vp, vp, vp, vp, vp, vp]),
             "oracle_median_build": (i, [vp, i, vp, vp, vp]),
-            "oracle_trace_cis": (d, [vp, vp, vp, vp, vp, i, i, vp, vp, i]),
+            "oracle_trace_cis": (d, [vp, vp, vp, vp, vp, i, i, vp, vp, vp, i]),
             "oracle_trace_dis": (d, [vp, vp, vp, vp, vp, i, i, vp, i]),
+            "oracle_trace_kernel": (i, [vp, vp, vp, vp, vp, i, vp, i, i]),
             "oracle_trace_brute": (None, [vp, i, vp, vp, i, vp]),
             "oracle_diffuse_bounce": (i, [vp, vp, vp, vp, i, u64, vp, vp, vp, vp]),
+            "oracle_shadow_rays": (i, [vp, vp, vp, vp, i, vp, vp, vp, vp, vp]),
+            "oracle_jitter_triangles": (None, [vp, vp, vp, i, C.c_float, u64, u32]),
             "oracle_max_threads": (i, []),
         }
         for name, (res, args) in sigs.items():
@@ -120,16 +124,35 @@ def median_build(tris: np.ndarray) -> dict:
     return {"nodes": nodes[:nn].copy(), "tris": tout, "order": order}
 
 
-def trace_cis(tris, nodes, o, d, t, w, h=1, stats=False, threads=0):
-    """Child-is closest hit (bbvh/bbvh.h:407-455).  Returns (hits, ms, stats)."""
+def trace_cis(tris, nodes, o, d, t, w, h=1, stats=False, threads=0, first=False):
+    """Child-is closest hit (bbvh/bbvh.h:407-455).  Returns (hits, ms, stats),
+    plus the first accepted record per ray when first=True."""
     n = w * h
     hits = np.zeros(n, HIT_DTYPE)
     st = np.zeros(2 * n, np.uint32) if stats else None
+    fr = np.zeros(n, HIT_DTYPE) if first else None
     tris, nodes = np.ascontiguousarray(tris), np.ascontiguousarray(nodes)
     o, d, t = _f32(o), _f32(d), _f32(t)
     ms = lib().oracle_trace_cis(_p(tris), _p(nodes), _p(o), _p(d), _p(t), w, h, _p(hits), _p(st),
-                                threads)
-    return hits, ms, st
+                                _p(fr), threads)
+    return (hits, ms, st, fr) if first else (hits, ms, st)
+
+
+def trace_kernel(tris, nodes, o, d, t, any_hit=False, hits_in=None, threads=0):
+    """The GPU tracers' semantics (kernels.cu:777-855, shadow :857-939).
+    hits_in: the records the rays start from (default: the reset sentinel).
+    Returns (hits, rays that needed more than 32 stack entries)."""
+    n = len(t)
+    if hits_in is None:
+        hits = np.zeros(n, HIT_DTYPE)
+        hits["t"], hits["beta"], hits["gamma"] = F32MAX, 1.0, 2.0
+    else:
+        hits = np.array(hits_in, dtype=HIT_DTYPE, copy=True)
+    tris, nodes = np.ascontiguousarray(tris), np.ascontiguousarray(nodes)
+    o, d, t = _f32(o), _f32(d), _f32(t)
+    deep = lib().oracle_trace_kernel(_p(tris), _p(nodes), _p(o), _p(d), _p(t), n, _p(hits),
+                                     int(bool(any_hit)), threads)
+    return hits, int(deep)
 
 
 def trace_dis(tris, nodes, o, d, t, w, h=1, threads=0):
@@ -161,6 +184,29 @@ def diffuse_bounce(hits, tris, o, d, seed):
     k = lib().oracle_diffuse_bounce(_p(hits), _p(tris), _p(o), _p(d), n, C.c_uint64(seed), _p(oo),
                                     _p(od), _p(ot), _p(src))
     return oo[:3 * k], od[:3 * k], ot[:k], src[:k]
+
+
+def shadow_rays(hits, tris, o, d, light):
+    n = len(hits)
+    oo = np.zeros(3 * n, np.float32)
+    od = np.zeros(3 * n, np.float32)
+    ot = np.zeros(n, np.float32)
+    src = np.zeros(n, np.uint32)
+    hits, tris = np.ascontiguousarray(hits), np.ascontiguousarray(tris)
+    o, d, lp = _f32(o), _f32(d), _f32(light)
+    k = lib().oracle_shadow_rays(_p(hits), _p(tris), _p(o), _p(d), n, _p(lp), _p(oo), _p(od),
+                                 _p(ot), _p(src))
+    return oo[:3 * k], od[:3 * k], ot[:k], src[:k]
+
+
+def jitter_triangles(verts, idx, sigma, seed, frame):
+    n = len(idx) // 3
+    out = np.zeros(n, TRI_DTYPE)
+    verts = np.ascontiguousarray(verts, dtype=np.float32)
+    idx = np.ascontiguousarray(idx, dtype=np.uint32)
+    lib().oracle_jitter_triangles(_p(out), _p(verts), _p(idx), n, float(sigma),
+                                  C.c_uint64(seed), frame)
+    return out
 
 
 def max_threads() -> int:

@@ -479,11 +479,12 @@ int oracle_median_build(const rta_triangle *tris_in, int n, rta_node *nodes_out,
 // values: closest.t starts at max_t and ref at 0 (traversal_state::reset,
 // librta/raytrav.h:29-33); beta/gamma of a miss are left 0 here (the
 // reference leaves them stale).  stats (optional, 2 words per ray) gets the
-// number of inner nodes visited and of triangle tests.  Returns the wall
-// time of the parallel loop in ms (bbvh.h:394,404).
+// number of inner nodes visited and of triangle tests.  first (optional)
+// gets the first record the ray accepted (t = FLT_MAX if none).  Returns the
+// wall time of the parallel loop in ms (bbvh.h:394,404).
 double oracle_trace_cis(const rta_triangle *tris, const rta_node *nodes, const float *ro,
                         const float *rd, const float *rmax, int w, int h, rta_hit *hits,
-                        uint32_t *stats, int threads) {
+                        uint32_t *stats, rta_hit *first, int threads) {
 	if (threads > 0) omp_set_num_threads(threads);
 	double t0 = now_ms();
 #pragma omp parallel for schedule(dynamic, 32)
@@ -499,6 +500,12 @@ double oracle_trace_cis(const rta_triangle *tris, const rta_node *nodes, const f
 			is_best.gamma = 0;
 			is_best.ref = 0;
 			uint32_t n_inner = 0, n_tris = 0;
+			rta_hit first_rec;
+			first_rec.t = FLT_MAX;
+			first_rec.beta = 0;
+			first_rec.gamma = 0;
+			first_rec.ref = 0;
+			bool accepted = false;
 			int sp = 0;
 			stack[0] = 0;
 			while (sp >= 0) {
@@ -537,12 +544,15 @@ double oracle_trace_cis(const rta_triangle *tris, const rta_node *nodes, const f
 								is_best.beta = be;
 								is_best.gamma = ga;
 								is_best.ref = (uint32_t)(offset + i);
+								if (!accepted) first_rec = is_best;
+								accepted = true;
 							}
 						}
 					}
 				}
 			}
 			hits[id] = is_best;
+			if (first) first[id] = first_rec;
 			if (stats) {
 				stats[2 * id] = n_inner;
 				stats[2 * id + 1] = n_tris;
@@ -603,6 +613,87 @@ double oracle_trace_dis(const rta_triangle *tris, const rta_node *nodes, const f
 		}
 	}
 	return now_ms() - t0;
+}
+
+// The GPU tracers' semantics over the same node array: k::trace_cis
+// (bbvh-cuda/kernels.cu:777-855) and, with any_hit, k::trace_shadow_cis
+// (:857-939).  closest starts as the incoming record with t = FLT_MAX
+// (:793-794); at an inner node both child boxes are tested, a child is kept
+// iff hit, dist < closest.t and dist <= max_t (:815-816); with both kept the
+// nearer one (left on a tie) is entered and the other pushed (:818-827);
+// with one kept it is entered; otherwise the stack is popped.  A leaf tests
+// its triangles in order and keeps t < closest.t && t <= max_t (:838-846);
+// the shadow form stops at the first kept triangle and then ends the ray
+// (:918-931).  The reference's stack has 32 entries and no check; here the
+// stack is larger and rays that would have needed more than 32 entries are
+// counted in the return value.  hits: in/out.
+int oracle_trace_kernel(const rta_triangle *tris, const rta_node *nodes, const float *ro,
+                        const float *rd, const float *rmax, int n_rays, rta_hit *hits,
+                        int any_hit, int threads) {
+	if (threads > 0) omp_set_num_threads(threads);
+	int overflow = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : overflow)
+	for (int id = 0; id < n_rays; ++id) {
+		uint32_t stack[256];
+		const float *origin = ro + 3 * (size_t)id, *dir = rd + 3 * (size_t)id;
+		const float t_max = rmax[id];
+		rta_hit closest = hits[id];
+		closest.t = FLT_MAX;
+		int sp = -1;
+		bool deep = false;
+		const rta_node *curr = &nodes[0];
+		while (true) {
+			if (curr->type_left_elems & 1u) {
+				const uint32_t l = curr->type_left_elems >> 1, r = curr->right_tris;
+				float dist_l = FLT_MAX, dist_r = FLT_MAX;
+				bool hit_l = intersect_aabb(nodes[l].box, origin, dir, dist_l);
+				bool hit_r = intersect_aabb(nodes[r].box, origin, dir, dist_r);
+				if (dist_l >= closest.t || dist_l > t_max) hit_l = false;
+				if (dist_r >= closest.t || dist_r > t_max) hit_r = false;
+				if (hit_l && hit_r) {
+					if (sp + 1 >= 32) deep = true;
+					if (sp + 1 >= 256) break;  // unreachable for trees of depth < 256
+					if (dist_l <= dist_r) {
+						stack[++sp] = r;
+						curr = &nodes[l];
+					} else {
+						stack[++sp] = l;
+						curr = &nodes[r];
+					}
+				} else if (hit_l) {
+					curr = &nodes[l];
+				} else if (hit_r) {
+					curr = &nodes[r];
+				} else if (sp >= 0) {
+					curr = &nodes[stack[sp--]];
+				} else {
+					break;
+				}
+			} else {
+				const uint32_t elems = curr->type_left_elems >> 1, offset = curr->right_tris;
+				for (uint32_t i = 0; i < elems; ++i) {
+					float t, be, ga;
+					if (intersect_tri_opt(tris[offset + i], origin, dir, t, be, ga)) {
+						if (t < closest.t && t <= t_max) {
+							closest.t = t;
+							closest.beta = be;
+							closest.gamma = ga;
+							closest.ref = offset + i;
+							if (any_hit) break;
+						}
+					}
+				}
+				if (any_hit && closest.t != FLT_MAX) break;
+				if (sp >= 0)
+					curr = &nodes[stack[sp--]];
+				else
+					break;
+			}
+		}
+		hits[id] = closest;
+		if (deep) ++overflow;
+	}
+	return overflow;
 }
 
 // bruteforce_tracer::trace_ray, brute-force/bruteforce.cpp:162-173: first
@@ -708,6 +799,88 @@ int oracle_diffuse_bounce(const rta_hit *hits, const rta_triangle *tris, const f
 		++k;
 	}
 	return k;
+}
+
+// Shadow rays towards a point light, the repo's own definition (as
+// rta_shadow_rays on the GPU): origin as for the bounce, unit direction to
+// the light, max_t = distance (0 with direction n for a light at the
+// origin).  Returns the number of rays.
+int oracle_shadow_rays(const rta_hit *hits, const rta_triangle *tris, const float *ro,
+                       const float *rd, int n_rays, const float *light, float *out_o,
+                       float *out_d, float *out_t, uint32_t *src_index) {
+	int k = 0;
+	for (int i = 0; i < n_rays; ++i) {
+		const rta_hit &h = hits[i];
+		if (h.t == FLT_MAX) continue;
+		const rta_triangle &tr = tris[h.ref];
+		const float e1x = tr.b.x - tr.a.x, e1y = tr.b.y - tr.a.y, e1z = tr.b.z - tr.a.z;
+		const float e2x = tr.c.x - tr.a.x, e2y = tr.c.y - tr.a.y, e2z = tr.c.z - tr.a.z;
+		float nx = e1y * e2z - e1z * e2y;
+		float ny = e1z * e2x - e1x * e2z;
+		float nz = e1x * e2y - e1y * e2x;
+		const float nl = sqrtf(nx * nx + ny * ny + nz * nz);
+		nx = nx / nl;
+		ny = ny / nl;
+		nz = nz / nl;
+		const float *o = ro + 3 * (size_t)i, *d = rd + 3 * (size_t)i;
+		if (nx * d[0] + ny * d[1] + nz * d[2] > 0.0f) {
+			nx = -nx;
+			ny = -ny;
+			nz = -nz;
+		}
+		const float px = o[0] + h.t * d[0], py = o[1] + h.t * d[1], pz = o[2] + h.t * d[2];
+		const float sx = px + 1e-4f * nx, sy = py + 1e-4f * ny, sz = pz + 1e-4f * nz;
+		out_o[3 * k + 0] = sx;
+		out_o[3 * k + 1] = sy;
+		out_o[3 * k + 2] = sz;
+		const float vx = light[0] - sx, vy = light[1] - sy, vz = light[2] - sz;
+		const float vl = sqrtf(vx * vx + vy * vy + vz * vz);
+		if (vl > 0.0f) {
+			out_d[3 * k + 0] = vx / vl;
+			out_d[3 * k + 1] = vy / vl;
+			out_d[3 * k + 2] = vz / vl;
+			out_t[k] = vl;
+		} else {
+			out_d[3 * k + 0] = nx;
+			out_d[3 * k + 1] = ny;
+			out_d[3 * k + 2] = nz;
+			out_t[k] = 0.0f;
+		}
+		src_index[k] = (uint32_t)i;
+		++k;
+	}
+	return k;
+}
+
+// Per-frame vertex jitter of the dynamic scene, the repo's own definition
+// (as rta_jitter_triangles on the GPU): every vertex moves by sigma times an
+// Irwin-Hall(4) variate per axis (four 16-bit uniforms of one hash keyed on
+// the frame and vertex id, centred and scaled to unit variance).
+void oracle_jitter_triangles(rta_triangle *out, const float *verts, const uint32_t *idx,
+                             int n_tris, float sigma, uint64_t seed, uint32_t frame) {
+	const uint64_t key = mix64(seed ^ mix64(0x6a09e667f3bcc909ull + frame));
+	const float k = 1.0f / 65536.0f;
+	for (int i = 0; i < n_tris; ++i) {
+		float p[9];
+		for (int c = 0; c < 3; ++c) {
+			const uint32_t v = idx[3 * (size_t)i + c];
+			for (int a = 0; a < 3; ++a) {
+				const uint64_t h = mix64(key + 3ull * v + (uint64_t)a);
+				const float u0 = (float)(uint32_t)(h & 0xffffu) * k;
+				const float u1 = (float)(uint32_t)((h >> 16) & 0xffffu) * k;
+				const float u2 = (float)(uint32_t)((h >> 32) & 0xffffu) * k;
+				const float u3 = (float)(uint32_t)(h >> 48) * k;
+				const float off = sigma * ((((u0 + u1) + (u2 + u3)) - 2.0f) * 1.7320508f);
+				p[3 * c + a] = verts[3 * (size_t)v + a] + off;
+			}
+		}
+		rta_triangle &t = out[i];
+		memset(&t, 0, sizeof t);
+		t.a = {p[0], p[1], p[2]};
+		t.b = {p[3], p[4], p[5]};
+		t.c = {p[6], p[7], p[8]};
+		t.material_index = -1;
+	}
 }
 
 int oracle_max_threads(void) { return omp_get_max_threads(); }

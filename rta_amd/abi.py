@@ -94,6 +94,12 @@ HIP_SIGNATURES = {
     "rta_hip_event_synchronize": [_vp],
     "rta_hip_event_elapsed_ms": [_pf, _vp, _vp],
     "rta_hip_copy_kernel": [_vp, _vp, _sz, _vp],
+    "rta_hip_graph_begin": [_vp],
+    "rta_hip_graph_end": [_vp, C.POINTER(_vp)],
+    "rta_hip_graph_launch": [_vp, _vp],
+    "rta_hip_graph_destroy": [_vp],
+    "rta_jitter_triangles": [_vp, _vp, _vp, _i, _f, _u64, _u32, _vp, _vp],
+    "rta_hip_add_u32": [_vp, _u32, _vp],
     "rta_hip_check_division": [_u64, _u64, C.POINTER(C.c_uint64), _vp],
     "rta_hip_check_division_pairs": [_u32, _u32, C.POINTER(C.c_uint64), _vp],
     "rta_lbvh_init_triangles": [_vp, _vp, _vp, _vp, _vp, _i, _vp],
@@ -114,14 +120,20 @@ HIP_SIGNATURES = {
     "rta_trace_cis_packed": [_vp, _vp, _vp, _u32, _u32, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp],
     "rta_trace_cis_persistent": [_vp, _vp, _vp, _u32, _u32, _vp, _vp, _vp, _i, _vp, _vp, _vp,
                                  _vp, _vp, _vp],
+    "rta_trace_shadow_cis_packed": [_vp, _vp, _vp, _u32, _u32, _vp, _vp, _vp, _i, _vp, _vp, _vp,
+                                    _vp],
+    "rta_trace_shadow_cis_persistent": [_vp, _vp, _vp, _u32, _u32, _vp, _vp, _vp, _i, _vp, _vp,
+                                        _vp, _vp, _vp, _vp],
     "rta_bounce_work_bytes": [_i, _psz],
     "rta_diffuse_bounce": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _u64, _vp, _sz, _vp],
+    "rta_shadow_rays": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp, _sz, _vp],
 }
 
 HOST_SIGNATURES = {
     "rta_scene_icosphere": [_i, _f, _vp, _f, _u64, _vp],
     "rta_scene_heightfield": [_i, _i, _f, _f, _f, _f, _f, _u64, _i, _f, _f, _f, _f, _vp],
     "rta_scene_random_triangles": [_i, _u64, _f, _f, _f, _f, _vp],
+    "rta_scene_instances": [_i, _i, _u64, _pi, _vp, _vp],
     "rta_host_shirley_rays": [_vp, _vp, _vp, _f, _f, _i, _i, _vp, _vp, _vp, _f],
 }
 

@@ -207,6 +207,8 @@ class CisTracer:
     waves that refill idle lanes; all give identical results."""
 
     LAYOUTS = ("plain", "packed", "persistent")
+    PACKED_ENTRY = "rta_trace_cis_packed"
+    PERSISTENT_ENTRY = "rta_trace_cis_persistent"
 
     def __init__(self, raygen: RayBuffers, bvh: BinaryLbvh, bouncer: GpuBouncer,
                  layout: str = "persistent", stream=None, config: TraceConfig | None = None):
@@ -243,14 +245,14 @@ class CisTracer:
                  ptr(stats), ptr(self._err), ptr(self.stream))
         elif self.layout == "packed":
             pk = self.bvh.packed(self.stream)
-            call("rta_trace_cis_packed", ptr(pk.tri4), ptr(pk.pairs), ptr(pk.flags), pk.root_link,
+            call(self.PACKED_ENTRY, ptr(pk.tri4), ptr(pk.pairs), ptr(pk.flags), pk.root_link,
                  pk.root_index,
                  ptr(rg.gpu_origin), ptr(rg.gpu_direction), ptr(rg.gpu_maxt), n, ptr(ray_count),
                  ptr(hits), ptr(self._err), ptr(self.stream))
         else:
             pk = self.bvh.packed(self.stream)
             cfg = C.byref(self.config) if self.config is not None else None
-            call("rta_trace_cis_persistent", ptr(pk.tri4), ptr(pk.pairs), ptr(pk.flags),
+            call(self.PERSISTENT_ENTRY, ptr(pk.tri4), ptr(pk.pairs), ptr(pk.flags),
                  pk.root_link, pk.root_index, ptr(rg.gpu_origin), ptr(rg.gpu_direction),
                  ptr(rg.gpu_maxt), n, ptr(ray_count), ptr(hits), ptr(self._err), ptr(self._queue),
                  cfg, ptr(self.stream))
@@ -284,6 +286,43 @@ class CisTracer:
             self.bouncer.bounce()
             if not self.bouncer.trace_further_bounces():
                 break
+
+
+class CisShadowTracer(CisTracer):
+    """bbvh_gpu_cis_shadow_tracer (bbvh-cuda/bbvh-cuda.h:265-291): the
+    closest-hit traversal that ends at the first triangle a leaf accepts
+    (trace_shadow_cis, kernels.cu:857-939).  Packed and persistent layouts."""
+
+    LAYOUTS = ("packed", "persistent")
+    PACKED_ENTRY = "rta_trace_shadow_cis_packed"
+    PERSISTENT_ENTRY = "rta_trace_shadow_cis_persistent"
+
+    def identification(self) -> str:
+        return f"bbvh-hip cis shadow tracer ({self.layout})"
+
+    def launch(self, n_rays: int | None = None, stats: DeviceBuffer | None = None,
+               ray_count: DeviceBuffer | None = None) -> None:
+        if stats is not None:
+            raise ValueError("the shadow tracer has no instrumented form")
+        super().launch(n_rays, None, ray_count)
+
+
+def shadow_rays(hits: DeviceBuffer, tris: DeviceBuffer, rays: RayBuffers, n_rays: int, light,
+                out: RayBuffers | None = None, stream=None):
+    """One shadow ray per hit towards a point light (see rta_shadow_rays).
+    Returns the ray buffers, the source ray index of each and the count."""
+    out = RayBuffers(n_rays) if out is None else out
+    src = DeviceBuffer(4 * n_rays)
+    cnt = DeviceBuffer(4)
+    sz = C.c_size_t(0)
+    call("rta_bounce_work_bytes", n_rays, C.byref(sz))
+    work = DeviceBuffer(sz.value)
+    lp = (C.c_float * 3)(*[float(x) for x in light])
+    call("rta_shadow_rays", ptr(out.gpu_origin), ptr(out.gpu_direction), ptr(out.gpu_maxt),
+         ptr(src), ptr(cnt), ptr(hits), ptr(tris), ptr(rays.gpu_origin), ptr(rays.gpu_direction),
+         n_rays, lp, ptr(work), C.c_size_t(sz.value), ptr(stream))
+    count = int(cnt.download(np.uint32, 1)[0])
+    return out, src, count
 
 
 def diffuse_bounce(hits: DeviceBuffer, tris: DeviceBuffer, rays: RayBuffers, n_rays: int,

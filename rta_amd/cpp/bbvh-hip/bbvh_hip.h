@@ -196,11 +196,18 @@ public:
 	binary_lbvh<box_t, tri_t> *bvh;
 	trace_layout layout;
 	rta_trace_config config;
+	//! the shadow form (bbvh_gpu_cis_shadow_tracer, bbvh-cuda.h:265-291):
+	//! stop at the first triangle a leaf accepts; packed/persistent only
+	bool shadow;
 
 	bbvh_gpu_cis_tracer(rta::ray_generator *gen, binary_lbvh<box_t, tri_t> *bvh, rta::bouncer *b,
 	                    trace_layout layout = trace_layout::persistent,
-	                    rta_trace_config cfg = rta_trace_config{64, 16, 16, 0, 0})
-	    : gpu_raytracer<box_t, tri_t>(gen, b, bvh), bvh(bvh), layout(layout), config(cfg) {
+	                    rta_trace_config cfg = rta_trace_config{64, 16, 16, 0, 0},
+	                    bool shadow = false)
+	    : gpu_raytracer<box_t, tri_t>(gen, b, bvh), bvh(bvh), layout(layout), config(cfg),
+	      shadow(shadow) {
+		if (shadow && layout == trace_layout::plain)
+			throw std::runtime_error("the shadow tracer needs the packed or persistent layout");
 		void *p = 0;
 		check(rta_hip_malloc(&p, 8), "error words");
 		d_err = (uint32_t *)p;
@@ -208,10 +215,12 @@ public:
 		check(rta_hip_fill_u32(d_err, 0, 1, 0), "clear error word");
 	}
 	bbvh_gpu_cis_tracer(const bbvh_gpu_cis_tracer &o)
-	    : bbvh_gpu_cis_tracer(o.raygen, o.bvh, o.bouncer, o.layout, o.config) {}
+	    : bbvh_gpu_cis_tracer(o.raygen, o.bvh, o.bouncer, o.layout, o.config, o.shadow) {}
 	~bbvh_gpu_cis_tracer() override { rta_hip_free(d_err); }
 
-	std::string identification() override { return "bbvh-hip closest-hit child-is tracer"; }
+	std::string identification() override {
+		return shadow ? "bbvh-hip child-is shadow tracer" : "bbvh-hip closest-hit child-is tracer";
+	}
 	bool supports_max_t() override { return true; }
 	bbvh_gpu_cis_tracer *copy() override { return new bbvh_gpu_cis_tracer(*this); }
 	void acceleration_structure(basic_acceleration_structure<box_t, tri_t> *as) override {
@@ -233,16 +242,17 @@ public:
 			      "rta_trace_cis");
 			break;
 		case trace_layout::packed:
-			check(rta_trace_cis_packed(bvh->tri4, bvh->pairs, bvh->flags, bvh->root_link,
-			                           bvh->root_index, rg->gpu_origin, rg->gpu_direction,
-			                           rg->gpu_maxt, n, 0, hits, d_err, 0),
-			      "rta_trace_cis_packed");
+			check((shadow ? rta_trace_shadow_cis_packed : rta_trace_cis_packed)(
+			          bvh->tri4, bvh->pairs, bvh->flags, bvh->root_link, bvh->root_index,
+			          rg->gpu_origin, rg->gpu_direction, rg->gpu_maxt, n, 0, hits, d_err, 0),
+			      "packed trace");
 			break;
 		case trace_layout::persistent:
-			check(rta_trace_cis_persistent(bvh->tri4, bvh->pairs, bvh->flags, bvh->root_link,
-			                               bvh->root_index, rg->gpu_origin, rg->gpu_direction,
-			                               rg->gpu_maxt, n, 0, hits, d_err, d_queue, &config, 0),
-			      "rta_trace_cis_persistent");
+			check((shadow ? rta_trace_shadow_cis_persistent : rta_trace_cis_persistent)(
+			          bvh->tri4, bvh->pairs, bvh->flags, bvh->root_link, bvh->root_index,
+			          rg->gpu_origin, rg->gpu_direction, rg->gpu_maxt, n, 0, hits, d_err, d_queue,
+			          &config, 0),
+			      "persistent trace");
 			break;
 		}
 		check(rta_hip_device_synchronize(), "trace");

@@ -6,6 +6,8 @@
 //   -b lbvh        GPU LBVH build (the only builder of this plugin)
 //   -t cis         child-is traversal (the only traversal of this plugin)
 //   -l cpu|2f4     accepted for compatibility; both byte images are one
+//   -s             shadow tracer: stop at the first accepted triangle
+//                  (bbvh-cuda/main.cpp:46,93; packed/persistent only)
 //   --trav plain|packed|persistent   traversal kernel (default persistent)
 //   --block N --refill N --steps N --leaf-min N   persistent launch shape
 #include <stdio.h>
@@ -26,13 +28,14 @@ struct Cmdline {
 	bool verbose = false;
 	hip::trace_layout layout = hip::trace_layout::persistent;
 	rta_trace_config config{64, 16, 16, 0, 0};
+	bool shadow = false;
 };
 Cmdline cmdline;
 char plugin_name[] = "bbvh-hip";
 
 int usage(const char *msg, const char *arg) {
 	std::cerr << "bbvh-hip: " << msg << (arg ? arg : "") << std::endl;
-	std::cerr << "usage: bbvh-hip [-v] [-b lbvh] [-t cis] [-l cpu|2f4] "
+	std::cerr << "usage: bbvh-hip [-v] [-s] [-b lbvh] [-t cis] [-l cpu|2f4] "
 	             "[--trav plain|packed|persistent] [--block N] [--refill N] [--steps N] [--leaf-min N]"
 	          << std::endl;
 	return 1;
@@ -47,6 +50,7 @@ char *description() { return (char *)"bbvh-hip: binary LBVH built and traced on 
 void initialize() { hip::plugin_with_hip_loaded = true; }
 
 int parse_cmdline(int argc, char **argv) {
+	cmdline = Cmdline();
 	if (argc > 0) argv[0] = plugin_name;
 	for (int i = 1; i < argc; ++i) {
 		std::string a = argv[i];
@@ -80,12 +84,16 @@ int parse_cmdline(int argc, char **argv) {
 			else if (a == "--refill") cmdline.config.refill_min = x;
 			else if (a == "--leaf-min") cmdline.config.leaf_min = x;
 			else cmdline.config.steps = x;
-		} else if (a == "-s" || a == "--shadow" || a == "-A" || a == "--ailabox") {
+		} else if (a == "-s" || a == "--shadow") {
+			cmdline.shadow = true;
+		} else if (a == "-A" || a == "--ailabox") {
 			return usage("not provided by this plugin: ", a.c_str());
 		} else {
 			return usage("unknown option ", a.c_str());
 		}
 	}
+	if (cmdline.shadow && cmdline.layout == hip::trace_layout::plain)
+		return usage("the shadow tracer needs --trav packed or persistent", nullptr);
 	return 0;
 }
 
@@ -107,7 +115,8 @@ rt_set create_rt_set(basic_flat_triangle_list<simple_triangle> &triangle_lists, 
 	rt_set set;
 	set.as = bvh;
 	set.ctor = ctor;
-	set.rt = new hip::bbvh_gpu_cis_tracer<box_t, tri_t>(0, bvh, 0, cmdline.layout, cmdline.config);
+	set.rt = new hip::bbvh_gpu_cis_tracer<box_t, tri_t>(0, bvh, 0, cmdline.layout, cmdline.config,
+	                                                   cmdline.shadow);
 	set.rgen = new hip::cam_ray_generator_shirley(w, h);
 	return set;
 }

@@ -4,7 +4,7 @@
 // metric `average rps per frame: <K> K` (rps = w*h*1000/ms of the first
 // trace_rays() timing of each frame, rta/main.cpp:404-407,461).
 //
-//   rta-hip [-r WxH] [--scene icosphere[:S] | config2[:GRID:RANDOM]]
+//   rta-hip [-r WxH] [--scene icosphere[:S] | terrain[:GRID:RANDOM]]
 //           [--frames N] [--pos x,y,z] [--target x,y,z] [--up x,y,z]
 //           [--fovy F] [--rebuild] [--dump-hits FILE] [--dump-bvh FILE]
 //           [plugin] [-- plugin options]
@@ -44,7 +44,7 @@ struct Options {
 bool parse_vec(const char *s, vec3f &v) { return sscanf(s, "%f,%f,%f", &v.x, &v.y, &v.z) == 3; }
 
 int usage() {
-	std::cerr << "usage: rta-hip [-r WxH] [--scene icosphere[:S]|config2[:GRID:RANDOM]] "
+	std::cerr << "usage: rta-hip [-r WxH] [--scene icosphere[:S]|terrain[:GRID:RANDOM]] "
 	             "[--frames N] [--pos x,y,z] [--target x,y,z] [--up x,y,z] [--fovy F] "
 	             "[--rebuild] [--dump-hits FILE] [--dump-bvh FILE] [plugin] [-- plugin options]"
 	          << std::endl;
@@ -61,7 +61,7 @@ std::vector<simple_triangle> make_scene(const std::string &spec, Options &o) {
 		int n = rta_scene_icosphere(s, 1.0f, nullptr, 0.0f, 0, nullptr);
 		t.resize(n);
 		rta_scene_icosphere(s, 1.0f, nullptr, 0.0f, 0, (rta_triangle *)t.data());
-	} else if (parts[0] == "config2") {
+	} else if (parts[0] == "terrain") {
 		int grid = parts.size() > 1 ? atoi(parts[1].c_str()) : 1024;
 		int rnd = parts.size() > 2 ? atoi(parts[2].c_str()) : 200000;
 		int nh = rta_scene_heightfield(grid, grid, 0, 0, 1, 1, 0, 7, 6, 0.2f, 2.0f, 0.5f, 4.0f, nullptr);

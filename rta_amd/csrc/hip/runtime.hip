@@ -29,6 +29,10 @@ __global__ void fill_u32_kernel(uint32_t *dst, uint32_t v, size_t n) {
 	for (; i < n; i += stride) dst[i] = v;
 }
 
+__global__ void add_u32_kernel(uint32_t *word, uint32_t delta) {
+	if (threadIdx.x == 0) *word += delta;
+}
+
 __global__ void copy16_kernel(uint4 *__restrict__ dst, const uint4 *__restrict__ src, size_t n) {
 	size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
 	size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -110,6 +114,13 @@ int rta_hip_fill_u32(uint32_t *dst, uint32_t value, size_t count, rta_stream s) 
 	return RTA_OK;
 }
 
+int rta_hip_add_u32(uint32_t *word, uint32_t delta, rta_stream s) {
+	if (!word) return invalid("rta_hip_add_u32: word is NULL");
+	hipLaunchKernelGGL(add_u32_kernel, dim3(1), dim3(64), 0, as_stream(s), word, delta);
+	RTA_LAUNCH_CHECK("add_u32_kernel");
+	return RTA_OK;
+}
+
 int rta_hip_copy_kernel(void *dst, const void *src, size_t bytes, rta_stream s) {
 	if (bytes == 0) return RTA_OK;
 	if (!dst || !src || (bytes & 15)) return invalid("rta_hip_copy_kernel: bad arguments");
@@ -175,6 +186,49 @@ int rta_hip_event_synchronize(rta_event e) {
 int rta_hip_event_elapsed_ms(float *ms, rta_event start, rta_event stop) {
 	if (!ms || !start || !stop) return invalid("rta_hip_event_elapsed_ms: NULL");
 	RTA_HIP_TRY(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+	return RTA_OK;
+}
+
+// A captured graph and its executable form.
+struct rta_graph_rec {
+	hipGraph_t graph;
+	hipGraphExec_t exec;
+};
+
+int rta_hip_graph_begin(rta_stream s) {
+	if (!s) return invalid("rta_hip_graph_begin: capture needs a created stream, not NULL");
+	RTA_HIP_TRY(hipStreamBeginCapture(as_stream(s), hipStreamCaptureModeThreadLocal));
+	return RTA_OK;
+}
+
+int rta_hip_graph_end(rta_stream s, rta_graph *g) {
+	if (!s || !g) return invalid("rta_hip_graph_end: NULL argument");
+	hipGraph_t graph = nullptr;
+	RTA_HIP_TRY(hipStreamEndCapture(as_stream(s), &graph));
+	hipGraphExec_t exec = nullptr;
+	hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+	if (e != hipSuccess) {
+		(void)hipGraphDestroy(graph);
+		return hip_status(e, "hipGraphInstantiate");
+	}
+	*g = (rta_graph) new rta_graph_rec{graph, exec};
+	return RTA_OK;
+}
+
+int rta_hip_graph_launch(rta_graph g, rta_stream s) {
+	if (!g) return invalid("rta_hip_graph_launch: NULL graph");
+	RTA_HIP_TRY(hipGraphLaunch(((rta_graph_rec *)g)->exec, as_stream(s)));
+	return RTA_OK;
+}
+
+int rta_hip_graph_destroy(rta_graph g) {
+	if (!g) return RTA_OK;
+	rta_graph_rec *r = (rta_graph_rec *)g;
+	hipError_t e1 = hipGraphExecDestroy(r->exec);
+	hipError_t e2 = hipGraphDestroy(r->graph);
+	delete r;
+	if (e1 != hipSuccess) return hip_status(e1, "hipGraphExecDestroy");
+	if (e2 != hipSuccess) return hip_status(e2, "hipGraphDestroy");
 	return RTA_OK;
 }
 

@@ -1,0 +1,68 @@
+// Per-frame geometry of the dynamic scene: every vertex of an indexed mesh
+// moves by a seeded Gaussian offset (sigma per axis) and the triangle
+// records the LBVH builder reads are written from the moved vertices.
+#include "common.hpp"
+
+namespace rta_hip {
+
+// Offset of vertex `vid` along `axis` in frame `frame`: sigma times an
+// Irwin-Hall(4) variate (four 16-bit uniforms from one hash, shifted to mean
+// 0 and scaled to unit variance).  Plain binary32 operations in a fixed
+// order, so the host restatement gives the same bits.
+__device__ inline float vertex_offset(uint64_t frame_key, uint32_t vid, int axis, float sigma) {
+	const uint64_t h = mix64(frame_key + 3ull * vid + (uint64_t)axis);
+	const float k = 1.0f / 65536.0f;
+	const float u0 = (float)(uint32_t)(h & 0xffffu) * k;
+	const float u1 = (float)(uint32_t)((h >> 16) & 0xffffu) * k;
+	const float u2 = (float)(uint32_t)((h >> 32) & 0xffffu) * k;
+	const float u3 = (float)(uint32_t)(h >> 48) * k;
+	return sigma * ((((u0 + u1) + (u2 + u3)) - 2.0f) * 1.7320508f);
+}
+
+__device__ __host__ inline uint64_t frame_key_of(uint64_t seed, uint32_t frame) {
+	return mix64(seed ^ mix64(0x6a09e667f3bcc909ull + frame));
+}
+
+// frame_word (device, may be NULL) overrides `frame`, so a captured graph
+// can advance the frame number itself (rta_hip_add_u32).
+__global__ __launch_bounds__(256) void jitter_triangles_kernel(
+    rta_triangle *__restrict__ out, const float *__restrict__ verts,
+    const uint32_t *__restrict__ idx, int n_tris, float sigma, uint64_t seed, uint32_t frame,
+    const uint32_t *__restrict__ frame_word) {
+	const int i = blockIdx.x * 256 + threadIdx.x;
+	if (i >= n_tris) return;
+	const uint64_t frame_key = frame_key_of(seed, frame_word ? *frame_word : frame);
+	float p[9];
+	for (int c = 0; c < 3; ++c) {
+		const uint32_t v = idx[3 * (size_t)i + c];
+		for (int a = 0; a < 3; ++a)
+			p[3 * c + a] = verts[3 * (size_t)v + a] + vertex_offset(frame_key, v, a, sigma);
+	}
+	rta_triangle t;
+	t.a = {p[0], p[1], p[2]};
+	t.b = {p[3], p[4], p[5]};
+	t.c = {p[6], p[7], p[8]};
+	t.na = {0.0f, 0.0f, 0.0f};
+	t.nb = t.na;
+	t.nc = t.na;
+	t.ta = {0.0f, 0.0f};
+	t.tb = t.ta;
+	t.tc = t.ta;
+	t.material_index = -1;
+	out[i] = t;
+}
+
+}  // namespace rta_hip
+
+using namespace rta_hip;
+
+extern "C" int rta_jitter_triangles(rta_triangle *out, const float *verts, const uint32_t *idx,
+                                    int n_tris, float sigma, uint64_t seed, uint32_t frame,
+                                    const uint32_t *frame_word, rta_stream s) {
+	if (!out || !verts || !idx || n_tris <= 0 || !(sigma >= 0.0f))
+		return invalid("rta_jitter_triangles: bad arguments");
+	hipLaunchKernelGGL(jitter_triangles_kernel, dim3(grid_for(n_tris, 256)), dim3(256), 0,
+	                   as_stream(s), out, verts, idx, n_tris, sigma, seed, frame, frame_word);
+	RTA_LAUNCH_CHECK("jitter_triangles_kernel");
+	return RTA_OK;
+}

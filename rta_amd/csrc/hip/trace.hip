@@ -229,7 +229,10 @@ __device__ inline void pair_boxes(const float4 &q0, const float4 &q1, const floa
 	}
 }
 
-template <int BLOCK>
+// One thread per ray over the child-pair layout.  ANY_HIT selects the
+// shadow form (trace_shadow_cis, kernels.cu:857-939): the first triangle a
+// leaf accepts ends that leaf and, once the leaf is done, the ray.
+template <int BLOCK, bool ANY_HIT>
 __global__ __launch_bounds__(BLOCK) void trace_cis_packed_kernel(
     const float4 *__restrict__ tri4, const float4 *__restrict__ pairs,
     const uint32_t *__restrict__ flags, uint32_t root_word, const float *__restrict__ ro,
@@ -287,11 +290,13 @@ __global__ __launch_bounds__(BLOCK) void trace_cis_packed_kernel(
 						closest.beta = be;
 						closest.gamma = ga;
 						closest.ref = i;
+						if (ANY_HIT) break;
 					}
 				}
 				if (__float_as_uint(v0.w) != 0u) break;
 				++i;
 			}
+			if (ANY_HIT && closest.t != FLT_MAX) break;
 		}
 		if (sp < 0) break;
 		curr = stack[sp--][lt];
@@ -319,8 +324,9 @@ __global__ void fill_queue_kernel(uint32_t *queue) {
 // when a lane advances, never what it does next, so every ray runs the
 // identical traversal and results equal trace_cis_packed_kernel's; a miss
 // stores only t = FLT_MAX over the reset sentinel (the reference writes the
-// sentinel back with t = FLT_MAX, kernels.cu:793-794,853).
-template <int BLOCK>
+// sentinel back with t = FLT_MAX, kernels.cu:793-794,853).  ANY_HIT as in
+// trace_cis_packed_kernel.
+template <int BLOCK, bool ANY_HIT>
 __global__ __launch_bounds__(BLOCK) void trace_cis_persistent_kernel(
     const float4 *__restrict__ tri4, const float4 *__restrict__ pairs,
     const uint32_t *__restrict__ flags, uint32_t root_word, const float *__restrict__ ro,
@@ -395,7 +401,7 @@ __global__ __launch_bounds__(BLOCK) void trace_cis_persistent_kernel(
 				hr = hr && dr < ct && dr <= tmax;
 				const uint32_t wl = __float_as_uint(q3.x), wr = __float_as_uint(q3.y);
 				if (hl && hr) {
-					// nearer child first, the left one on a tie (kernels.cu:822-829)
+					// nearer child first, the left one on a tie (kernels.cu:818-827)
 					const bool left_first = dl <= dr;
 					if (sp + 1 >= kStack) { atomicAdd(err, 1u); done = true; }
 					else { stack[++sp][lt] = left_first ? wr : wl; curr = left_first ? wl : wr; }
@@ -415,12 +421,16 @@ __global__ __launch_bounds__(BLOCK) void trace_cis_persistent_kernel(
 					float t, be, ga;
 					if (tri_test(v0.x, v0.y, v0.z, v1.x, v1.y, v1.z, v2.x, v2.y, v2.z, ox, oy, oz,
 					             dx, dy, dz, t, be, ga)) {
-						if (t < ct && t <= tmax) { ct = t; cb = be; cg = ga; cref = i; }
+						if (t < ct && t <= tmax) {
+							ct = t; cb = be; cg = ga; cref = i;
+							if (ANY_HIT) break;
+						}
 					}
 					if (__float_as_uint(v0.w) != 0u) break;
 					++i;
 				}
-				if (sp >= 0) curr = stack[sp--][lt];
+				if (ANY_HIT && ct != FLT_MAX) done = true;
+				else if (sp >= 0) curr = stack[sp--][lt];
 				else done = true;
 			}
 			if (done) {
@@ -525,11 +535,45 @@ __device__ inline void bounce_ray(const rta_hit &h, const rta_triangle *tris, fl
 	d_out[0] = vx; d_out[1] = vy; d_out[2] = vz;
 }
 
+// Shadow ray of a hit towards a point light: origin as for the bounce
+// (hit point + 1e-4 * normal facing the incoming ray), direction the unit
+// vector to the light, max_t the distance to it.  A light at the origin
+// itself gives direction n and max_t 0 (never occluded).
+__device__ inline void shadow_ray(const rta_hit &h, const rta_triangle *tris, float ox, float oy,
+                                  float oz, float dx, float dy, float dz, float lx, float ly,
+                                  float lz, float *o_out, float *d_out, float *t_out) {
+	const float *f = reinterpret_cast<const float *>(tris + h.ref);
+	const float e1x = f[3] - f[0], e1y = f[4] - f[1], e1z = f[5] - f[2];
+	const float e2x = f[6] - f[0], e2y = f[7] - f[1], e2z = f[8] - f[2];
+	float nx = e1y * e2z - e1z * e2y;
+	float ny = e1z * e2x - e1x * e2z;
+	float nz = e1x * e2y - e1y * e2x;
+	const float nl = sqrtf(nx * nx + ny * ny + nz * nz);
+	nx = nx / nl; ny = ny / nl; nz = nz / nl;
+	if (nx * dx + ny * dy + nz * dz > 0.0f) { nx = -nx; ny = -ny; nz = -nz; }
+	const float px = ox + h.t * dx, py = oy + h.t * dy, pz = oz + h.t * dz;
+	const float sx = px + 1e-4f * nx, sy = py + 1e-4f * ny, sz = pz + 1e-4f * nz;
+	o_out[0] = sx; o_out[1] = sy; o_out[2] = sz;
+	float vx = lx - sx, vy = ly - sy, vz = lz - sz;
+	const float vl = sqrtf(vx * vx + vy * vy + vz * vz);
+	if (vl > 0.0f) {
+		d_out[0] = vx / vl; d_out[1] = vy / vl; d_out[2] = vz / vl;
+		*t_out = vl;
+	} else {
+		d_out[0] = nx; d_out[1] = ny; d_out[2] = nz;
+		*t_out = 0.0f;
+	}
+}
+
+// Writes one derived ray per hit, compacted in ray order.  SHADOW selects
+// shadow rays (light at lx, ly, lz) over diffuse bounces (seed).
+template <bool SHADOW>
 __global__ __launch_bounds__(kBounceBlock) void bounce_emit_kernel(
     float *__restrict__ out_o, float *__restrict__ out_d, float *__restrict__ out_t,
     uint32_t *__restrict__ src_index, const uint32_t *__restrict__ block_offsets,
     const rta_hit *__restrict__ hits, const rta_triangle *__restrict__ tris,
-    const float *__restrict__ ro, const float *__restrict__ rd, int n, uint64_t seed) {
+    const float *__restrict__ ro, const float *__restrict__ rd, int n, uint64_t seed, float lx,
+    float ly, float lz) {
 	__shared__ uint32_t wsum[kBounceBlock / kWave];
 	const int i = blockIdx.x * kBounceBlock + threadIdx.x;
 	rta_hit h;
@@ -544,12 +588,16 @@ __global__ __launch_bounds__(kBounceBlock) void bounce_emit_kernel(
 	for (int w = 0; w < wv; ++w) before += wsum[w];
 	const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
 	const uint32_t k = block_offsets[blockIdx.x] + before + (uint32_t)__popcll(b & lt);
-	float o[3], d[3];
-	bounce_ray(h, tris, ro[3 * i], ro[3 * i + 1], ro[3 * i + 2], rd[3 * i], rd[3 * i + 1],
-	           rd[3 * i + 2], seed, (uint32_t)i, o, d);
+	float o[3], d[3], t = FLT_MAX;
+	if (SHADOW)
+		shadow_ray(h, tris, ro[3 * i], ro[3 * i + 1], ro[3 * i + 2], rd[3 * i], rd[3 * i + 1],
+		           rd[3 * i + 2], lx, ly, lz, o, d, &t);
+	else
+		bounce_ray(h, tris, ro[3 * i], ro[3 * i + 1], ro[3 * i + 2], rd[3 * i], rd[3 * i + 1],
+		           rd[3 * i + 2], seed, (uint32_t)i, o, d);
 	out_o[3 * (size_t)k + 0] = o[0]; out_o[3 * (size_t)k + 1] = o[1]; out_o[3 * (size_t)k + 2] = o[2];
 	out_d[3 * (size_t)k + 0] = d[0]; out_d[3 * (size_t)k + 1] = d[1]; out_d[3 * (size_t)k + 2] = d[2];
-	out_t[k] = FLT_MAX;
+	out_t[k] = t;
 	src_index[k] = (uint32_t)i;
 }
 
@@ -557,32 +605,102 @@ __global__ __launch_bounds__(kBounceBlock) void bounce_emit_kernel(
 
 namespace rta_hip {
 
-template <int BLOCK>
+// Grid of the persistent kernel: as many blocks as fit on every CU (or
+// blocks_per_cu), never more than the rays need.  The CU count and the
+// occupancy are queried once per device and kernel instance.
+template <int BLOCK, bool ANY_HIT>
 int launch_persistent(const float *tri4, const rta_node_pair *pairs, const uint32_t *flags,
                       uint32_t root_word, const float *ro, const float *rd, const float *rmax,
                       int n_rays, const uint32_t *ray_count, rta_hit *hits, uint32_t *err,
                       uint32_t *queue, int refill_min, int steps, int blocks_per_cu,
                       int leaf_min, hipStream_t st) {
-	static int cus = 0, occ = 0;
-	if (cus == 0) {
-		int dev = 0;
-		RTA_HIP_TRY(hipGetDevice(&dev));
-		RTA_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+	constexpr int kMaxDevices = 64;
+	static int cus[kMaxDevices] = {}, occ[kMaxDevices] = {};
+	int dev = 0;
+	RTA_HIP_TRY(hipGetDevice(&dev));
+	if (dev < 0 || dev >= kMaxDevices) return invalid("launch_persistent: device id out of range");
+	if (cus[dev] == 0) {
+		int c = 0, o = 0;
+		RTA_HIP_TRY(hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev));
 		RTA_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-		    &occ, trace_cis_persistent_kernel<BLOCK>, BLOCK, 0));
-		if (occ < 1) occ = 1;
+		    &o, trace_cis_persistent_kernel<BLOCK, ANY_HIT>, BLOCK, 0));
+		occ[dev] = o < 1 ? 1 : o;
+		cus[dev] = c;
 	}
-	const int bpc = blocks_per_cu > 0 ? blocks_per_cu : occ;
-	const long long want = (long long)cus * bpc;
+	const int bpc = blocks_per_cu > 0 ? blocks_per_cu : occ[dev];
+	const long long want = (long long)cus[dev] * bpc;
 	const long long need = ((long long)n_rays + BLOCK - 1) / BLOCK;
 	const unsigned blocks = (unsigned)(want < need ? want : need);
 	hipLaunchKernelGGL(fill_queue_kernel, dim3(1), dim3(64), 0, st, queue);
 	RTA_LAUNCH_CHECK("fill_queue_kernel");
-	hipLaunchKernelGGL(trace_cis_persistent_kernel<BLOCK>, dim3(blocks), dim3(BLOCK), 0, st,
-	                   (const float4 *)tri4, (const float4 *)pairs, flags, root_word, ro, rd, rmax,
-	                   n_rays, ray_count, hits, err, queue, refill_min, steps, leaf_min);
+	hipLaunchKernelGGL((trace_cis_persistent_kernel<BLOCK, ANY_HIT>), dim3(blocks), dim3(BLOCK), 0,
+	                   st, (const float4 *)tri4, (const float4 *)pairs, flags, root_word, ro, rd,
+	                   rmax, n_rays, ray_count, hits, err, queue, refill_min, steps, leaf_min);
 	RTA_LAUNCH_CHECK("trace_cis_persistent_kernel");
 	return RTA_OK;
+}
+
+template <bool ANY_HIT>
+int trace_packed(const char *name, const float *tri4, const rta_node_pair *pairs,
+                 const uint32_t *flags, uint32_t root_link, uint32_t root_index,
+                 const float *ray_orig, const float *ray_dir, const float *max_t, int n_rays,
+                 const uint32_t *ray_count, rta_hit *hits, uint32_t *err, hipStream_t st) {
+	if (!tri4 || !pairs || !flags || !ray_orig || !ray_dir || !max_t || !hits || !err ||
+	    n_rays <= 0) {
+		set_error(std::string(name) + ": bad arguments");
+		return RTA_E_INVALID;
+	}
+	// the root as its parent's pair record would describe it
+	const uint32_t root_word = link_inner(root_link) ? 1u : (root_index << 1);
+	const unsigned tiles = grid_for(n_rays, kTraceBlock);
+	const unsigned blocks = (tiles + kXcds - 1) / kXcds * kXcds;
+	hipLaunchKernelGGL((trace_cis_packed_kernel<kTraceBlock, ANY_HIT>), dim3(blocks),
+	                   dim3(kTraceBlock), 0, st, (const float4 *)tri4, (const float4 *)pairs, flags,
+	                   root_word, ray_orig, ray_dir, max_t, n_rays, ray_count, hits, err);
+	RTA_LAUNCH_CHECK("trace_cis_packed_kernel");
+	return RTA_OK;
+}
+
+template <bool ANY_HIT>
+int trace_persistent(const char *name, const float *tri4, const rta_node_pair *pairs,
+                     const uint32_t *flags, uint32_t root_link, uint32_t root_index,
+                     const float *ray_orig, const float *ray_dir, const float *max_t, int n_rays,
+                     const uint32_t *ray_count, rta_hit *hits, uint32_t *err, uint32_t *queue,
+                     const rta_trace_config *cfg, hipStream_t st) {
+	if (!tri4 || !pairs || !flags || !ray_orig || !ray_dir || !max_t || !hits || !err ||
+	    !queue || n_rays <= 0) {
+		set_error(std::string(name) + ": bad arguments");
+		return RTA_E_INVALID;
+	}
+	rta_trace_config c = {64, 16, 16, 0, 0};
+	if (cfg) c = *cfg;
+	if (c.leaf_min == 0) c.leaf_min = 6;
+	if (c.refill_min < 1 || c.refill_min > 64 || c.steps < 1 || c.blocks_per_cu < 0 ||
+	    c.leaf_min < 1 || c.leaf_min > 64) {
+		set_error(std::string(name) + ": bad config");
+		return RTA_E_INVALID;
+	}
+	const uint32_t root_word = link_inner(root_link) ? 1u : (root_index << 1);
+	switch (c.block) {
+	case 64:
+		return launch_persistent<64, ANY_HIT>(tri4, pairs, flags, root_word, ray_orig, ray_dir,
+		                                      max_t, n_rays, ray_count, hits, err, queue,
+		                                      c.refill_min, c.steps, c.blocks_per_cu, c.leaf_min,
+		                                      st);
+	case 128:
+		return launch_persistent<128, ANY_HIT>(tri4, pairs, flags, root_word, ray_orig, ray_dir,
+		                                       max_t, n_rays, ray_count, hits, err, queue,
+		                                       c.refill_min, c.steps, c.blocks_per_cu, c.leaf_min,
+		                                       st);
+	case 256:
+		return launch_persistent<256, ANY_HIT>(tri4, pairs, flags, root_word, ray_orig, ray_dir,
+		                                       max_t, n_rays, ray_count, hits, err, queue,
+		                                       c.refill_min, c.steps, c.blocks_per_cu, c.leaf_min,
+		                                       st);
+	default:
+		set_error(std::string(name) + ": block must be 64, 128 or 256");
+		return RTA_E_INVALID;
+	}
 }
 
 }  // namespace rta_hip
@@ -672,18 +790,19 @@ int rta_trace_cis_packed(const float *tri4, const rta_node_pair *pairs, const ui
                          uint32_t root_link, uint32_t root_index, const float *ray_orig,
                          const float *ray_dir, const float *max_t, int n_rays,
                          const uint32_t *ray_count, rta_hit *hits, uint32_t *err, rta_stream s) {
-	if (!tri4 || !pairs || !flags || !ray_orig || !ray_dir || !max_t || !hits || !err ||
-	    n_rays <= 0)
-		return invalid("rta_trace_cis_packed: bad arguments");
-	// the root as its parent's pair record would describe it
-	const uint32_t root_word = link_inner(root_link) ? 1u : (root_index << 1);
-	const unsigned tiles = grid_for(n_rays, kTraceBlock);
-	const unsigned blocks = (tiles + kXcds - 1) / kXcds * kXcds;
-	hipLaunchKernelGGL(trace_cis_packed_kernel<kTraceBlock>, dim3(blocks), dim3(kTraceBlock), 0,
-	                   as_stream(s), (const float4 *)tri4, (const float4 *)pairs, flags, root_word,
-	                   ray_orig, ray_dir, max_t, n_rays, ray_count, hits, err);
-	RTA_LAUNCH_CHECK("trace_cis_packed_kernel");
-	return RTA_OK;
+	return trace_packed<false>("rta_trace_cis_packed", tri4, pairs, flags, root_link, root_index,
+	                           ray_orig, ray_dir, max_t, n_rays, ray_count, hits, err,
+	                           as_stream(s));
+}
+
+int rta_trace_shadow_cis_packed(const float *tri4, const rta_node_pair *pairs,
+                                const uint32_t *flags, uint32_t root_link, uint32_t root_index,
+                                const float *ray_orig, const float *ray_dir, const float *max_t,
+                                int n_rays, const uint32_t *ray_count, rta_hit *hits,
+                                uint32_t *err, rta_stream s) {
+	return trace_packed<true>("rta_trace_shadow_cis_packed", tri4, pairs, flags, root_link,
+	                          root_index, ray_orig, ray_dir, max_t, n_rays, ray_count, hits, err,
+	                          as_stream(s));
 }
 
 int rta_trace_cis_persistent(const float *tri4, const rta_node_pair *pairs,
@@ -691,33 +810,20 @@ int rta_trace_cis_persistent(const float *tri4, const rta_node_pair *pairs,
                              const float *ray_orig, const float *ray_dir, const float *max_t,
                              int n_rays, const uint32_t *ray_count, rta_hit *hits, uint32_t *err,
                              uint32_t *queue, const rta_trace_config *cfg, rta_stream s) {
-	if (!tri4 || !pairs || !flags || !ray_orig || !ray_dir || !max_t || !hits || !err ||
-	    !queue || n_rays <= 0)
-		return invalid("rta_trace_cis_persistent: bad arguments");
-	rta_trace_config c = {64, 16, 16, 0, 0};
-	if (cfg) c = *cfg;
-	if (c.leaf_min == 0) c.leaf_min = 6;
-	if (c.refill_min < 1 || c.refill_min > 64 || c.steps < 1 || c.blocks_per_cu < 0 ||
-	    c.leaf_min < 1 || c.leaf_min > 64)
-		return invalid("rta_trace_cis_persistent: bad config");
-	const uint32_t root_word = link_inner(root_link) ? 1u : (root_index << 1);
-	hipStream_t st = as_stream(s);
-	switch (c.block) {
-	case 64:
-		return launch_persistent<64>(tri4, pairs, flags, root_word, ray_orig, ray_dir, max_t, n_rays,
-		                             ray_count, hits, err, queue, c.refill_min, c.steps,
-		                             c.blocks_per_cu, c.leaf_min, st);
-	case 128:
-		return launch_persistent<128>(tri4, pairs, flags, root_word, ray_orig, ray_dir, max_t,
-		                              n_rays, ray_count, hits, err, queue, c.refill_min, c.steps,
-		                              c.blocks_per_cu, c.leaf_min, st);
-	case 256:
-		return launch_persistent<256>(tri4, pairs, flags, root_word, ray_orig, ray_dir, max_t,
-		                              n_rays, ray_count, hits, err, queue, c.refill_min, c.steps,
-		                              c.blocks_per_cu, c.leaf_min, st);
-	default:
-		return invalid("rta_trace_cis_persistent: block must be 64, 128 or 256");
-	}
+	return trace_persistent<false>("rta_trace_cis_persistent", tri4, pairs, flags, root_link,
+	                               root_index, ray_orig, ray_dir, max_t, n_rays, ray_count, hits,
+	                               err, queue, cfg, as_stream(s));
+}
+
+int rta_trace_shadow_cis_persistent(const float *tri4, const rta_node_pair *pairs,
+                                    const uint32_t *flags, uint32_t root_link,
+                                    uint32_t root_index, const float *ray_orig,
+                                    const float *ray_dir, const float *max_t, int n_rays,
+                                    const uint32_t *ray_count, rta_hit *hits, uint32_t *err,
+                                    uint32_t *queue, const rta_trace_config *cfg, rta_stream s) {
+	return trace_persistent<true>("rta_trace_shadow_cis_persistent", tri4, pairs, flags,
+	                              root_link, root_index, ray_orig, ray_dir, max_t, n_rays,
+	                              ray_count, hits, err, queue, cfg, as_stream(s));
 }
 
 int rta_bounce_work_bytes(int n_rays, size_t *bytes) {
@@ -726,30 +832,61 @@ int rta_bounce_work_bytes(int n_rays, size_t *bytes) {
 	return RTA_OK;
 }
 
-int rta_diffuse_bounce(float *out_orig, float *out_dir, float *out_maxt, uint32_t *src_index,
-                       uint32_t *count, const rta_hit *hits, const rta_triangle *tris,
-                       const float *ray_orig, const float *ray_dir, int n_rays, uint64_t seed,
-                       void *work, size_t work_bytes, rta_stream s) {
+}  // extern "C"
+
+namespace rta_hip {
+
+template <bool SHADOW>
+int emit_rays(const char *name, float *out_orig, float *out_dir, float *out_maxt,
+              uint32_t *src_index, uint32_t *count, const rta_hit *hits, const rta_triangle *tris,
+              const float *ray_orig, const float *ray_dir, int n_rays, uint64_t seed,
+              const float *light, void *work, size_t work_bytes, hipStream_t st) {
 	if (!out_orig || !out_dir || !out_maxt || !src_index || !count || !hits || !tris ||
-	    !ray_orig || !ray_dir || !work || n_rays <= 0)
-		return invalid("rta_diffuse_bounce: bad arguments");
+	    !ray_orig || !ray_dir || !work || n_rays <= 0 || (SHADOW && !light)) {
+		set_error(std::string(name) + ": bad arguments");
+		return RTA_E_INVALID;
+	}
 	size_t need = 0;
 	rta_bounce_work_bytes(n_rays, &need);
 	if (work_bytes < need) {
-		set_error("rta_diffuse_bounce: work buffer too small");
+		set_error(std::string(name) + ": work buffer too small");
 		return RTA_E_WORKSPACE;
 	}
 	const unsigned nb = grid_for(n_rays, kBounceBlock);
 	uint32_t *bc = reinterpret_cast<uint32_t *>(work);
-	hipStream_t st = as_stream(s);
 	hipLaunchKernelGGL(bounce_count_kernel, dim3(nb), dim3(kBounceBlock), 0, st, hits, n_rays, bc);
 	RTA_LAUNCH_CHECK("bounce_count_kernel");
 	hipLaunchKernelGGL(bounce_scan_kernel, dim3(1), dim3(1024), 0, st, bc, (int)nb, count);
 	RTA_LAUNCH_CHECK("bounce_scan_kernel");
-	hipLaunchKernelGGL(bounce_emit_kernel, dim3(nb), dim3(kBounceBlock), 0, st, out_orig, out_dir,
-	                   out_maxt, src_index, bc, hits, tris, ray_orig, ray_dir, n_rays, seed);
+	const float lx = SHADOW ? light[0] : 0.0f, ly = SHADOW ? light[1] : 0.0f,
+	            lz = SHADOW ? light[2] : 0.0f;
+	hipLaunchKernelGGL(bounce_emit_kernel<SHADOW>, dim3(nb), dim3(kBounceBlock), 0, st, out_orig,
+	                   out_dir, out_maxt, src_index, bc, hits, tris, ray_orig, ray_dir, n_rays,
+	                   seed, lx, ly, lz);
 	RTA_LAUNCH_CHECK("bounce_emit_kernel");
 	return RTA_OK;
+}
+
+}  // namespace rta_hip
+
+extern "C" {
+
+int rta_diffuse_bounce(float *out_orig, float *out_dir, float *out_maxt, uint32_t *src_index,
+                       uint32_t *count, const rta_hit *hits, const rta_triangle *tris,
+                       const float *ray_orig, const float *ray_dir, int n_rays, uint64_t seed,
+                       void *work, size_t work_bytes, rta_stream s) {
+	return emit_rays<false>("rta_diffuse_bounce", out_orig, out_dir, out_maxt, src_index, count,
+	                        hits, tris, ray_orig, ray_dir, n_rays, seed, nullptr, work, work_bytes,
+	                        as_stream(s));
+}
+
+int rta_shadow_rays(float *out_orig, float *out_dir, float *out_maxt, uint32_t *src_index,
+                    uint32_t *count, const rta_hit *hits, const rta_triangle *tris,
+                    const float *ray_orig, const float *ray_dir, int n_rays, const float *light,
+                    void *work, size_t work_bytes, rta_stream s) {
+	return emit_rays<true>("rta_shadow_rays", out_orig, out_dir, out_maxt, src_index, count, hits,
+	                       tris, ray_orig, ray_dir, n_rays, 0, light, work, work_bytes,
+	                       as_stream(s));
 }
 
 }  // extern "C"

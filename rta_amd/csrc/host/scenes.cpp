@@ -173,6 +173,97 @@ int rta_scene_heightfield(int nx, int ny, float x0, float y0, float x1, float y1
 	return (int)count;
 }
 
+int rta_scene_instances(int n_inst, int g, uint64_t seed, int *n_verts, float *verts,
+                        uint32_t *idx) {
+	if (n_inst <= 0 || g < 2 || (g & (g - 1)) != 0 || !n_verts) return -1;
+	const long long tris_per = 2LL * g * g;
+	if (tris_per * n_inst > 0x7fffffffLL) return -1;
+	int s = 0;
+	while ((8LL << (2 * s)) < tris_per) ++s;  // 8 * 4^s == 2 g^2
+	const long long hf_verts = (long long)(g + 1) * (g + 1);
+	const long long sp_verts = (long long)g * g + 2;
+	const int side = (int)std::ceil(std::sqrt((double)n_inst));
+	auto is_sphere = [&](int k) { return (mix64(seed ^ mix64(0x1157ull + (uint64_t)k)) & 1) != 0; };
+	long long nv = 0;
+	for (int k = 0; k < n_inst; ++k) nv += is_sphere(k) ? sp_verts : hf_verts;
+	if (nv > 0x7fffffffLL) return -1;
+	*n_verts = (int)nv;
+	const int count = (int)(tris_per * n_inst);
+	if (!verts || !idx) return count;
+
+	// unit geodesic sphere: octahedron subdivided s times, shared vertices
+	std::vector<D3> sv = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+	std::vector<uint32_t> sf = {4, 0, 2, 4, 2, 1, 4, 1, 3, 4, 3, 0,
+	                            5, 2, 0, 5, 1, 2, 5, 3, 1, 5, 0, 3};
+	for (int it = 0; it < s; ++it) {
+		std::map<std::pair<uint32_t, uint32_t>, uint32_t> mid;
+		auto midpoint = [&](uint32_t a, uint32_t b) {
+			std::pair<uint32_t, uint32_t> key(std::min(a, b), std::max(a, b));
+			auto f = mid.find(key);
+			if (f != mid.end()) return f->second;
+			sv.push_back(normalized(0.5 * (sv[a] + sv[b])));
+			const uint32_t id = (uint32_t)sv.size() - 1;
+			mid[key] = id;
+			return id;
+		};
+		std::vector<uint32_t> nf;
+		nf.reserve(sf.size() * 4);
+		for (size_t i = 0; i < sf.size(); i += 3) {
+			const uint32_t a = sf[i], b = sf[i + 1], c = sf[i + 2];
+			const uint32_t ab = midpoint(a, b), bc = midpoint(b, c), ca = midpoint(c, a);
+			const uint32_t t4[12] = {a, ab, ca, b, bc, ab, c, ca, bc, ab, bc, ca};
+			nf.insert(nf.end(), t4, t4 + 12);
+		}
+		sf.swap(nf);
+	}
+
+	long long vbase = 0, t = 0;
+	for (int k = 0; k < n_inst; ++k) {
+		const double cx = (double)(k % side), cy = (double)(k / side);
+		const uint64_t hk = mix64(seed ^ mix64(0x2000ull + (uint64_t)k));
+		float *vp = verts + 3 * vbase;
+		if (is_sphere(k)) {
+			const double r = 0.3 + 0.1 * unit(mix64(hk + 1));
+			const D3 c = {cx + 0.5, cy + 0.5, 0.1 + r};
+			for (size_t i = 0; i < sv.size(); ++i) {
+				const D3 p = c + r * sv[i];
+				vp[3 * i] = (float)p.x; vp[3 * i + 1] = (float)p.y; vp[3 * i + 2] = (float)p.z;
+			}
+			for (size_t i = 0; i < sf.size(); ++i) idx[3 * t + i] = (uint32_t)(vbase + sf[i]);
+			vbase += (long long)sv.size();
+		} else {
+			ValueNoise vn{mix64(hk + 2)};
+			const int vx = g + 1;
+			for (int j = 0; j <= g; ++j)
+				for (int i = 0; i <= g; ++i) {
+					const double x = cx + (double)i / g, y = cy + (double)j / g;
+					double hsum = 0, amp = 1, fr = 4.0, wsum = 0;
+					for (int o = 0; o < 6; ++o) {
+						ValueNoise vo{mix64(vn.seed + (uint64_t)o)};
+						hsum += amp * vo(fr * x, fr * y);
+						wsum += amp;
+						amp *= 0.5;
+						fr *= 2.0;
+					}
+					const size_t q = (size_t)j * vx + i;
+					vp[3 * q] = (float)x; vp[3 * q + 1] = (float)y;
+					vp[3 * q + 2] = (float)(0.2 * hsum / wsum);
+				}
+			uint32_t *ip = idx + 3 * t;
+			for (int j = 0; j < g; ++j)
+				for (int i = 0; i < g; ++i) {
+					const uint32_t v00 = (uint32_t)(vbase + (long long)j * vx + i), v10 = v00 + 1,
+					               v01 = v00 + (uint32_t)vx, v11 = v01 + 1;
+					const uint32_t q[6] = {v00, v10, v11, v00, v11, v01};
+					for (int e = 0; e < 6; ++e) *ip++ = q[e];
+				}
+			vbase += hf_verts;
+		}
+		t += tris_per;
+	}
+	return count;
+}
+
 int rta_scene_random_triangles(int count, uint64_t seed, float lo, float hi, float edge_min,
                                float edge_max, rta_triangle *out) {
 	if (count < 0) return -1;

@@ -1,0 +1,219 @@
+"""The dynamic frame: per-frame vertex jitter, full LBVH rebuild, primary
+rays, diffuse bounces and one shadow ray per hit, all enqueued on one
+stream through the C-ABI so that the whole frame can be captured into a
+graph once and replayed.
+
+Stage order of one frame (every ray count stays on the device):
+
+    jitter -> build -> pack -> primary rays -> trace (closest hit)
+    for level 0..bounces:
+        shadow rays of level's hits -> trace (shadow)
+        if level < bounces: diffuse bounce -> trace (closest hit)
+    frame counter += 1
+
+The reference's host loop (rta/main.cpp:389-407, librta/raytrav.h:411-421)
+rebuilds with -R and traces one bounce pass per trace_rays(); here the
+whole frame is one stream of launches with no host round trip.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from .abi import FLT_MAX, HIT_DTYPE, NODE_DTYPE, TRI_DTYPE, TraceConfig, call, ptr
+from .bvh import RayBuffers
+from .device import DeviceBuffer, Event, Stream
+
+
+class Graph:
+    """A captured stream of launches (rta_hip_graph_*)."""
+
+    def __init__(self, handle: C.c_void_p):
+        self._h = handle
+
+    def launch(self, stream: Stream) -> None:
+        call("rta_hip_graph_launch", self._h, ptr(stream))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            try:
+                call("rta_hip_graph_destroy", self._h)
+            except Exception:
+                pass
+            self._h = None
+
+
+class DynamicFrame:
+    """Buffers and launch sequence of one frame of the dynamic scene."""
+
+    def __init__(self, verts: np.ndarray, idx: np.ndarray, camera, light, sigma: float,
+                 seed: int, bounces: int = 2, shadows: bool = True,
+                 config: TraceConfig | None = None, stream: Stream | None = None):
+        self.n = len(idx) // 3
+        if self.n < 2:
+            raise ValueError("the dynamic frame needs at least two triangles")
+        self.cam, self.light = camera, tuple(float(x) for x in light)
+        self.sigma, self.seed, self.bounces, self.shadows = sigma, seed, bounces, shadows
+        self.config = config if config is not None else TraceConfig(64, 16, 16, 0, 6)
+        self.stream = stream if stream is not None else Stream()
+        n, R = self.n, camera.w * camera.h
+        self.R = R
+        self.verts = DeviceBuffer.from_array(np.ascontiguousarray(verts, np.float32))
+        self.idx = DeviceBuffer.from_array(np.ascontiguousarray(idx, np.uint32))
+        self.tris = DeviceBuffer(104 * n)
+        self.nodes = DeviceBuffer(32 * (2 * n - 1))
+        self.index = DeviceBuffer(4 * n)
+        self.status = DeviceBuffer(4)
+        sz = C.c_size_t(0)
+        call("rta_lbvh_workspace_bytes", n, C.byref(sz))
+        self.ws, self.ws_bytes = DeviceBuffer(sz.value), sz.value
+        self.pairs = DeviceBuffer(64 * (2 * n - 1))
+        self.tri4 = DeviceBuffer(48 * n)
+        self.flags = DeviceBuffer(4)
+        # level k rays / hits (k = 0 primary), shadow rays / hits per level
+        self.rays = [RayBuffers(R) for _ in range(bounces + 1)]
+        self.hits = [DeviceBuffer(16 * R) for _ in range(bounces + 1)]
+        self.counts = [DeviceBuffer(4) for _ in range(bounces + 1)]
+        self.srays = [RayBuffers(R) for _ in range(bounces + 1)] if shadows else []
+        self.shits = [DeviceBuffer(16 * R) for _ in range(bounces + 1)] if shadows else []
+        self.scounts = [DeviceBuffer(4) for _ in range(bounces + 1)] if shadows else []
+        self.src = DeviceBuffer(4 * R)
+        call("rta_bounce_work_bytes", R, C.byref(sz))
+        self.bwork, self.bwork_bytes = DeviceBuffer(sz.value), sz.value
+        self.err = DeviceBuffer(4)
+        self.err.fill_u32(0)
+        self.queue = DeviceBuffer(4)
+        self.frame_word = DeviceBuffer(4)
+        self.frame_word.fill_u32(0)
+        self.counts[0].fill_u32(R)
+        self.events: list[Event] = []
+        self._root = None
+        # the fills above ran on the default stream; the frame's own stream
+        # does not wait for it
+        call("rta_hip_device_synchronize")
+
+    # ------------------------------------------------------------ stages
+
+    def _trace(self, rays: RayBuffers, count: DeviceBuffer, hits: DeviceBuffer,
+               shadow: bool) -> None:
+        s = ptr(self.stream)
+        call("rta_reset_intersections", ptr(hits), self.R, 1, s)
+        entry = "rta_trace_shadow_cis_persistent" if shadow else "rta_trace_cis_persistent"
+        call(entry, ptr(self.tri4), ptr(self.pairs), ptr(self.flags), self._root[0], self._root[1],
+             ptr(rays.gpu_origin), ptr(rays.gpu_direction), ptr(rays.gpu_maxt), self.R,
+             ptr(count), ptr(hits), ptr(self.err), ptr(self.queue), C.byref(self.config), s)
+
+    def _mark(self, timed: bool) -> None:
+        if timed:
+            ev = Event()
+            ev.record(self.stream)
+            self.events.append(ev)
+
+    def enqueue(self, frame: int | None = None, timed: bool = False) -> None:
+        """Enqueue one frame.  frame=None takes the frame number from the
+        device counter (and advances it), which is what a captured graph
+        does on every replay."""
+        if self._root is None:
+            raise RuntimeError("call prime() once before enqueueing frames")
+        s = ptr(self.stream)
+        f3 = lambda v: (C.c_float * 3)(*v)
+        self.events = []
+        self._mark(timed)
+        fw = ptr(self.frame_word) if frame is None else None
+        call("rta_jitter_triangles", ptr(self.tris), ptr(self.verts), ptr(self.idx), self.n,
+             self.sigma, C.c_uint64(self.seed), 0 if frame is None else frame, fw, s)
+        self._mark(timed)
+        call("rta_lbvh_build", ptr(self.tris), self.n, ptr(self.nodes), ptr(self.index),
+             ptr(self.ws), C.c_size_t(self.ws_bytes), ptr(self.status), None, s)
+        call("rta_bvh_pack", ptr(self.pairs), ptr(self.tri4), ptr(self.flags), ptr(self.nodes),
+             2 * self.n - 1, ptr(self.tris), self.n, s)
+        self._mark(timed)
+        cam, p = self.cam, self.rays[0]
+        call("rta_setup_shirley_rays", ptr(p.gpu_direction), ptr(p.gpu_origin), ptr(p.gpu_maxt),
+             cam.fovy, cam.aspect, cam.w, cam.h, f3(cam.view_dir), f3(cam.pos), f3(cam.up),
+             float(FLT_MAX), s)
+        self._trace(self.rays[0], self.counts[0], self.hits[0], shadow=False)
+        self._mark(timed)
+        for k in range(self.bounces + 1):
+            if self.shadows:
+                lp = f3(self.light)
+                call("rta_shadow_rays", ptr(self.srays[k].gpu_origin),
+                     ptr(self.srays[k].gpu_direction), ptr(self.srays[k].gpu_maxt),
+                     ptr(self.src), ptr(self.scounts[k]), ptr(self.hits[k]), ptr(self.tris),
+                     ptr(self.rays[k].gpu_origin), ptr(self.rays[k].gpu_direction), self.R, lp,
+                     ptr(self.bwork), C.c_size_t(self.bwork_bytes), s)
+                self._trace(self.srays[k], self.scounts[k], self.shits[k], shadow=True)
+                self._mark(timed)
+            if k < self.bounces:
+                nxt = self.rays[k + 1]
+                call("rta_diffuse_bounce", ptr(nxt.gpu_origin), ptr(nxt.gpu_direction),
+                     ptr(nxt.gpu_maxt), ptr(self.src), ptr(self.counts[k + 1]),
+                     ptr(self.hits[k]), ptr(self.tris), ptr(self.rays[k].gpu_origin),
+                     ptr(self.rays[k].gpu_direction), self.R,
+                     C.c_uint64(bounce_seed(self.seed, k + 1)), ptr(self.bwork),
+                     C.c_size_t(self.bwork_bytes), s)
+                self._trace(nxt, self.counts[k + 1], self.hits[k + 1], shadow=False)
+                self._mark(timed)
+        if frame is None:
+            call("rta_hip_add_u32", ptr(self.frame_word), 1, s)
+
+    def prime(self) -> None:
+        """One eager jitter + build to learn the root's link words (an
+        inner node for any scene whose Morton codes are not all equal);
+        replays bake them in."""
+        s = ptr(self.stream)
+        call("rta_jitter_triangles", ptr(self.tris), ptr(self.verts), ptr(self.idx), self.n,
+             self.sigma, C.c_uint64(self.seed), 0, None, s)
+        call("rta_lbvh_build", ptr(self.tris), self.n, ptr(self.nodes), ptr(self.index),
+             ptr(self.ws), C.c_size_t(self.ws_bytes), ptr(self.status), None, s)
+        self.stream.synchronize()
+        r = self.nodes.download(NODE_DTYPE, 1)[0]
+        if not (int(r["type_left_elems"]) & 1):
+            raise RuntimeError("all Morton codes are equal: the root is a leaf")
+        self._root = (int(r["type_left_elems"]), int(r["right_tris"]))
+
+    def capture(self) -> Graph:
+        """Capture one frame (frame number from the device counter)."""
+        call("rta_hip_graph_begin", ptr(self.stream))
+        h = C.c_void_p()
+        try:
+            self.enqueue(None)
+        finally:
+            # always end the capture so the stream is usable again
+            call("rta_hip_graph_end", ptr(self.stream), C.byref(h))
+        return Graph(h)
+
+    # ----------------------------------------------------------- results
+
+    def set_frame_counter(self, frame: int) -> None:
+        self.frame_word.fill_u32(frame, 1, self.stream)
+
+    def check_errors(self) -> None:
+        e = int(self.err.download(np.uint32, 1)[0])
+        if e:
+            raise RuntimeError(f"{e} rays overflowed the traversal stack")
+
+    def ray_counts(self) -> dict:
+        c = {"primary": self.R}
+        for k in range(1, self.bounces + 1):
+            c[f"bounce{k}"] = int(self.counts[k].download(np.uint32, 1)[0])
+        for k in range(len(self.scounts)):
+            c[f"shadow{k}"] = int(self.scounts[k].download(np.uint32, 1)[0])
+        return c
+
+    def results(self) -> dict:
+        """Host copies of everything a parity check needs."""
+        out = {"tris": self.tris.download(TRI_DTYPE, self.n),
+               "nodes": self.nodes.download(NODE_DTYPE, 2 * self.n - 1),
+               "counts": self.ray_counts()}
+        out["hits"] = [h.download(HIT_DTYPE, self.R) for h in self.hits]
+        out["rays"] = [r.download() for r in self.rays]
+        out["shadow_hits"] = [h.download(HIT_DTYPE, self.R) for h in self.shits]
+        out["shadow_rays"] = [r.download() for r in self.srays]
+        return out
+
+
+def bounce_seed(seed: int, level: int) -> int:
+    """Seed of the diffuse bounce that produces level `level` rays."""
+    return (seed * 0x9E3779B1 + level) & 0xFFFFFFFFFFFFFFFF

@@ -80,18 +80,18 @@ class Camera:
 
 # ------------------------------------------------------- configurations
 
-def config1_scene() -> np.ndarray:
+def sphere_scene() -> np.ndarray:
     """Icosphere of radius 1 subdivided 5 times: 20,480 triangles."""
     return icosphere(5, 1.0)
 
 
-def config1_camera(w: int = 512, h: int = 512) -> Camera:
+def sphere_camera(w: int = 512, h: int = 512) -> Camera:
     """Pinhole at distance 3 looking at the sphere, 60 degree vertical FOV."""
     return Camera(pos=(0.0, 0.0, 3.0), target=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fovy=60.0,
                   w=w, h=h)
 
 
-def config2_scene(grid: int = 1024, n_random: int = 200_000, seed: int = 7) -> np.ndarray:
+def terrain_scene(grid: int = 1024, n_random: int = 200_000, seed: int = 7) -> np.ndarray:
     """1024x1024 heightfield over [0,1]^2 (6-octave value noise, amplitude 0.2,
     lacunarity 2, gain 0.5) plus 200,000 small random triangles in the unit
     cube: 2,297,152 triangles at full size."""
@@ -100,7 +100,7 @@ def config2_scene(grid: int = 1024, n_random: int = 200_000, seed: int = 7) -> n
     return np.concatenate([hf, rnd])
 
 
-def config2_camera(w: int = 1920, h: int = 1080) -> Camera:
+def terrain_camera(w: int = 1920, h: int = 1080) -> Camera:
     """Above the heightfield looking down at ~50 degrees; ~96% of the
     pixel rays hit geometry."""
     return Camera(pos=(0.5, 0.05, 0.6), target=(0.5, 0.55, 0.0), up=(0.0, 0.0, 1.0),
@@ -108,3 +108,34 @@ def config2_camera(w: int = 1920, h: int = 1080) -> Camera:
 
 
 BOUNCE_SEED = 0x5EED_B0B0
+
+
+def instances(n_inst: int = 64, g: int = 256, seed: int = 11):
+    """Indexed mesh of the dynamic scene (rta_scene_instances): n_inst
+    heightfield patches / geodesic spheres of 2*g*g triangles each on a grid
+    of unit cells.  Returns (verts float32[3V], idx uint32[3T])."""
+    nv = C.c_int(0)
+    nt = host().rta_scene_instances(n_inst, g, seed, C.byref(nv), None, None)
+    if nt < 0:
+        raise ValueError("rta_scene_instances: bad arguments")
+    verts = np.zeros(3 * nv.value, np.float32)
+    idx = np.zeros(3 * nt, np.uint32)
+    host().rta_scene_instances(n_inst, g, seed, C.byref(nv), ptr(verts), ptr(idx))
+    return verts, idx
+
+
+def dynamic_camera(w: int = 3840, h: int = 2160, n_inst: int = 64) -> Camera:
+    """Looking across the instance grid from one side, slightly above it."""
+    side = float(int(np.ceil(np.sqrt(n_inst))))
+    return Camera(pos=(0.5 * side, -0.35 * side, 0.6 * side), target=(0.5 * side, 0.55 * side, 0.0),
+                  up=(0.0, 0.0, 1.0), fovy=50.0, w=w, h=h)
+
+
+def dynamic_light(n_inst: int = 64):
+    """Point light above one corner of the instance grid."""
+    side = float(int(np.ceil(np.sqrt(n_inst))))
+    return (0.8 * side, 0.2 * side, 0.5 * side)
+
+
+DYNAMIC_SIGMA = 1e-3
+DYNAMIC_SEED = 0xD1A_2024

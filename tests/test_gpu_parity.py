@@ -50,7 +50,7 @@ SCENES = {
     "icosphere2": lambda: scenes.icosphere(2),
     "clustered": lambda: clustered(5000, 37, seed=5),
     "random_small": lambda: scenes.random_triangles(777, seed=3),
-    "config1": scenes.config1_scene,
+    "sphere20k": scenes.sphere_scene,
     "hf_small": lambda: np.concatenate([scenes.heightfield(64, 64, seed=3),
                                         scenes.random_triangles(3000, seed=4)]),
 }
@@ -189,7 +189,7 @@ def test_radix_sort_stable(gpu, n, dup):
 
 
 def test_shirley_rays_match_host(gpu):
-    cam = scenes.config2_camera(333, 177)
+    cam = scenes.terrain_camera(333, 177)
     o, d, t = cam.host_rays()
     rg = rta_amd.ShirleyRayGenerator(cam.w, cam.h)
     rg.setup(cam.pos, cam.view_dir, cam.up, cam.fovy)
@@ -237,11 +237,11 @@ def assert_hits_equal(got, ref):
 
 
 @pytest.mark.parametrize("layout", LAYOUTS)
-def test_trace_config1_bit_exact(gpu, layout):
-    say(f"trace config1 layout={layout}")
-    tris = scenes.config1_scene()
+def test_trace_sphere20k_bit_exact(gpu, layout):
+    say(f"trace sphere20k layout={layout}")
+    tris = scenes.sphere_scene()
     ref_b = oracle.lbvh_build(tris)
-    cam = scenes.config1_camera()
+    cam = scenes.sphere_camera()
     o, d, t = cam.host_rays()
     ref, _, ref_st = oracle.trace_cis(ref_b["tris"], ref_b["nodes"], o, d, t, cam.w, cam.h,
                                       stats=True)
@@ -288,7 +288,7 @@ def test_trace_median_tree(gpu):
     say("trace median tree")
     tris = scenes.icosphere(4, jitter=0.03, seed=9)
     mb = oracle.median_build(tris)
-    cam = scenes.config1_camera(200, 200)
+    cam = scenes.sphere_camera(200, 200)
     o, d, t = cam.host_rays()
     ref, _, _ = oracle.trace_cis(mb["tris"], mb["nodes"], o, d, t, 200, 200)
     bvh = rta_amd.BinaryLbvh(DeviceBuffer.from_array(mb["tris"]), len(tris),
@@ -324,9 +324,9 @@ def test_persistent_launch_shapes(gpu, shape):
 
 def test_bounce_bit_exact(gpu):
     say("bounce")
-    tris = scenes.config1_scene()
+    tris = scenes.sphere_scene()
     ref_b = oracle.lbvh_build(tris)
-    cam = scenes.config1_camera(256, 256)
+    cam = scenes.sphere_camera(256, 256)
     o, d, t = cam.host_rays()
     ref, _, _ = oracle.trace_cis(ref_b["tris"], ref_b["nodes"], o, d, t, 256, 256)
     bo, bd, bt, bsrc = oracle.diffuse_bounce(ref, ref_b["tris"], o, d, seed=scenes.BOUNCE_SEED)
@@ -348,3 +348,143 @@ def test_bounce_bit_exact(gpu):
     ref2, _, _ = oracle.trace_cis(ref_b["tris"], ref_b["nodes"], bo, bd, bt, count)
     got2, _ = trace_gpu(bvh, go, gd, gt, "packed")
     assert_hits_equal(got2, ref2)
+
+
+# ------------------------------- shadow tracer, finite max_t, ray chains
+
+def _mixed_scene():
+    return np.concatenate([scenes.icosphere(3, jitter=0.05, seed=21),
+                           scenes.random_triangles(4000, seed=22, lo=-1.2, hi=1.2)])
+
+
+def _segment_rays(n, seed):
+    """Rays between random point pairs: finite max_t = segment length, plus
+    every fifth ray unbounded."""
+    rng = np.random.default_rng(seed)
+    o = rng.uniform(-1.5, 1.5, size=(n, 3)).astype(np.float32)
+    p = rng.uniform(-1.5, 1.5, size=(n, 3)).astype(np.float32)
+    v = (p - o).astype(np.float32)
+    ln = np.linalg.norm(v, axis=1).astype(np.float32)
+    d = (v / ln[:, None]).astype(np.float32)
+    ln[::5] = F32MAX
+    d[n // 2::9, 0] = 0.0                       # some axis-parallel rays
+    return o.reshape(-1), d.reshape(-1), ln
+
+
+def _trace_gpu_kind(bvh, o, d, t, layout, shadow):
+    rays = rta_amd.RayBuffers.from_host(o, d, t)
+    bouncer = rta_amd.GpuBouncer(len(t))
+    cls = rta_amd.CisShadowTracer if shadow else rta_amd.CisTracer
+    tracer = cls(rays, bvh, bouncer, layout=layout)
+    tracer.prepare_trace()
+    tracer.launch()
+    rta_amd.synchronize()
+    tracer.check_errors()
+    return bouncer.intersections()
+
+
+def _assert_records_equal(got, exp):
+    for f in ("t", "beta", "gamma"):
+        assert np.array_equal(bits(got[f]), bits(exp[f])), f
+    assert np.array_equal(got["ref"], exp["ref"])
+
+
+@pytest.mark.parametrize("layout", ["packed", "persistent"])
+@pytest.mark.parametrize("shadow", [False, True])
+def test_kernel_semantics_finite_max_t(gpu, layout, shadow):
+    """Closest-hit and shadow traversal against the oracle's restatement of
+    the GPU kernels, every field of every record (misses keep the reset
+    sentinel apart from t)."""
+    say(f"kernel semantics layout={layout} shadow={shadow}")
+    tris = _mixed_scene()
+    ref_b = oracle.lbvh_build(tris)
+    o, d, t = _segment_rays(40_000, seed=23)
+    bvh = rta_amd.LbvhConstructor().build(DeviceBuffer.from_array(tris), len(tris))
+    got = _trace_gpu_kind(bvh, o, d, t, layout, shadow)
+    sentinel = np.zeros(len(t), HIT_DTYPE)
+    reset = DeviceBuffer(16 * len(t))
+    call("rta_reset_intersections", ptr(reset), len(t), 1, None)
+    sentinel[:] = reset.download(HIT_DTYPE, len(t))
+    exp, deep = oracle.trace_kernel(ref_b["tris"], ref_b["nodes"], o, d, t, any_hit=shadow,
+                                    hits_in=sentinel)
+    assert deep == 0
+    hit = exp["t"] != F32MAX
+    assert 5000 < hit.sum() < len(t) - 5000
+    if layout == "persistent":
+        # a persistent miss stores t only, leaving the rest of the sentinel
+        _assert_records_equal(got, exp)
+    else:
+        _assert_records_equal(got[hit], exp[hit])
+        assert (got["t"][~hit] == F32MAX).all()
+
+
+def test_shadow_rays_bit_exact(gpu):
+    say("shadow rays")
+    tris = scenes.sphere_scene()
+    ref_b = oracle.lbvh_build(tris)
+    cam = scenes.sphere_camera(256, 256)
+    o, d, t = cam.host_rays()
+    ref, _, _ = oracle.trace_cis(ref_b["tris"], ref_b["nodes"], o, d, t, 256, 256)
+    light = (4.0, 0.0, 1.0)
+    so, sd, st, ssrc = oracle.shadow_rays(ref, ref_b["tris"], o, d, np.float32(light))
+    bvh = rta_amd.LbvhConstructor().build(DeviceBuffer.from_array(tris), len(tris))
+    rays = rta_amd.RayBuffers.from_host(o, d, t)
+    bouncer = rta_amd.GpuBouncer(len(t))
+    tracer = rta_amd.CisTracer(rays, bvh, bouncer)
+    tracer.prepare_trace()
+    tracer.launch()
+    out, src, count = rta_amd.shadow_rays(bouncer.gpu_last_intersection, bvh.triangle_data, rays,
+                                          len(t), light)
+    assert count == len(st)
+    go, gd, gt = out.download(count)
+    assert np.array_equal(src.download(np.uint32, count), ssrc)
+    assert np.array_equal(bits(go), bits(so))
+    assert np.array_equal(bits(gd), bits(sd))
+    assert np.array_equal(bits(gt), bits(st))
+    exp, _ = oracle.trace_kernel(ref_b["tris"], ref_b["nodes"], so, sd, st, any_hit=True)
+    got = _trace_gpu_kind(bvh, go, gd, gt, "persistent", shadow=True)
+    hit = exp["t"] != F32MAX
+    assert 1000 < hit.sum() < count - 1000
+    _assert_records_equal(got[hit], exp[hit])
+    assert (got["t"][~hit] == F32MAX).all()
+
+
+def test_two_bounce_chain(gpu):
+    """primary -> bounce -> trace -> bounce -> trace entirely on the GPU
+    (ray counts stay on the device) against the same chain in the oracle."""
+    say("two-bounce chain")
+    tris = np.concatenate([scenes.heightfield(96, 96, seed=31),
+                           scenes.random_triangles(5000, seed=32)])
+    ref_b = oracle.lbvh_build(tris)
+    cam = scenes.Camera(pos=(0.5, -0.3, 0.6), target=(0.5, 0.5, 0.0), up=(0, 0, 1), fovy=50.0,
+                        w=192, h=128)
+    o, d, t = cam.host_rays()
+    R = len(t)
+    bvh = rta_amd.LbvhConstructor().build(DeviceBuffer.from_array(tris), len(tris))
+    rays = rta_amd.RayBuffers.from_host(o, d, t)
+    hits = [rta_amd.GpuBouncer(R) for _ in range(3)]
+    tr = rta_amd.CisTracer(rays, bvh, hits[0])
+    tr.prepare_trace()
+    tr.launch()
+    exp0, _ = oracle.trace_kernel(ref_b["tris"], ref_b["nodes"], o, d, t)
+    cur_rays, cur_o, cur_d, cur_exp = rays, o, d, exp0
+    for k in (1, 2):
+        seed = scenes.BOUNCE_SEED + k
+        b_rays, _, count = rta_amd.diffuse_bounce(hits[k - 1].gpu_last_intersection,
+                                                  bvh.triangle_data, cur_rays, R, seed)
+        bo, bd, bt, _ = oracle.diffuse_bounce(cur_exp, ref_b["tris"], cur_o, cur_d, seed)
+        assert count == len(bt) and count > 1000
+        go, gd, gt = b_rays.download(count)
+        assert np.array_equal(bits(go), bits(bo)) and np.array_equal(bits(gd), bits(bd))
+        tr = rta_amd.CisTracer(b_rays, bvh, hits[k])
+        tr.prepare_trace()                      # resets all R records
+        tr.launch(n_rays=count)
+        rta_amd.synchronize()
+        tr.check_errors()
+        got = hits[k].intersections()
+        exp, _ = oracle.trace_kernel(ref_b["tris"], ref_b["nodes"], bo, bd, bt)
+        _assert_records_equal(got[:count], exp)
+        assert (got["t"][count:] == F32MAX).all()
+        cur_rays, cur_o, cur_d = b_rays, go, gd
+        cur_exp = np.concatenate([exp, got[count:]])
+        cur_exp["t"][count:] = F32MAX

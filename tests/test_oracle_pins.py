@@ -215,8 +215,8 @@ def test_lbvh_duplicate_codes_collapse():
     assert leaves == [(int(s), int(e - s)) for s, e in zip(starts, ends)]
 
 
-def test_lbvh_icosphere_config1():
-    _check_lbvh_against_pins(scenes.config1_scene())
+def test_lbvh_icosphere_20k():
+    _check_lbvh_against_pins(scenes.sphere_scene())
 
 
 def test_lbvh_edge_sizes():
@@ -312,7 +312,7 @@ def test_cis_matches_brute_force():
 
 def test_cis_lbvh_vs_median_tree():
     tris = scenes.icosphere(4)
-    cam = scenes.config1_camera(96, 96)
+    cam = scenes.sphere_camera(96, 96)
     o, d, t = cam.host_rays()
     lb = oracle.lbvh_build(tris)
     mb = oracle.median_build(tris)
@@ -329,7 +329,7 @@ def test_cis_lbvh_vs_median_tree():
 def test_dis_equals_cis_hits():
     tris = scenes.icosphere(3)
     out = oracle.lbvh_build(tris)
-    cam = scenes.config1_camera(64, 64)
+    cam = scenes.sphere_camera(64, 64)
     o, d, t = cam.host_rays()
     h1, _, _ = oracle.trace_cis(out["tris"], out["nodes"], o, d, t, 64, 64)
     h2, _ = oracle.trace_dis(out["tris"], out["nodes"], o, d, t, 64, 64)
@@ -349,7 +349,7 @@ def test_cis_stats_counts():
 def test_cis_finite_max_t():
     tris = scenes.icosphere(3)
     out = oracle.lbvh_build(tris)
-    cam = scenes.config1_camera(32, 32)
+    cam = scenes.sphere_camera(32, 32)
     o, d, t = cam.host_rays()
     tm = np.full_like(t, 2.5)                  # the sphere front is at t ~ 2
     h, _, _ = oracle.trace_cis(out["tris"], out["nodes"], o, d, tm, 32, 32)
@@ -368,7 +368,7 @@ def test_cis_finite_max_t():
 def test_bounce_properties():
     tris = scenes.icosphere(3)
     out = oracle.lbvh_build(tris)
-    cam = scenes.config1_camera(64, 64)
+    cam = scenes.sphere_camera(64, 64)
     o, d, t = cam.host_rays()
     h, _, _ = oracle.trace_cis(out["tris"], out["nodes"], o, d, t, 64, 64)
     bo, bd, bt, src = oracle.diffuse_bounce(h, out["tris"], o, d, seed=3)
@@ -387,3 +387,122 @@ def test_bounce_properties():
     assert (cos > -1e-6).all()
     assert abs(cos.mean() - 2.0 / 3.0) < 0.02
     assert (bt == F32MAX).all()
+
+
+# ------------------------------------------- GPU-kernel semantics, shadow
+
+def test_kernel_semantics_equal_cpu_tracer():
+    """trace_kernel restates the GPU kernel (push the far child, enter the
+    near one); trace_cis the CPU plugin (push both).  For max_t = FLT_MAX
+    both must give identical hit records: two independent traversal codes."""
+    tris = np.concatenate([scenes.icosphere(3, jitter=0.05, seed=5),
+                           scenes.random_triangles(3000, seed=6, lo=-1.2, hi=1.2)])
+    out = oracle.lbvh_build(tris)
+    o, d, t = _rays_at(out["tris"], 3000, seed=12)
+    cpu, _, _ = oracle.trace_cis(out["tris"], out["nodes"], o, d, t, len(t))
+    gpu, deep = oracle.trace_kernel(out["tris"], out["nodes"], o, d, t)
+    assert deep == 0
+    hit = cpu["t"] != F32MAX
+    assert hit.sum() > 1000
+    assert np.array_equal(hit, gpu["t"] != F32MAX)
+    for f in ("t", "beta", "gamma", "ref"):
+        assert np.array_equal(cpu[f][hit], gpu[f][hit])
+    # a GPU miss keeps the incoming (reset) record apart from t
+    assert (gpu["beta"][~hit] == 1.0).all() and (gpu["gamma"][~hit] == 2.0).all()
+
+
+def test_shadow_is_first_accepted_triangle():
+    """The shadow tracer runs the closest-hit traversal up to its first
+    accepted triangle: with max_t = FLT_MAX its record equals the first
+    record the CPU closest-hit tracer accepts (recorded by separate code)."""
+    tris = np.concatenate([scenes.icosphere(3, jitter=0.05, seed=7),
+                           scenes.random_triangles(3000, seed=8, lo=-1.2, hi=1.2)])
+    out = oracle.lbvh_build(tris)
+    o, d, t = _rays_at(out["tris"], 3000, seed=13)
+    _, _, _, first = oracle.trace_cis(out["tris"], out["nodes"], o, d, t, len(t), first=True)
+    closest, _ = oracle.trace_kernel(out["tris"], out["nodes"], o, d, t)
+    sh, deep = oracle.trace_kernel(out["tris"], out["nodes"], o, d, t, any_hit=True)
+    assert deep == 0
+    hit = first["t"] != F32MAX
+    assert np.array_equal(hit, sh["t"] != F32MAX)
+    for f in ("t", "beta", "gamma", "ref"):
+        assert np.array_equal(first[f][hit], sh[f][hit])
+    # near-first order makes the first hit the closest on most rays; the
+    # check above is only meaningful because some rays differ
+    assert (sh["ref"][hit] != closest["ref"][hit]).sum() >= 10
+    assert (sh["t"][hit] >= closest["t"][hit]).all()
+
+
+def test_shadow_finite_max_t():
+    """Occlusion within max_t: the shadow tracer reports a hit exactly when
+    the closest-hit kernel finds one within max_t; the reported triangle
+    re-intersects to the same record and lies within max_t."""
+    tris = np.concatenate([scenes.icosphere(3, jitter=0.05, seed=9),
+                           scenes.random_triangles(3000, seed=10, lo=-1.2, hi=1.2)])
+    out = oracle.lbvh_build(tris)
+    o, d, _ = _rays_at(out["tris"], 3000, seed=14)
+    rng = np.random.default_rng(15)
+    tm = rng.uniform(0.2, 3.0, len(o) // 3).astype(np.float32)
+    closest, _ = oracle.trace_kernel(out["tris"], out["nodes"], o, d, tm)
+    sh, _ = oracle.trace_kernel(out["tris"], out["nodes"], o, d, tm, any_hit=True)
+    hit = sh["t"] != F32MAX
+    assert 500 < hit.sum() < len(tm) - 500
+    assert np.array_equal(hit, closest["t"] != F32MAX)
+    assert (sh["t"][hit] <= tm[hit]).all()
+    for r in np.flatnonzero(hit)[:300]:
+        ok, tt, be, ga = oracle.intersect_tri(out["tris"][sh["ref"][r]], o[3 * r:3 * r + 3],
+                                              d[3 * r:3 * r + 3])
+        assert ok and (tt, be, ga) == (sh["t"][r], sh["beta"][r], sh["gamma"][r])
+
+
+def test_shadow_against_brute_force():
+    """Occlusion agrees with a brute-force search for any triangle within
+    max_t (up to box-boundary grazing, as for closest hits)."""
+    tris = scenes.random_triangles(400, seed=16, lo=-1.0, hi=1.0, edge_min=0.05,
+                                   edge_max=0.4)
+    out = oracle.lbvh_build(tris)
+    rng = np.random.default_rng(17)
+    n = 4000
+    o = rng.uniform(-1.5, 1.5, size=(n, 3)).astype(np.float32)
+    p = rng.uniform(-1.5, 1.5, size=(n, 3)).astype(np.float32)
+    v = (p - o).astype(np.float32)
+    ln = np.linalg.norm(v, axis=1).astype(np.float32)
+    d = (v / ln[:, None]).astype(np.float32)
+    sh, _ = oracle.trace_kernel(out["tris"], out["nodes"], o.reshape(-1), d.reshape(-1), ln,
+                                any_hit=True)
+    brute = oracle.trace_brute(out["tris"], o.reshape(-1), d.reshape(-1))
+    occl_b = brute["t"] <= ln
+    occl_s = sh["t"] != F32MAX
+    assert 500 < occl_b.sum() < n - 500
+    assert (occl_b == occl_s).mean() > 0.999
+
+
+def test_shadow_ray_properties():
+    tris = scenes.icosphere(3)
+    out = oracle.lbvh_build(tris)
+    cam = scenes.sphere_camera(64, 64)
+    o, d, t = cam.host_rays()
+    h, _, _ = oracle.trace_cis(out["tris"], out["nodes"], o, d, t, 64, 64)
+    light = np.array([4.0, 0.0, 1.0], np.float32)
+    so, sd, st, src = oracle.shadow_rays(h, out["tris"], o, d, light)
+    bo, _, _, bsrc = oracle.diffuse_bounce(h, out["tris"], o, d, seed=3)
+    hit = np.flatnonzero(h["t"] != F32MAX)
+    assert np.array_equal(src, hit.astype(np.uint32)) and np.array_equal(src, bsrc)
+    # same offset origin as the bounce rays, unit directions reaching the light
+    assert np.array_equal(so, bo)
+    so, sd = so.reshape(-1, 3).astype(np.float64), sd.reshape(-1, 3).astype(np.float64)
+    assert np.allclose(np.linalg.norm(sd, axis=1), 1.0, atol=1e-6)
+    end = so + st[:, None].astype(np.float64) * sd
+    assert np.allclose(end, light.astype(np.float64), atol=2e-6 * 6)
+    # rays from the lit side of a convex sphere are unoccluded, the rest hit it
+    sh, _ = oracle.trace_kernel(out["tris"], out["nodes"], so.astype(np.float32).reshape(-1),
+                                sd.astype(np.float32).reshape(-1), st, any_hit=True)
+    # cosine between the sphere normal and the light direction; facets of the
+    # 3-times subdivided icosphere tilt by a few degrees from it
+    nrm = so / np.linalg.norm(so, axis=1, keepdims=True)
+    facing = (nrm * sd).sum(axis=1)
+    lit = facing > 0.2
+    dark = facing < -0.2
+    assert lit.sum() > 100 and dark.sum() > 100
+    assert (sh["t"][lit] == F32MAX).all()
+    assert (sh["t"][dark] != F32MAX).all()

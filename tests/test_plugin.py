@@ -45,7 +45,9 @@ def test_plugin_option_parsing():
     assert parse("--trav", "packed", "--block", "64", "--refill", "8", "--steps", "4",
                  "--leaf-min", "12") == 0
     assert parse("-b", "median") != 0        # no CPU-built tree in this plugin
-    assert parse("-s") != 0                  # any-hit tracer not provided
+    assert parse("-s") == 0                  # shadow tracer (persistent layout)
+    assert parse("-s", "--trav", "plain") != 0   # no shadow form of the plain kernel
+    assert parse("-A") != 0                  # Aila's box variant is not provided
     assert parse("--bogus") != 0
 
 
@@ -69,9 +71,9 @@ def test_driver_frame_matches_oracle(gpu, tmp_path, trav):
     assert r.returncode == 0, r.stdout + r.stderr
     assert "average rps per frame:" in r.stdout
     got = np.fromfile(dump, dtype=abi.HIT_DTYPE)
-    tris = scenes.config1_scene()
+    tris = scenes.sphere_scene()
     ref_b = oracle.lbvh_build(tris)
-    o, d, t = scenes.config1_camera(256, 256).host_rays()
+    o, d, t = scenes.sphere_camera(256, 256).host_rays()
     ref, _, _ = oracle.trace_cis(ref_b["tris"], ref_b["nodes"], o, d, t, 256, 256)
     hit = ref["t"] != F32MAX
     assert hit.mean() > 0.2
@@ -83,7 +85,7 @@ def test_driver_frame_matches_oracle(gpu, tmp_path, trav):
 @pytest.mark.gpu
 def test_driver_rebuild_and_dump(gpu, tmp_path):
     dump = str(tmp_path / "tree.bvh")
-    r = run_driver("-r", "320x180", "--scene", "config2:64:500", "--frames", "2", "--rebuild",
+    r = run_driver("-r", "320x180", "--scene", "terrain:64:500", "--frames", "2", "--rebuild",
                    "--dump-bvh", dump, "bbvh-hip", "--", "-v")
     assert r.returncode == 0, r.stdout + r.stderr
     assert "LBVH build of" in r.stdout
@@ -91,3 +93,24 @@ def test_driver_rebuild_and_dump(gpu, tmp_path):
     n = 2 * 64 * 64 + 500
     assert lines[0] == "bvh" and int(lines[1]) == 2 * n - 1
     assert lines[2].split()[1] == "I"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("trav", ["packed", "persistent"])
+def test_driver_shadow_tracer(gpu, tmp_path, trav):
+    """`-s` selects the shadow tracer: its records are the oracle's
+    restatement of trace_shadow_cis on the same tree and rays."""
+    dump = str(tmp_path / "hits.bin")
+    r = run_driver("-r", "256x256", "--frames", "1", "--dump-hits", dump, "bbvh-hip", "--",
+                   "-b", "lbvh", "-s", "--trav", trav)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = np.fromfile(dump, dtype=abi.HIT_DTYPE)
+    tris = scenes.sphere_scene()
+    ref_b = oracle.lbvh_build(tris)
+    o, d, t = scenes.sphere_camera(256, 256).host_rays()
+    ref, _ = oracle.trace_kernel(ref_b["tris"], ref_b["nodes"], o, d, t, any_hit=True)
+    hit = ref["t"] != F32MAX
+    assert hit.mean() > 0.2
+    assert np.array_equal(got["t"] != F32MAX, hit)
+    for f in ("t", "beta", "gamma", "ref"):
+        assert np.array_equal(got[f][hit], ref[f][hit])
