@@ -76,6 +76,7 @@ typedef struct rta_lbvh_times {
 typedef void *rta_stream;
 typedef void *rta_event;
 typedef void *rta_graph;
+typedef void *rta_batch;
 
 enum {
 	RTA_OK = 0,
@@ -314,6 +315,57 @@ int rta_trace_shadow_cis_persistent(const float *tri4,
                                     rta_hit *hits, uint32_t *err,
                                     uint32_t *queue,
                                     const rta_trace_config *cfg, rta_stream s);
+
+/* ------------------------------------------------ batches of many meshes */
+
+/* A batch: up to 256 independent meshes (1 .. 2^24-1 triangles each), each
+ * with its own rays, built and traced together: every stage of the LBVH
+ * build is one launch over all meshes and all rays are traced by one
+ * persistent launch.  No counterpart in the reference (one mesh per
+ * plugin instance); every mesh's results equal a separate build/trace of it.
+ *
+ * Layouts (mesh m, offsets from rta_batch_offsets):
+ *   triangles  concatenated, mesh m at tri_off[m]; the build reorders each
+ *              mesh's range in place, as rta_lbvh_build does;
+ *   nodes      mesh m's 2n-1 nodes at node_off[m], with child and triangle
+ *              indices local to the mesh (exactly rta_lbvh_build's array);
+ *   indices    mesh m's sorted->original map at tri_off[m], local;
+ *   pairs      mesh m's pair records at node_off[m]; tri4 records at
+ *              3*tri_off[m] (float4 units); flags one word per mesh;
+ *   rays/hits  mesh m's rays at ray_off[m]; hit refs local to the mesh.
+ * rta_batch_create copies the size tables to the device (synchronously);
+ * n_rays may be NULL (no rays). */
+int rta_batch_create(int n_meshes, const int *n_tris, const int *n_rays, rta_batch *batch);
+int rta_batch_destroy(rta_batch batch);
+int rta_batch_sizes(rta_batch batch, long long *total_tris, long long *total_nodes,
+                    long long *total_rays);
+int rta_batch_offsets(rta_batch batch, uint32_t *tri_off, uint32_t *node_off,
+                      uint32_t *ray_off);
+int rta_batch_workspace_bytes(rta_batch batch, size_t *bytes);
+/* All seven build stages for every mesh (status as for rta_lbvh_build). */
+int rta_batch_build(rta_batch batch, rta_triangle *tris, rta_node *nodes, uint32_t *indices,
+                    void *workspace, size_t workspace_bytes, uint32_t *status, rta_stream s);
+/* rta_bvh_pack of every mesh; also records each mesh's root word. */
+int rta_batch_pack(rta_batch batch, rta_node_pair *pairs, float *tri4, uint32_t *flags,
+                   const rta_node *nodes, const rta_triangle *tris, rta_stream s);
+/* Closest-hit (any_hit = 0) or shadow (any_hit = 1) traversal of all rays
+ * of all meshes in one persistent launch; results as the single-mesh
+ * persistent forms.  Reset the hits first. */
+int rta_batch_trace_persistent(rta_batch batch, const float *tri4, const rta_node_pair *pairs,
+                               const uint32_t *flags, const float *ray_orig, const float *ray_dir,
+                               const float *max_t, rta_hit *hits, uint32_t *err, uint32_t *queue,
+                               int any_hit, const rta_trace_config *cfg, rta_stream s);
+/* Autotuner of the persistent launch shape on this batch: coordinate
+ * descent over leaf_min, refill_min (the queue chunk a wave takes), steps
+ * and block, `rounds` rounds, each candidate timed as the best of `reps`
+ * launches (synchronous).  *best: the starting shape (block 0 = defaults);
+ * on return the fastest found and *best_ms its time.  The hits of the last
+ * launch are valid results. */
+int rta_batch_autotune(rta_batch batch, const float *tri4, const rta_node_pair *pairs,
+                       const uint32_t *flags, const float *ray_orig, const float *ray_dir,
+                       const float *max_t, rta_hit *hits, uint32_t *err, uint32_t *queue,
+                       int any_hit, int rounds, int reps, rta_trace_config *best,
+                       float *best_ms, rta_stream s);
 
 /* -------------------------------------------------------------- bouncing */
 

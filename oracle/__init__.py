@@ -50,7 +50,7 @@ def lib() -> C.CDLL:
             "oracle_median_build": (i, [vp, i, vp, vp, vp]),
             "oracle_trace_cis": (d, [vp, vp, vp, vp, vp, i, i, vp, vp, vp, i]),
             "oracle_trace_dis": (d, [vp, vp, vp, vp, vp, i, i, vp, i]),
-            "oracle_trace_kernel": (i, [vp, vp, vp, vp, vp, i, vp, i, i]),
+            "oracle_trace_kernel": (i, [vp, vp, vp, vp, vp, i, vp, i, vp, vp, i]),
             "oracle_trace_brute": (None, [vp, i, vp, vp, i, vp]),
             "oracle_diffuse_bounce": (i, [vp, vp, vp, vp, i, u64, vp, vp, vp, vp]),
             "oracle_shadow_rays": (i, [vp, vp, vp, vp, i, vp, vp, vp, vp, vp]),
@@ -138,10 +138,13 @@ def trace_cis(tris, nodes, o, d, t, w, h=1, stats=False, threads=0, first=False)
     return (hits, ms, st, fr) if first else (hits, ms, st)
 
 
-def trace_kernel(tris, nodes, o, d, t, any_hit=False, hits_in=None, threads=0):
+def trace_kernel(tris, nodes, o, d, t, any_hit=False, hits_in=None, threads=0, bound=None,
+                 stats=False):
     """The GPU tracers' semantics (kernels.cu:777-855, shadow :857-939).
     hits_in: the records the rays start from (default: the reset sentinel).
-    Returns (hits, rays that needed more than 32 stack entries)."""
+    bound: optional per-ray inclusive bound on child-box entry distances.
+    Returns (hits, rays that needed more than 32 stack entries) and, with
+    stats=True, per-ray (inner-node visits, triangle tests)."""
     n = len(t)
     if hits_in is None:
         hits = np.zeros(n, HIT_DTYPE)
@@ -150,8 +153,12 @@ def trace_kernel(tris, nodes, o, d, t, any_hit=False, hits_in=None, threads=0):
         hits = np.array(hits_in, dtype=HIT_DTYPE, copy=True)
     tris, nodes = np.ascontiguousarray(tris), np.ascontiguousarray(nodes)
     o, d, t = _f32(o), _f32(d), _f32(t)
+    b = None if bound is None else _f32(bound)
+    st = np.zeros(2 * n, np.uint32) if stats else None
     deep = lib().oracle_trace_kernel(_p(tris), _p(nodes), _p(o), _p(d), _p(t), n, _p(hits),
-                                     int(bool(any_hit)), threads)
+                                     int(bool(any_hit)), _p(b), _p(st), threads)
+    if stats:
+        return hits, int(deep), st.reshape(-1, 2)
     return hits, int(deep)
 
 

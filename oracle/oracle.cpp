@@ -629,7 +629,7 @@ double oracle_trace_dis(const rta_triangle *tris, const rta_node *nodes, const f
 // counted in the return value.  hits: in/out.
 int oracle_trace_kernel(const rta_triangle *tris, const rta_node *nodes, const float *ro,
                         const float *rd, const float *rmax, int n_rays, rta_hit *hits,
-                        int any_hit, int threads) {
+                        int any_hit, const float *bound, uint32_t *stats, int threads) {
 	if (threads > 0) omp_set_num_threads(threads);
 	int overflow = 0;
 #pragma omp parallel for schedule(dynamic, 64) reduction(+ : overflow)
@@ -637,6 +637,10 @@ int oracle_trace_kernel(const rta_triangle *tris, const rta_node *nodes, const f
 		uint32_t stack[256];
 		const float *origin = ro + 3 * (size_t)id, *dir = rd + 3 * (size_t)id;
 		const float t_max = rmax[id];
+		// optional extra inclusive bound on child boxes (no reference
+		// counterpart: the seeded traversal's prune, dist > bound)
+		const float b = bound ? bound[id] : FLT_MAX;
+		uint32_t n_inner = 0, n_tests = 0;
 		rta_hit closest = hits[id];
 		closest.t = FLT_MAX;
 		int sp = -1;
@@ -644,12 +648,15 @@ int oracle_trace_kernel(const rta_triangle *tris, const rta_node *nodes, const f
 		const rta_node *curr = &nodes[0];
 		while (true) {
 			if (curr->type_left_elems & 1u) {
+				++n_inner;
 				const uint32_t l = curr->type_left_elems >> 1, r = curr->right_tris;
 				float dist_l = FLT_MAX, dist_r = FLT_MAX;
 				bool hit_l = intersect_aabb(nodes[l].box, origin, dir, dist_l);
 				bool hit_r = intersect_aabb(nodes[r].box, origin, dir, dist_r);
 				if (dist_l >= closest.t || dist_l > t_max) hit_l = false;
 				if (dist_r >= closest.t || dist_r > t_max) hit_r = false;
+				if (dist_l > b) hit_l = false;
+				if (dist_r > b) hit_r = false;
 				if (hit_l && hit_r) {
 					if (sp + 1 >= 32) deep = true;
 					if (sp + 1 >= 256) break;  // unreachable for trees of depth < 256
@@ -673,6 +680,7 @@ int oracle_trace_kernel(const rta_triangle *tris, const rta_node *nodes, const f
 				const uint32_t elems = curr->type_left_elems >> 1, offset = curr->right_tris;
 				for (uint32_t i = 0; i < elems; ++i) {
 					float t, be, ga;
+					++n_tests;
 					if (intersect_tri_opt(tris[offset + i], origin, dir, t, be, ga)) {
 						if (t < closest.t && t <= t_max) {
 							closest.t = t;
@@ -691,6 +699,10 @@ int oracle_trace_kernel(const rta_triangle *tris, const rta_node *nodes, const f
 			}
 		}
 		hits[id] = closest;
+		if (stats) {
+			stats[2 * (size_t)id] = n_inner;
+			stats[2 * (size_t)id + 1] = n_tests;
+		}
 		if (deep) ++overflow;
 	}
 	return overflow;

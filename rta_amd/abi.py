@@ -127,6 +127,18 @@ HIP_SIGNATURES = {
     "rta_bounce_work_bytes": [_i, _psz],
     "rta_diffuse_bounce": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _u64, _vp, _sz, _vp],
     "rta_shadow_rays": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp, _sz, _vp],
+    "rta_batch_create": [_i, _vp, _vp, C.POINTER(_vp)],
+    "rta_batch_destroy": [_vp],
+    "rta_batch_sizes": [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+                        C.POINTER(C.c_longlong)],
+    "rta_batch_offsets": [_vp, _vp, _vp, _vp],
+    "rta_batch_workspace_bytes": [_vp, _psz],
+    "rta_batch_build": [_vp, _vp, _vp, _vp, _vp, _sz, _vp, _vp],
+    "rta_batch_pack": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "rta_batch_trace_persistent": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp,
+                                   _vp],
+    "rta_batch_autotune": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _vp,
+                           _pf, _vp],
 }
 
 HOST_SIGNATURES = {

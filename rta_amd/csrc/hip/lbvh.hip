@@ -2,265 +2,48 @@
 // (bbvh-cuda/lbvh.h:153-211, bbvh-cuda/lbvh-kernels.cu) as hand-written HIP
 // kernels.  Results are bit-identical to the reference's semantics; where
 // the reference is racy the behaviour chosen here is deterministic and
-// documented at the kernel.
-#include "common.hpp"
+// documented at the stage body (lbvh_body.hpp).  This file holds the
+// single-mesh kernels and entry points; batch.hip runs the same bodies for
+// many meshes per launch.
+#include "lbvh_body.hpp"
 
 #include <cstdio>
 
 namespace rta_hip {
 
-constexpr int kInitBlock = 256;
-constexpr int kTriWords = sizeof(rta_triangle) / 8;   // 13 x 8 bytes
-constexpr float kBoundsInit = 100000000000.0f;        // lbvh-kernels.cu:519
-
-// ---------------------------------------------------------- wave helpers
-
-__device__ inline float wave_min(float v) {
-	for (int off = kWave / 2; off > 0; off >>= 1) v = sel_min(v, __shfl_xor(v, off));
-	return v;
-}
-__device__ inline float wave_max(float v) {
-	for (int off = kWave / 2; off > 0; off >>= 1) v = sel_max(v, __shfl_xor(v, off));
-	return v;
-}
-
-// ------------------------------------------------ stage 1: initTriangles
-
-// One block handles 256 consecutive triangles (26,624 contiguous bytes):
-// a coalesced 8-byte-word copy src -> scratch staged through LDS, then each
-// thread forms its triangle's box (lbvh-kernels.cu:141-153) and centroid
-// ((a+b)+c)/3.0f (lbvh-kernels.cu:60-63) from LDS, and the block reduces
-// its boxes into one partial scene box.
+// One block per 256 triangles; its partial scene box goes to
+// partials[blockIdx.x].
 __global__ __launch_bounds__(kInitBlock) void init_triangles_kernel(
     rta_triangle *__restrict__ scratch, const rta_triangle *__restrict__ src,
     rta_aabb *__restrict__ boxes, rta_float3 *__restrict__ centers,
     rta_aabb *__restrict__ partials, int n) {
-	__shared__ uint2 words[kInitBlock * kTriWords];
-	__shared__ float red[6][kInitBlock / kWave];
-
-	const int tid = threadIdx.x;
-	const long long base = (long long)blockIdx.x * kInitBlock;
-	const int count = (int)((n - base) < kInitBlock ? (n - base) : kInitBlock);
-	const int nwords = count * kTriWords;
-	const uint2 *gsrc = reinterpret_cast<const uint2 *>(src + base);
-	uint2 *gdst = reinterpret_cast<uint2 *>(scratch + base);
-	for (int w = tid; w < nwords; w += kInitBlock) {
-		uint2 v = gsrc[w];
-		words[w] = v;
-		gdst[w] = v;
-	}
-	__syncthreads();
-
-	float mn[3] = {kBoundsInit, kBoundsInit, kBoundsInit};
-	float mx[3] = {-kBoundsInit, -kBoundsInit, -kBoundsInit};
-	if (tid < count) {
-		const float *f = reinterpret_cast<const float *>(&words[tid * kTriWords]);
-		float lo[3], hi[3], ce[3];
-		for (int c = 0; c < 3; ++c) {
-			float va = f[c], vb = f[3 + c], vc = f[6 + c];
-			lo[c] = sel_min(sel_min(va, vb), vc);
-			hi[c] = sel_max(sel_max(va, vb), vc);
-			ce[c] = ((va + vb) + vc) / 3.0f;
-			mn[c] = lo[c];
-			mx[c] = hi[c];
-		}
-		rta_aabb b;
-		b.min = {lo[0], lo[1], lo[2]};
-		b.max = {hi[0], hi[1], hi[2]};
-		boxes[base + tid] = b;
-		centers[base + tid] = {ce[0], ce[1], ce[2]};
-	}
-	// block reduction of the boxes (min/max are exact; the order only
-	// matters for the sign of a zero, which cannot change a Morton code)
-	const int lane = tid % kWave, wv = tid / kWave;
-	for (int c = 0; c < 3; ++c) {
-		float a = wave_min(mn[c]);
-		float b = wave_max(mx[c]);
-		if (lane == 0) { red[c][wv] = a; red[3 + c][wv] = b; }
-	}
-	__syncthreads();
-	if (tid == 0) {
-		float r[6];
-		for (int c = 0; c < 3; ++c) { r[c] = kBoundsInit; r[3 + c] = -kBoundsInit; }
-		for (int w = 0; w < kInitBlock / kWave; ++w)
-			for (int c = 0; c < 3; ++c) {
-				r[c] = sel_min(r[c], red[c][w]);
-				r[3 + c] = sel_max(r[3 + c], red[3 + c][w]);
-			}
-		rta_aabb p;
-		p.min = {r[0], r[1], r[2]};
-		p.max = {r[3], r[4], r[5]};
-		partials[blockIdx.x] = p;
-	}
+	init_triangles_block(scratch, src, boxes, centers, partials + blockIdx.x, n, blockIdx.x);
 }
-
-// ----------------------------------------------- stage 2: voxel bounds
-
-constexpr int kBoundsBlock = 1024;
 
 __global__ __launch_bounds__(kBoundsBlock) void voxel_bounds_kernel(
     rta_aabb *__restrict__ bounds, const rta_aabb *__restrict__ partials, int np) {
-	__shared__ float red[6][kBoundsBlock / kWave];
-	float mn[3] = {kBoundsInit, kBoundsInit, kBoundsInit};
-	float mx[3] = {-kBoundsInit, -kBoundsInit, -kBoundsInit};
-	for (int i = threadIdx.x; i < np; i += kBoundsBlock) {
-		rta_aabb p = partials[i];
-		mn[0] = sel_min(mn[0], p.min.x); mn[1] = sel_min(mn[1], p.min.y); mn[2] = sel_min(mn[2], p.min.z);
-		mx[0] = sel_max(mx[0], p.max.x); mx[1] = sel_max(mx[1], p.max.y); mx[2] = sel_max(mx[2], p.max.z);
-	}
-	const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
-	for (int c = 0; c < 3; ++c) {
-		float a = wave_min(mn[c]);
-		float b = wave_max(mx[c]);
-		if (lane == 0) { red[c][wv] = a; red[3 + c][wv] = b; }
-	}
-	__syncthreads();
-	if (threadIdx.x == 0) {
-		float r[6];
-		for (int c = 0; c < 3; ++c) { r[c] = kBoundsInit; r[3 + c] = -kBoundsInit; }
-		for (int w = 0; w < kBoundsBlock / kWave; ++w)
-			for (int c = 0; c < 3; ++c) {
-				r[c] = sel_min(r[c], red[c][w]);
-				r[3 + c] = sel_max(r[3 + c], red[3 + c][w]);
-			}
-		rta_aabb b;
-		b.min = {r[0], r[1], r[2]};
-		b.max = {r[3], r[4], r[5]};
-		*bounds = b;
-	}
+	voxel_bounds_block(bounds, partials, np);
 }
 
-// ------------------------------------------------ stage 3: Morton codes
-
-// d = GRIDSIZE/(max-min) is formed per thread with the same correctly
-// rounded division the reference does once on the host
-// (lbvh-kernels.cu:414-417), so every thread sees the identical d.
 __global__ __launch_bounds__(256) void morton_kernel(
     uint32_t *__restrict__ codes, uint32_t *__restrict__ indices,
     const rta_float3 *__restrict__ centers, const rta_aabb *__restrict__ bounds,
     uint32_t *__restrict__ status, int n) {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n) return;
-	const rta_aabb vb = *bounds;
-	const float dx = 1024.0f / (vb.max.x - vb.min.x);
-	const float dy = 1024.0f / (vb.max.y - vb.min.y);
-	const float dz = 1024.0f / (vb.max.z - vb.min.z);
-	const rta_float3 c = centers[i];
-	const uint32_t x = f2u_rz_sat((c.x - vb.min.x) * dx);
-	const uint32_t y = f2u_rz_sat((c.y - vb.min.y) * dy);
-	const uint32_t z = f2u_rz_sat((c.z - vb.min.z) * dz);
-	const uint32_t code = morton_encode_10bit(x, y, z);
+	const uint32_t code = morton_of(centers, bounds, i);
 	codes[i] = code;
 	indices[i] = (uint32_t)i;
 	if (code == 0u && status) atomicOr(status, (uint32_t)RTA_LBVH_CODE_ZERO);
 }
 
-// ----------------------------------------- stage 5: Karras hierarchy
-
-// Common prefix length of the augmented keys (code, position):
-// clz(ci^cj), or 32+clz(i^j) for equal codes, -1 outside [0,n)
-// (__cpl, lbvh-kernels.cu:84-92).
-__device__ inline int cpl(int i, int j, const uint32_t *__restrict__ codes, int n) {
-	if (j < 0 || j >= n) return -1;
-	const uint32_t ci = codes[i], cj = codes[j];
-	if (ci == cj) return __clz((int)((uint32_t)i ^ (uint32_t)j)) + 32;
-	return __clz((int)(ci ^ cj));
-}
-
-// Thread per node (k::build_tree_karras, lbvh-kernels.cu:186-256).  The
-// direction/range/split searches are the reference's, including its
-// ceil-halving split search that always runs in +1 direction from the
-// lower end of the range.  Inner nodes also clear their visit counter and
-// record their sorted range [from, to] for the bottom-up pass.
 __global__ __launch_bounds__(256) void karras_kernel(
     rta_node *__restrict__ nodes, uint32_t *__restrict__ parents,
     uint32_t *__restrict__ visit, uint2 *__restrict__ ranges,
     const uint32_t *__restrict__ codes, int n) {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= 2 * n - 1) return;
-	uint2 *links = reinterpret_cast<uint2 *>(&nodes[i]);
-	if (i >= n - 1) {
-		// leaf: tris = i-(n-1), elems = 1, leaf bit clear
-		*links = make_uint2(1u << 1, (uint32_t)(i - (n - 1)));
-		return;
-	}
-	int d = (cpl(i, i + 1, codes, n) - cpl(i, i - 1, codes, n)) >= 0 ? 1 : -1;
-	const int cpl_min = cpl(i, i - d, codes, n);
-	int l_max = 2;
-	while (cpl(i, i + l_max * d, codes, n) > cpl_min) l_max *= 2;
-	int l = 0;
-	for (int t = l_max / 2; t >= 1; t /= 2)
-		if (cpl(i, i + (l + t) * d, codes, n) > cpl_min) l += t;
-	const int j = i + l * d;
-	const int from = j < i ? j : i;
-	const int to = j < i ? i : j;
-
-	const int cpl_node = cpl(from, to, codes, n);
-	int s = 0;
-	for (int t = (l / 2 + l % 2); t >= 1; t = (t / 2 + t % 2)) {
-		if (cpl(from, from + (s + t), codes, n) > cpl_node) s += t;
-		if (t <= 1) break;
-	}
-	const int mid = from + s;
-	int left = mid, right = mid + 1;
-	if (from == mid) left += n - 1;
-	if (to == mid + 1) right += n - 1;
-	*links = make_uint2(((uint32_t)left << 1) | 1u, (uint32_t)right);
-	parents[left] = (uint32_t)i;
-	parents[right] = (uint32_t)i;
-	visit[i] = 0u;
-	ranges[i] = make_uint2((uint32_t)from, (uint32_t)to);
-}
-
-// ----------------------------------------- stage 6: bottom-up boxes
-
-// Bottom-up boxes (k::build_bounding_boxes_karras2, lbvh-kernels.cu:258-341):
-// every leaf climbs towards the root; at each inner node the first arriving
-// child stops and the second forms the union and continues.  Deterministic
-// form of the reference:
-//  * a separate visit counter decides first/second arrival (the reference
-//    sums representative codes into the flag, so a first arrival with code
-//    0 also stops the second thread, lbvh-kernels.cu:288-291);
-//  * the union is MIN(left, right) / MAX(left, right) with the reference's
-//    `(a<b)?a:b`, fixing the sign of zero that the reference leaves to
-//    arrival order;
-//  * merge rule: both children are leaves and their representative codes
-//    (code of the leaf, or codes[i] for inner node i, :261) are equal
-//    (`code == old` in the reference) -> the parent becomes one leaf with
-//    tris = min, elems = sum (lbvh-kernels.cu:302-330).
-//
-// Hierarchical schedule.  A subtree covers a contiguous range of sorted
-// positions, so all communication inside a chunk of consecutive leaves
-// stays in one workgroup:
-//  level 1: a 1024-thread block owns 1024 leaves; counters, boxes and leaf
-//           state live in LDS; a climb stops when the parent's range leaves
-//           the chunk and the child is recorded in the block's frontier
-//           slots (at most 2 per tree level, so <= 126 for depth <= 63);
-//  level k: one block merges the frontiers of kMergeFan level-(k-1) chunks
-//           with workgroup-scope atomics on the global visit counters,
-//           until one block covers all leaves and reaches the root.
-// No step needs device-scope cache maintenance; results do not depend on
-// the schedule.
-
-constexpr int kChunk = 1024;
-constexpr int kSlots = 128;
-constexpr int kMergeFan = 64;
-constexpr uint32_t kStateNone = 0u;           // lelems: node not completed
-constexpr uint32_t kStateInner = 0xffffffffu; // lelems: completed inner node
-
-__device__ inline rta_aabb union_lr(const rta_aabb &lb, const rta_aabb &rb) {
-	rta_aabb u;
-	u.min.x = sel_min(lb.min.x, rb.min.x);
-	u.min.y = sel_min(lb.min.y, rb.min.y);
-	u.min.z = sel_min(lb.min.z, rb.min.z);
-	u.max.x = sel_max(lb.max.x, rb.max.x);
-	u.max.y = sel_max(lb.max.y, rb.max.y);
-	u.max.z = sel_max(lb.max.z, rb.max.z);
-	return u;
-}
-
-__device__ inline uint32_t rep_code(const uint32_t *codes, uint32_t node, uint32_t leaf0) {
-	return node >= leaf0 ? codes[node - leaf0] : codes[node];
+	karras_node(nodes, parents, visit, ranges, codes, n, i);
 }
 
 __global__ __launch_bounds__(kChunk) void bottom_up_chunk_kernel(
@@ -269,112 +52,9 @@ __global__ __launch_bounds__(kChunk) void bottom_up_chunk_kernel(
     const uint32_t *__restrict__ parents, const uint2 *__restrict__ ranges,
     uint32_t *__restrict__ slots, uint32_t *__restrict__ counts, uint32_t *__restrict__ status,
     int n) {
-	__shared__ float lb_leaf[6][kChunk];
-	__shared__ float lb_int[6][kChunk];
-	__shared__ uint32_t lvisit[kChunk];
-	__shared__ uint32_t ltris[kChunk];
-	__shared__ uint32_t lelems[kChunk];
-	__shared__ uint32_t front[kSlots];
-	__shared__ uint32_t nfront;
-
-	const int t = threadIdx.x;
-	const uint32_t base = blockIdx.x * kChunk;
-	const uint32_t leaf0 = (uint32_t)(n - 1);
-	const uint32_t end = min(base + (uint32_t)kChunk, (uint32_t)n);
-	lvisit[t] = 0;
-	lelems[t] = kStateNone;
-	if (t == 0) nfront = 0;
-	const uint32_t k = base + t;
-	rta_aabb box;
-	if (k < (uint32_t)n) {
-		box = boxes[indices[k]];
-		lb_leaf[0][t] = box.min.x; lb_leaf[1][t] = box.min.y; lb_leaf[2][t] = box.min.z;
-		lb_leaf[3][t] = box.max.x; lb_leaf[4][t] = box.max.y; lb_leaf[5][t] = box.max.z;
-	}
-	__syncthreads();
-
-	if (k < (uint32_t)n) {
-		uint32_t cur = leaf0 + k;
-		for (int level = 0; cur != 0u && level < 128; ++level) {
-			const uint32_t parent = parents[cur];
-			const uint2 rg = ranges[parent];
-			if (rg.x < base || rg.y >= end) {
-				const uint32_t slot = atomicAdd(&nfront, 1u);
-				if (slot < (uint32_t)kSlots) front[slot] = cur;
-				else atomicOr(status, 2u);
-				break;
-			}
-			const uint32_t p = parent - base;
-			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-			const uint32_t old = atomicAdd(&lvisit[p], 1u);
-			if (old == 0u) break;
-			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-
-			const uint2 pl = *reinterpret_cast<const uint2 *>(&nodes[parent]);
-			const uint32_t left = pl.x >> 1, right = pl.y;
-			const bool cur_is_left = (left == cur);
-			const uint32_t sib = cur_is_left ? right : left;
-			rta_aabb sbox;
-			if (sib >= leaf0) {
-				const uint32_t q = sib - leaf0 - base;
-				sbox.min = {lb_leaf[0][q], lb_leaf[1][q], lb_leaf[2][q]};
-				sbox.max = {lb_leaf[3][q], lb_leaf[4][q], lb_leaf[5][q]};
-			} else {
-				const uint32_t q = sib - base;
-				sbox.min = {lb_int[0][q], lb_int[1][q], lb_int[2][q]};
-				sbox.max = {lb_int[3][q], lb_int[4][q], lb_int[5][q]};
-			}
-			box = cur_is_left ? union_lr(box, sbox) : union_lr(sbox, box);
-			lb_int[0][p] = box.min.x; lb_int[1][p] = box.min.y; lb_int[2][p] = box.min.z;
-			lb_int[3][p] = box.max.x; lb_int[4][p] = box.max.y; lb_int[5][p] = box.max.z;
-
-			// children state: original leaf, collapsed inner node, or inner
-			uint32_t state = kStateInner, ptris = 0;
-			if (rep_code(codes, cur, leaf0) == rep_code(codes, sib, leaf0)) {
-				uint32_t le, lt, re, rt;
-				bool l_leaf, r_leaf;
-				if (left >= leaf0) { l_leaf = true; le = 1; lt = left - leaf0; }
-				else { le = lelems[left - base]; lt = ltris[left - base]; l_leaf = le != kStateInner; }
-				if (right >= leaf0) { r_leaf = true; re = 1; rt = right - leaf0; }
-				else { re = lelems[right - base]; rt = ltris[right - base]; r_leaf = re != kStateInner; }
-				if (l_leaf && r_leaf) { state = re + le; ptris = lt < rt ? lt : rt; }
-			}
-			ltris[p] = ptris;
-			lelems[p] = state;
-			cur = parent;
-		}
-	}
-	__syncthreads();
-
-	// write-out: leaf boxes, completed inner nodes, frontier
-	if (k < (uint32_t)n) {
-		float *f = reinterpret_cast<float *>(&nodes[leaf0 + k]) + 2;
-		f[0] = lb_leaf[0][t]; f[1] = lb_leaf[1][t]; f[2] = lb_leaf[2][t];
-		f[3] = lb_leaf[3][t]; f[4] = lb_leaf[4][t]; f[5] = lb_leaf[5][t];
-	}
-	if (k < leaf0 && lelems[t] != kStateNone) {
-		float *f = reinterpret_cast<float *>(&nodes[k]) + 2;
-		f[0] = lb_int[0][t]; f[1] = lb_int[1][t]; f[2] = lb_int[2][t];
-		f[3] = lb_int[3][t]; f[4] = lb_int[4][t]; f[5] = lb_int[5][t];
-		if (lelems[t] != kStateInner)
-			*reinterpret_cast<uint2 *>(&nodes[k]) = make_uint2(lelems[t] << 1, ltris[t]);
-	}
-	const uint32_t nf = min(nfront, (uint32_t)kSlots);
-	if ((uint32_t)t < nf) slots[(size_t)blockIdx.x * kSlots + t] = front[t];
-	if (t == 0) counts[blockIdx.x] = nf;
-}
-
-__device__ inline rta_aabb load_box(const rta_node *node) {
-	const float *f = reinterpret_cast<const float *>(node) + 2;
-	rta_aabb b;
-	b.min = {f[0], f[1], f[2]};
-	b.max = {f[3], f[4], f[5]};
-	return b;
-}
-__device__ inline void store_box(rta_node *node, const rta_aabb &b) {
-	float *f = reinterpret_cast<float *>(node) + 2;
-	f[0] = b.min.x; f[1] = b.min.y; f[2] = b.min.z;
-	f[3] = b.max.x; f[4] = b.max.y; f[5] = b.max.z;
+	bottom_up_chunk(nodes, boxes, codes, indices, parents, ranges,
+	                slots + (size_t)blockIdx.x * kSlots, counts + blockIdx.x, status, n,
+	                (int)blockIdx.x);
 }
 
 // One block merges the frontier slots of up to kMergeFan chunks of the
@@ -385,67 +65,14 @@ __global__ __launch_bounds__(1024) void bottom_up_merge_kernel(
     const uint2 *__restrict__ ranges, const uint32_t *__restrict__ in_slots,
     const uint32_t *__restrict__ in_counts, int n_in, uint32_t *__restrict__ out_slots,
     uint32_t *__restrict__ out_counts, uint32_t *__restrict__ status, long long chunk, int n) {
-	__shared__ uint32_t prefix[kMergeFan + 1];
-	__shared__ uint32_t front[kSlots];
-	__shared__ uint32_t nfront;
-	const int t = threadIdx.x;
 	const int g0 = blockIdx.x * kMergeFan;
 	const int ng = min(kMergeFan, n_in - g0);
-	if (t == 0) {
-		uint32_t acc = 0;
-		prefix[0] = 0;
-		for (int g = 0; g < ng; ++g) {
-			acc += in_counts[g0 + g];
-			prefix[g + 1] = acc;
-		}
-		nfront = 0;
-	}
-	__syncthreads();
-	const uint32_t leaf0 = (uint32_t)(n - 1);
 	const uint32_t lo = (uint32_t)(blockIdx.x * chunk);
 	const uint32_t hi = (uint32_t)min((long long)n, (long long)(blockIdx.x + 1) * chunk);
-	const uint32_t total = prefix[ng];
-	for (uint32_t f = t; f < total; f += blockDim.x) {
-		int g = 0;
-		while (prefix[g + 1] <= f) ++g;
-		uint32_t cur = in_slots[(size_t)(g0 + g) * kSlots + (f - prefix[g])];
-		rta_aabb box = load_box(&nodes[cur]);
-		for (int level = 0; cur != 0u && level < 128; ++level) {
-			const uint32_t parent = parents[cur];
-			const uint2 rg = ranges[parent];
-			if (rg.x < lo || rg.y >= hi) {
-				const uint32_t slot = atomicAdd(&nfront, 1u);
-				if (slot < (uint32_t)kSlots) front[slot] = cur;
-				else atomicOr(status, 2u);
-				break;
-			}
-			const uint32_t old = __hip_atomic_fetch_add(&visit[parent], 1u, __ATOMIC_ACQ_REL,
-			                                            __HIP_MEMORY_SCOPE_WORKGROUP);
-			if (old == 0u) break;
-			const uint2 pl = *reinterpret_cast<const uint2 *>(&nodes[parent]);
-			const uint32_t left = pl.x >> 1, right = pl.y;
-			const bool cur_is_left = (left == cur);
-			const uint32_t sib = cur_is_left ? right : left;
-			const rta_aabb sbox = load_box(&nodes[sib]);
-			box = cur_is_left ? union_lr(box, sbox) : union_lr(sbox, box);
-			store_box(&nodes[parent], box);
-			if (rep_code(codes, cur, leaf0) == rep_code(codes, sib, leaf0)) {
-				const uint2 ll = *reinterpret_cast<const uint2 *>(&nodes[left]);
-				const uint2 rl = *reinterpret_cast<const uint2 *>(&nodes[right]);
-				if (!link_inner(ll.x) && !link_inner(rl.x)) {
-					const uint32_t tris = ll.y < rl.y ? ll.y : rl.y;
-					const uint32_t elems = (rl.x >> 1) + (ll.x >> 1);
-					*reinterpret_cast<uint2 *>(&nodes[parent]) = make_uint2(elems << 1, tris);
-				}
-			}
-			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-			cur = parent;
-		}
-	}
-	__syncthreads();
-	const uint32_t nf = min(nfront, (uint32_t)kSlots);
-	if (out_slots && (uint32_t)t < nf) out_slots[(size_t)blockIdx.x * kSlots + t] = front[t];
-	if (out_counts && t == 0) out_counts[blockIdx.x] = nf;
+	bottom_up_merge(nodes, visit, codes, parents, ranges, in_slots + (size_t)g0 * kSlots,
+	                in_counts + g0, ng,
+	                out_slots ? out_slots + (size_t)blockIdx.x * kSlots : nullptr,
+	                out_counts ? out_counts + blockIdx.x : nullptr, status, lo, hi, n);
 }
 
 struct BoxesWork {
@@ -479,23 +106,12 @@ static BoxesWork carve_boxes(void *base, int n) {
 	return w;
 }
 
-// -------------------------------------------------- stage 7: permute
-
-// out[i] = scratch[indices[i]] (invert_triangle_permutation_cuda,
-// lbvh-kernels.cu:557-578).  Each thread moves one 104-byte triangle as
-// thirteen 8-byte words.
 __global__ __launch_bounds__(256) void permute_kernel(
     rta_triangle *__restrict__ out, const rta_triangle *__restrict__ scratch,
     const uint32_t *__restrict__ indices, int n) {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n) return;
-	const uint2 *s = reinterpret_cast<const uint2 *>(scratch + indices[i]);
-	uint2 *d = reinterpret_cast<uint2 *>(out + i);
-	uint2 w[kTriWords];
-#pragma unroll
-	for (int k = 0; k < kTriWords; ++k) w[k] = s[k];
-#pragma unroll
-	for (int k = 0; k < kTriWords; ++k) d[k] = w[k];
+	permute_one(out, scratch, indices, i);
 }
 
 // --------------------------------------------------------- workspace

@@ -9,7 +9,7 @@
 //      scatter to digit_base + tile offset + rank.
 // No kernel waits on another block, so there is no spin loop anywhere.
 // A stable sort is unique, so the output equals std::stable_sort by code.
-#include "common.hpp"
+#include "internal.hpp"
 
 namespace rta_hip {
 
@@ -143,6 +143,49 @@ __global__ __launch_bounds__(kThreads) void rs_scatter_kernel(
 
 static inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
+size_t radix_work_bytes(int n) {
+	const size_t tiles = ((size_t)n + kTile - 1) / kTile;
+	return 2 * align256((size_t)n * 4) + align256(tiles * kBins * 4) + align256(kBins * 4);
+}
+
+int radix_sort_pairs(uint32_t *keys, uint32_t *vals, int n, int first_shift, int passes,
+                     void *work, size_t work_bytes, hipStream_t st) {
+	if (work_bytes < radix_work_bytes(n)) {
+		set_error("radix sort: work buffer too small");
+		return RTA_E_WORKSPACE;
+	}
+	const int tiles = (int)(((size_t)n + kTile - 1) / kTile);
+	char *p = reinterpret_cast<char *>(work);
+	uint32_t *alt_keys = reinterpret_cast<uint32_t *>(p);
+	p += align256((size_t)n * 4);
+	uint32_t *alt_vals = reinterpret_cast<uint32_t *>(p);
+	p += align256((size_t)n * 4);
+	uint32_t *hist = reinterpret_cast<uint32_t *>(p);
+	p += align256((size_t)tiles * kBins * 4);
+	uint32_t *totals = reinterpret_cast<uint32_t *>(p);
+
+	uint32_t *kin = keys, *vin = vals, *kout = alt_keys, *vout = alt_vals;
+	for (int pass = 0; pass < passes; ++pass) {
+		const int shift = first_shift + pass * kBits;
+		hipLaunchKernelGGL(rs_hist_kernel, dim3(tiles), dim3(kThreads), 0, st, kin, n, shift, hist,
+		                   tiles);
+		RTA_LAUNCH_CHECK("rs_hist_kernel");
+		hipLaunchKernelGGL(rs_scan_kernel, dim3(kBins), dim3(kThreads), 0, st, hist, tiles, totals);
+		RTA_LAUNCH_CHECK("rs_scan_kernel");
+		hipLaunchKernelGGL(rs_scatter_kernel, dim3(tiles), dim3(kThreads), 0, st, kin, vin, kout,
+		                   vout, n, shift, hist, totals, tiles);
+		RTA_LAUNCH_CHECK("rs_scatter_kernel");
+		uint32_t *tk = kin; kin = kout; kout = tk;
+		uint32_t *tv = vin; vin = vout; vout = tv;
+	}
+	if (kin != keys) {
+		// an odd number of passes ends in the alternate buffers
+		RTA_HIP_TRY(hipMemcpyAsync(keys, kin, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+		RTA_HIP_TRY(hipMemcpyAsync(vals, vin, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+	}
+	return RTA_OK;
+}
+
 }  // namespace rta_hip
 
 using namespace rta_hip;
@@ -151,8 +194,7 @@ extern "C" {
 
 int rta_lbvh_sort_work_bytes(int n, size_t *bytes) {
 	if (n <= 0 || !bytes) return invalid("rta_lbvh_sort_work_bytes: bad arguments");
-	const size_t tiles = ((size_t)n + kTile - 1) / kTile;
-	*bytes = 2 * align256((size_t)n * 4) + align256(tiles * kBins * 4) + align256(kBins * 4);
+	*bytes = radix_work_bytes(n);
 	return RTA_OK;
 }
 
@@ -166,34 +208,9 @@ int rta_lbvh_sort(uint32_t *codes, uint32_t *indices, int n, void *work, size_t 
 		set_error("rta_lbvh_sort: work buffer too small");
 		return RTA_E_WORKSPACE;
 	}
-	const int tiles = (int)(((size_t)n + kTile - 1) / kTile);
-	char *p = reinterpret_cast<char *>(work);
-	uint32_t *alt_keys = reinterpret_cast<uint32_t *>(p);
-	p += align256((size_t)n * 4);
-	uint32_t *alt_vals = reinterpret_cast<uint32_t *>(p);
-	p += align256((size_t)n * 4);
-	uint32_t *hist = reinterpret_cast<uint32_t *>(p);
-	p += align256((size_t)tiles * kBins * 4);
-	uint32_t *totals = reinterpret_cast<uint32_t *>(p);
-
-	uint32_t *kin = codes, *vin = indices, *kout = alt_keys, *vout = alt_vals;
-	hipStream_t st = as_stream(s);
-	for (int pass = 0; pass < kPasses; ++pass) {
-		const int shift = pass * kBits;
-		hipLaunchKernelGGL(rs_hist_kernel, dim3(tiles), dim3(kThreads), 0, st, kin, n, shift, hist,
-		                   tiles);
-		RTA_LAUNCH_CHECK("rs_hist_kernel");
-		hipLaunchKernelGGL(rs_scan_kernel, dim3(kBins), dim3(kThreads), 0, st, hist, tiles, totals);
-		RTA_LAUNCH_CHECK("rs_scan_kernel");
-		hipLaunchKernelGGL(rs_scatter_kernel, dim3(tiles), dim3(kThreads), 0, st, kin, vin, kout,
-		                   vout, n, shift, hist, totals, tiles);
-		RTA_LAUNCH_CHECK("rs_scatter_kernel");
-		uint32_t *tk = kin; kin = kout; kout = tk;
-		uint32_t *tv = vin; vin = vout; vout = tv;
-	}
 	// an even number of passes leaves the result in codes/indices
 	static_assert(kPasses % 2 == 0, "result must end in the caller's buffers");
-	return RTA_OK;
+	return radix_sort_pairs(codes, indices, n, 0, kPasses, work, work_bytes, as_stream(s));
 }
 
 }  // extern "C"

@@ -38,18 +38,16 @@ __global__ __launch_bounds__(256) void jitter_triangles_kernel(
 		for (int a = 0; a < 3; ++a)
 			p[3 * c + a] = verts[3 * (size_t)v + a] + vertex_offset(frame_key, v, a, sigma);
 	}
-	rta_triangle t;
-	t.a = {p[0], p[1], p[2]};
-	t.b = {p[3], p[4], p[5]};
-	t.c = {p[6], p[7], p[8]};
-	t.na = {0.0f, 0.0f, 0.0f};
-	t.nb = t.na;
-	t.nc = t.na;
-	t.ta = {0.0f, 0.0f};
-	t.tb = t.ta;
-	t.tc = t.ta;
-	t.material_index = -1;
-	out[i] = t;
+	// the 104-byte record as 26 words: a, b, c; zero normals and texcoords;
+	// material -1; the trailing alignment word 0 as well
+	uint32_t w[26];
+	for (int k = 0; k < 9; ++k) w[k] = __float_as_uint(p[k]);
+	for (int k = 9; k < 24; ++k) w[k] = 0u;
+	w[24] = 0xffffffffu;
+	w[25] = 0u;
+	uint2 *dst = reinterpret_cast<uint2 *>(out + i);
+#pragma unroll
+	for (int k = 0; k < 13; ++k) dst[k] = make_uint2(w[2 * k], w[2 * k + 1]);
 }
 
 }  // namespace rta_hip

@@ -9,7 +9,9 @@
 // pushed; leaves test their triangles in index order and accept
 // t < closest.t && t <= max_t.  Box and triangle tests are the exact ones
 // of librta/intersect.h (see common.hpp).
-#include "common.hpp"
+#include "internal.hpp"
+
+#include <algorithm>
 
 namespace rta_hip {
 
@@ -152,16 +154,21 @@ __device__ inline uint32_t pair_word(const rta_node &c, uint32_t id) {
 	return link_inner(c.type_left_elems) ? ((id << 1) | 1u) : (c.right_tris << 1);
 }
 
-__global__ void pack_tris_kernel(float4 *__restrict__ tri4, const rta_triangle *__restrict__ tris,
-                                 uint32_t *__restrict__ flags, int n) {
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i == 0) *flags = 0u;
-	if (i >= n) return;
+__device__ inline void pack_tri(float4 *__restrict__ tri4, const rta_triangle *__restrict__ tris,
+                                int i) {
 	const float *f = reinterpret_cast<const float *>(tris + i);
 	const float ax = f[0], ay = f[1], az = f[2];
 	tri4[3 * i + 0] = make_float4(ax, ay, az, 0.0f);
 	tri4[3 * i + 1] = make_float4(ax - f[3], ay - f[4], az - f[5], 0.0f);
 	tri4[3 * i + 2] = make_float4(ax - f[6], ay - f[7], az - f[8], 0.0f);
+}
+
+__global__ void pack_tris_kernel(float4 *__restrict__ tri4, const rta_triangle *__restrict__ tris,
+                                 uint32_t *__restrict__ flags, int n) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i == 0) *flags = 0u;
+	if (i >= n) return;
+	pack_tri(tri4, tris, i);
 }
 
 __device__ inline void mark_leaf_end(float4 *tri4, const rta_node &leaf) {
@@ -177,11 +184,21 @@ __device__ inline bool box_fast_ok(const rta_aabb &b) {
 // One pair record per inner node; flags bit 0 is set when some tested box
 // has a coordinate outside the fast-safe range (then every ray uses the
 // IEEE division path).
+__device__ inline void pack_pair(rta_node_pair *__restrict__ pairs, float4 *__restrict__ tri4,
+                                 uint32_t *__restrict__ flags, const rta_node *__restrict__ nodes,
+                                 int x);
+
 __global__ void pack_pairs_kernel(rta_node_pair *__restrict__ pairs, float4 *__restrict__ tri4,
                                   uint32_t *__restrict__ flags, const rta_node *__restrict__ nodes,
                                   int n_nodes) {
 	const int x = blockIdx.x * blockDim.x + threadIdx.x;
 	if (x >= n_nodes) return;
+	pack_pair(pairs, tri4, flags, nodes, x);
+}
+
+__device__ inline void pack_pair(rta_node_pair *__restrict__ pairs, float4 *__restrict__ tri4,
+                                 uint32_t *__restrict__ flags, const rta_node *__restrict__ nodes,
+                                 int x) {
 	const rta_node nx = nodes[x];
 	if (!link_inner(nx.type_left_elems)) {
 		if (x == 0) mark_leaf_end(tri4, nx);  // a leaf root has no parent record
@@ -204,6 +221,36 @@ __global__ void pack_pairs_kernel(rta_node_pair *__restrict__ pairs, float4 *__r
 	if (!link_inner(nl.type_left_elems)) mark_leaf_end(tri4, nl);
 	if (!link_inner(nr.type_left_elems)) mark_leaf_end(tri4, nr);
 	if (!box_fast_ok(nl.box) || !box_fast_ok(nr.box)) atomicOr(flags, 1u);
+}
+
+// Batched packing: mesh m's pair records at pairs[node_off], its tri4
+// records at tri4[3 * tri_off], its flag word flags[m] (cleared before) and
+// its root word (the pair-record word of the root) root_words[m].
+__global__ void batch_pack_tris_kernel(const BatchItem *__restrict__ items,
+                                       const BatchMesh *__restrict__ meshes,
+                                       float4 *__restrict__ tri4,
+                                       const rta_triangle *__restrict__ tris) {
+	const BatchItem it = items[blockIdx.x];
+	const BatchMesh m = meshes[it.mesh];
+	const int i = (int)it.block * 256 + (int)threadIdx.x;
+	if (i >= (int)m.n) return;
+	pack_tri(tri4 + 3 * (size_t)m.tri_off, tris + m.tri_off, i);
+}
+
+__global__ void batch_pack_pairs_kernel(const BatchItem *__restrict__ items,
+                                        const BatchMesh *__restrict__ meshes,
+                                        rta_node_pair *__restrict__ pairs, float4 *__restrict__ tri4,
+                                        uint32_t *__restrict__ flags,
+                                        const rta_node *__restrict__ nodes,
+                                        uint32_t *__restrict__ root_words) {
+	const BatchItem it = items[blockIdx.x];
+	const BatchMesh m = meshes[it.mesh];
+	const int x = (int)it.block * 256 + (int)threadIdx.x;
+	if (x >= 2 * (int)m.n - 1) return;
+	const rta_node *mn = nodes + m.node_off;
+	pack_pair(pairs + m.node_off, tri4 + 3 * (size_t)m.tri_off, flags + it.mesh, mn, x);
+	if (x == 0)
+		root_words[it.mesh] = link_inner(mn[0].type_left_elems) ? 1u : (mn[0].right_tris << 1);
 }
 
 // -------------------------------------------- packed CIS (fast path)
@@ -326,24 +373,32 @@ __global__ void fill_queue_kernel(uint32_t *queue) {
 // stores only t = FLT_MAX over the reset sentinel (the reference writes the
 // sentinel back with t = FLT_MAX, kernels.cu:793-794,853).  ANY_HIT as in
 // trace_cis_packed_kernel.
-template <int BLOCK, bool ANY_HIT>
+//
+// BATCHED: rays of many meshes in one launch.  Ray r belongs to the mesh
+// whose [ray_off, ray_off + n_rays) holds it; that mesh's pair records start
+// at pairs[node_off], its tri4 records at tri4[3 * tri_off], its flag word
+// is flags[mesh] and its root word root_words[mesh]; hit refs are local to
+// the mesh.
+template <int BLOCK, bool ANY_HIT, bool BATCHED>
 __global__ __launch_bounds__(BLOCK) void trace_cis_persistent_kernel(
     const float4 *__restrict__ tri4, const float4 *__restrict__ pairs,
     const uint32_t *__restrict__ flags, uint32_t root_word, const float *__restrict__ ro,
     const float *__restrict__ rd, const float *__restrict__ rmax, int n_rays,
     const uint32_t *__restrict__ ray_count, rta_hit *__restrict__ hits,
     uint32_t *__restrict__ err, uint32_t *__restrict__ queue, int refill_min, int steps,
-    int leaf_min) {
+    int leaf_min, const BatchMesh *__restrict__ meshes, int n_meshes,
+    const uint32_t *__restrict__ root_words) {
 	__shared__ uint32_t stack[kStack][BLOCK];
 	const int lt = threadIdx.x;
 	const int lane = __lane_id();
 	const uint64_t lower = lane == 0 ? 0ull : (~0ull >> (64 - lane));
 	uint32_t n = (uint32_t)n_rays;
 	if (ray_count) n = min(n, *ray_count);
-	const bool scene_fast = (*flags & 1u) == 0u;
+	const bool scene_fast = BATCHED ? false : (*flags & 1u) == 0u;
 
 	bool active = false, exhausted = false;
 	uint32_t ray = 0, curr = 0;
+	uint32_t node_base = 0, tri_base = 0;
 	int sp = -1;
 	uint32_t guard = 0;
 	float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, yx = 0, yy = 0, yz = 0, tmax = 0;
@@ -367,10 +422,24 @@ __global__ __launch_bounds__(BLOCK) void trace_cis_persistent_kernel(
 					ox = ro[3 * (size_t)ray + 0]; oy = ro[3 * (size_t)ray + 1]; oz = ro[3 * (size_t)ray + 2];
 					dx = rd[3 * (size_t)ray + 0]; dy = rd[3 * (size_t)ray + 1]; dz = rd[3 * (size_t)ray + 2];
 					tmax = rmax[ray];
-					fast = scene_fast && ray_nz_ok(ox, oy, oz, dx, dy, dz);
+					if (BATCHED) {
+						// last mesh whose first ray is <= ray
+						int lo = 0, hi = n_meshes;
+						while (hi - lo > 1) {
+							const int mid = (lo + hi) >> 1;
+							if (meshes[mid].ray_off <= ray) lo = mid;
+							else hi = mid;
+						}
+						node_base = meshes[lo].node_off;
+						tri_base = meshes[lo].tri_off;
+						curr = root_words[lo];
+						fast = (flags[lo] & 1u) == 0u && ray_nz_ok(ox, oy, oz, dx, dy, dz);
+					} else {
+						curr = root_word;
+						fast = scene_fast && ray_nz_ok(ox, oy, oz, dx, dy, dz);
+					}
 					yx = 1.0f / dx; yy = 1.0f / dy; yz = 1.0f / dz;
 					ct = FLT_MAX;
-					curr = root_word;
 					sp = -1;
 					guard = 0;
 					active = true;
@@ -392,7 +461,7 @@ __global__ __launch_bounds__(BLOCK) void trace_cis_persistent_kernel(
 			bool done = false;
 			if (++guard >= kMaxSteps) { atomicAdd(err, 1u); done = true; }
 			else if (!leaf_step) {
-				const float4 *p = pairs + 4 * (size_t)(curr >> 1);
+				const float4 *p = pairs + 4 * ((size_t)node_base + (curr >> 1));
 				const float4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
 				float dl, dr;
 				bool hl, hr;
@@ -414,10 +483,11 @@ __global__ __launch_bounds__(BLOCK) void trace_cis_persistent_kernel(
 				}
 			} else {
 				uint32_t i = curr >> 1;
+				const float4 *tb = tri4 + 3 * (size_t)tri_base;
 				while (true) {
-					const float4 v0 = tri4[3 * (size_t)i + 0];
-					const float4 v1 = tri4[3 * (size_t)i + 1];
-					const float4 v2 = tri4[3 * (size_t)i + 2];
+					const float4 v0 = tb[3 * (size_t)i + 0];
+					const float4 v1 = tb[3 * (size_t)i + 1];
+					const float4 v2 = tb[3 * (size_t)i + 2];
 					float t, be, ga;
 					if (tri_test(v0.x, v0.y, v0.z, v1.x, v1.y, v1.z, v2.x, v2.y, v2.z, ox, oy, oz,
 					             dx, dy, dz, t, be, ga)) {
@@ -608,12 +678,13 @@ namespace rta_hip {
 // Grid of the persistent kernel: as many blocks as fit on every CU (or
 // blocks_per_cu), never more than the rays need.  The CU count and the
 // occupancy are queried once per device and kernel instance.
-template <int BLOCK, bool ANY_HIT>
+template <int BLOCK, bool ANY_HIT, bool BATCHED>
 int launch_persistent(const float *tri4, const rta_node_pair *pairs, const uint32_t *flags,
                       uint32_t root_word, const float *ro, const float *rd, const float *rmax,
                       int n_rays, const uint32_t *ray_count, rta_hit *hits, uint32_t *err,
                       uint32_t *queue, int refill_min, int steps, int blocks_per_cu,
-                      int leaf_min, hipStream_t st) {
+                      int leaf_min, const BatchMesh *meshes, int n_meshes,
+                      const uint32_t *root_words, hipStream_t st) {
 	constexpr int kMaxDevices = 64;
 	static int cus[kMaxDevices] = {}, occ[kMaxDevices] = {};
 	int dev = 0;
@@ -623,7 +694,7 @@ int launch_persistent(const float *tri4, const rta_node_pair *pairs, const uint3
 		int c = 0, o = 0;
 		RTA_HIP_TRY(hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev));
 		RTA_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-		    &o, trace_cis_persistent_kernel<BLOCK, ANY_HIT>, BLOCK, 0));
+		    &o, trace_cis_persistent_kernel<BLOCK, ANY_HIT, BATCHED>, BLOCK, 0));
 		occ[dev] = o < 1 ? 1 : o;
 		cus[dev] = c;
 	}
@@ -633,11 +704,49 @@ int launch_persistent(const float *tri4, const rta_node_pair *pairs, const uint3
 	const unsigned blocks = (unsigned)(want < need ? want : need);
 	hipLaunchKernelGGL(fill_queue_kernel, dim3(1), dim3(64), 0, st, queue);
 	RTA_LAUNCH_CHECK("fill_queue_kernel");
-	hipLaunchKernelGGL((trace_cis_persistent_kernel<BLOCK, ANY_HIT>), dim3(blocks), dim3(BLOCK), 0,
-	                   st, (const float4 *)tri4, (const float4 *)pairs, flags, root_word, ro, rd,
-	                   rmax, n_rays, ray_count, hits, err, queue, refill_min, steps, leaf_min);
+	hipLaunchKernelGGL((trace_cis_persistent_kernel<BLOCK, ANY_HIT, BATCHED>), dim3(blocks),
+	                   dim3(BLOCK), 0, st, (const float4 *)tri4, (const float4 *)pairs, flags,
+	                   root_word, ro, rd, rmax, n_rays, ray_count, hits, err, queue, refill_min,
+	                   steps, leaf_min, meshes, n_meshes, root_words);
 	RTA_LAUNCH_CHECK("trace_cis_persistent_kernel");
 	return RTA_OK;
+}
+
+// Validated launch shape with defaults filled in.
+static int resolve_config(const char *name, const rta_trace_config *cfg, rta_trace_config &c) {
+	c = {64, 16, 16, 0, 0};
+	if (cfg) c = *cfg;
+	if (c.leaf_min == 0) c.leaf_min = 6;
+	if (c.refill_min < 1 || c.refill_min > 64 || c.steps < 1 || c.blocks_per_cu < 0 ||
+	    c.leaf_min < 1 || c.leaf_min > 64 || (c.block != 64 && c.block != 128 && c.block != 256)) {
+		set_error(std::string(name) + ": bad config (block 64/128/256, refill_min and leaf_min "
+		                              "1..64, steps >= 1, blocks_per_cu >= 0)");
+		return RTA_E_INVALID;
+	}
+	return RTA_OK;
+}
+
+template <bool ANY_HIT, bool BATCHED>
+int launch_persistent_cfg(const rta_trace_config &c, const float *tri4, const rta_node_pair *pairs,
+                          const uint32_t *flags, uint32_t root_word, const float *ro,
+                          const float *rd, const float *rmax, int n_rays,
+                          const uint32_t *ray_count, rta_hit *hits, uint32_t *err,
+                          uint32_t *queue, const BatchMesh *meshes, int n_meshes,
+                          const uint32_t *root_words, hipStream_t st) {
+	switch (c.block) {
+	case 64:
+		return launch_persistent<64, ANY_HIT, BATCHED>(
+		    tri4, pairs, flags, root_word, ro, rd, rmax, n_rays, ray_count, hits, err, queue,
+		    c.refill_min, c.steps, c.blocks_per_cu, c.leaf_min, meshes, n_meshes, root_words, st);
+	case 128:
+		return launch_persistent<128, ANY_HIT, BATCHED>(
+		    tri4, pairs, flags, root_word, ro, rd, rmax, n_rays, ray_count, hits, err, queue,
+		    c.refill_min, c.steps, c.blocks_per_cu, c.leaf_min, meshes, n_meshes, root_words, st);
+	default:
+		return launch_persistent<256, ANY_HIT, BATCHED>(
+		    tri4, pairs, flags, root_word, ro, rd, rmax, n_rays, ray_count, hits, err, queue,
+		    c.refill_min, c.steps, c.blocks_per_cu, c.leaf_min, meshes, n_meshes, root_words, st);
+	}
 }
 
 template <bool ANY_HIT>
@@ -672,35 +781,13 @@ int trace_persistent(const char *name, const float *tri4, const rta_node_pair *p
 		set_error(std::string(name) + ": bad arguments");
 		return RTA_E_INVALID;
 	}
-	rta_trace_config c = {64, 16, 16, 0, 0};
-	if (cfg) c = *cfg;
-	if (c.leaf_min == 0) c.leaf_min = 6;
-	if (c.refill_min < 1 || c.refill_min > 64 || c.steps < 1 || c.blocks_per_cu < 0 ||
-	    c.leaf_min < 1 || c.leaf_min > 64) {
-		set_error(std::string(name) + ": bad config");
-		return RTA_E_INVALID;
-	}
+	rta_trace_config c;
+	const int rc = resolve_config(name, cfg, c);
+	if (rc != RTA_OK) return rc;
 	const uint32_t root_word = link_inner(root_link) ? 1u : (root_index << 1);
-	switch (c.block) {
-	case 64:
-		return launch_persistent<64, ANY_HIT>(tri4, pairs, flags, root_word, ray_orig, ray_dir,
-		                                      max_t, n_rays, ray_count, hits, err, queue,
-		                                      c.refill_min, c.steps, c.blocks_per_cu, c.leaf_min,
-		                                      st);
-	case 128:
-		return launch_persistent<128, ANY_HIT>(tri4, pairs, flags, root_word, ray_orig, ray_dir,
-		                                       max_t, n_rays, ray_count, hits, err, queue,
-		                                       c.refill_min, c.steps, c.blocks_per_cu, c.leaf_min,
-		                                       st);
-	case 256:
-		return launch_persistent<256, ANY_HIT>(tri4, pairs, flags, root_word, ray_orig, ray_dir,
-		                                       max_t, n_rays, ray_count, hits, err, queue,
-		                                       c.refill_min, c.steps, c.blocks_per_cu, c.leaf_min,
-		                                       st);
-	default:
-		set_error(std::string(name) + ": block must be 64, 128 or 256");
-		return RTA_E_INVALID;
-	}
+	return launch_persistent_cfg<ANY_HIT, false>(c, tri4, pairs, flags, root_word, ray_orig,
+	                                             ray_dir, max_t, n_rays, ray_count, hits, err,
+	                                             queue, nullptr, 0, nullptr, st);
 }
 
 }  // namespace rta_hip
@@ -824,6 +911,116 @@ int rta_trace_shadow_cis_persistent(const float *tri4, const rta_node_pair *pair
 	return trace_persistent<true>("rta_trace_shadow_cis_persistent", tri4, pairs, flags,
 	                              root_link, root_index, ray_orig, ray_dir, max_t, n_rays,
 	                              ray_count, hits, err, queue, cfg, as_stream(s));
+}
+
+int rta_batch_pack(rta_batch b, rta_node_pair *pairs, float *tri4, uint32_t *flags,
+                   const rta_node *nodes, const rta_triangle *tris, rta_stream s) {
+	if (!b || !pairs || !tri4 || !flags || !nodes || !tris)
+		return invalid("rta_batch_pack: NULL argument");
+	const BatchPlan &p = *(const BatchPlan *)b;
+	hipStream_t st = as_stream(s);
+	const int rc = rta_hip_fill_u32(flags, 0u, (size_t)p.n_meshes, s);
+	if (rc != RTA_OK) return rc;
+	hipLaunchKernelGGL(batch_pack_tris_kernel, dim3((unsigned)p.tri_blocks), dim3(256), 0, st,
+	                   p.d_items + p.tri_blocks_off, p.d_meshes, (float4 *)tri4, tris);
+	RTA_LAUNCH_CHECK("batch_pack_tris_kernel");
+	hipLaunchKernelGGL(batch_pack_pairs_kernel, dim3((unsigned)p.node_blocks), dim3(256), 0, st,
+	                   p.d_items + p.node_blocks_off, p.d_meshes, pairs, (float4 *)tri4, flags,
+	                   nodes, p.d_root_words);
+	RTA_LAUNCH_CHECK("batch_pack_pairs_kernel");
+	return RTA_OK;
+}
+
+int rta_batch_trace_persistent(rta_batch b, const float *tri4, const rta_node_pair *pairs,
+                               const uint32_t *flags, const float *ray_orig, const float *ray_dir,
+                               const float *max_t, rta_hit *hits, uint32_t *err, uint32_t *queue,
+                               int any_hit, const rta_trace_config *cfg, rta_stream s) {
+	if (!b || !tri4 || !pairs || !flags || !ray_orig || !ray_dir || !max_t || !hits || !err ||
+	    !queue)
+		return invalid("rta_batch_trace_persistent: NULL argument");
+	const BatchPlan &p = *(const BatchPlan *)b;
+	if (p.total_rays <= 0) return invalid("rta_batch_trace_persistent: the batch has no rays");
+	rta_trace_config c;
+	const int rc = resolve_config("rta_batch_trace_persistent", cfg, c);
+	if (rc != RTA_OK) return rc;
+	hipStream_t st = as_stream(s);
+	if (any_hit)
+		return launch_persistent_cfg<true, true>(c, tri4, pairs, flags, 0u, ray_orig, ray_dir,
+		                                         max_t, (int)p.total_rays, nullptr, hits, err,
+		                                         queue, p.d_meshes, p.n_meshes, p.d_root_words, st);
+	return launch_persistent_cfg<false, true>(c, tri4, pairs, flags, 0u, ray_orig, ray_dir, max_t,
+	                                          (int)p.total_rays, nullptr, hits, err, queue,
+	                                          p.d_meshes, p.n_meshes, p.d_root_words, st);
+}
+
+// Coordinate descent over the launch shape: starting from `*best` (or the
+// defaults), each round tries every candidate of one parameter at a time
+// (leaf_min, refill_min, steps, block) with the others fixed and keeps the
+// fastest, timing each candidate as the best of `reps` launches.  Results do
+// not depend on the shape, so the hits of the last launch are valid.
+int rta_batch_autotune(rta_batch b, const float *tri4, const rta_node_pair *pairs,
+                       const uint32_t *flags, const float *ray_orig, const float *ray_dir,
+                       const float *max_t, rta_hit *hits, uint32_t *err, uint32_t *queue,
+                       int any_hit, int rounds, int reps, rta_trace_config *best, float *best_ms,
+                       rta_stream s) {
+	if (!best) return invalid("rta_batch_autotune: best is NULL");
+	if (rounds < 1 || reps < 1) return invalid("rta_batch_autotune: rounds and reps must be >= 1");
+	rta_trace_config cur;
+	int rc = resolve_config("rta_batch_autotune", best->block ? best : nullptr, cur);
+	if (rc != RTA_OK) return rc;
+	hipStream_t st = as_stream(s);
+	hipEvent_t e0, e1;
+	RTA_HIP_TRY(hipEventCreate(&e0));
+	hipError_t he = hipEventCreate(&e1);
+	if (he != hipSuccess) {
+		(void)hipEventDestroy(e0);
+		return hip_status(he, "hipEventCreate");
+	}
+	auto measure = [&](const rta_trace_config &c, float &ms) -> int {
+		ms = 1e30f;
+		for (int r = 0; r < reps; ++r) {
+			hipError_t e = hipEventRecord(e0, st);
+			if (e != hipSuccess) return hip_status(e, "hipEventRecord");
+			int k = rta_batch_trace_persistent(b, tri4, pairs, flags, ray_orig, ray_dir, max_t, hits,
+			                                   err, queue, any_hit, &c, s);
+			if (k != RTA_OK) return k;
+			e = hipEventRecord(e1, st);
+			if (e == hipSuccess) e = hipEventSynchronize(e1);
+			float t = 0;
+			if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+			if (e != hipSuccess) return hip_status(e, "autotune timing");
+			ms = std::min(ms, t);
+		}
+		return RTA_OK;
+	};
+	float cur_ms = 0;
+	rc = measure(cur, cur_ms);
+	static const int leafs[] = {2, 4, 6, 8, 12};
+	static const int refills[] = {4, 8, 16, 24, 32};
+	static const int stepss[] = {8, 16, 24, 32, 64};
+	static const int blocks[] = {64, 128};
+	for (int round = 0; round < rounds && rc == RTA_OK; ++round) {
+		for (int param = 0; param < 4 && rc == RTA_OK; ++param) {
+			const int *vals = param == 0 ? leafs : param == 1 ? refills : param == 2 ? stepss : blocks;
+			const int nv = param == 3 ? 2 : 5;
+			for (int v = 0; v < nv && rc == RTA_OK; ++v) {
+				rta_trace_config c = cur;
+				int &field = param == 0 ? c.leaf_min : param == 1 ? c.refill_min
+				             : param == 2 ? c.steps : c.block;
+				if (field == vals[v]) continue;
+				field = vals[v];
+				float ms = 0;
+				rc = measure(c, ms);
+				if (rc == RTA_OK && ms < cur_ms) { cur = c; cur_ms = ms; }
+			}
+		}
+	}
+	(void)hipEventDestroy(e0);
+	(void)hipEventDestroy(e1);
+	if (rc != RTA_OK) return rc;
+	*best = cur;
+	if (best_ms) *best_ms = cur_ms;
+	return RTA_OK;
 }
 
 int rta_bounce_work_bytes(int n_rays, size_t *bytes) {

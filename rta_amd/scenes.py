@@ -139,3 +139,51 @@ def dynamic_light(n_inst: int = 64):
 
 DYNAMIC_SIGMA = 1e-3
 DYNAMIC_SEED = 0xD1A_2024
+
+
+def batch_meshes(n_meshes: int = 256, seed: int = 3, tris=(1_000, 200_000),
+                 rays=(4_096, 262_144)):
+    """The many-meshes workload: n_meshes independent meshes whose triangle
+    counts are log-uniform in `tris`, each a jittered icosphere (the largest
+    subdivision 20*4^s not above the drawn count, radius 1, at the origin)
+    or a value-noise heightfield patch over [0,1]^2 (g x g quads with 2g^2
+    close to the count), chosen by a seeded coin; each with its own rays,
+    log-uniform in `rays`: half from a random camera 3 units outside the
+    mesh towards random points of its box, half from uniform origins in the
+    box in uniform directions.  Returns (meshes, rays) lists; rays are
+    (origins[3r], dirs[3r], max_t[r]) float32."""
+    rng = np.random.default_rng(seed)
+    meshes, ray_sets = [], []
+    for k in range(n_meshes):
+        target = float(np.exp(rng.uniform(np.log(tris[0]), np.log(tris[1]))))
+        if rng.uniform() < 0.5 and target >= 20:
+            s = int(np.floor(np.log(target / 20.0) / np.log(4.0)))
+            s = max(0, min(s, 7))
+            m = icosphere(s, 1.0, jitter=0.02, seed=seed * 1000 + k)
+        else:
+            g = max(1, int(round(np.sqrt(target / 2.0))))
+            m = heightfield(g, g, seed=seed * 1000 + k)
+        meshes.append(m)
+        n = int(np.exp(rng.uniform(np.log(rays[0]), np.log(rays[1]))))
+        ray_sets.append(mesh_rays(m, n, rng))
+    return meshes, ray_sets
+
+
+def mesh_rays(m: np.ndarray, n: int, rng):
+    """n rays for mesh m: half from a random camera 3 units outside the
+    mesh's box centre towards random points of the box, half from uniform
+    origins in the box in uniform directions; max_t = FLT_MAX."""
+    v = np.concatenate([m["a"], m["b"], m["c"]]).astype(np.float64)
+    lo, hi = v.min(axis=0), v.max(axis=0)
+    n1 = n // 2
+    ctr = 0.5 * (lo + hi)
+    u = rng.normal(size=3)
+    cam = ctr + 3.0 * u / np.linalg.norm(u)
+    d1 = rng.uniform(lo, hi, size=(n1, 3)) - cam
+    o1 = np.broadcast_to(cam, (n1, 3))
+    o2 = rng.uniform(lo, hi, size=(n - n1, 3))
+    d2 = rng.normal(size=(n - n1, 3))
+    o = np.concatenate([o1, o2]).astype(np.float32)
+    d = np.concatenate([d1, d2])
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    return o.reshape(-1), d.reshape(-1), np.full(n, FLT_MAX, np.float32)
