@@ -144,7 +144,13 @@ def test_dynamic_frame_graph_replays(gpu):
         assert np.array_equal(bits(got["nodes"]), bits(e["nodes"]))
         assert got["counts"] == e["counts"]
         for g, x in zip(got["hits"] + got["shadow_hits"], e["hits"] + e["shadow_hits"]):
-            assert np.array_equal(bits(g), bits(x))
+            # a miss keeps the stale ref of whatever the buffer held before
+            # (the reset writes t, beta, gamma only, cuda-kernels.cu:26-28)
+            hit = x["t"] != F32MAX
+            assert np.array_equal(g["t"] != F32MAX, hit)
+            for f in ("t", "beta", "gamma"):
+                assert np.array_equal(bits(g[f]), bits(x[f]))
+            assert np.array_equal(g["ref"][hit], x["ref"][hit])
         # unrelated eager work between replays
         rta_amd.synchronize()
     del graph
