@@ -469,18 +469,20 @@ __global__ __launch_bounds__(BLOCK) void trace_cis_persistent_kernel(
 				hl = hl && dl < ct && dl <= tmax;
 				hr = hr && dr < ct && dr <= tmax;
 				const uint32_t wl = __float_as_uint(q3.x), wr = __float_as_uint(q3.y);
-				if (hl && hr) {
-					// nearer child first, the left one on a tie (kernels.cu:818-827)
-					const bool left_first = dl <= dr;
-					if (sp + 1 >= kStack) { atomicAdd(err, 1u); done = true; }
-					else { stack[++sp][lt] = left_first ? wr : wl; curr = left_first ? wl : wr; }
-				} else if (hl || hr) {
-					curr = hl ? wl : wr;
-				} else if (sp >= 0) {
-					curr = stack[sp--][lt];
-				} else {
-					done = true;
-				}
+				// The reference's decision (kernels.cu:818-835) without
+				// branches: both kept -> enter the nearer, the left one on a
+				// tie, and push the other; one kept -> enter it; none -> pop,
+				// or finish on an empty stack.  The far child is written one
+				// slot above the top on every step (after the top is read);
+				// only a push makes that slot part of the stack.
+				const bool both = hl && hr, any = hl || hr;
+				const bool enter_left = both ? (dl <= dr) : hl;
+				const uint32_t popped = stack[sp >= 0 ? sp : 0][lt];
+				stack[sp + 1 < kStack ? sp + 1 : kStack - 1][lt] = enter_left ? wr : wl;
+				curr = any ? (enter_left ? wl : wr) : popped;
+				if (both && sp + 1 >= kStack) { atomicAdd(err, 1u); done = true; }
+				done = done || (!any && sp < 0);
+				sp += both ? 1 : (any ? 0 : -1);
 			} else {
 				uint32_t i = curr >> 1;
 				const float4 *tb = tri4 + 3 * (size_t)tri_base;
@@ -499,9 +501,10 @@ __global__ __launch_bounds__(BLOCK) void trace_cis_persistent_kernel(
 					if (__float_as_uint(v0.w) != 0u) break;
 					++i;
 				}
-				if (ANY_HIT && ct != FLT_MAX) done = true;
-				else if (sp >= 0) curr = stack[sp--][lt];
-				else done = true;
+				// pop, or finish (empty stack, or a shadow ray's first hit)
+				curr = stack[sp >= 0 ? sp : 0][lt];
+				done = (ANY_HIT && ct != FLT_MAX) || sp < 0;
+				--sp;
 			}
 			if (done) {
 				if (ct != FLT_MAX) {
