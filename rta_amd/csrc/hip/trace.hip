@@ -16,7 +16,10 @@
 namespace rta_hip {
 
 constexpr int kTraceBlock = 128;
-constexpr int kStack = 32;   // the reference's stack depth (kernels.cu:787)
+#ifndef RTA_TRACE_STACK
+#define RTA_TRACE_STACK 32
+#endif
+constexpr int kStack = RTA_TRACE_STACK;   // the reference's stack depth is 32 (kernels.cu:787)
 // Upper bound on loop iterations of one ray: a malformed tree ends in an
 // error count instead of a hang.  A valid tree needs at most
 // 2*nodes iterations; 2^26 covers every tree this library can hold.
@@ -400,16 +403,22 @@ __global__ __launch_bounds__(BLOCK) void trace_cis_persistent_kernel(
 	uint32_t ray = 0, curr = 0;
 	uint32_t node_base = 0, tri_base = 0;
 	int sp = -1;
-	uint32_t guard = 0;
 	float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, yx = 0, yy = 0, yz = 0, tmax = 0;
 	float ct = FLT_MAX, cb = 0, cg = 0;
 	uint32_t cref = 0;
 	bool fast = false;
+	// Wave-steps since the last refill.  Every wave-step advances at least
+	// one lane, so a valid tree (at most 2 * nodes < kMaxSteps steps per
+	// ray) never lets this pass 64 * kMaxSteps; a malformed one ends in an
+	// error count per active ray instead of an endless loop.
+	uint64_t wave_steps = 0;
+	constexpr uint64_t kWaveGuard = 64ull * kMaxSteps;
 
 	while (true) {
 		const uint64_t idle = __ballot(!active);
 		const int nidle = __popcll(idle);
 		if (!exhausted && nidle >= refill_min) {
+			wave_steps = 0;
 			const int leader = __ffsll((unsigned long long)idle) - 1;
 			uint32_t base = 0;
 			if (lane == leader) base = atomicAdd(queue, (uint32_t)nidle);
@@ -441,7 +450,6 @@ __global__ __launch_bounds__(BLOCK) void trace_cis_persistent_kernel(
 					yx = 1.0f / dx; yy = 1.0f / dy; yz = 1.0f / dz;
 					ct = FLT_MAX;
 					sp = -1;
-					guard = 0;
 					active = true;
 				}
 			}
@@ -455,11 +463,16 @@ __global__ __launch_bounds__(BLOCK) void trace_cis_persistent_kernel(
 			const uint64_t inner_lanes = __ballot(at_inner);
 			const uint64_t leaf_lanes = __ballot(at_leaf);
 			if ((inner_lanes | leaf_lanes) == 0ull) break;
-			// wave-uniform choice of this step's kind
-			const bool leaf_step = inner_lanes == 0ull || __popcll(leaf_lanes) >= leaf_min;
-			if (!(leaf_step ? at_leaf : at_inner)) continue;
+			// Wave-uniform choice of this step's kind: a leaf step once
+			// leaf_min of 64 lanes, scaled to the lanes still active, wait at
+			// leaves (a draining wave does not hold its last rays there).
+			const int n_leaf = __popcll(leaf_lanes);
+			const int thr = ((n_leaf + __popcll(inner_lanes)) * leaf_min + 63) >> 6;
+			const bool leaf_step = inner_lanes == 0ull || n_leaf >= thr;
+			const bool trip = ++wave_steps > kWaveGuard;
+			if (!(trip ? active : (leaf_step ? at_leaf : at_inner))) continue;
 			bool done = false;
-			if (++guard >= kMaxSteps) { atomicAdd(err, 1u); done = true; }
+			if (trip) { atomicAdd(err, 1u); done = true; }
 			else if (!leaf_step) {
 				const float4 *p = pairs + 4 * ((size_t)node_base + (curr >> 1));
 				const float4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
