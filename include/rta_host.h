@@ -59,6 +59,42 @@ int rta_host_shirley_rays(float *dirs, float *origins, float *max_t, float fovy,
                           int w, int h, const float *view_dir, const float *pos, const float *up,
                           float maxt);
 
+/* ---- the reference's measurement protocol (rta/main.cpp:200-462) ----
+ * Functions below return 0 (or a count) on success and a negative value on
+ * failure, with the message in rta_host_last_error(). */
+const char *rta_host_last_error(void);
+
+/* Bounds of all triangle vertices as compute_aabb forms them
+ * (librta/aabb.h:10-49): start from the first triangle's vertex a, update
+ * on strict < / >.  n >= 1. */
+int rta_host_scene_bounds(const rta_triangle *tris, int n, float *bb_min, float *bb_max);
+
+/* Camera of one spherical-series sample (rta/main.cpp:338-346, 355-373):
+ * pos = centre + normalize(vertex) * |max-min| * distance_factor, with
+ * centre = min + (max-min)/2; dir = normalize(-vertex) (towards the
+ * coordinate origin); up from (0,0,1), or (0,1,0) when |dot(dir, z)| > 0.8,
+ * then up = cross(dir, cross(dir, up)).  The series renders with fovy 45. */
+int rta_host_series_camera(const float *vertex, const float *bb_min, const float *bb_max,
+                           float distance_factor, float *pos, float *dir, float *up);
+
+/* Camera positions of the axial series (rta/main.cpp:296-309): sample i is
+ * the point (anchor.x, anchor.x, anchor.x) -- the reference builds it from
+ * the x component three times -- rotated about `axis` (normalised here) by
+ * float((2 * 3.1416 / samples) * i) radians.  out: 3 floats per sample. */
+int rta_host_axial_positions(const float *axis, const float *anchor, int samples, float *out);
+
+/* Distinct vertex positions of an ASCII .ply file (x, y, z properties of
+ * element "vertex"), first occurrence first, as rta/main.cpp:247-270 keeps
+ * them.  Returns the count; at most max_out are written (out may be NULL). */
+int rta_host_ply_vertices(const char *path, float *out, int max_out);
+
+/* Triangles of a Wavefront .obj file: v/vt/vn records, faces with v,
+ * v/t, v//n or v/t/n corners (negative indices count back), polygons split
+ * into fans around their first corner.  Normals and texcoords are taken
+ * where given (else 0); material_index is -1.  Returns the triangle count;
+ * at most max_out are written (out may be NULL). */
+int rta_host_load_obj(const char *path, rta_triangle *out, int max_out);
+
 #ifdef __cplusplus
 }
 #endif

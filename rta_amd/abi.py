@@ -147,6 +147,12 @@ HOST_SIGNATURES = {
     "rta_scene_random_triangles": [_i, _u64, _f, _f, _f, _f, _vp],
     "rta_scene_instances": [_i, _i, _u64, _pi, _vp, _vp],
     "rta_host_shirley_rays": [_vp, _vp, _vp, _f, _f, _i, _i, _vp, _vp, _vp, _f],
+    "rta_host_last_error": [],
+    "rta_host_scene_bounds": [_vp, _i, _vp, _vp],
+    "rta_host_series_camera": [_vp, _vp, _vp, _f, _vp, _vp, _vp],
+    "rta_host_axial_positions": [_vp, _vp, _i, _vp],
+    "rta_host_ply_vertices": [C.c_char_p, _vp, _i],
+    "rta_host_load_obj": [C.c_char_p, _vp, _i],
 }
 
 
@@ -167,9 +173,10 @@ def _load(path: str, sigs: dict) -> C.CDLL:
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = C.c_int
-    if hasattr(lib, "rta_hip_last_error"):
-        lib.rta_hip_last_error.argtypes = []
-        lib.rta_hip_last_error.restype = C.c_char_p
+    for name in ("rta_hip_last_error", "rta_host_last_error"):
+        if hasattr(lib, name):
+            getattr(lib, name).argtypes = []
+            getattr(lib, name).restype = C.c_char_p
     return lib
 
 

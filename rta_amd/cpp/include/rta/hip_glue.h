@@ -8,6 +8,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 
+#include <fstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -93,6 +94,21 @@ public:
 		check(rta_setup_shirley_rays(gpu_direction, gpu_origin, gpu_maxt, fovy, aspect, w, h,
 		                             &dir.x, &position.x, &up.x, FLT_MAX, 0),
 		      "rta_setup_shirley_rays");
+	}
+	//! the reference's .ray text format: "ray", "w h", then per ray
+	//! "id 2 ox oy oz dx dy dz 0 max_t" (librta/raytrav.h:353-366)
+	void dump_rays(const std::string &filename) override {
+		const size_t n = (size_t)w * h;
+		std::vector<float> o(3 * n), d(3 * n), t(n);
+		check(rta_hip_memcpy_dtoh(o.data(), gpu_origin, 12 * n), "download origins");
+		check(rta_hip_memcpy_dtoh(d.data(), gpu_direction, 12 * n), "download directions");
+		check(rta_hip_memcpy_dtoh(t.data(), gpu_maxt, 4 * n), "download max_t");
+		std::ofstream out(filename.c_str());
+		out << "ray\n" << w << " " << h << "\n";
+		for (size_t i = 0; i < n; ++i)
+			out << i << " " << 2 << " " << o[3 * i] << " " << o[3 * i + 1] << " " << o[3 * i + 2]
+			    << " " << d[3 * i] << " " << d[3 * i + 1] << " " << d[3 * i + 2] << " 0 " << t[i]
+			    << "\n";
 	}
 	std::string identification() override { return "HIP pinhole ray generator (Shirley)"; }
 };

@@ -73,6 +73,8 @@ public:
 	      ray_max_t((size_t)res_x * res_y) {}
 	virtual ~ray_generator() {}
 	virtual void generate_rays() = 0;
+	//! text dump of the rays (the .ray format, librta/raytrav.h:295,353-366)
+	virtual void dump_rays(const std::string &) {}
 	uint res_x() const { return res_w; }
 	uint res_y() const { return res_h; }
 	vec3f *origin(uint x, uint y) { return &origins[(size_t)y * res_w + x]; }
