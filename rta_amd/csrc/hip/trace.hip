@@ -407,18 +407,32 @@ __global__ __launch_bounds__(BLOCK) void trace_cis_persistent_kernel(
 	float ct = FLT_MAX, cb = 0, cg = 0;
 	uint32_t cref = 0;
 	bool fast = false;
-	// Wave-steps since the last refill.  Every wave-step advances at least
-	// one lane, so a valid tree (at most 2 * nodes < kMaxSteps steps per
-	// ray) never lets this pass 64 * kMaxSteps; a malformed one ends in an
-	// error count per active ray instead of an endless loop.
-	uint64_t wave_steps = 0;
-	constexpr uint64_t kWaveGuard = 64ull * kMaxSteps;
+	// Rounds (of at most `steps` wave-steps) since the last refill.  Every
+	// wave-step advances at least one lane and each lane holds at most one
+	// ray per refill, so with a valid tree (at most 2 * nodes < kMaxSteps
+	// steps per ray) a wave takes fewer than 64 * kMaxSteps wave-steps, and
+	// so at most round_limit rounds, between refills; a malformed tree ends
+	// in an error count per active ray instead of an endless loop.
+	uint64_t rounds = 0;
+	const uint64_t round_limit = 64ull * kMaxSteps / (uint64_t)steps + 2;
+	// write the ray's record (hits[ray].t = FLT_MAX alone on a miss) and
+	// free the lane
+	auto finish = [&]() {
+		if (ct != FLT_MAX) {
+			rta_hit hres;
+			hres.t = ct; hres.beta = cb; hres.gamma = cg; hres.ref = cref;
+			hits[ray] = hres;
+		} else {
+			hits[ray].t = FLT_MAX;
+		}
+		active = false;
+	};
 
 	while (true) {
 		const uint64_t idle = __ballot(!active);
 		const int nidle = __popcll(idle);
 		if (!exhausted && nidle >= refill_min) {
-			wave_steps = 0;
+			rounds = 0;
 			const int leader = __ffsll((unsigned long long)idle) - 1;
 			uint32_t base = 0;
 			if (lane == leader) base = atomicAdd(queue, (uint32_t)nidle);
@@ -457,23 +471,28 @@ __global__ __launch_bounds__(BLOCK) void trace_cis_persistent_kernel(
 		// with refill_min <= 64 an all-idle wave has just refilled, so an
 		// all-idle wave here means the queue is exhausted
 		if (__ballot(active) == 0ull) break;
+		if (++rounds > round_limit) {
+			// malformed tree: every active ray ends with an error count
+			if (active) {
+				atomicAdd(err, 1u);
+				finish();
+			}
+			continue;
+		}
 		for (int k = 0; k < steps; ++k) {
-			const bool at_inner = active && (curr & 1u) != 0u;
-			const bool at_leaf = active && (curr & 1u) == 0u;
-			const uint64_t inner_lanes = __ballot(at_inner);
-			const uint64_t leaf_lanes = __ballot(at_leaf);
-			if ((inner_lanes | leaf_lanes) == 0ull) break;
+			const bool is_inner = (curr & 1u) != 0u;
+			const uint64_t act = __ballot(active);
+			if (act == 0ull) break;
+			const uint64_t inner_lanes = __ballot(active && is_inner);
 			// Wave-uniform choice of this step's kind: a leaf step once
 			// leaf_min of 64 lanes, scaled to the lanes still active, wait at
 			// leaves (a draining wave does not hold its last rays there).
-			const int n_leaf = __popcll(leaf_lanes);
-			const int thr = ((n_leaf + __popcll(inner_lanes)) * leaf_min + 63) >> 6;
+			const int n_leaf = __popcll(act & ~inner_lanes);
+			const int thr = (__popcll(act) * leaf_min + 63) >> 6;
 			const bool leaf_step = inner_lanes == 0ull || n_leaf >= thr;
-			const bool trip = ++wave_steps > kWaveGuard;
-			if (!(trip ? active : (leaf_step ? at_leaf : at_inner))) continue;
-			bool done = false;
-			if (trip) { atomicAdd(err, 1u); done = true; }
-			else if (!leaf_step) {
+			if (!active || is_inner == leaf_step) continue;
+			bool done, overflow = false;
+			if (!leaf_step) {
 				const float4 *p = pairs + 4 * ((size_t)node_base + (curr >> 1));
 				const float4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
 				float dl, dr;
@@ -487,14 +506,15 @@ __global__ __launch_bounds__(BLOCK) void trace_cis_persistent_kernel(
 				// tie, and push the other; one kept -> enter it; none -> pop,
 				// or finish on an empty stack.  The far child is written one
 				// slot above the top on every step (after the top is read);
-				// only a push makes that slot part of the stack.
+				// only a push makes that slot part of the stack.  A push onto
+				// a full stack ends the ray with an error count.
 				const bool both = hl && hr, any = hl || hr;
 				const bool enter_left = both ? (dl <= dr) : hl;
 				const uint32_t popped = stack[sp >= 0 ? sp : 0][lt];
 				stack[sp + 1 < kStack ? sp + 1 : kStack - 1][lt] = enter_left ? wr : wl;
 				curr = any ? (enter_left ? wl : wr) : popped;
-				if (both && sp + 1 >= kStack) { atomicAdd(err, 1u); done = true; }
-				done = done || (!any && sp < 0);
+				overflow = both && sp + 1 >= kStack;
+				done = overflow || (!any && sp < 0);
 				sp += both ? 1 : (any ? 0 : -1);
 			} else {
 				uint32_t i = curr >> 1;
@@ -520,14 +540,8 @@ __global__ __launch_bounds__(BLOCK) void trace_cis_persistent_kernel(
 				--sp;
 			}
 			if (done) {
-				if (ct != FLT_MAX) {
-					rta_hit hres;
-					hres.t = ct; hres.beta = cb; hres.gamma = cg; hres.ref = cref;
-					hits[ray] = hres;
-				} else {
-					hits[ray].t = FLT_MAX;
-				}
-				active = false;
+				if (overflow) atomicAdd(err, 1u);
+				finish();
 			}
 		}
 	}
