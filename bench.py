@@ -50,8 +50,8 @@ def parse():
     p.add_argument("--trav", choices=("persistent", "packed"), default="persistent")
     p.add_argument("--block", type=int, default=64)
     p.add_argument("--refill", type=int, default=16)
-    p.add_argument("--steps-per-refill", dest="steps_per_refill", type=int, default=16)
-    p.add_argument("--leaf-min", dest="leaf_min", type=int, default=6)
+    p.add_argument("--steps-per-refill", dest="steps_per_refill", type=int, default=32)
+    p.add_argument("--leaf-min", dest="leaf_min", type=int, default=10)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run a 1 GiB device copy kernel (to scale HBM traffic counters)")
@@ -104,7 +104,7 @@ def batch_main(args) -> None:
     cfg, tune_ms = b.autotune(rounds=2, reps=2)
     log(f"autotuned {cfg.as_tuple()} -> {tune_ms:.3f} ms")
     trace_ms = timed(lambda: b.trace(config=cfg), max(args.steps, 3))
-    default_ms = timed(lambda: b.trace(config=TraceConfig(64, 16, 16, 0, 6)), 3)
+    default_ms = timed(lambda: b.trace(config=TraceConfig(64, 16, 32, 0, 10)), 3)
     b.check_errors()
 
     # the same meshes one build + one trace launch at a time

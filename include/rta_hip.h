@@ -274,9 +274,9 @@ int rta_trace_cis_packed(const float *tri4, const rta_node_pair *pairs,
  * threads), refill_min (a wave fetches new rays once at least this many of
  * its 64 lanes are idle, 1..64), steps (traversal steps between refill
  * checks), blocks_per_cu (0 = as many as fit) and leaf_min (a wave tests
- * leaf triangles once at least this many of its lanes wait at a leaf, or no
- * lane is at an inner node; 1..64, 0 = default 6).  None of these changes
- * a result. */
+ * leaf triangles once at least ceil(leaf_min * active / 64) of its active
+ * lanes wait at a leaf, or no lane is at an inner node; 1..64, 0 = default
+ * 10).  None of these changes a result. */
 typedef struct rta_trace_config {
 	int block, refill_min, steps, blocks_per_cu, leaf_min;
 } rta_trace_config;
@@ -285,7 +285,7 @@ typedef struct rta_trace_config {
  * that refill idle lanes from a queue counter (`queue`, one device word of
  * scratch, cleared by this call with a kernel).  Same results, except that
  * a miss only stores t = FLT_MAX and leaves beta, gamma and ref as they
- * were (the reset sentinel).  cfg may be NULL (defaults 64/16/16/0/6). */
+ * were (the reset sentinel).  cfg may be NULL (defaults 64/16/32/0/10). */
 int rta_trace_cis_persistent(const float *tri4, const rta_node_pair *pairs,
                              const uint32_t *flags, uint32_t root_link,
                              uint32_t root_index, const float *ray_orig,

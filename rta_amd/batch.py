@@ -53,7 +53,7 @@ class MeshBatch:
         self.err = DeviceBuffer(4)
         self.err.fill_u32(0)
         self.queue = DeviceBuffer(4)
-        self.config = TraceConfig(64, 16, 16, 0, 6)
+        self.config = TraceConfig(64, 16, 32, 0, 10)
         # uploads and fills above ran on the default stream
         call("rta_hip_device_synchronize")
 

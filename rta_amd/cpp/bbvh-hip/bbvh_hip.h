@@ -202,7 +202,7 @@ public:
 
 	bbvh_gpu_cis_tracer(rta::ray_generator *gen, binary_lbvh<box_t, tri_t> *bvh, rta::bouncer *b,
 	                    trace_layout layout = trace_layout::persistent,
-	                    rta_trace_config cfg = rta_trace_config{64, 16, 16, 0, 0},
+	                    rta_trace_config cfg = rta_trace_config{64, 16, 32, 0, 0},
 	                    bool shadow = false)
 	    : gpu_raytracer<box_t, tri_t>(gen, b, bvh), bvh(bvh), layout(layout), config(cfg),
 	      shadow(shadow) {

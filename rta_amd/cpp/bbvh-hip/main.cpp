@@ -27,7 +27,7 @@ namespace {
 struct Cmdline {
 	bool verbose = false;
 	hip::trace_layout layout = hip::trace_layout::persistent;
-	rta_trace_config config{64, 16, 16, 0, 0};
+	rta_trace_config config{64, 16, 32, 0, 0};
 	bool shadow = false;
 };
 Cmdline cmdline;

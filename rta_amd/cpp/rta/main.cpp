@@ -247,12 +247,13 @@ int main(int argc, char **argv) {
 			const char *v = next();
 			if (!v) return usage();
 			o.samples = atoi(v);
-		} else if (a == "--sphere-file") {
+		} else if (a == "--sphere-file" || a == "--sphere") {
+			// argp takes unique prefixes: bbvh-cuda-bench.sh writes --sphere
 			const char *v = next();
 			if (!v) return usage();
 			o.sphere_file = v;
 			o.sphere = true;
-		} else if (a == "--distance-factor") {
+		} else if (a == "--distance-factor" || a == "--dist") {
 			const char *v = next();
 			if (!v) return usage();
 			o.distance_factor = (float)atof(v);

@@ -744,9 +744,9 @@ int launch_persistent(const float *tri4, const rta_node_pair *pairs, const uint3
 
 // Validated launch shape with defaults filled in.
 static int resolve_config(const char *name, const rta_trace_config *cfg, rta_trace_config &c) {
-	c = {64, 16, 16, 0, 0};
+	c = {64, 16, 32, 0, 0};
 	if (cfg) c = *cfg;
-	if (c.leaf_min == 0) c.leaf_min = 6;
+	if (c.leaf_min == 0) c.leaf_min = 10;
 	if (c.refill_min < 1 || c.refill_min > 64 || c.steps < 1 || c.blocks_per_cu < 0 ||
 	    c.leaf_min < 1 || c.leaf_min > 64 || (c.block != 64 && c.block != 128 && c.block != 256)) {
 		set_error(std::string(name) + ": bad config (block 64/128/256, refill_min and leaf_min "

@@ -55,7 +55,7 @@ class DynamicFrame:
             raise ValueError("the dynamic frame needs at least two triangles")
         self.cam, self.light = camera, tuple(float(x) for x in light)
         self.sigma, self.seed, self.bounces, self.shadows = sigma, seed, bounces, shadows
-        self.config = config if config is not None else TraceConfig(64, 16, 16, 0, 6)
+        self.config = config if config is not None else TraceConfig(64, 16, 32, 0, 10)
         self.stream = stream if stream is not None else Stream()
         n, R = self.n, camera.w * camera.h
         self.R = R

@@ -209,6 +209,9 @@ def test_driver_series_argument_checks(tmp_path):
     assert r.returncode == 1 and "--samples" in r.stderr
     r = run_driver("--sphere-file", str(tmp_path / "none.ply"), timeout=60)
     assert r.returncode == 1 and "cannot open" in r.stderr
+    # the abbreviation bbvh-cuda-bench.sh uses
+    r = run_driver("--sphere", str(tmp_path / "none.ply"), "--dist", "1.5", timeout=60)
+    assert r.returncode == 1 and "cannot open" in r.stderr
     # -f picks one series out of several
     r = run_driver("-p", "0,0,3", "--sphere-file", str(tmp_path / "none.ply"), "-f", "s",
                    timeout=60)
