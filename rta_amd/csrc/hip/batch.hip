@@ -110,15 +110,14 @@ __global__ __launch_bounds__(1024) void batch_merge_kernel(
 	                (int)m.n);
 }
 
-__global__ __launch_bounds__(256) void batch_permute_kernel(
+__global__ __launch_bounds__(kPermuteBlock) void batch_permute_kernel(
     const BatchItem *__restrict__ items, const BatchMesh *__restrict__ meshes,
     rta_triangle *__restrict__ out, const rta_triangle *__restrict__ scratch,
     const uint32_t *__restrict__ indices) {
 	const BatchItem it = items[blockIdx.x];
 	const BatchMesh m = meshes[it.mesh];
-	const int i = (int)it.block * 256 + (int)threadIdx.x;
-	if (i >= (int)m.n) return;
-	permute_one(out + m.tri_off, scratch + m.tri_off, indices + m.tri_off, i);
+	permute_block(out + m.tri_off, scratch + m.tri_off, indices + m.tri_off, (int)m.n,
+	              (int)it.block);
 }
 
 static inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }

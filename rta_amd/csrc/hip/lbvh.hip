@@ -106,12 +106,10 @@ static BoxesWork carve_boxes(void *base, int n) {
 	return w;
 }
 
-__global__ __launch_bounds__(256) void permute_kernel(
+__global__ __launch_bounds__(kPermuteBlock) void permute_kernel(
     rta_triangle *__restrict__ out, const rta_triangle *__restrict__ scratch,
     const uint32_t *__restrict__ indices, int n) {
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= n) return;
-	permute_one(out, scratch, indices, i);
+	permute_block(out, scratch, indices, n, (int)blockIdx.x);
 }
 
 // --------------------------------------------------------- workspace
@@ -246,8 +244,8 @@ int rta_lbvh_permute(rta_triangle *out, const rta_triangle *scratch, const uint3
                      int n, rta_stream s) {
 	if (n <= 0) return invalid("rta_lbvh_permute: n must be > 0");
 	if (!out || !scratch || !indices) return invalid("rta_lbvh_permute: NULL pointer");
-	hipLaunchKernelGGL(permute_kernel, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(s), out,
-	                   scratch, indices, n);
+	hipLaunchKernelGGL(permute_kernel, dim3(grid_for(n, kPermuteBlock)), dim3(kPermuteBlock), 0,
+	                   as_stream(s), out, scratch, indices, n);
 	RTA_LAUNCH_CHECK("permute_kernel");
 	return RTA_OK;
 }

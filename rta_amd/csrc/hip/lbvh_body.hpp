@@ -255,6 +255,19 @@ __device__ inline uint32_t rep_code(const uint32_t *codes, uint32_t node, uint32
 
 // Level 1 for leaves [chunk*kChunk, +kChunk) of a mesh; the frontier goes
 // to slots[0..kSlots) and its size to *count.
+//
+// An inner node whose range lies inside the chunk has its index inside it
+// too (Karras: a node's index is one end of its range), and so do its
+// children.  The block therefore first loads, coalesced, everything the
+// climbs read: representative codes, each such node's children as 16-bit
+// local references (kLocalLeaf marks a leaf), and each node's parent as a
+// local index, kParentExit when the parent's range leaves the chunk (the
+// climb stops and records a frontier entry) or kParentRoot for the root.
+// The climbs then touch LDS only.
+constexpr uint16_t kLocalLeaf = 0x8000u;
+constexpr uint16_t kParentExit = 0xffffu;
+constexpr uint16_t kParentRoot = 0xfffeu;
+
 __device__ inline void bottom_up_chunk(rta_node *__restrict__ nodes,
                                        const rta_aabb *__restrict__ boxes,
                                        const uint32_t *__restrict__ codes,
@@ -268,6 +281,10 @@ __device__ inline void bottom_up_chunk(rta_node *__restrict__ nodes,
 	__shared__ uint32_t lvisit[kChunk];
 	__shared__ uint32_t ltris[kChunk];
 	__shared__ uint32_t lelems[kChunk];
+	__shared__ uint32_t lcode[kChunk];
+	__shared__ uint32_t lkids[kChunk];      // left | right << 16, ~0u: range leaves the chunk
+	__shared__ uint16_t lpar_leaf[kChunk];
+	__shared__ uint16_t lpar_int[kChunk];
 	__shared__ uint32_t front[kSlots];
 	__shared__ uint32_t nfront;
 
@@ -275,46 +292,68 @@ __device__ inline void bottom_up_chunk(rta_node *__restrict__ nodes,
 	const uint32_t base = (uint32_t)chunk * kChunk;
 	const uint32_t leaf0 = (uint32_t)(n - 1);
 	const uint32_t end = min(base + (uint32_t)kChunk, (uint32_t)n);
+	const uint32_t k = base + t;          // leaf k and inner node k
 	lvisit[t] = 0;
 	lelems[t] = kStateNone;
 	if (t == 0) nfront = 0;
-	const uint32_t k = base + t;
 	rta_aabb box;
 	if (k < (uint32_t)n) {
 		box = boxes[indices[k]];
 		lb_leaf[0][t] = box.min.x; lb_leaf[1][t] = box.min.y; lb_leaf[2][t] = box.min.z;
 		lb_leaf[3][t] = box.max.x; lb_leaf[4][t] = box.max.y; lb_leaf[5][t] = box.max.z;
+		lcode[t] = codes[k];
 	}
+	const bool inner = k < leaf0 && k < end;
+	if (inner) {
+		const uint2 rg = ranges[k];
+		uint32_t kids = ~0u;
+		if (rg.x >= base && rg.y < end) {
+			const uint2 pl = *reinterpret_cast<const uint2 *>(&nodes[k]);
+			const uint32_t l = pl.x >> 1, r = pl.y;
+			const uint32_t ll = l >= leaf0 ? (l - leaf0 - base) | kLocalLeaf : l - base;
+			const uint32_t rl = r >= leaf0 ? (r - leaf0 - base) | kLocalLeaf : r - base;
+			kids = ll | (rl << 16);
+		}
+		lkids[t] = kids;
+	}
+	__syncthreads();
+	auto local_parent = [&](uint32_t node) -> uint16_t {
+		if (node == 0u) return kParentRoot;
+		const uint32_t p = parents[node];
+		if (p < base || p >= end || p >= leaf0 || lkids[p - base] == ~0u) return kParentExit;
+		return (uint16_t)(p - base);
+	};
+	if (k < (uint32_t)n) lpar_leaf[t] = local_parent(leaf0 + k);
+	if (inner) lpar_int[t] = local_parent(k);
 	__syncthreads();
 
 	if (k < (uint32_t)n) {
-		uint32_t cur = leaf0 + k;
-		for (int level = 0; cur != 0u && level < 128; ++level) {
-			const uint32_t parent = parents[cur];
-			const uint2 rg = ranges[parent];
-			if (rg.x < base || rg.y >= end) {
+		uint32_t cur = (uint32_t)t | kLocalLeaf;
+		uint32_t p = lpar_leaf[t];
+		for (int level = 0; level < 128; ++level) {
+			if (p == kParentRoot) break;
+			if (p == kParentExit) {
+				const uint32_t g = (cur & kLocalLeaf) ? leaf0 + base + (cur & 0x3ffu) : base + cur;
 				const uint32_t slot = atomicAdd(&nfront, 1u);
-				if (slot < (uint32_t)kSlots) front[slot] = cur;
+				if (slot < (uint32_t)kSlots) front[slot] = g;
 				else atomicOr(status, 2u);
 				break;
 			}
-			const uint32_t p = parent - base;
 			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
 			const uint32_t old = atomicAdd(&lvisit[p], 1u);
 			if (old == 0u) break;
 			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 
-			const uint2 pl = *reinterpret_cast<const uint2 *>(&nodes[parent]);
-			const uint32_t left = pl.x >> 1, right = pl.y;
+			const uint32_t kids = lkids[p];
+			const uint32_t left = kids & 0xffffu, right = kids >> 16;
 			const bool cur_is_left = (left == cur);
 			const uint32_t sib = cur_is_left ? right : left;
+			const uint32_t q = sib & 0x3ffu;
 			rta_aabb sbox;
-			if (sib >= leaf0) {
-				const uint32_t q = sib - leaf0 - base;
+			if (sib & kLocalLeaf) {
 				sbox.min = {lb_leaf[0][q], lb_leaf[1][q], lb_leaf[2][q]};
 				sbox.max = {lb_leaf[3][q], lb_leaf[4][q], lb_leaf[5][q]};
 			} else {
-				const uint32_t q = sib - base;
 				sbox.min = {lb_int[0][q], lb_int[1][q], lb_int[2][q]};
 				sbox.max = {lb_int[3][q], lb_int[4][q], lb_int[5][q]};
 			}
@@ -322,20 +361,24 @@ __device__ inline void bottom_up_chunk(rta_node *__restrict__ nodes,
 			lb_int[0][p] = box.min.x; lb_int[1][p] = box.min.y; lb_int[2][p] = box.min.z;
 			lb_int[3][p] = box.max.x; lb_int[4][p] = box.max.y; lb_int[5][p] = box.max.z;
 
-			// children state: original leaf, collapsed inner node, or inner
+			// children state: original leaf, collapsed inner node, or inner.
+			// The representative code of leaf j and of inner node j are both
+			// codes[j] (lbvh-kernels.cu:261), i.e. lcode of the local index.
 			uint32_t state = kStateInner, ptris = 0;
-			if (rep_code(codes, cur, leaf0) == rep_code(codes, sib, leaf0)) {
+			if (lcode[cur & 0x3ffu] == lcode[q]) {
 				uint32_t le, lt, re, rt;
 				bool l_leaf, r_leaf;
-				if (left >= leaf0) { l_leaf = true; le = 1; lt = left - leaf0; }
-				else { le = lelems[left - base]; lt = ltris[left - base]; l_leaf = le != kStateInner; }
-				if (right >= leaf0) { r_leaf = true; re = 1; rt = right - leaf0; }
-				else { re = lelems[right - base]; rt = ltris[right - base]; r_leaf = re != kStateInner; }
+				const uint32_t li = left & 0x3ffu, ri = right & 0x3ffu;
+				if (left & kLocalLeaf) { l_leaf = true; le = 1; lt = base + li; }
+				else { le = lelems[li]; lt = ltris[li]; l_leaf = le != kStateInner; }
+				if (right & kLocalLeaf) { r_leaf = true; re = 1; rt = base + ri; }
+				else { re = lelems[ri]; rt = ltris[ri]; r_leaf = re != kStateInner; }
 				if (l_leaf && r_leaf) { state = re + le; ptris = lt < rt ? lt : rt; }
 			}
 			ltris[p] = ptris;
 			lelems[p] = state;
-			cur = parent;
+			cur = p;
+			p = lpar_int[p];
 		}
 	}
 	__syncthreads();
@@ -447,17 +490,38 @@ __device__ inline void bottom_up_merge(rta_node *__restrict__ nodes, uint32_t *_
 // -------------------------------------------------- stage 7: permute
 
 // out[i] = scratch[indices[i]] (invert_triangle_permutation_cuda,
-// lbvh-kernels.cu:557-578): one 104-byte triangle as thirteen 8-byte words.
-__device__ inline void permute_one(rta_triangle *__restrict__ out,
-                                   const rta_triangle *__restrict__ scratch,
-                                   const uint32_t *__restrict__ indices, int i) {
-	const uint2 *s = reinterpret_cast<const uint2 *>(scratch + indices[i]);
-	uint2 *d = reinterpret_cast<uint2 *>(out + i);
-	uint2 w[kTriWords];
+// lbvh-kernels.cu:557-578) for the kPermuteBlock triangles of one block,
+// as 13 * kPermuteBlock consecutive 8-byte output words: word w comes from
+// word w % 13 of triangle indices[block * kPermuteBlock + w / 13].  Every
+// store of a wave is contiguous and each scattered source record is read
+// by 13 neighbouring lanes.  All threads of the block must call it.
+constexpr int kPermuteBlock = 256;
+__device__ inline void permute_block(rta_triangle *__restrict__ out,
+                                     const rta_triangle *__restrict__ scratch,
+                                     const uint32_t *__restrict__ indices, int n, int block) {
+	__shared__ uint32_t src[kPermuteBlock];
+	const int t = threadIdx.x;
+	const int base = block * kPermuteBlock;
+	const int count = min(kPermuteBlock, n - base);
+	if (t < count) src[t] = indices[base + t];
+	__syncthreads();
+	const uint2 *s = reinterpret_cast<const uint2 *>(scratch);
+	uint2 *d = reinterpret_cast<uint2 *>(out + base);
+	const int nwords = count * kTriWords;
+	uint2 v[kTriWords];
 #pragma unroll
-	for (int k = 0; k < kTriWords; ++k) w[k] = s[k];
+	for (int k = 0; k < kTriWords; ++k) {
+		const int w = t + k * kPermuteBlock;
+		if (w < nwords) {
+			const int r = w / kTriWords;
+			v[k] = s[(size_t)src[r] * kTriWords + (w - r * kTriWords)];
+		}
+	}
 #pragma unroll
-	for (int k = 0; k < kTriWords; ++k) d[k] = w[k];
+	for (int k = 0; k < kTriWords; ++k) {
+		const int w = t + k * kPermuteBlock;
+		if (w < nwords) d[w] = v[k];
+	}
 }
 
 }  // namespace rta_hip
