@@ -281,6 +281,11 @@ def cpu_baseline(tris_h, bo, bd, bt, n_bounce, w, max_rays):
     t0 = time.perf_counter()
     mb = oracle.median_build(tris_h)
     build_s = time.perf_counter() - t0
+    # the CPU reference point for the LBVH build: the oracle's single-threaded
+    # restatement of the seven lbvh-kernels.cu stages on the same triangles
+    t0 = time.perf_counter()
+    oracle.lbvh_build(tris_h)
+    lbvh_s = time.perf_counter() - t0
     rows = n_bounce // w
     if max_rays:
         rows = min(rows, max(1, max_rays // w))
@@ -301,6 +306,9 @@ def cpu_baseline(tris_h, bo, bd, bt, n_bounce, w, max_rays):
                    f"oracle's child-is tracer on its spatial-median tree (built single-threaded "
                    f"in {build_s:.1f} s, not timed), OpenMP schedule(dynamic,32), {cpu}"),
         "median_build_s": build_s,
+        "lbvh_build_s": lbvh_s,
+        "lbvh_build_mtri_s": len(tris_h) / lbvh_s / 1e6,
+        "lbvh_build_threads": 1,
     }
 
 

@@ -5,8 +5,8 @@
 //   1. per-tile digit histograms (LDS counters)           hist[digit][tile]
 //   2. per-digit exclusive scan over tiles (one block per digit), digit totals
 //   3. per-tile stable ranking with wave ballots (match-any over the digit
-//      bits, popcount of the lower-lane mask) and LDS running counters, then
-//      scatter to digit_base + tile offset + rank.
+//      bits, popcount of the lower-lane mask) and LDS prefix counts, then a
+//      digit-grouped write of the tile (contiguous runs per digit).
 // No kernel waits on another block, so there is no spin loop anywhere.
 // A stable sort is unique, so the output equals std::stable_sort by code.
 #include "internal.hpp"
@@ -83,34 +83,52 @@ __global__ __launch_bounds__(kThreads) void rs_scan_kernel(uint32_t *__restrict_
 	if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
 
+// Per tile: rank every key among the tile's keys of its digit in index
+// order (stable), then write the tile's keys grouped by digit so that each
+// digit's run is stored contiguously.
+//  1. every wave ranks its 64 keys of item `it` with ballots (match-any over
+//     the digit bits, popcount of the lower lanes) and records the per-digit
+//     count of (it, wave) in LDS;
+//  2. one thread per digit turns the counts over (it, wave) -- the tile's
+//     index order -- into exclusive prefixes and the digit's tile total;
+//  3. a key's position in the tile is its digit's start in the tile plus
+//     its (it, wave) prefix plus its rank; keys and payloads go to LDS there,
+//     and the block writes them out in tile order, key j of digit d to
+//     digit_base[d] + (j - start of d in the tile).
 __global__ __launch_bounds__(kThreads) void rs_scatter_kernel(
     const uint32_t *__restrict__ keys_in, const uint32_t *__restrict__ vals_in,
     uint32_t *__restrict__ keys_out, uint32_t *__restrict__ vals_out, int n, int shift,
     const uint32_t *__restrict__ hist, const uint32_t *__restrict__ totals, int tiles) {
-	__shared__ uint32_t base[kBins];
-	__shared__ uint32_t running[kBins];
-	__shared__ uint32_t wcnt[kWaves][kBins];
+	__shared__ union {
+		uint16_t cnt[kItems][kWaves][kBins];  // counts, then their prefixes
+		uint32_t kv[2][kTile];                // the tile's keys and payloads by digit
+	} sm;
+	__shared__ uint32_t gbase[kBins];        // where digit d of this tile starts in the output
+	__shared__ uint32_t lbase[kBins];        // where digit d starts within the tile
 	__shared__ uint32_t wave_sums[kWaves];
 
 	const int t = threadIdx.x;
 	const int wv = t / kWave;
+	uint32_t *cnt_words = reinterpret_cast<uint32_t *>(&sm.cnt[0][0][0]);
+	for (int i = t; i < kItems * kWaves * kBins / 2; i += kThreads) cnt_words[i] = 0u;
 	{
 		uint32_t dummy;
 		const uint32_t digit_base = block_exclusive_scan(totals[t], wave_sums, dummy);
-		base[t] = digit_base + hist[(size_t)t * tiles + blockIdx.x];
-		running[t] = 0;
-		for (int w = 0; w < kWaves; ++w) wcnt[w][t] = 0;
+		gbase[t] = digit_base + hist[(size_t)t * tiles + blockIdx.x];
 	}
 	__syncthreads();
 
 	const uint64_t lt = lanemask_lt();
 	const long long tile_base = (long long)blockIdx.x * kTile;
+	const int count = (int)min((long long)kTile, (long long)n - tile_base);
+	uint32_t key[kItems], val[kItems], pos[kItems];
+#pragma unroll
 	for (int it = 0; it < kItems; ++it) {
-		const long long idx = tile_base + it * kThreads + t;
-		const bool valid = idx < n;
-		const uint32_t key = valid ? keys_in[idx] : 0u;
-		const uint32_t val = valid ? vals_in[idx] : 0u;
-		const uint32_t d = (key >> shift) & (kBins - 1);
+		const int j = it * kThreads + t;
+		const bool valid = j < count;
+		key[it] = valid ? keys_in[tile_base + j] : 0u;
+		val[it] = valid ? vals_in[tile_base + j] : 0u;
+		const uint32_t d = (key[it] >> shift) & (kBins - 1);
 		uint64_t match = __ballot(valid);
 #pragma unroll
 		for (int b = 0; b < kBits; ++b) {
@@ -118,26 +136,48 @@ __global__ __launch_bounds__(kThreads) void rs_scatter_kernel(
 			const uint64_t m = __ballot(bit);
 			match &= bit ? m : ~m;
 		}
-		const uint32_t rank = (uint32_t)__popcll(match & lt);
-		const uint32_t cnt = (uint32_t)__popcll(match);
-		const bool leader = valid && rank == 0;
-		if (leader) wcnt[wv][d] = cnt;
-		__syncthreads();
-		if (valid) {
-			uint32_t prefix = 0;
-			for (int w = 0; w < wv; ++w) prefix += wcnt[w][d];
-			const uint32_t pos = base[d] + running[d] + prefix + rank;
-			if (pos < (uint32_t)n) {  // always true for consistent histograms
-				keys_out[pos] = key;
-				vals_out[pos] = val;
-			}
+		pos[it] = (uint32_t)__popcll(match & lt);
+		if (valid && pos[it] == 0u) sm.cnt[it][wv][d] = (uint16_t)__popcll(match);
+	}
+	__syncthreads();
+	uint32_t run = 0;
+	for (int g = 0; g < kItems * kWaves; g += 16) {
+		uint32_t c[16];
+#pragma unroll
+		for (int q = 0; q < 16; ++q) c[q] = sm.cnt[(g + q) / kWaves][(g + q) % kWaves][t];
+#pragma unroll
+		for (int q = 0; q < 16; ++q) {
+			sm.cnt[(g + q) / kWaves][(g + q) % kWaves][t] = (uint16_t)run;
+			run += c[q];
 		}
-		__syncthreads();
-		if (leader) {
-			atomicAdd(&running[d], cnt);
-			wcnt[wv][d] = 0;
+	}
+	{
+		uint32_t dummy;
+		lbase[t] = block_exclusive_scan(run, wave_sums, dummy);
+	}
+	__syncthreads();
+#pragma unroll
+	for (int it = 0; it < kItems; ++it) {
+		const uint32_t d = (key[it] >> shift) & (kBins - 1);
+		pos[it] += lbase[d] + sm.cnt[it][wv][d];
+	}
+	__syncthreads();  // the counts are dead; their LDS now holds the tile
+#pragma unroll
+	for (int it = 0; it < kItems; ++it) {
+		if (it * kThreads + t < count) {
+			sm.kv[0][pos[it]] = key[it];
+			sm.kv[1][pos[it]] = val[it];
 		}
-		__syncthreads();
+	}
+	__syncthreads();
+	for (int j = t; j < count; j += kThreads) {
+		const uint32_t k = sm.kv[0][j];
+		const uint32_t d = (k >> shift) & (kBins - 1);
+		const uint32_t o = gbase[d] + ((uint32_t)j - lbase[d]);
+		if (o < (uint32_t)n) {  // always true for consistent histograms
+			keys_out[o] = k;
+			vals_out[o] = sm.kv[1][j];
+		}
 	}
 }
 
