@@ -449,9 +449,18 @@ __device__ inline void bottom_up_merge(rta_node *__restrict__ nodes, uint32_t *_
 		while (prefix[g + 1] <= f) ++g;
 		uint32_t cur = in_slots[(size_t)g * kSlots + (f - prefix[g])];
 		rta_aabb box = load_box(&nodes[cur]);
+		uint32_t code_cur = rep_code(codes, cur, leaf0);
+		uint32_t parent = cur != 0u ? parents[cur] : 0u;
+		// Three dependent rounds per level: (a) what earlier stages wrote
+		// and nothing changes before this thread does -- the parent's range,
+		// links and code and the grandparent; (b) the visit counter; (c)
+		// after its acquire, what the sibling subtree's thread wrote -- the
+		// sibling's box and the children's (possibly collapsed) links.
 		for (int level = 0; cur != 0u && level < 128; ++level) {
-			const uint32_t parent = parents[cur];
 			const uint2 rg = ranges[parent];
+			const uint2 pl = *reinterpret_cast<const uint2 *>(&nodes[parent]);
+			const uint32_t grand = parent != 0u ? parents[parent] : 0u;
+			const uint32_t code_parent = codes[parent];
 			if (rg.x < lo || rg.y >= hi) {
 				const uint32_t slot = atomicAdd(&nfront, 1u);
 				if (slot < (uint32_t)kSlots) front[slot] = cur;
@@ -461,24 +470,24 @@ __device__ inline void bottom_up_merge(rta_node *__restrict__ nodes, uint32_t *_
 			const uint32_t old = __hip_atomic_fetch_add(&visit[parent], 1u, __ATOMIC_ACQ_REL,
 			                                            __HIP_MEMORY_SCOPE_WORKGROUP);
 			if (old == 0u) break;
-			const uint2 pl = *reinterpret_cast<const uint2 *>(&nodes[parent]);
 			const uint32_t left = pl.x >> 1, right = pl.y;
 			const bool cur_is_left = (left == cur);
 			const uint32_t sib = cur_is_left ? right : left;
 			const rta_aabb sbox = load_box(&nodes[sib]);
+			const uint32_t code_sib = rep_code(codes, sib, leaf0);
+			const uint2 ll = *reinterpret_cast<const uint2 *>(&nodes[left]);
+			const uint2 rl = *reinterpret_cast<const uint2 *>(&nodes[right]);
 			box = cur_is_left ? union_lr(box, sbox) : union_lr(sbox, box);
 			store_box(&nodes[parent], box);
-			if (rep_code(codes, cur, leaf0) == rep_code(codes, sib, leaf0)) {
-				const uint2 ll = *reinterpret_cast<const uint2 *>(&nodes[left]);
-				const uint2 rl = *reinterpret_cast<const uint2 *>(&nodes[right]);
-				if (!link_inner(ll.x) && !link_inner(rl.x)) {
-					const uint32_t tris = ll.y < rl.y ? ll.y : rl.y;
-					const uint32_t elems = (rl.x >> 1) + (ll.x >> 1);
-					*reinterpret_cast<uint2 *>(&nodes[parent]) = make_uint2(elems << 1, tris);
-				}
+			if (code_cur == code_sib && !link_inner(ll.x) && !link_inner(rl.x)) {
+				const uint32_t tris = ll.y < rl.y ? ll.y : rl.y;
+				const uint32_t elems = (rl.x >> 1) + (ll.x >> 1);
+				*reinterpret_cast<uint2 *>(&nodes[parent]) = make_uint2(elems << 1, tris);
 			}
 			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
 			cur = parent;
+			parent = grand;
+			code_cur = code_parent;
 		}
 	}
 	__syncthreads();
