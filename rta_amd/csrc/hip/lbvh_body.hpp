@@ -316,15 +316,17 @@ __device__ inline void bottom_up_chunk(rta_node *__restrict__ nodes,
 		}
 		lkids[t] = kids;
 	}
+	// parents are loaded with the rest; only the decision needs lkids
+	const uint32_t leaf_parent = k < (uint32_t)n && leaf0 + k != 0u ? parents[leaf0 + k] : 0u;
+	const uint32_t inner_parent = inner && k != 0u ? parents[k] : 0u;
 	__syncthreads();
-	auto local_parent = [&](uint32_t node) -> uint16_t {
+	auto local_parent = [&](uint32_t node, uint32_t p) -> uint16_t {
 		if (node == 0u) return kParentRoot;
-		const uint32_t p = parents[node];
 		if (p < base || p >= end || p >= leaf0 || lkids[p - base] == ~0u) return kParentExit;
 		return (uint16_t)(p - base);
 	};
-	if (k < (uint32_t)n) lpar_leaf[t] = local_parent(leaf0 + k);
-	if (inner) lpar_int[t] = local_parent(k);
+	if (k < (uint32_t)n) lpar_leaf[t] = local_parent(leaf0 + k, leaf_parent);
+	if (inner) lpar_int[t] = local_parent(k, inner_parent);
 	__syncthreads();
 
 	if (k < (uint32_t)n) {
