@@ -541,6 +541,13 @@ def main():
         "primary_mrays_s": round(primary_mrays, 3),
         "bounce_mrays_s": round(bounce_mrays, 3),
         "build_mtri_s": round(build_mtri, 3),
+        # compulsory bytes of the build (read 104 B triangles, write 32 B
+        # nodes, 4 B indices and the 104 B reordered triangles) over its time
+        "build_roofline": {
+            "bound": "hbm", "bytes_per_tri": 276, "unit": "GB/s", "peak": HBM_PEAK_GBS,
+            "achieved": round(276 * n / (avg["build"] * 1e-3) / 1e9, 2),
+            "frac": round(276 * n / (avg["build"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        },
         "build_ms": round(avg["build"], 4),
         "pack_ms": round(avg["pack"], 4),
         "bounce_gen_ms": round(avg["bounce_gen"], 4),
