@@ -118,3 +118,63 @@ def test_terrain_benchmark_workload_bit_exact(gpu):
     hit = records_equal(hits_b.download(HIT_DTYPE, nb), ref_b, "bounce")
     assert 0.05 < hit.mean() < 0.95
     say("done")
+
+
+def test_batch_workload_full_size_bit_exact(gpu):
+    """The 256-mesh workload (7,031,454 triangles, 15,327,936 rays): every
+    mesh's tree, permutation, reordered triangles and hits from the batched
+    launches equal the oracle's build and trace of that mesh alone."""
+    from rta_amd.batch import MeshBatch
+
+    say("batch: generate")
+    meshes, rays = scenes.batch_meshes(256)
+    assert sum(len(m) for m in meshes) == 7_031_454
+    assert sum(len(r[2]) for r in rays) == 15_327_936
+    b = MeshBatch(meshes, rays)
+    say("batch: build + trace")
+    b.build()
+    b.trace()
+    b.stream.synchronize()
+    b.check_errors()
+    assert int(b.status.download(np.uint32, 1)[0]) & ~1 == 0
+    say("batch: compare")
+    for m, (mesh, (o, d, t)) in enumerate(zip(meshes, rays)):
+        got = b.mesh_results(m)
+        exp = oracle.lbvh_build(mesh)
+        assert np.array_equal(got["indices"], exp["indices"]), f"mesh {m} indices"
+        assert np.array_equal(bits(got["tris"]), bits(exp["tris"])), f"mesh {m} tris"
+        for f in ("type_left_elems", "right_tris", "min", "max"):
+            assert np.array_equal(got["nodes"][f], exp["nodes"][f]), f"mesh {m} nodes.{f}"
+        ref, deep = oracle.trace_kernel(exp["tris"], exp["nodes"], o, d, t)
+        assert deep == 0
+        g, hit = got["hits"], ref["t"] != F32MAX
+        assert np.array_equal(g["t"] != F32MAX, hit), f"mesh {m} hit flags"
+        for f in ("t", "beta", "gamma"):
+            assert np.array_equal(bits(g[f][hit]), bits(ref[f][hit])), f"mesh {m} {f}"
+        assert np.array_equal(g["ref"][hit], ref["ref"][hit]), f"mesh {m} ref"
+    say("batch: done")
+
+
+def test_dynamic_frame_full_size_bit_exact(gpu):
+    """One eager frame of the 8,388,608-triangle dynamic scene at 3840x2160
+    (jitter, rebuild, primary rays, two diffuse bounces, a shadow ray per
+    hit of each level) against the oracle's chain of the same frame."""
+    from rta_amd.frame import DynamicFrame
+    from test_dynamic import SEED, _compare, _oracle_frame
+
+    say("dynamic: generate")
+    v, i = scenes.instances(64, 256, seed=scenes.DYNAMIC_SEED)
+    assert len(i) // 3 == 8_388_608
+    cam = scenes.dynamic_camera(3840, 2160, n_inst=64)
+    light = scenes.dynamic_light(64)
+    fr = DynamicFrame(v, i, cam, light, scenes.DYNAMIC_SIGMA, SEED, bounces=2, shadows=True)
+    fr.prime()
+    say("dynamic: frame 1")
+    fr.enqueue(frame=1)
+    fr.stream.synchronize()
+    fr.check_errors()
+    res = fr.results()
+    say("dynamic: oracle")
+    exp = _oracle_frame(v, i, cam, light, 1)
+    _compare(res, exp)
+    say("dynamic: done")
