@@ -408,6 +408,64 @@ int rta_shadow_rays(float *out_orig, float *out_dir, float *out_maxt,
                     const float *ray_dir, int n_rays, const float *light,
                     void *work, size_t work_bytes, rta_stream s);
 
+/* ------------------------------------------- learned hit predictor (MLP)
+ * No counterpart in the reference.  A small network proposes, per ray, the
+ * leaf clusters most likely to hold its closest hit (cluster c = sorted
+ * triangle positions [c*n/1024, (c+1)*n/1024)) and a log hit distance;
+ * rta_trace_seed_clusters turns the proposals into a max_t seed and the
+ * exact traversal decides every result.  Network: 102 encoded inputs
+ * (origin mapped to [-1,1] by the scene box, unit direction, sin/cos of
+ * 2^f*pi*v for f = 0..7), three ReLU layers of 64, 1024 cluster logits + 1
+ * log distance; bf16 weights and activations, FP32 accumulation (MFMA),
+ * FP32 master weights and Adam.  Parameters are one packed FP32 array of
+ * rta_pred_param_count floats: W1[64][112], b1[64], W2[64][64], b2[64],
+ * W3[64][64], b3[64], W4[1040][64], b4[1040] (weights out-major; padding
+ * columns/rows stay zero). */
+typedef void *rta_pred;
+int rta_pred_param_count(int *count);
+/* State for batches of up to max_batch rays, He-uniform weights from seed. */
+int rta_pred_create(int max_batch, uint64_t seed, rta_pred *pred);
+int rta_pred_destroy(rta_pred pred);
+int rta_pred_get_params(rta_pred pred, float *host);
+int rta_pred_set_params(rta_pred pred, const float *host);
+int rta_pred_get_grads(rta_pred pred, float *host);
+int rta_pred_zero_grads(rta_pred pred, rta_stream s);
+/* Forward pass of n rays (device arrays; scene_min/max: 3 host floats).
+ * out (n x 1040 f32), top (n x 4 cluster ids, best first) and logdist (n)
+ * are optional device outputs.  Keeps the activations for a backward pass. */
+int rta_pred_forward(rta_pred pred, const float *ray_orig, const float *ray_dir, int n,
+                     const float *scene_min, const float *scene_max, float *out,
+                     int32_t *top, float *logdist, rta_stream s);
+/* Gradients of the last forward pass (added to the gradient buffer):
+ * softmax cross-entropy of the 1024 logits against label[i] (a cluster, or
+ * -1 for a ray without hit, which contributes nothing) plus the squared
+ * error of the log distance against target[i], averaged over the n rays;
+ * the loss is added to *loss (device float). */
+int rta_pred_backward(rta_pred pred, const int32_t *label, const float *target, int n,
+                      float *loss, rta_stream s);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) with the accumulated
+ * gradients; step counts from 1. */
+int rta_pred_adam(rta_pred pred, float lr, int step, rta_stream s);
+
+/* Karras ranges (sorted positions [from, to], 2 words per inner node, n-1
+ * nodes) of the last rta_lbvh_build made with this workspace. */
+int rta_lbvh_copy_ranges(uint32_t *ranges, const void *workspace, int n, rta_stream s);
+
+/* Hit-distance seeds from predicted clusters: for each ray, the closest hit
+ * among the triangles of its 4 clusters top[4*i..] (cluster c = sorted
+ * positions [c*n_tris/1024, (c+1)*n_tris/1024)), found by the packed
+ * traversal restricted to children whose range (inner: `ranges`; leaf: its
+ * first triangle) meets a candidate cluster; t_seed[i] = its t or FLT_MAX.
+ * Every such t is the t of a real hit of the ray (the same triangle test
+ * and max_t), so the closest hit lies at or before it. */
+int rta_trace_seed_clusters(const float *tri4, const rta_node_pair *pairs, const uint32_t *flags,
+                            uint32_t root_link, uint32_t root_index, const uint32_t *ranges,
+                            int n_tris, const float *ray_orig, const float *ray_dir,
+                            const float *max_t, int n_rays, const int32_t *top, float *t_seed,
+                            uint32_t *err, rta_stream s);
+/* out[i] = min(max_t[i], t_seed[i]): the max_t a seeded traversal runs with. */
+int rta_seed_max_t(float *out, const float *max_t, const float *t_seed, int n, rta_stream s);
+
 #ifdef __cplusplus
 }
 #endif

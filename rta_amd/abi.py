@@ -139,6 +139,20 @@ HIP_SIGNATURES = {
                                    _vp],
     "rta_batch_autotune": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _vp,
                            _pf, _vp],
+    "rta_pred_param_count": [_pi],
+    "rta_pred_create": [_i, _u64, C.POINTER(_vp)],
+    "rta_pred_destroy": [_vp],
+    "rta_pred_get_params": [_vp, _vp],
+    "rta_pred_set_params": [_vp, _vp],
+    "rta_pred_get_grads": [_vp, _vp],
+    "rta_pred_zero_grads": [_vp, _vp],
+    "rta_pred_forward": [_vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp],
+    "rta_pred_backward": [_vp, _vp, _vp, _i, _vp, _vp],
+    "rta_pred_adam": [_vp, _f, _i, _vp],
+    "rta_lbvh_copy_ranges": [_vp, _vp, _i, _vp],
+    "rta_trace_seed_clusters": [_vp, _vp, _vp, _u32, _u32, _vp, _i, _vp, _vp, _vp, _i, _vp, _vp,
+                                _vp, _vp],
+    "rta_seed_max_t": [_vp, _vp, _vp, _i, _vp],
 }
 
 HOST_SIGNATURES = {

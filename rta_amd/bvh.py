@@ -138,6 +138,13 @@ class LbvhConstructor:
                 print("lbvh stage times [ms]:", self.last_times)
         return BinaryLbvh(tris, n, self._nodes, self._idx)
 
+    def ranges(self, stream=None) -> DeviceBuffer:
+        """Karras ranges [from, to] (sorted positions) of the inner nodes of
+        the last build, two words per node, for rta_trace_seed_clusters."""
+        out = DeviceBuffer(8 * max(self._n - 1, 1))
+        call("rta_lbvh_copy_ranges", ptr(out), ptr(self._ws), self._n, ptr(stream))
+        return out
+
 
 class RayBuffers:
     """Device ray arrays: origin[3R], direction[3R], max_t[R]."""

@@ -256,6 +256,16 @@ int rta_lbvh_workspace_bytes(int n, size_t *bytes) {
 	return RTA_OK;
 }
 
+int rta_lbvh_copy_ranges(uint32_t *ranges, const void *workspace, int n, rta_stream s) {
+	if (n <= 0 || !ranges || !workspace) return invalid("rta_lbvh_copy_ranges: bad arguments");
+	if (n == 1) return RTA_OK;
+	LbvhWorkspace w = carve(const_cast<void *>(workspace), n);
+	BoxesWork b = carve_boxes(w.boxes_work, n);
+	RTA_HIP_TRY(hipMemcpyAsync(ranges, b.ranges, (size_t)(n - 1) * 8, hipMemcpyDeviceToDevice,
+	                           as_stream(s)));
+	return RTA_OK;
+}
+
 int rta_lbvh_build(rta_triangle *tris, int n, rta_node *nodes, uint32_t *indices, void *workspace,
                    size_t workspace_bytes, uint32_t *status, rta_lbvh_times *times,
                    rta_stream s) {

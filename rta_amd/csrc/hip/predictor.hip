@@ -1,0 +1,522 @@
+// Learned hit predictor: a small MLP from an encoded ray to scores over the
+// LBVH's leaf clusters plus a log hit distance, with hand-written BF16 MFMA
+// kernels for inference and training (FP32 accumulate, FP32 master weights,
+// Adam).  The predictor has no counterpart in the reference; it only
+// proposes candidate clusters whose hit distance seeds a ray's max_t, and the
+// exact traversal decides every result.
+//
+// Network (widths padded to multiples of 16 for the 16x16x16 MFMA tiles):
+//   x[112]  = 6 normalised inputs (origin mapped to [-1,1] by the scene box,
+//             unit direction) and sin/cos(2^f pi v) for f = 0..7 of each
+//             (102 features), zero padded
+//   h1..h3  = ReLU(W h + b), 64 wide, bf16
+//   out[1040] = W4 h3 + b4: 1024 cluster logits, the log hit distance at
+//             column 1024, zero padding
+// Weights are stored out-major ([out][in], the input index contiguous), so
+// both MFMA operands of the forward pass are one 8-byte load per lane.
+#include "common.hpp"
+
+#include <cstring>
+#include <new>
+#include <vector>
+
+namespace rta_hip {
+
+namespace pred {
+
+constexpr int kIn = 102, kInP = 112;
+constexpr int kHid = 64;
+constexpr int kClusters = 1024;
+constexpr int kOut = kClusters + 1, kOutP = 1040;
+constexpr int kTop = 4;
+
+// parameter offsets (floats) in the packed master array
+constexpr int oW1 = 0, oB1 = oW1 + kHid * kInP;
+constexpr int oW2 = oB1 + kHid, oB2 = oW2 + kHid * kHid;
+constexpr int oW3 = oB2 + kHid, oB3 = oW3 + kHid * kHid;
+constexpr int oW4 = oB3 + kHid, oB4 = oW4 + kOutP * kHid;
+constexpr int kParams = oB4 + kOutP;  // 83,152 with padding (81,537 used)
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// float -> bf16, round to nearest even (finite inputs)
+__device__ __host__ inline uint16_t to_bf16(float f) {
+	uint32_t u;
+	memcpy(&u, &f, 4);
+	u += 0x7fffu + ((u >> 16) & 1u);
+	return (uint16_t)(u >> 16);
+}
+__device__ inline float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ inline v4s load4(const uint16_t *p) {
+	const uint2 w = *reinterpret_cast<const uint2 *>(p);
+	v4s r;
+	r[0] = (short)(w.x & 0xffffu); r[1] = (short)(w.x >> 16);
+	r[2] = (short)(w.y & 0xffffu); r[3] = (short)(w.y >> 16);
+	return r;
+}
+
+// ---------------------------------------------------------- encoding
+
+__global__ __launch_bounds__(256) void encode_kernel(uint16_t *__restrict__ x,
+                                                     const float *__restrict__ ro,
+                                                     const float *__restrict__ rd, int n,
+                                                     float3 lo, float3 scale) {
+	const int r = blockIdx.x * 256 + threadIdx.x;
+	if (r >= n) return;
+	float v[6];
+	v[0] = (ro[3 * (size_t)r + 0] - lo.x) * scale.x - 1.0f;
+	v[1] = (ro[3 * (size_t)r + 1] - lo.y) * scale.y - 1.0f;
+	v[2] = (ro[3 * (size_t)r + 2] - lo.z) * scale.z - 1.0f;
+	v[3] = rd[3 * (size_t)r + 0];
+	v[4] = rd[3 * (size_t)r + 1];
+	v[5] = rd[3 * (size_t)r + 2];
+	uint16_t *row = x + (size_t)r * kInP;
+	for (int j = 0; j < 6; ++j) row[j] = to_bf16(v[j]);
+	for (int j = 0; j < 6; ++j) {
+		float f = 1.0f;
+		for (int o = 0; o < 8; ++o) {
+			row[6 + 16 * j + 2 * o] = to_bf16(sinpif(f * v[j]));
+			row[6 + 16 * j + 2 * o + 1] = to_bf16(cospif(f * v[j]));
+			f *= 2.0f;
+		}
+	}
+	for (int j = kIn; j < kInP; ++j) row[j] = 0;
+}
+
+// ------------------------------------------------------------ GEMMs
+
+// One wave, one 16x16 tile of Y = X W^T: X [m][k] row-major (ldx), W [nn][k]
+// row-major (ldw); rows >= m read as zero.  Lane l holds the tile elements
+// (4*(l/16) + i, l%16), i = 0..3.
+__device__ inline v4f tile_xwt(const uint16_t *X, int ldx, const uint16_t *W, int ldw, int K,
+                               int r0, int c0, int m) {
+	const int lane = __lane_id();
+	const int row = r0 + (lane & 15), col = c0 + (lane & 15), kq = 4 * (lane >> 4);
+	v4f acc = {0.f, 0.f, 0.f, 0.f};
+	for (int k0 = 0; k0 < K; k0 += 16) {
+		v4s a = {0, 0, 0, 0};
+		if (row < m) a = load4(X + (size_t)row * ldx + k0 + kq);
+		const v4s b = load4(W + (size_t)col * ldw + k0 + kq);
+		acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+	}
+	return acc;
+}
+
+// Y = act(X W^T + b): one tile per wave, 4 waves per block.  OUT_F32: the
+// last layer (f32, no activation); otherwise ReLU into bf16.
+template <bool OUT_F32>
+__global__ __launch_bounds__(256) void fwd_kernel(void *__restrict__ y, int ldy,
+                                                  const uint16_t *__restrict__ x, int ldx,
+                                                  const uint16_t *__restrict__ w,
+                                                  const float *__restrict__ b, int K, int N,
+                                                  int m) {
+	const int tiles_n = N / 16;
+	const int tile = blockIdx.x * 4 + threadIdx.x / kWave;
+	const int tiles = ((m + 15) / 16) * tiles_n;
+	if (tile >= tiles) return;
+	const int r0 = (tile / tiles_n) * 16, c0 = (tile % tiles_n) * 16;
+	const v4f acc = tile_xwt(x, ldx, w, K, K, r0, c0, m);
+	const int lane = __lane_id();
+	const int col = c0 + (lane & 15);
+	for (int i = 0; i < 4; ++i) {
+		const int row = r0 + 4 * (lane >> 4) + i;
+		if (row >= m) continue;
+		const float v = acc[i] + b[col];
+		if (OUT_F32) reinterpret_cast<float *>(y)[(size_t)row * ldy + col] = v;
+		else reinterpret_cast<uint16_t *>(y)[(size_t)row * ldy + col] = to_bf16(v > 0.f ? v : 0.f);
+	}
+}
+
+// dX = (dY W) * [x_prev > 0]: dY [m][N] (lddy), W [N][K] row-major; tile of
+// dX [m][K].  The B operand B[n][k] = W[n][k] is read with stride K.
+__global__ __launch_bounds__(256) void bwd_data_kernel(uint16_t *__restrict__ dx,
+                                                       const uint16_t *__restrict__ dy, int lddy,
+                                                       const uint16_t *__restrict__ w, int K,
+                                                       int N, const uint16_t *__restrict__ xprev,
+                                                       int m) {
+	const int tiles_k = K / 16;
+	const int tile = blockIdx.x * 4 + threadIdx.x / kWave;
+	const int tiles = ((m + 15) / 16) * tiles_k;
+	if (tile >= tiles) return;
+	const int r0 = (tile / tiles_k) * 16, c0 = (tile % tiles_k) * 16;
+	const int lane = __lane_id();
+	const int row = r0 + (lane & 15), col = c0 + (lane & 15), kq = 4 * (lane >> 4);
+	v4f acc = {0.f, 0.f, 0.f, 0.f};
+	for (int n0 = 0; n0 < N; n0 += 16) {
+		v4s a = {0, 0, 0, 0};
+		if (row < m) a = load4(dy + (size_t)row * lddy + n0 + kq);
+		v4s bb;
+		for (int i = 0; i < 4; ++i) bb[i] = (short)w[(size_t)(n0 + kq + i) * K + col];
+		acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bb, acc, 0, 0, 0);
+	}
+	for (int i = 0; i < 4; ++i) {
+		const int r = r0 + 4 * (lane >> 4) + i;
+		if (r >= m) continue;
+		const size_t at = (size_t)r * K + col;
+		dx[at] = xprev[at] != 0 ? to_bf16(acc[i]) : (uint16_t)0;
+	}
+}
+
+// dW += dY^T X over a chunk of kRowChunk rows (fp32 atomics): dY [m][N]
+// (lddy), X [m][K] (ldx); tile of dW [N][K].  Both operands are read with a
+// row stride.
+constexpr int kRowChunk = 256;
+__global__ __launch_bounds__(256) void bwd_weight_kernel(float *__restrict__ dw,
+                                                         const uint16_t *__restrict__ dy, int lddy,
+                                                         const uint16_t *__restrict__ x, int ldx,
+                                                         int K, int N, int m) {
+	const int tiles_k = K / 16, tiles_n = N / 16;
+	const int chunks = (m + kRowChunk - 1) / kRowChunk;
+	const int tile = blockIdx.x * 4 + threadIdx.x / kWave;
+	if (tile >= tiles_n * tiles_k * chunks) return;
+	const int chunk = tile / (tiles_n * tiles_k);
+	const int t2 = tile % (tiles_n * tiles_k);
+	const int n0 = (t2 / tiles_k) * 16, k0 = (t2 % tiles_k) * 16;
+	const int lane = __lane_id();
+	const int nn = n0 + (lane & 15), kk = k0 + (lane & 15), rq = 4 * (lane >> 4);
+	v4f acc = {0.f, 0.f, 0.f, 0.f};
+	const int rend = min(m, (chunk + 1) * kRowChunk);
+	for (int rb = chunk * kRowChunk; rb < rend; rb += 16) {
+		v4s a, b;
+		for (int i = 0; i < 4; ++i) {
+			const int r = rb + rq + i;
+			a[i] = r < m ? (short)dy[(size_t)r * lddy + nn] : (short)0;
+			b[i] = r < m ? (short)x[(size_t)r * ldx + kk] : (short)0;
+		}
+		acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+	}
+	for (int i = 0; i < 4; ++i)
+		atomicAdd(&dw[(size_t)(n0 + rq + i) * K + kk], acc[i]);
+}
+
+// db += column sums of dY over kRowChunk-row chunks
+__global__ __launch_bounds__(256) void bwd_bias_kernel(float *__restrict__ db,
+                                                       const uint16_t *__restrict__ dy, int lddy,
+                                                       int N, int m) {
+	const int col = blockIdx.x * 256 + threadIdx.x;
+	if (col >= N) return;
+	const int r0 = blockIdx.y * kRowChunk;
+	const int r1 = min(m, r0 + kRowChunk);
+	float s = 0.f;
+	for (int r = r0; r < r1; ++r) s += from_bf16(dy[(size_t)r * lddy + col]);
+	atomicAdd(&db[col], s);
+}
+
+// --------------------------------------------------------- loss, top-k
+
+// One wave per row: softmax cross-entropy over the 1024 cluster logits with
+// the label's cluster and a squared error on the log distance, both for
+// rows with a hit (label >= 0), averaged over the m rows.  Writes dOut
+// (bf16, zero on padding columns and on rows without a hit) and adds the
+// loss to *loss.
+__global__ __launch_bounds__(256) void loss_kernel(uint16_t *__restrict__ dout,
+                                                   const float *__restrict__ out,
+                                                   const int32_t *__restrict__ label,
+                                                   const float *__restrict__ target, int m,
+                                                   float *__restrict__ loss) {
+	const int row = blockIdx.x * 4 + threadIdx.x / kWave;
+	if (row >= m) return;
+	const int lane = __lane_id();
+	const float *o = out + (size_t)row * kOutP;
+	uint16_t *g = dout + (size_t)row * kOutP;
+	const int lab = label[row];
+	if (lab < 0) {
+		for (int c = lane; c < kOutP; c += kWave) g[c] = 0;
+		return;
+	}
+	float mx = -FLT_MAX;
+	for (int c = lane; c < kClusters; c += kWave) mx = fmaxf(mx, o[c]);
+	for (int off = kWave / 2; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+	float se = 0.f;
+	for (int c = lane; c < kClusters; c += kWave) se += expf(o[c] - mx);
+	for (int off = kWave / 2; off > 0; off >>= 1) se += __shfl_xor(se, off);
+	const float inv_m = 1.0f / (float)m;
+	for (int c = lane; c < kOutP; c += kWave) {
+		float gv = 0.f;
+		if (c < kClusters) gv = (expf(o[c] - mx) / se - (c == lab ? 1.f : 0.f)) * inv_m;
+		else if (c == kClusters) gv = 2.f * (o[c] - target[row]) * inv_m;
+		g[c] = to_bf16(gv);
+	}
+	if (lane == 0) {
+		const float d = o[kClusters] - target[row];
+		atomicAdd(loss, ((logf(se) + mx - o[lab]) + d * d) * inv_m);
+	}
+}
+
+// One wave per row: the kTop highest logits (ties: lower cluster first) and
+// the predicted log distance.
+__global__ __launch_bounds__(256) void topk_kernel(int32_t *__restrict__ top,
+                                                   float *__restrict__ logdist,
+                                                   const float *__restrict__ out, int m) {
+	const int row = blockIdx.x * 4 + threadIdx.x / kWave;
+	if (row >= m) return;
+	const int lane = __lane_id();
+	const float *o = out + (size_t)row * kOutP;
+	float v[kClusters / kWave];
+	for (int j = 0; j < kClusters / kWave; ++j) v[j] = o[lane + kWave * j];
+	for (int t = 0; t < kTop; ++t) {
+		float best = -FLT_MAX;
+		int bi = 0x7fffffff;
+		for (int j = 0; j < kClusters / kWave; ++j)
+			if (v[j] > best) { best = v[j]; bi = lane + kWave * j; }
+		for (int off = kWave / 2; off > 0; off >>= 1) {
+			const float ob = __shfl_xor(best, off);
+			const int oi = __shfl_xor(bi, off);
+			if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+		}
+		if (top && lane == 0) top[(size_t)row * kTop + t] = bi;
+		if ((bi & (kWave - 1)) == lane) v[bi / kWave] = -FLT_MAX;
+	}
+	if (logdist && lane == 0) logdist[row] = o[kClusters];
+}
+
+// Adam on the packed parameters; refreshes the bf16 weight copies.
+__global__ __launch_bounds__(256) void adam_kernel(float *__restrict__ p, float *__restrict__ mo,
+                                                   float *__restrict__ ve,
+                                                   const float *__restrict__ g,
+                                                   uint16_t *__restrict__ wb, float lr, float c1,
+                                                   float c2) {
+	const int i = blockIdx.x * 256 + threadIdx.x;
+	if (i >= kParams) return;
+	const float gi = g[i];
+	const float m1 = 0.9f * mo[i] + 0.1f * gi;
+	const float v1 = 0.999f * ve[i] + 0.001f * gi * gi;
+	mo[i] = m1;
+	ve[i] = v1;
+	const float pi = p[i] - lr * (m1 * c1) / (sqrtf(v1 * c2) + 1e-8f);
+	p[i] = pi;
+	wb[i] = to_bf16(pi);
+}
+
+__global__ __launch_bounds__(256) void to_bf16_kernel(uint16_t *__restrict__ wb,
+                                                      const float *__restrict__ p) {
+	const int i = blockIdx.x * 256 + threadIdx.x;
+	if (i < kParams) wb[i] = to_bf16(p[i]);
+}
+
+// counter-based uniform in [-1, 1)
+__host__ inline float init_uniform(uint64_t seed, uint64_t i) {
+	uint64_t z = seed * 0x9e3779b97f4a7c15ull + i + 1;
+	z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+	z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+	z ^= z >> 31;
+	return (float)(z >> 40) * (2.0f / 16777216.0f) - 1.0f;
+}
+
+struct State {
+	int max_batch = 0;
+	float *params = nullptr, *mom = nullptr, *vel = nullptr, *grads = nullptr;
+	uint16_t *wb = nullptr;      // bf16 copy of params (weights used from it)
+	uint16_t *x = nullptr, *h1 = nullptr, *h2 = nullptr, *h3 = nullptr;
+	float *out = nullptr;
+	uint16_t *dout = nullptr, *dh3 = nullptr, *dh2 = nullptr, *dh1 = nullptr;
+	int n_fwd = 0;               // rows of the last forward pass
+	std::vector<void *> allocs;
+	template <typename T> int alloc(T *&p, size_t count) {
+		void *q = nullptr;
+		if (hipMalloc(&q, count * sizeof(T)) != hipSuccess) return RTA_E_INVALID;
+		allocs.push_back(q);
+		p = reinterpret_cast<T *>(q);
+		return RTA_OK;
+	}
+	~State() {
+		for (void *q : allocs) (void)hipFree(q);
+	}
+};
+
+inline unsigned blocks_for(long long items, int per_block) {
+	return (unsigned)((items + per_block - 1) / per_block);
+}
+
+}  // namespace pred
+}  // namespace rta_hip
+
+using namespace rta_hip;
+using namespace rta_hip::pred;
+
+extern "C" {
+
+int rta_pred_param_count(int *count) {
+	if (!count) return invalid("rta_pred_param_count: NULL pointer");
+	*count = kParams;
+	return RTA_OK;
+}
+
+int rta_pred_create(int max_batch, uint64_t seed, rta_pred *out) {
+	if (!out || max_batch <= 0 || max_batch > (1 << 24))
+		return invalid("rta_pred_create: max_batch in 1..2^24 and a non-NULL handle");
+	State *p = new (std::nothrow) State;
+	if (!p) return invalid("rta_pred_create: out of host memory");
+	p->max_batch = max_batch;
+	const size_t B = ((size_t)max_batch + 15) / 16 * 16;
+	int rc = RTA_OK;
+	rc |= p->alloc(p->params, kParams);
+	rc |= p->alloc(p->mom, kParams);
+	rc |= p->alloc(p->vel, kParams);
+	rc |= p->alloc(p->grads, kParams);
+	rc |= p->alloc(p->wb, kParams);
+	rc |= p->alloc(p->x, B * kInP);
+	rc |= p->alloc(p->h1, B * kHid);
+	rc |= p->alloc(p->h2, B * kHid);
+	rc |= p->alloc(p->h3, B * kHid);
+	rc |= p->alloc(p->out, B * kOutP);
+	rc |= p->alloc(p->dout, B * kOutP);
+	rc |= p->alloc(p->dh3, B * kHid);
+	rc |= p->alloc(p->dh2, B * kHid);
+	rc |= p->alloc(p->dh1, B * kHid);
+	if (rc != RTA_OK) {
+		delete p;
+		set_error("rta_pred_create: device allocation failed");
+		return RTA_E_INVALID;
+	}
+	// He-uniform weights on the used entries, zero biases and padding
+	std::vector<float> h(kParams, 0.0f);
+	auto fill = [&](int off, int rows, int cols_p, int cols, int fan_in) {
+		const float a = sqrtf(6.0f / (float)fan_in);
+		for (int r = 0; r < rows; ++r)
+			for (int c = 0; c < cols; ++c)
+				h[off + r * cols_p + c] = a * init_uniform(seed, (uint64_t)(off + r * cols_p + c));
+	};
+	fill(oW1, kHid, kInP, kIn, kIn);
+	fill(oW2, kHid, kHid, kHid, kHid);
+	fill(oW3, kHid, kHid, kHid, kHid);
+	fill(oW4, kOut, kHid, kHid, kHid);
+	RTA_HIP_TRY(hipMemcpy(p->params, h.data(), kParams * 4, hipMemcpyHostToDevice));
+	RTA_HIP_TRY(hipMemset(p->mom, 0, kParams * 4));
+	RTA_HIP_TRY(hipMemset(p->vel, 0, kParams * 4));
+	RTA_HIP_TRY(hipMemset(p->grads, 0, kParams * 4));
+	hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for(kParams, 256)), dim3(256), 0, 0, p->wb,
+	                   p->params);
+	RTA_LAUNCH_CHECK("to_bf16_kernel");
+	RTA_HIP_TRY(hipDeviceSynchronize());
+	*out = p;
+	return RTA_OK;
+}
+
+int rta_pred_destroy(rta_pred h) {
+	delete reinterpret_cast<State *>(h);
+	return RTA_OK;
+}
+
+int rta_pred_get_params(rta_pred h, float *host) {
+	State *p = reinterpret_cast<State *>(h);
+	if (!p || !host) return invalid("rta_pred_get_params: NULL pointer");
+	RTA_HIP_TRY(hipMemcpy(host, p->params, kParams * 4, hipMemcpyDeviceToHost));
+	return RTA_OK;
+}
+
+int rta_pred_set_params(rta_pred h, const float *host) {
+	State *p = reinterpret_cast<State *>(h);
+	if (!p || !host) return invalid("rta_pred_set_params: NULL pointer");
+	RTA_HIP_TRY(hipMemcpy(p->params, host, kParams * 4, hipMemcpyHostToDevice));
+	hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for(kParams, 256)), dim3(256), 0, 0, p->wb,
+	                   p->params);
+	RTA_LAUNCH_CHECK("to_bf16_kernel");
+	RTA_HIP_TRY(hipDeviceSynchronize());
+	return RTA_OK;
+}
+
+int rta_pred_get_grads(rta_pred h, float *host) {
+	State *p = reinterpret_cast<State *>(h);
+	if (!p || !host) return invalid("rta_pred_get_grads: NULL pointer");
+	RTA_HIP_TRY(hipMemcpy(host, p->grads, kParams * 4, hipMemcpyDeviceToHost));
+	return RTA_OK;
+}
+
+int rta_pred_zero_grads(rta_pred h, rta_stream s) {
+	State *p = reinterpret_cast<State *>(h);
+	if (!p) return invalid("rta_pred_zero_grads: NULL handle");
+	return rta_hip_fill_u32(reinterpret_cast<uint32_t *>(p->grads), 0u, kParams, s);
+}
+
+int rta_pred_forward(rta_pred h, const float *o, const float *d, int n, const float *scene_min,
+                     const float *scene_max, float *out, int32_t *top, float *logdist,
+                     rta_stream s) {
+	State *p = reinterpret_cast<State *>(h);
+	if (!p || !o || !d || !scene_min || !scene_max) return invalid("rta_pred_forward: NULL pointer");
+	if (n <= 0 || n > p->max_batch) return invalid("rta_pred_forward: n must be in 1..max_batch");
+	hipStream_t st = as_stream(s);
+	float3 lo = make_float3(scene_min[0], scene_min[1], scene_min[2]);
+	float3 sc = make_float3(2.0f / (scene_max[0] - scene_min[0]), 2.0f / (scene_max[1] - scene_min[1]),
+	                        2.0f / (scene_max[2] - scene_min[2]));
+	hipLaunchKernelGGL(encode_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, p->x, o, d, n, lo,
+	                   sc);
+	RTA_LAUNCH_CHECK("encode_kernel");
+	const long long rt = (n + 15) / 16;
+	hipLaunchKernelGGL(fwd_kernel<false>, dim3(blocks_for(rt * (kHid / 16), 4)), dim3(256), 0, st,
+	                   (void *)p->h1, kHid, p->x, kInP, p->wb + oW1, p->params + oB1, kInP, kHid, n);
+	hipLaunchKernelGGL(fwd_kernel<false>, dim3(blocks_for(rt * (kHid / 16), 4)), dim3(256), 0, st,
+	                   (void *)p->h2, kHid, p->h1, kHid, p->wb + oW2, p->params + oB2, kHid, kHid, n);
+	hipLaunchKernelGGL(fwd_kernel<false>, dim3(blocks_for(rt * (kHid / 16), 4)), dim3(256), 0, st,
+	                   (void *)p->h3, kHid, p->h2, kHid, p->wb + oW3, p->params + oB3, kHid, kHid, n);
+	hipLaunchKernelGGL(fwd_kernel<true>, dim3(blocks_for(rt * (kOutP / 16), 4)), dim3(256), 0, st,
+	                   (void *)p->out, kOutP, p->h3, kHid, p->wb + oW4, p->params + oB4, kHid, kOutP,
+	                   n);
+	RTA_LAUNCH_CHECK("fwd_kernel");
+	if (out)
+		RTA_HIP_TRY(hipMemcpyAsync(out, p->out, (size_t)n * kOutP * 4, hipMemcpyDeviceToDevice, st));
+	if (top || logdist) {
+		hipLaunchKernelGGL(topk_kernel, dim3(blocks_for(n, 4)), dim3(256), 0, st, top, logdist,
+		                   p->out, n);
+		RTA_LAUNCH_CHECK("topk_kernel");
+	}
+	p->n_fwd = n;
+	return RTA_OK;
+}
+
+int rta_pred_backward(rta_pred h, const int32_t *label, const float *target, int n, float *loss,
+                      rta_stream s) {
+	State *p = reinterpret_cast<State *>(h);
+	if (!p || !label || !target || !loss) return invalid("rta_pred_backward: NULL pointer");
+	if (n != p->n_fwd) return invalid("rta_pred_backward: n differs from the last forward pass");
+	hipStream_t st = as_stream(s);
+	hipLaunchKernelGGL(loss_kernel, dim3(blocks_for(n, 4)), dim3(256), 0, st, p->dout, p->out,
+	                   label, target, n, loss);
+	RTA_LAUNCH_CHECK("loss_kernel");
+	const long long rt = (n + 15) / 16;
+	const int chunks = (n + kRowChunk - 1) / kRowChunk;
+	float *g = p->grads;
+	// layer 4: dW4, db4, dh3
+	hipLaunchKernelGGL(bwd_weight_kernel, dim3(blocks_for((long long)(kOutP / 16) * (kHid / 16) * chunks, 4)),
+	                   dim3(256), 0, st, g + oW4, p->dout, kOutP, p->h3, kHid, kHid, kOutP, n);
+	hipLaunchKernelGGL(bwd_bias_kernel, dim3(blocks_for(kOutP, 256), chunks), dim3(256), 0, st,
+	                   g + oB4, p->dout, kOutP, kOutP, n);
+	hipLaunchKernelGGL(bwd_data_kernel, dim3(blocks_for(rt * (kHid / 16), 4)), dim3(256), 0, st,
+	                   p->dh3, p->dout, kOutP, p->wb + oW4, kHid, kOutP, p->h3, n);
+	// layer 3
+	hipLaunchKernelGGL(bwd_weight_kernel, dim3(blocks_for((long long)(kHid / 16) * (kHid / 16) * chunks, 4)),
+	                   dim3(256), 0, st, g + oW3, p->dh3, kHid, p->h2, kHid, kHid, kHid, n);
+	hipLaunchKernelGGL(bwd_bias_kernel, dim3(blocks_for(kHid, 256), chunks), dim3(256), 0, st,
+	                   g + oB3, p->dh3, kHid, kHid, n);
+	hipLaunchKernelGGL(bwd_data_kernel, dim3(blocks_for(rt * (kHid / 16), 4)), dim3(256), 0, st,
+	                   p->dh2, p->dh3, kHid, p->wb + oW3, kHid, kHid, p->h2, n);
+	// layer 2
+	hipLaunchKernelGGL(bwd_weight_kernel, dim3(blocks_for((long long)(kHid / 16) * (kHid / 16) * chunks, 4)),
+	                   dim3(256), 0, st, g + oW2, p->dh2, kHid, p->h1, kHid, kHid, kHid, n);
+	hipLaunchKernelGGL(bwd_bias_kernel, dim3(blocks_for(kHid, 256), chunks), dim3(256), 0, st,
+	                   g + oB2, p->dh2, kHid, kHid, n);
+	hipLaunchKernelGGL(bwd_data_kernel, dim3(blocks_for(rt * (kHid / 16), 4)), dim3(256), 0, st,
+	                   p->dh1, p->dh2, kHid, p->wb + oW2, kHid, kHid, p->h1, n);
+	// layer 1 (no input gradient)
+	hipLaunchKernelGGL(bwd_weight_kernel, dim3(blocks_for((long long)(kHid / 16) * (kInP / 16) * chunks, 4)),
+	                   dim3(256), 0, st, g + oW1, p->dh1, kHid, p->x, kInP, kInP, kHid, n);
+	hipLaunchKernelGGL(bwd_bias_kernel, dim3(blocks_for(kHid, 256), chunks), dim3(256), 0, st,
+	                   g + oB1, p->dh1, kHid, kHid, n);
+	RTA_LAUNCH_CHECK("backward kernels");
+	return RTA_OK;
+}
+
+int rta_pred_adam(rta_pred h, float lr, int step, rta_stream s) {
+	State *p = reinterpret_cast<State *>(h);
+	if (!p || step < 1) return invalid("rta_pred_adam: NULL handle or step < 1");
+	const float c1 = 1.0f / (1.0f - powf(0.9f, (float)step));
+	const float c2 = 1.0f / (1.0f - powf(0.999f, (float)step));
+	hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(kParams, 256)), dim3(256), 0, as_stream(s),
+	                   p->params, p->mom, p->vel, p->grads, p->wb, lr, c1, c2);
+	RTA_LAUNCH_CHECK("adam_kernel");
+	return RTA_OK;
+}
+
+}  // extern "C"

@@ -354,6 +354,99 @@ __global__ __launch_bounds__(BLOCK) void trace_cis_packed_kernel(
 	hits[tid] = closest;
 }
 
+// ------------------------------------- seeding from predicted clusters
+
+// Closest hit of ray tid among the triangles of its kSeedTop candidate
+// clusters (sorted positions [c*n/1024, (c+1)*n/1024)), by the packed
+// traversal restricted to children whose sorted range meets a candidate
+// range: an inner child by its Karras range, a leaf child by its first
+// triangle.  Any triangle found is a real hit of the ray, so its t (or
+// FLT_MAX) bounds the ray's closest hit from above.
+constexpr int kSeedTop = 4;
+constexpr int kSeedClusters = 1024;
+
+__global__ __launch_bounds__(kTraceBlock) void seed_clusters_kernel(
+    const float4 *__restrict__ tri4, const float4 *__restrict__ pairs,
+    const uint32_t *__restrict__ flags, uint32_t root_word, const uint2 *__restrict__ ranges,
+    int n_tris, const float *__restrict__ ro, const float *__restrict__ rd,
+    const float *__restrict__ rmax, int n_rays, const int32_t *__restrict__ top,
+    float *__restrict__ t_seed, uint32_t *__restrict__ err) {
+	__shared__ uint32_t stack[kStack][kTraceBlock];
+	const int lt = threadIdx.x;
+	const int tid = blockIdx.x * kTraceBlock + lt;
+	if (tid >= n_rays) return;
+	uint32_t clo[kSeedTop], chi[kSeedTop];
+	for (int k = 0; k < kSeedTop; ++k) {
+		const uint32_t c = (uint32_t)top[(size_t)tid * kSeedTop + k] & (kSeedClusters - 1);
+		clo[k] = (uint32_t)(((uint64_t)c * (uint32_t)n_tris) / kSeedClusters);
+		chi[k] = (uint32_t)(((uint64_t)(c + 1) * (uint32_t)n_tris) / kSeedClusters);
+	}
+	auto wanted = [&](uint32_t word) {
+		uint32_t a, b;
+		if (word & 1u) { const uint2 r = ranges[word >> 1]; a = r.x; b = r.y; }
+		else { a = b = word >> 1; }
+		bool ok = false;
+		for (int k = 0; k < kSeedTop; ++k) ok = ok || (a < chi[k] && b >= clo[k]);
+		return ok;
+	};
+	const float ox = ro[3 * tid + 0], oy = ro[3 * tid + 1], oz = ro[3 * tid + 2];
+	const float dx = rd[3 * tid + 0], dy = rd[3 * tid + 1], dz = rd[3 * tid + 2];
+	const float tmax = rmax[tid];
+	const bool fast = ((*flags & 1u) == 0u) && ray_nz_ok(ox, oy, oz, dx, dy, dz);
+	const float yx = 1.0f / dx, yy = 1.0f / dy, yz = 1.0f / dz;
+	float ct = FLT_MAX;
+	uint32_t curr = root_word;
+	int sp = -1;
+	if (!wanted(curr)) {  // only a leaf root outside every candidate cluster
+		t_seed[tid] = FLT_MAX;
+		return;
+	}
+	for (uint32_t guard = 0;; ++guard) {
+		if (guard >= kMaxSteps) { atomicAdd(err, 1u); break; }
+		if (curr & 1u) {
+			const float4 *p = pairs + 4 * (size_t)(curr >> 1);
+			const float4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
+			float dl, dr;
+			bool hl, hr;
+			pair_boxes(q0, q1, q2, fast, ox, oy, oz, dx, dy, dz, yx, yy, yz, hl, dl, hr, dr);
+			const uint32_t wl = __float_as_uint(q3.x), wr = __float_as_uint(q3.y);
+			hl = hl && dl < ct && dl <= tmax && wanted(wl);
+			hr = hr && dr < ct && dr <= tmax && wanted(wr);
+			if (hl && hr) {
+				if (sp + 1 >= kStack) { atomicAdd(err, 1u); break; }
+				if (dl <= dr) { stack[++sp][lt] = wr; curr = wl; }
+				else          { stack[++sp][lt] = wl; curr = wr; }
+				continue;
+			}
+			if (hl) { curr = wl; continue; }
+			if (hr) { curr = wr; continue; }
+		} else {
+			uint32_t i = curr >> 1;
+			while (true) {
+				const float4 v0 = tri4[3 * (size_t)i + 0];
+				const float4 v1 = tri4[3 * (size_t)i + 1];
+				const float4 v2 = tri4[3 * (size_t)i + 2];
+				float t, be, ga;
+				if (tri_test(v0.x, v0.y, v0.z, v1.x, v1.y, v1.z, v2.x, v2.y, v2.z, ox, oy, oz, dx,
+				             dy, dz, t, be, ga) &&
+				    t < ct && t <= tmax)
+					ct = t;
+				if (__float_as_uint(v0.w) != 0u) break;
+				++i;
+			}
+		}
+		if (sp < 0) break;
+		curr = stack[sp--][lt];
+	}
+	t_seed[tid] = ct;
+}
+
+__global__ void seed_max_t_kernel(float *__restrict__ out, const float *__restrict__ max_t,
+                                  const float *__restrict__ t_seed, int n) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) out[i] = fminf(max_t[i], t_seed[i]);
+}
+
 // ---------------------------- persistent packed CIS with ray refill
 
 __global__ void fill_queue_kernel(uint32_t *queue) {
@@ -941,6 +1034,31 @@ int rta_trace_shadow_cis_persistent(const float *tri4, const rta_node_pair *pair
 	return trace_persistent<true>("rta_trace_shadow_cis_persistent", tri4, pairs, flags,
 	                              root_link, root_index, ray_orig, ray_dir, max_t, n_rays,
 	                              ray_count, hits, err, queue, cfg, as_stream(s));
+}
+
+int rta_trace_seed_clusters(const float *tri4, const rta_node_pair *pairs, const uint32_t *flags,
+                            uint32_t root_link, uint32_t root_index, const uint32_t *ranges,
+                            int n_tris, const float *ray_orig, const float *ray_dir,
+                            const float *max_t, int n_rays, const int32_t *top, float *t_seed,
+                            uint32_t *err, rta_stream s) {
+	if (!tri4 || !pairs || !flags || !ranges || !ray_orig || !ray_dir || !max_t || !top ||
+	    !t_seed || !err || n_rays <= 0 || n_tris <= 0)
+		return invalid("rta_trace_seed_clusters: bad arguments");
+	const uint32_t root_word = link_inner(root_link) ? 1u : (root_index << 1);
+	hipLaunchKernelGGL(seed_clusters_kernel, dim3(grid_for(n_rays, kTraceBlock)),
+	                   dim3(kTraceBlock), 0, as_stream(s), (const float4 *)tri4,
+	                   (const float4 *)pairs, flags, root_word, (const uint2 *)ranges, n_tris,
+	                   ray_orig, ray_dir, max_t, n_rays, top, t_seed, err);
+	RTA_LAUNCH_CHECK("seed_clusters_kernel");
+	return RTA_OK;
+}
+
+int rta_seed_max_t(float *out, const float *max_t, const float *t_seed, int n, rta_stream s) {
+	if (!out || !max_t || !t_seed || n <= 0) return invalid("rta_seed_max_t: bad arguments");
+	hipLaunchKernelGGL(seed_max_t_kernel, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(s), out,
+	                   max_t, t_seed, n);
+	RTA_LAUNCH_CHECK("seed_max_t_kernel");
+	return RTA_OK;
 }
 
 int rta_batch_pack(rta_batch b, rta_node_pair *pairs, float *tri4, uint32_t *flags,
