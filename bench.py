@@ -53,6 +53,9 @@ def parse():
     p.add_argument("--steps-per-refill", dest="steps_per_refill", type=int, default=32)
     p.add_argument("--leaf-min", dest="leaf_min", type=int, default=10)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    p.add_argument("--no-predictor", action="store_true",
+                   help="skip the hit-predicted leg (training + seeded traversal)")
+    p.add_argument("--predictor-epochs", dest="predictor_epochs", type=int, default=8)
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run a 1 GiB device copy kernel (to scale HBM traffic counters)")
     p.add_argument("--cpu-rays", type=int, default=0,
@@ -312,6 +315,179 @@ def cpu_baseline(tris_h, bo, bd, bt, n_bounce, w, max_rays):
     }
 
 
+PRED_TRAIN_SEEDS = (100, 101, 102, 103)     # offsets from scenes.BOUNCE_SEED
+PRED_EVAL_SEEDS = (0, 1)                     # 0 = the headline bounce set
+PRED_FLOPS_PER_RAY = 2 * (102 * 64 + 64 * 64 + 64 * 64 + 64 * 1025)   # 160,640
+
+
+def predictor_leg(args, tris_h, tris, nodes, packed, ranges, prim, hits_p, R, tcfg, s):
+    """Plain against hit-predicted closest-hit traversal of bounce rays.
+
+    The network is trained on the diffuse bounce rays of four seeds and
+    evaluated on two held-out seeds (the first is the headline bounce set).
+    Hit-predicted = inference (top-4 clusters), the restricted traversal of
+    those clusters (a max_t seed), then the unchanged persistent traversal
+    with max_t' = min(max_t, seed); all three inside the timing.  Every
+    held-out ray's record is compared with the plain traversal's."""
+    import rta_amd
+    from rta_amd import predictor as P
+    from rta_amd import scenes
+    from rta_amd.abi import HIT_DTYPE, call, ptr
+    from rta_amd.device import DeviceBuffer, Event
+
+    n = len(tris_h)
+    sp = ptr(s)
+    sz = C.c_size_t(0)
+    call("rta_bounce_work_bytes", R, C.byref(sz))
+    bwork, bwork_bytes = DeviceBuffer(sz.value), C.c_size_t(sz.value)
+    src = DeviceBuffer(4 * R)
+    err = DeviceBuffer(4)
+    err.fill_u32(0)
+    queue = DeviceBuffer(4)
+    fmax = np.float32(rta_amd.FLT_MAX)
+
+    def persistent(rays, cnt, maxt, hits):
+        call("rta_trace_cis_persistent", ptr(packed.tri4), ptr(packed.pairs), ptr(packed.flags),
+             packed.root_link, packed.root_index, ptr(rays.gpu_origin), ptr(rays.gpu_direction),
+             ptr(maxt), R, ptr(cnt), ptr(hits), ptr(err), ptr(queue), C.byref(tcfg), sp)
+
+    def bounce_set(seed, rays, cnt, hits):
+        call("rta_diffuse_bounce", ptr(rays.gpu_origin), ptr(rays.gpu_direction),
+             ptr(rays.gpu_maxt), ptr(src), ptr(cnt), ptr(hits_p), ptr(tris), ptr(prim.gpu_origin),
+             ptr(prim.gpu_direction), R, C.c_uint64(seed), ptr(bwork), bwork_bytes, sp)
+        call("rta_reset_intersections", ptr(hits), R, 1, sp)
+        persistent(rays, cnt, rays.gpu_maxt, hits)
+        s.synchronize()
+        return int(cnt.download(np.uint32, 1)[0])
+
+    # training data: bounce rays of four seeds and their plain records
+    t0 = time.perf_counter()
+    tmp, tcnt, thits = rta_amd.RayBuffers(R), DeviceBuffer(4), DeviceBuffer(16 * R)
+    o_tr, d_tr, lab_tr, tgt_tr = [], [], [], []
+    for k in PRED_TRAIN_SEEDS:
+        m = bounce_set(scenes.BOUNCE_SEED + k, tmp, tcnt, thits)
+        o, d, _ = tmp.download(m)
+        lab, tgt = P.cluster_labels(thits.download(HIT_DTYPE, m), n)
+        o_tr.append(o), d_tr.append(d), lab_tr.append(lab), tgt_tr.append(tgt)
+    o_tr, d_tr = np.concatenate(o_tr), np.concatenate(d_tr)
+    lab_tr, tgt_tr = np.concatenate(lab_tr), np.concatenate(tgt_tr)
+    n_tr = len(lab_tr)
+    del tmp, thits
+
+    evals = []
+    for k in PRED_EVAL_SEEDS:
+        e = {"seed": scenes.BOUNCE_SEED + k, "rays": rta_amd.RayBuffers(R), "cnt": DeviceBuffer(4),
+             "plain": DeviceBuffer(16 * R), "pred": DeviceBuffer(16 * R)}
+        e["m"] = bounce_set(e["seed"], e["rays"], e["cnt"], e["plain"])
+        evals.append(e)
+    m_max = max(e["m"] for e in evals)
+    log(f"predictor: {n_tr} training rays, held-out {[e['m'] for e in evals]} "
+        f"({time.perf_counter() - t0:.1f} s)")
+
+    allv = np.concatenate([tris_h["a"], tris_h["b"], tris_h["c"]])
+    pred = P.HitPredictor(m_max, seed=1, scene_min=allv.min(axis=0), scene_max=allv.max(axis=0))
+    o_b, d_b = DeviceBuffer.from_array(o_tr), DeviceBuffer.from_array(d_tr)
+    l_b, g_b = DeviceBuffer.from_array(lab_tr), DeviceBuffer.from_array(tgt_tr)
+    losses, train_s = pred.train(o_b, d_b, l_b, g_b, n_tr, epochs=args.predictor_epochs,
+                                 batch=8192, lr=2e-3, stream=s)
+    del o_b, d_b, l_b, g_b
+    log(f"predictor trained: {args.predictor_epochs} epochs in {train_s:.1f} s, loss "
+        f"{losses[0]:.3f} -> {losses[-1]:.3f}")
+
+    top, t_seed, maxt2 = DeviceBuffer(16 * m_max), DeviceBuffer(4 * m_max), DeviceBuffer(4 * m_max)
+    ev = [Event() for _ in range(5)]
+    reps = max(args.steps, 3)
+    out = []
+    for e in evals:
+        m, rays = e["m"], e["rays"]
+        # plain: the same persistent traversal as the headline figure
+        plain_ms = 0.0
+        for _ in range(reps):
+            call("rta_reset_intersections", ptr(e["plain"]), R, 1, sp)
+            ev[0].record(s)
+            persistent(rays, e["cnt"], rays.gpu_maxt, e["plain"])
+            ev[1].record(s)
+            ev[1].synchronize()
+            plain_ms += ev[0].elapsed_ms(ev[1]) / reps
+        part = {"inference": 0.0, "seeding": 0.0, "traversal": 0.0}
+        total_ms = 0.0
+        for _ in range(reps):
+            call("rta_reset_intersections", ptr(e["pred"]), R, 1, sp)
+            ev[0].record(s)
+            pred.forward(rays.gpu_origin, rays.gpu_direction, m, top=top, stream=s)
+            ev[1].record(s)
+            call("rta_trace_seed_clusters", ptr(packed.tri4), ptr(packed.pairs),
+                 ptr(packed.flags), packed.root_link, packed.root_index, ptr(ranges), n,
+                 ptr(rays.gpu_origin), ptr(rays.gpu_direction), ptr(rays.gpu_maxt), m, ptr(top),
+                 ptr(t_seed), ptr(err), sp)
+            call("rta_seed_max_t", ptr(maxt2), ptr(rays.gpu_maxt), ptr(t_seed), m, sp)
+            ev[2].record(s)
+            persistent(rays, e["cnt"], maxt2, e["pred"])
+            ev[3].record(s)
+            ev[3].synchronize()
+            part["inference"] += ev[0].elapsed_ms(ev[1]) / reps
+            part["seeding"] += ev[1].elapsed_ms(ev[2]) / reps
+            part["traversal"] += ev[2].elapsed_ms(ev[3]) / reps
+            total_ms += ev[0].elapsed_ms(ev[3]) / reps
+        if int(err.download(np.uint32, 1)[0]):
+            raise SystemExit("hit-predicted leg: traversal reported stack/guard errors")
+        hp = e["plain"].download(HIT_DTYPE, m)
+        hq = e["pred"].download(HIT_DTYPE, m)
+        hit = hp["t"] != fmax
+        same = (hq["t"] != fmax) == hit
+        for f in ("t", "beta", "gamma"):
+            same &= (hq[f].view(np.uint32) == hp[f].view(np.uint32)) | ~hit
+        same &= (hq["ref"] == hp["ref"]) | ~hit
+        seeds = t_seed.download(np.float32, m)
+        tops = top.download(np.int32, 4 * m).reshape(m, 4)
+        lab, _ = P.cluster_labels(hp, n)
+        rec = {
+            "bounce_seed": e["seed"], "rays": m,
+            "plain_mrays_s": round(m / (plain_ms * 1e3), 3),
+            "hit_predicted_mrays_s": round(m / (total_ms * 1e3), 3),
+            "plain_ms": round(plain_ms, 4), "hit_predicted_ms": round(total_ms, 4),
+            "hit_predicted_breakdown_ms": {k: round(v, 4) for k, v in part.items()},
+            "exact_share": float(same.mean()),
+            "hit_share": round(float(hit.mean()), 4),
+            "top4_accuracy": round(P.top4_accuracy(tops, lab), 4),
+            "seeded_share": round(float((seeds != fmax).mean()), 4),
+            "seed_is_closest_share": round(float((seeds[hit] == hp["t"][hit]).mean()), 4),
+            "inference_tflop_s": round(PRED_FLOPS_PER_RAY * m / (part["inference"] * 1e-3) / 1e12,
+                                       3),
+        }
+        # inner-node visits per ray with and without the seed
+        stats = DeviceBuffer(8 * m)
+        scratch = DeviceBuffer(16 * m)
+        vis = []
+        for mt in (rays.gpu_maxt, maxt2):
+            call("rta_reset_intersections", ptr(scratch), m, 1, sp)
+            call("rta_trace_cis", ptr(tris), n, ptr(nodes), ptr(rays.gpu_origin),
+                 ptr(rays.gpu_direction), ptr(mt), m, 1, ptr(scratch), ptr(stats), ptr(err), sp)
+            s.synchronize()
+            vt = stats.download(np.uint32, 2 * m).reshape(-1, 2).astype(np.float64)
+            vis.append((float(vt[:, 0].mean()), float(vt[:, 1].mean())))
+        rec["V_plain"], rec["T_plain"] = round(vis[0][0], 3), round(vis[0][1], 3)
+        rec["V_seeded"], rec["T_seeded"] = round(vis[1][0], 3), round(vis[1][1], 3)
+        log(f"predictor seed {e['seed']}: plain {rec['plain_mrays_s']:.1f} Mrays/s, "
+            f"hit-predicted {rec['hit_predicted_mrays_s']:.1f} Mrays/s "
+            f"({rec['hit_predicted_breakdown_ms']}), top-4 {rec['top4_accuracy']:.3f}, "
+            f"exact {rec['exact_share']:.6f}, V {rec['V_plain']:.1f} -> {rec['V_seeded']:.1f}")
+        if rec["exact_share"] != 1.0:
+            raise SystemExit("hit-predicted leg: seeded records differ from the plain ones")
+        out.append(rec)
+    pred.close()
+    return {
+        "network": "102-64-64-64-1025 MLP, bf16 weights/activations, fp32 MFMA accumulation",
+        "flops_per_ray": PRED_FLOPS_PER_RAY,
+        "mfma_peak_tflop_s": 2500.0,
+        "training": {"rays": n_tr, "epochs": args.predictor_epochs, "batch": 8192, "lr": 2e-3,
+                     "seconds": round(train_s, 2),
+                     "loss_first_last": [round(losses[0], 4), round(losses[-1], 4)],
+                     "bounce_seeds": [scenes.BOUNCE_SEED + k for k in PRED_TRAIN_SEEDS]},
+        "held_out": out,
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -508,6 +684,17 @@ def main():
             traffic = round(rec["fetch_bytes"] + rec["write_bytes"])
             traffic_src = os.path.relpath(pmc_file, ROOT)
 
+    hit_predicted = None
+    if not args.no_predictor and default_workload:
+        from rta_amd.bvh import PackedBvh
+
+        packed = PackedBvh(pairs, tri4, pflags, root[0], root[1])
+        ranges = DeviceBuffer(8 * (n - 1))
+        call("rta_lbvh_copy_ranges", ptr(ranges), ptr(ws), n, sp)
+        hit_predicted = predictor_leg(args, tris_h, tris, nodes, packed, ranges, prim, hits_p, R,
+                                      tcfg, s)
+        del ranges
+
     cpu = None
     if not args.no_cpu:
         log("CPU baseline (oracle restatement of the reference CPU plugin)")
@@ -562,6 +749,7 @@ def main():
             "kernel": f"trace_cis_{args.trav}_kernel (bounce rays)",
             "bytes_per_ray": round(b_ray, 1), "V": round(V, 3), "T": round(T, 3),
         },
+        "hit_predicted": hit_predicted,
         "cpu_baseline": cpu,
         "device": dev_name,
     }

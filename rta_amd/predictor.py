@@ -124,7 +124,7 @@ class HitPredictor:
         losses = []
         t0 = time.perf_counter()
         for _ in range(epochs):
-            self.loss.fill_u32(0)
+            self.loss.fill_u32(0, stream=stream)
             for s0 in rng.permutation(starts):
                 m = int(min(batch, n - s0))
                 call("rta_pred_zero_grads", self.h, sp)
