@@ -325,10 +325,11 @@ def predictor_leg(args, tris_h, tris, nodes, packed, ranges, prim, hits_p, R, tc
 
     The network is trained on the diffuse bounce rays of four seeds and
     evaluated on two held-out seeds (the first is the headline bounce set).
-    Hit-predicted = inference (top-4 clusters), the restricted traversal of
-    those clusters (a max_t seed), then the unchanged persistent traversal
-    with max_t' = min(max_t, seed); all three inside the timing.  Every
-    held-out ray's record is compared with the plain traversal's."""
+    The classes are a fixed cut of the tree into 1024 treelets.
+    Hit-predicted = inference (top-4 treelets, fused kernel), the traversal
+    of those treelets (a max_t seed), then the unchanged persistent
+    traversal with max_t' = min(max_t, seed); all three inside the timing.
+    Every held-out ray's record is compared with the plain traversal's."""
     import rta_amd
     from rta_amd import predictor as P
     from rta_amd import scenes
@@ -360,14 +361,24 @@ def predictor_leg(args, tris_h, tris, nodes, packed, ranges, prim, hits_p, R, tc
         s.synchronize()
         return int(cnt.download(np.uint32, 1)[0])
 
-    # training data: bounce rays of four seeds and their plain records
+    # the fixed cut into 1024 treelets (the network's classes)
     t0 = time.perf_counter()
+    cut = P.TreeletCut(packed, ranges, n, stream=s)
+    ev_cut = [Event(), Event()]
+    ev_cut[0].record(s)
+    cut = P.TreeletCut(packed, ranges, n, stream=s)
+    ev_cut[1].record(s)
+    ev_cut[1].synchronize()
+    cut_ms = ev_cut[0].elapsed_ms(ev_cut[1])
+    sizes = np.diff(np.append(cut.host()["first"].astype(np.int64), n))
+
+    # training data: bounce rays of four seeds and their plain records
     tmp, tcnt, thits = rta_amd.RayBuffers(R), DeviceBuffer(4), DeviceBuffer(16 * R)
     o_tr, d_tr, lab_tr, tgt_tr = [], [], [], []
     for k in PRED_TRAIN_SEEDS:
         m = bounce_set(scenes.BOUNCE_SEED + k, tmp, tcnt, thits)
         o, d, _ = tmp.download(m)
-        lab, tgt = P.cluster_labels(thits.download(HIT_DTYPE, m), n)
+        lab, tgt = cut.labels_host(thits, m, stream=s)
         o_tr.append(o), d_tr.append(d), lab_tr.append(lab), tgt_tr.append(tgt)
     o_tr, d_tr = np.concatenate(o_tr), np.concatenate(d_tr)
     lab_tr, tgt_tr = np.concatenate(lab_tr), np.concatenate(tgt_tr)
@@ -385,7 +396,7 @@ def predictor_leg(args, tris_h, tris, nodes, packed, ranges, prim, hits_p, R, tc
         f"({time.perf_counter() - t0:.1f} s)")
 
     allv = np.concatenate([tris_h["a"], tris_h["b"], tris_h["c"]])
-    pred = P.HitPredictor(m_max, seed=1, scene_min=allv.min(axis=0), scene_max=allv.max(axis=0))
+    pred = P.HitPredictor(8192, seed=1, scene_min=allv.min(axis=0), scene_max=allv.max(axis=0))
     o_b, d_b = DeviceBuffer.from_array(o_tr), DeviceBuffer.from_array(d_tr)
     l_b, g_b = DeviceBuffer.from_array(lab_tr), DeviceBuffer.from_array(tgt_tr)
     losses, train_s = pred.train(o_b, d_b, l_b, g_b, n_tr, epochs=args.predictor_epochs,
@@ -414,12 +425,12 @@ def predictor_leg(args, tris_h, tris, nodes, packed, ranges, prim, hits_p, R, tc
         for _ in range(reps):
             call("rta_reset_intersections", ptr(e["pred"]), R, 1, sp)
             ev[0].record(s)
-            pred.forward(rays.gpu_origin, rays.gpu_direction, m, top=top, stream=s)
+            pred.infer(rays.gpu_origin, rays.gpu_direction, m, top=top, stream=s)
             ev[1].record(s)
-            call("rta_trace_seed_clusters", ptr(packed.tri4), ptr(packed.pairs),
-                 ptr(packed.flags), packed.root_link, packed.root_index, ptr(ranges), n,
-                 ptr(rays.gpu_origin), ptr(rays.gpu_direction), ptr(rays.gpu_maxt), m, ptr(top),
-                 ptr(t_seed), ptr(err), sp)
+            call("rta_trace_seed_treelets", ptr(packed.tri4), ptr(packed.pairs),
+                 ptr(packed.flags), ptr(cut.table), ptr(cut.count), ptr(rays.gpu_origin),
+                 ptr(rays.gpu_direction), ptr(rays.gpu_maxt), m, ptr(top), ptr(t_seed), ptr(err),
+                 sp)
             call("rta_seed_max_t", ptr(maxt2), ptr(rays.gpu_maxt), ptr(t_seed), m, sp)
             ev[2].record(s)
             persistent(rays, e["cnt"], maxt2, e["pred"])
@@ -440,7 +451,7 @@ def predictor_leg(args, tris_h, tris, nodes, packed, ranges, prim, hits_p, R, tc
         same &= (hq["ref"] == hp["ref"]) | ~hit
         seeds = t_seed.download(np.float32, m)
         tops = top.download(np.int32, 4 * m).reshape(m, 4)
-        lab, _ = P.cluster_labels(hp, n)
+        lab, _ = cut.labels_host(e["plain"], m, stream=s)
         rec = {
             "bounce_seed": e["seed"], "rays": m,
             "plain_mrays_s": round(m / (plain_ms * 1e3), 3),
@@ -480,6 +491,9 @@ def predictor_leg(args, tris_h, tris, nodes, packed, ranges, prim, hits_p, R, tc
         "network": "102-64-64-64-1025 MLP, bf16 weights/activations, fp32 MFMA accumulation",
         "flops_per_ray": PRED_FLOPS_PER_RAY,
         "mfma_peak_tflop_s": 2500.0,
+        "treelets": {"count": int(len(sizes)), "cut_ms": round(cut_ms, 4),
+                     "triangles_min_median_max": [int(sizes.min()), int(np.median(sizes)),
+                                                  int(sizes.max())]},
         "training": {"rays": n_tr, "epochs": args.predictor_epochs, "batch": 8192, "lr": 2e-3,
                      "seconds": round(train_s, 2),
                      "loss_first_last": [round(losses[0], 4), round(losses[-1], 4)],

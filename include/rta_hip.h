@@ -410,10 +410,11 @@ int rta_shadow_rays(float *out_orig, float *out_dir, float *out_maxt,
 
 /* ------------------------------------------- learned hit predictor (MLP)
  * No counterpart in the reference.  A small network proposes, per ray, the
- * leaf clusters most likely to hold its closest hit (cluster c = sorted
- * triangle positions [c*n/1024, (c+1)*n/1024)) and a log hit distance;
- * rta_trace_seed_clusters turns the proposals into a max_t seed and the
- * exact traversal decides every result.  Network: 102 encoded inputs
+ * treelets most likely to hold its closest hit (treelet c = entry c of a
+ * fixed cut of the tree into at most 1024 subtrees, rta_treelet_cut) and a
+ * log hit distance; rta_trace_seed_treelets turns the proposals into a
+ * max_t seed and the exact traversal decides every result.  Network: 102
+ * encoded inputs
  * (origin mapped to [-1,1] by the scene box, unit direction, sin/cos of
  * 2^f*pi*v for f = 0..7), three ReLU layers of 64, 1024 cluster logits + 1
  * log distance; bf16 weights and activations, FP32 accumulation (MFMA),
@@ -447,22 +448,62 @@ int rta_pred_backward(rta_pred pred, const int32_t *label, const float *target, 
  * gradients; step counts from 1. */
 int rta_pred_adam(rta_pred pred, float lr, int step, rta_stream s);
 
+/* Fused inference for n rays (no batch limit): encoding, the four layers
+ * and the top-4 selection in one kernel, the logits kept in registers.
+ * top[4*i..] (optional): the 4 clusters with the largest keys, largest
+ * first, where a cluster's key is its logit with the low 10 mantissa bits
+ * replaced by 1023 - cluster; logdist[i] (optional): the log-distance
+ * output; logits (optional, n x 1040 floats, for testing): all outputs.
+ * The weights are those of the last rta_pred_set_params / rta_pred_adam on
+ * the same stream.  Same network and encoding as rta_pred_forward; the
+ * sums run in another order (16x16x32 tiles, bias first), so logits may
+ * differ from it in the last bits. */
+int rta_pred_infer(rta_pred pred, const float *ray_orig, const float *ray_dir, int n,
+                   const float *scene_min, const float *scene_max, int32_t *top,
+                   float *logdist, float *logits, rta_stream s);
+
 /* Karras ranges (sorted positions [from, to], 2 words per inner node, n-1
  * nodes) of the last rta_lbvh_build made with this workspace. */
 int rta_lbvh_copy_ranges(uint32_t *ranges, const void *workspace, int n, rta_stream s);
 
-/* Hit-distance seeds from predicted clusters: for each ray, the closest hit
- * among the triangles of its 4 clusters top[4*i..] (cluster c = sorted
- * positions [c*n_tris/1024, (c+1)*n_tris/1024)), found by the packed
- * traversal restricted to children whose range (inner: `ranges`; leaf: its
- * first triangle) meets a candidate cluster; t_seed[i] = its t or FLT_MAX.
- * Every such t is the t of a real hit of the ray (the same triangle test
- * and max_t), so the closest hit lies at or before it. */
-int rta_trace_seed_clusters(const float *tri4, const rta_node_pair *pairs, const uint32_t *flags,
-                            uint32_t root_link, uint32_t root_index, const uint32_t *ranges,
-                            int n_tris, const float *ray_orig, const float *ray_dir,
-                            const float *max_t, int n_rays, const int32_t *top, float *t_seed,
-                            uint32_t *err, rta_stream s);
+/* One treelet of a cut: the subtree rooted at `word` (a pair-record word:
+ * inner node id<<1|1, or a leaf's first triangle<<1) covering the sorted
+ * triangle positions [first, next treelet's first); box = its root's box
+ * (the whole tree's root: +-FLT_MAX). */
+typedef struct rta_treelet {
+	float lo[3], hi[3];
+	uint32_t word, first;
+} rta_treelet;
+
+/* A fixed cut of the packed tree into at most max_treelets (1..1024)
+ * subtrees, found in rounds: starting from the root, each round takes the
+ * inner treelets holding more than half as many triangles as the largest
+ * inner one and splits them, largest first (ties to the lower first
+ * position), as many as fit, into their two children.  table
+ * (max_treelets entries) receives the treelets in ascending `first` (so
+ * they partition [0, n_tris)), *count (device word) their number.  ranges:
+ * rta_lbvh_copy_ranges of the same build.  One workgroup. */
+int rta_treelet_cut(const rta_node_pair *pairs, uint32_t root_link, uint32_t root_index,
+                    const uint32_t *ranges, int n_tris, int max_treelets, rta_treelet *table,
+                    uint32_t *count, rta_stream s);
+
+/* Training labels from closest-hit records: label[i] = the treelet whose
+ * range holds hits[i].ref and target[i] = log(hits[i].t), or -1 and 0 for a
+ * miss (t = FLT_MAX). */
+int rta_treelet_labels(const rta_hit *hits, int n, const rta_treelet *table,
+                       const uint32_t *count, int32_t *label, float *target, rta_stream s);
+
+/* Hit-distance seeds from predicted treelets: for each ray, the closest hit
+ * among the triangles of its 4 treelets top[4*i..] (ids >= *count are
+ * skipped), each traversed from its root after a test of its box;
+ * t_seed[i] = that t, or FLT_MAX.  Every such t is the t of a real hit of
+ * the ray (the same triangle test, t <= max_t), so the closest hit lies at
+ * or before it. */
+int rta_trace_seed_treelets(const float *tri4, const rta_node_pair *pairs, const uint32_t *flags,
+                            const rta_treelet *table, const uint32_t *count,
+                            const float *ray_orig, const float *ray_dir, const float *max_t,
+                            int n_rays, const int32_t *top, float *t_seed, uint32_t *err,
+                            rta_stream s);
 /* out[i] = min(max_t[i], t_seed[i]): the max_t a seeded traversal runs with. */
 int rta_seed_max_t(float *out, const float *max_t, const float *t_seed, int n, rta_stream s);
 

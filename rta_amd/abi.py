@@ -30,10 +30,13 @@ AABB_DTYPE = np.dtype([("min", "<f4", 3), ("max", "<f4", 3)])
 NODE_DTYPE = np.dtype([("type_left_elems", "<u4"), ("right_tris", "<u4"),
                        ("min", "<f4", 3), ("max", "<f4", 3)])
 HIT_DTYPE = np.dtype([("t", "<f4"), ("beta", "<f4"), ("gamma", "<f4"), ("ref", "<u4")])
+# rta_treelet (include/rta_hip.h): root box, root pair-record word, first position
+TREELET_DTYPE = np.dtype([("lo", "<f4", 3), ("hi", "<f4", 3), ("word", "<u4"), ("first", "<u4")])
 TIMES_FIELDS = ("init_triangles", "voxel_bounds", "morton", "sort", "tree", "boxes",
                 "permute", "total")
 assert TRI_DTYPE.itemsize == 104 and AABB_DTYPE.itemsize == 24
 assert NODE_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 16
+assert TREELET_DTYPE.itemsize == 32
 
 FLT_MAX = np.float32(np.finfo(np.float32).max)
 
@@ -147,11 +150,13 @@ HIP_SIGNATURES = {
     "rta_pred_get_grads": [_vp, _vp],
     "rta_pred_zero_grads": [_vp, _vp],
     "rta_pred_forward": [_vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp],
+    "rta_pred_infer": [_vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp],
     "rta_pred_backward": [_vp, _vp, _vp, _i, _vp, _vp],
     "rta_pred_adam": [_vp, _f, _i, _vp],
     "rta_lbvh_copy_ranges": [_vp, _vp, _i, _vp],
-    "rta_trace_seed_clusters": [_vp, _vp, _vp, _u32, _u32, _vp, _i, _vp, _vp, _vp, _i, _vp, _vp,
-                                _vp, _vp],
+    "rta_treelet_cut": [_vp, _u32, _u32, _vp, _i, _i, _vp, _vp, _vp],
+    "rta_treelet_labels": [_vp, _i, _vp, _vp, _vp, _vp, _vp],
+    "rta_trace_seed_treelets": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp],
     "rta_seed_max_t": [_vp, _vp, _vp, _i, _vp],
 }
 

@@ -21,6 +21,7 @@ static_assert(sizeof(rta_aabb) == 24, "aabb layout");
 static_assert(sizeof(rta_node) == 32, "node layout");
 static_assert(sizeof(rta_hit) == 16, "hit layout");
 static_assert(sizeof(rta_node_pair) == 64, "pair layout");
+static_assert(sizeof(rta_treelet) == 32, "treelet layout");
 
 constexpr int kWave = 64;
 

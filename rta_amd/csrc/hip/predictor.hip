@@ -59,6 +59,24 @@ __device__ inline v4s load4(const uint16_t *p) {
 
 // ---------------------------------------------------------- encoding
 
+// sin and cos of 2^o pi v for o = o0..o0+3 (o0 = 0 or 4): sinpi/cospi at
+// 2^o0 v (an exact scaling), then double-angle steps
+// sin 2a = 2 sin a cos a, cos 2a = (cos a - sin a)(cos a + sin a); each step
+// at most doubles the error, so the last octave is within ~1e-6 of the
+// directly evaluated value.  e[2q] = sin, e[2q+1] = cos of octave o0+q.
+// Training and inference share this, so both see identical inputs.
+__device__ inline void encode_octaves(float v, int o0, float e[8]) {
+	const float a = (o0 ? 16.0f : 1.0f) * v;
+	float s = sinpif(a), c = cospif(a);
+	for (int q = 0; q < 4; ++q) {
+		e[2 * q] = s;
+		e[2 * q + 1] = c;
+		const float s2 = 2.0f * (s * c);
+		c = (c - s) * (c + s);
+		s = s2;
+	}
+}
+
 __global__ __launch_bounds__(256) void encode_kernel(uint16_t *__restrict__ x,
                                                      const float *__restrict__ ro,
                                                      const float *__restrict__ rd, int n,
@@ -74,14 +92,12 @@ __global__ __launch_bounds__(256) void encode_kernel(uint16_t *__restrict__ x,
 	v[5] = rd[3 * (size_t)r + 2];
 	uint16_t *row = x + (size_t)r * kInP;
 	for (int j = 0; j < 6; ++j) row[j] = to_bf16(v[j]);
-	for (int j = 0; j < 6; ++j) {
-		float f = 1.0f;
-		for (int o = 0; o < 8; ++o) {
-			row[6 + 16 * j + 2 * o] = to_bf16(sinpif(f * v[j]));
-			row[6 + 16 * j + 2 * o + 1] = to_bf16(cospif(f * v[j]));
-			f *= 2.0f;
+	for (int j = 0; j < 6; ++j)
+		for (int h = 0; h < 2; ++h) {
+			float e[8];
+			encode_octaves(v[j], 4 * h, e);
+			for (int q = 0; q < 8; ++q) row[6 + 16 * j + 8 * h + q] = to_bf16(e[q]);
 		}
-	}
 	for (int j = kIn; j < kInP; ++j) row[j] = 0;
 }
 
@@ -296,6 +312,250 @@ __global__ __launch_bounds__(256) void to_bf16_kernel(uint16_t *__restrict__ wb,
 	if (i < kParams) wb[i] = to_bf16(p[i]);
 }
 
+// ------------------------------------------------- fused inference
+//
+// One kernel from rays to top-4 clusters: activations stay in registers and
+// the 1040 logits of a ray never reach memory.  The products are computed
+// transposed, D = W H (rows = output features, columns = rays), with the
+// 16x16x32 bf16 MFMA: lane l supplies weight row l%16 and ray l%16, and the
+// 8 k-slots of lane group g = l/16.  Lane l then holds D[4g+i][l%16], which
+// is already the next layer's B operand once the k-slots are assigned as
+//   k-step s, group g, slot j<4  -> feature 32s + 4g + j       (tile 2s)
+//                      slot j>=4 -> feature 32s + 16 + 4g + j-4 (tile 2s+1)
+// so the weights are stored permuted to match (pack_infer_kernel), one 16-B
+// record per (tile, k-step, lane).  The first layer's slots are the encoded
+// inputs rotated by 6 (k' = feature - 6 for the 96 sin/cos features, then
+// the 6 raw inputs at k' = 96..101): slot (s, g, j) is octave block 4s+g,
+// which a lane computes by itself.
+//
+// Top-4 per ray: each logit becomes a key whose low 10 mantissa bits are
+// replaced by 1023 - cluster (distinct keys; the order of the keys is the
+// order of the logits truncated to 13 mantissa bits, ties to the lower
+// cluster for positive logits), kept in a sorted register quadruple by one
+// max and three med3 per key, and the four lanes of a ray merged by
+// shuffles at the end.
+
+typedef unsigned v4u __attribute__((ext_vector_type(4)));
+typedef __bf16 v8b __attribute__((ext_vector_type(8)));
+typedef float v2f __attribute__((ext_vector_type(2)));
+typedef __bf16 v2b __attribute__((ext_vector_type(2)));
+
+constexpr int kInferWaves = 16;                       // 1024 threads per block
+constexpr int kRec1 = 4 * 4 * 64;                     // layer 1: 4 tiles x 4 k-steps
+constexpr int kRecH = 4 * 2 * 64;                     // layers 2, 3
+constexpr int kRec4 = (kOutP / 16) * 2 * 64;          // layer 4: 65 tiles x 2 k-steps
+constexpr int pW1 = 0, pW2 = pW1 + kRec1, pW3 = pW2 + kRecH, pW4 = pW3 + kRecH;
+constexpr int kPackRecords = pW4 + kRec4;             // 16-B records (165,888 B)
+
+__device__ inline unsigned bf16x2(float a, float b) {
+	return __builtin_bit_cast(unsigned, __builtin_convertvector((v2f){a, b}, v2b));
+}
+
+__device__ inline v4f mfma32(v4u a, v4u b, v4f c) {
+	return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8b, a),
+	                                               __builtin_bit_cast(v8b, b), c, 0, 0, 0);
+}
+
+// wp[e] for element e = 8 * record + slot of the permuted bf16 weights
+__global__ __launch_bounds__(256) void pack_infer_kernel(uint16_t *__restrict__ wp,
+                                                         const uint16_t *__restrict__ wb) {
+	const int e = blockIdx.x * 256 + threadIdx.x;
+	if (e >= kPackRecords * 8) return;
+	const int rec = e >> 3, j = e & 7, lane = rec & 63, g = lane >> 4;
+	uint16_t val = 0;
+	if (rec < pW2) {
+		const int r = rec - pW1, t = r / (4 * 64), s = (r / 64) % 4;
+		const int kp = 32 * s + 8 * g + j;
+		const int col = kp < 96 ? kp + 6 : (kp < 102 ? kp - 96 : -1);
+		if (col >= 0) val = wb[oW1 + (16 * t + (lane & 15)) * kInP + col];
+	} else {
+		const int base = rec < pW3 ? pW2 : (rec < pW4 ? pW3 : pW4);
+		const int off = rec < pW3 ? oW2 : (rec < pW4 ? oW3 : oW4);
+		const int r = rec - base, t = r / (2 * 64), s = (r / 64) % 2;
+		const int k = j < 4 ? 32 * s + 4 * g + j : 32 * s + 16 + 4 * g + (j - 4);
+		val = wb[off + (16 * t + (lane & 15)) * kHid + k];
+	}
+	wp[e] = val;
+}
+
+// one 64-wide ReLU layer for two sets of 16 rays; in: KS k-steps of B
+// operands per set, out: the next layer's 2 k-steps per set
+template <int KS>
+__device__ inline void hidden_layer(v4u out[2][2], const v4u in[2][KS], const v4u *__restrict__ wp,
+                                    const float *__restrict__ bias, int lane) {
+	unsigned d[2][4][2];
+	#pragma unroll
+	for (int t = 0; t < 4; ++t) {
+		const float4 b = *reinterpret_cast<const float4 *>(bias + 16 * t + 4 * (lane >> 4));
+		v4f a0 = {b.x, b.y, b.z, b.w}, a1 = a0;
+		#pragma unroll
+		for (int s = 0; s < KS; ++s) {
+			const v4u w = wp[(t * KS + s) * 64 + lane];
+			a0 = mfma32(w, in[0][s], a0);
+			a1 = mfma32(w, in[1][s], a1);
+		}
+		d[0][t][0] = bf16x2(fmaxf(a0[0], 0.f), fmaxf(a0[1], 0.f));
+		d[0][t][1] = bf16x2(fmaxf(a0[2], 0.f), fmaxf(a0[3], 0.f));
+		d[1][t][0] = bf16x2(fmaxf(a1[0], 0.f), fmaxf(a1[1], 0.f));
+		d[1][t][1] = bf16x2(fmaxf(a1[2], 0.f), fmaxf(a1[3], 0.f));
+	}
+	#pragma unroll
+	for (int r = 0; r < 2; ++r)
+		#pragma unroll
+		for (int s = 0; s < 2; ++s)
+			out[r][s] = (v4u){d[r][2 * s][0], d[r][2 * s][1], d[r][2 * s + 1][0], d[r][2 * s + 1][1]};
+}
+
+// sorted (t[0] > t[1] > t[2] > t[3]) insertion of a key
+__device__ inline void insert4(float t[4], float x) {
+	const float n0 = __builtin_amdgcn_fmed3f(t[0], x, __builtin_inff());
+	const float n1 = __builtin_amdgcn_fmed3f(t[0], t[1], x);
+	const float n2 = __builtin_amdgcn_fmed3f(t[1], t[2], x);
+	const float n3 = __builtin_amdgcn_fmed3f(t[2], t[3], x);
+	t[0] = n0; t[1] = n1; t[2] = n2; t[3] = n3;
+}
+
+__device__ inline int key_cluster(float k) { return 1023 - (int)(__float_as_uint(k) & 0x3ffu); }
+
+// Persistent: each wave takes tiles of 32 rays (two sets of 16) from a
+// static stride; layer 4's permuted weights (133,120 B) and biases sit in
+// LDS, one block of 16 waves per CU.  LOGITS: also write every output (for
+// testing; a separate instance keeps the stores out of the fast loop).
+template <bool LOGITS>
+__global__ __launch_bounds__(1024) void infer_kernel(const v4u *__restrict__ wp,
+                                                     const float *__restrict__ params,
+                                                     const float *__restrict__ ro,
+                                                     const float *__restrict__ rd, int n,
+                                                     float3 lo, float3 scale,
+                                                     int32_t *__restrict__ top,
+                                                     float *__restrict__ logdist,
+                                                     float *__restrict__ logits) {
+	__shared__ v4u w4s[kRec4];
+	__shared__ v4f b4s[kOutP / 4];
+	for (int i = threadIdx.x; i < kRec4; i += kInferWaves * kWave) w4s[i] = wp[pW4 + i];
+	for (int i = threadIdx.x; i < kOutP / 4; i += kInferWaves * kWave)
+		b4s[i] = reinterpret_cast<const v4f *>(params + oB4)[i];
+	__syncthreads();
+	const int lane = __lane_id(), g = lane >> 4, col = lane & 15;
+	const int n_tiles = (n + 31) / 32;
+	for (int tile = blockIdx.x * kInferWaves + (int)(threadIdx.x / kWave); tile < n_tiles;
+	     tile += gridDim.x * kInferWaves) {
+		// layer 1 operands: octave blocks 4s+g of the lane's ray, s = 0..2,
+		// and the raw inputs (group 0) at s = 3
+		v4u x[2][4];
+		#pragma unroll
+		for (int r = 0; r < 2; ++r) {
+			const int ray = tile * 32 + r * 16 + col;
+			float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f;
+			if (ray < n) {
+				v0 = (ro[3 * (size_t)ray + 0] - lo.x) * scale.x - 1.0f;
+				v1 = (ro[3 * (size_t)ray + 1] - lo.y) * scale.y - 1.0f;
+				v2 = (ro[3 * (size_t)ray + 2] - lo.z) * scale.z - 1.0f;
+				v3 = rd[3 * (size_t)ray + 0];
+				v4 = rd[3 * (size_t)ray + 1];
+				v5 = rd[3 * (size_t)ray + 2];
+			}
+			// block 4s+g encodes input 2s + g/2, octaves 4*(g%2)..+3
+			const float in[3] = {(g & 2) ? v1 : v0, (g & 2) ? v3 : v2, (g & 2) ? v5 : v4};
+			#pragma unroll
+			for (int s = 0; s < 3; ++s) {
+				float e[8];
+				encode_octaves(in[s], (g & 1) ? 4 : 0, e);
+				x[r][s] = (v4u){bf16x2(e[0], e[1]), bf16x2(e[2], e[3]), bf16x2(e[4], e[5]),
+				                bf16x2(e[6], e[7])};
+			}
+			x[r][3] = g == 0 ? (v4u){bf16x2(v0, v1), bf16x2(v2, v3), bf16x2(v4, v5), 0u}
+			                 : (v4u){0u, 0u, 0u, 0u};
+		}
+		// the hidden layers' weights and biases are loaded per tile (from the
+		// caches): the empty asm hides their loop invariance, which would
+		// otherwise hoist all of them into registers and spill
+		const v4u *wq = wp;
+		const float *pq = params;
+		asm volatile("" : "+s"(wq), "+s"(pq));
+		v4u h1[2][2], h2[2][2], h3[2][2];
+		hidden_layer<4>(h1, x, wq + pW1, pq + oB1, lane);
+		hidden_layer<2>(h2, h1, wq + pW2, pq + oB2, lane);
+		hidden_layer<2>(h3, h2, wq + pW3, pq + oB3, lane);
+		// layer 4: 64 cluster tiles into the running top-4 of each lane,
+		// software-pipelined: the products of tile t+1 are issued and the
+		// operands of tile t+2 fetched before the keys of tile t are inserted;
+		// tile 64 (the log distance and padding) comes out of the last round
+		float tk[2][4];
+		#pragma unroll
+		for (int r = 0; r < 2; ++r)
+			#pragma unroll
+			for (int i = 0; i < 4; ++i) tk[r][i] = -__builtin_inff();
+		const unsigned low0 = 1023u - 4u * (unsigned)g;
+		v4u w0 = w4s[lane], w1 = w4s[64 + lane];
+		v4f bias = b4s[g];
+		v4f acc[2];
+		acc[0] = mfma32(w1, h3[0][1], mfma32(w0, h3[0][0], bias));
+		acc[1] = mfma32(w1, h3[1][1], mfma32(w0, h3[1][0], bias));
+		w0 = w4s[2 * 64 + lane];
+		w1 = w4s[3 * 64 + lane];
+		bias = b4s[4 + g];
+#pragma unroll 1
+		for (int t = 0; t < kClusters / 16; ++t) {
+			v4f nxt[2];
+			nxt[0] = mfma32(w1, h3[0][1], mfma32(w0, h3[0][0], bias));
+			nxt[1] = mfma32(w1, h3[1][1], mfma32(w0, h3[1][0], bias));
+			const int u = t + 2 < kOutP / 16 ? t + 2 : kOutP / 16 - 1;
+			w0 = w4s[(2 * u) * 64 + lane];
+			w1 = w4s[(2 * u + 1) * 64 + lane];
+			bias = b4s[4 * u + g];
+			#pragma unroll
+			for (int r = 0; r < 2; ++r)
+				#pragma unroll
+				for (int i = 0; i < 4; ++i) {
+					const unsigned low = low0 - 16u * (unsigned)t - (unsigned)i;
+					insert4(tk[r], __uint_as_float((__float_as_uint(acc[r][i]) & 0xfffffc00u) | low));
+				}
+			if (LOGITS)
+				#pragma unroll
+				for (int r = 0; r < 2; ++r) {
+					const int ray = tile * 32 + r * 16 + col;
+					if (ray < n)
+						#pragma unroll
+						for (int i = 0; i < 4; ++i)
+							logits[(size_t)ray * kOutP + 16 * t + 4 * g + i] = acc[r][i];
+				}
+			acc[0] = nxt[0];
+			acc[1] = nxt[1];
+		}
+		if (logdist || LOGITS) {
+			const int t = kClusters / 16;
+			#pragma unroll
+			for (int r = 0; r < 2; ++r) {
+				const int ray = tile * 32 + r * 16 + col;
+				if (ray >= n) continue;
+				if (logdist && g == 0) logdist[ray] = acc[r][0];
+				if (LOGITS)
+					#pragma unroll
+					for (int i = 0; i < 4; ++i)
+						logits[(size_t)ray * kOutP + 16 * t + 4 * g + i] = acc[r][i];
+			}
+		}
+		// merge the four lanes of each ray (groups g = 0..3)
+		#pragma unroll
+		for (int r = 0; r < 2; ++r) {
+			#pragma unroll
+			for (int off = 16; off <= 32; off <<= 1) {
+				float o[4];
+				#pragma unroll
+				for (int i = 0; i < 4; ++i) o[i] = __shfl_xor(tk[r][i], off);
+				#pragma unroll
+				for (int i = 0; i < 4; ++i) insert4(tk[r], o[i]);
+			}
+			const int ray = tile * 32 + r * 16 + col;
+			if (top && ray < n) {
+				const float k = g == 0 ? tk[r][0] : g == 1 ? tk[r][1] : g == 2 ? tk[r][2] : tk[r][3];
+				top[(size_t)ray * kTop + g] = key_cluster(k);
+			}
+		}
+	}
+}
+
 // counter-based uniform in [-1, 1)
 __host__ inline float init_uniform(uint64_t seed, uint64_t i) {
 	uint64_t z = seed * 0x9e3779b97f4a7c15ull + i + 1;
@@ -312,6 +572,8 @@ struct State {
 	uint16_t *x = nullptr, *h1 = nullptr, *h2 = nullptr, *h3 = nullptr;
 	float *out = nullptr;
 	uint16_t *dout = nullptr, *dh3 = nullptr, *dh2 = nullptr, *dh1 = nullptr;
+	uint16_t *wpack = nullptr;   // permuted bf16 weights of the fused inference kernel
+	int cus = 0;                 // compute units of the device the state lives on
 	int n_fwd = 0;               // rows of the last forward pass
 	std::vector<void *> allocs;
 	template <typename T> int alloc(T *&p, size_t count) {
@@ -366,6 +628,7 @@ int rta_pred_create(int max_batch, uint64_t seed, rta_pred *out) {
 	rc |= p->alloc(p->dh3, B * kHid);
 	rc |= p->alloc(p->dh2, B * kHid);
 	rc |= p->alloc(p->dh1, B * kHid);
+	rc |= p->alloc(p->wpack, (size_t)kPackRecords * 8);
 	if (rc != RTA_OK) {
 		delete p;
 		set_error("rta_pred_create: device allocation failed");
@@ -390,6 +653,10 @@ int rta_pred_create(int max_batch, uint64_t seed, rta_pred *out) {
 	hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for(kParams, 256)), dim3(256), 0, 0, p->wb,
 	                   p->params);
 	RTA_LAUNCH_CHECK("to_bf16_kernel");
+	int dev = 0;
+	RTA_HIP_TRY(hipGetDevice(&dev));
+	RTA_HIP_TRY(hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, dev));
+	if (p->cus < 1) p->cus = 1;
 	RTA_HIP_TRY(hipDeviceSynchronize());
 	*out = p;
 	return RTA_OK;
@@ -463,6 +730,37 @@ int rta_pred_forward(rta_pred h, const float *o, const float *d, int n, const fl
 		RTA_LAUNCH_CHECK("topk_kernel");
 	}
 	p->n_fwd = n;
+	return RTA_OK;
+}
+
+int rta_pred_infer(rta_pred h, const float *o, const float *d, int n, const float *scene_min,
+                   const float *scene_max, int32_t *top, float *logdist, float *logits,
+                   rta_stream s) {
+	State *p = reinterpret_cast<State *>(h);
+	if (!p || !o || !d || !scene_min || !scene_max) return invalid("rta_pred_infer: NULL pointer");
+	if (!top && !logdist && !logits) return invalid("rta_pred_infer: no output requested");
+	if (n < 0) return invalid("rta_pred_infer: n < 0");
+	if (n == 0) return RTA_OK;
+	hipStream_t st = as_stream(s);
+	// the permuted weights follow the current parameters (one small launch)
+	hipLaunchKernelGGL(pack_infer_kernel, dim3(blocks_for((long long)kPackRecords * 8, 256)),
+	                   dim3(256), 0, st, p->wpack, p->wb);
+	RTA_LAUNCH_CHECK("pack_infer_kernel");
+	float3 lo = make_float3(scene_min[0], scene_min[1], scene_min[2]);
+	float3 sc = make_float3(2.0f / (scene_max[0] - scene_min[0]), 2.0f / (scene_max[1] - scene_min[1]),
+	                        2.0f / (scene_max[2] - scene_min[2]));
+	const long long tiles = ((long long)n + 31) / 32;
+	const long long need = (tiles + kInferWaves - 1) / kInferWaves;
+	const unsigned blocks = (unsigned)(need < p->cus ? need : p->cus);
+	if (logits)
+		hipLaunchKernelGGL(infer_kernel<true>, dim3(blocks), dim3(kInferWaves * kWave), 0, st,
+		                   reinterpret_cast<const v4u *>(p->wpack), p->params, o, d, n, lo, sc, top,
+		                   logdist, logits);
+	else
+		hipLaunchKernelGGL(infer_kernel<false>, dim3(blocks), dim3(kInferWaves * kWave), 0, st,
+		                   reinterpret_cast<const v4u *>(p->wpack), p->params, o, d, n, lo, sc, top,
+		                   logdist, nullptr);
+	RTA_LAUNCH_CHECK("infer_kernel");
 	return RTA_OK;
 }
 

@@ -354,89 +354,237 @@ __global__ __launch_bounds__(BLOCK) void trace_cis_packed_kernel(
 	hits[tid] = closest;
 }
 
-// ------------------------------------- seeding from predicted clusters
+// -------------------------------- treelets and seeding from predictions
 
-// Closest hit of ray tid among the triangles of its kSeedTop candidate
-// clusters (sorted positions [c*n/1024, (c+1)*n/1024)), by the packed
-// traversal restricted to children whose sorted range meets a candidate
-// range: an inner child by its Karras range, a leaf child by its first
-// triangle.  Any triangle found is a real hit of the ray, so its t (or
-// FLT_MAX) bounds the ray's closest hit from above.
 constexpr int kSeedTop = 4;
-constexpr int kSeedClusters = 1024;
+constexpr int kMaxTreelets = 1024;
 
-__global__ __launch_bounds__(kTraceBlock) void seed_clusters_kernel(
+struct CutEntry {
+	float lo[3], hi[3];
+	uint32_t word, first, size;
+};
+
+// Bitonic sort of key/payload pairs over 1024 threads (one element each),
+// ascending or descending.  All threads of the block take part.
+template <bool DESC>
+__device__ inline void block_sort_1024(uint64_t *key, uint32_t *val, int t) {
+	for (int k = 2; k <= kMaxTreelets; k <<= 1)
+		for (int j = k >> 1; j > 0; j >>= 1) {
+			const int u = t ^ j;
+			if (u > t) {
+				const bool up = ((t & k) == 0) != DESC;
+				const uint64_t a = key[t], b = key[u];
+				if ((a > b) == up && a != b) {
+					key[t] = b; key[u] = a;
+					const uint32_t v = val[t]; val[t] = val[u]; val[u] = v;
+				}
+			}
+			__syncthreads();
+		}
+}
+
+// The cut in rounds (one block of 1024 threads), an approximation of
+// "always split the largest treelet": each round takes the inner treelets
+// holding more than half as many triangles as the largest inner one, sorts
+// them by (triangle count descending, first ascending), splits as many of
+// them as fit under max_treelets into their two children (boxes and words
+// from the pair record, ranges from the build), and stops when nothing fits
+// or nothing is splittable.  Every round splits at least one treelet.  The
+// result is sorted by first position.
+__global__ __launch_bounds__(kMaxTreelets) void treelet_cut_kernel(
+    const float4 *__restrict__ pairs, uint32_t root_word, const uint2 *__restrict__ ranges,
+    uint32_t n_tris, int max_treelets, rta_treelet *__restrict__ table,
+    uint32_t *__restrict__ count_out) {
+	__shared__ CutEntry ent[kMaxTreelets];
+	__shared__ uint64_t key[kMaxTreelets];
+	__shared__ uint32_t idx[kMaxTreelets];
+	__shared__ int s_count, s_splittable;
+	__shared__ uint32_t s_largest;
+	const int t = threadIdx.x;
+	if (t == 0) {
+		ent[0] = CutEntry{{-FLT_MAX, -FLT_MAX, -FLT_MAX}, {FLT_MAX, FLT_MAX, FLT_MAX}, root_word, 0u,
+		                  n_tris};
+		s_count = 1;
+	}
+	__syncthreads();
+	for (int round = 0; round < kMaxTreelets; ++round) {
+		const int K = s_count;
+		if (K >= max_treelets) break;
+		const bool inner = t < K && (ent[t].word & 1u);
+		if (t == 0) {
+			s_splittable = 0;
+			s_largest = 0u;
+		}
+		__syncthreads();
+		if (inner) atomicMax(&s_largest, ent[t].size);
+		__syncthreads();
+		uint64_t k = 0;
+		if (inner && 2ull * ent[t].size > (uint64_t)s_largest)
+			k = ((uint64_t)ent[t].size << 32) | (uint64_t)(0xffffffffu - ent[t].first);
+		key[t] = k;
+		idx[t] = (uint32_t)t;
+		if (k != 0) atomicAdd(&s_splittable, 1);
+		__syncthreads();
+		block_sort_1024<true>(key, idx, t);
+		const int splits = min(s_splittable, max_treelets - K);
+		if (splits == 0) break;
+		CutEntry a, b;
+		int e = -1;
+		if (t < splits) {
+			e = (int)idx[t];
+			const float4 *p = pairs + 4 * (size_t)(ent[e].word >> 1);
+			const float4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
+			a = CutEntry{{q0.x, q0.z, q1.x}, {q1.z, q2.x, q2.z}, __float_as_uint(q3.x), 0u, 1u};
+			b = CutEntry{{q0.y, q0.w, q1.y}, {q1.w, q2.y, q2.w}, __float_as_uint(q3.y), 0u, 1u};
+			auto locate = [&](CutEntry &c) {
+				if (c.word & 1u) {
+					const uint2 r = ranges[c.word >> 1];
+					c.first = r.x;
+					c.size = r.y - r.x + 1u;
+				} else {
+					c.first = c.word >> 1;   // a leaf is never split
+				}
+			};
+			locate(a);
+			locate(b);
+		}
+		__syncthreads();   // every read of ent before the writes below
+		if (t < splits) {
+			ent[e] = a;
+			ent[K + t] = b;
+		}
+		if (t == 0) s_count = K + splits;
+		__syncthreads();
+	}
+	const int K = s_count;
+	key[t] = t < K ? (uint64_t)ent[t].first : ~0ull;
+	idx[t] = (uint32_t)t;
+	__syncthreads();
+	block_sort_1024<false>(key, idx, t);
+	if (t < K) {
+		const CutEntry &c = ent[idx[t]];
+		rta_treelet o;
+		for (int j = 0; j < 3; ++j) { o.lo[j] = c.lo[j]; o.hi[j] = c.hi[j]; }
+		o.word = c.word;
+		o.first = c.first;
+		table[t] = o;
+	}
+	if (t == 0) *count_out = (uint32_t)K;
+}
+
+__global__ void treelet_labels_kernel(const rta_hit *__restrict__ hits, int n,
+                                      const rta_treelet *__restrict__ table,
+                                      const uint32_t *__restrict__ count, int32_t *__restrict__ label,
+                                      float *__restrict__ target) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const rta_hit h = hits[i];
+	if (h.t == FLT_MAX) {
+		label[i] = -1;
+		target[i] = 0.0f;
+		return;
+	}
+	// the last treelet whose first position is <= ref (table[0].first = 0)
+	uint32_t lo = 0, hi = *count;
+	while (hi - lo > 1) {
+		const uint32_t mid = (lo + hi) >> 1;
+		if (table[mid].first <= h.ref) lo = mid;
+		else hi = mid;
+	}
+	label[i] = (int32_t)lo;
+	target[i] = logf(h.t);
+}
+
+// Closest hit of ray tid among the triangles of its kSeedTop predicted
+// treelets, each traversed from its root (packed CIS, near child first)
+// once the ray meets the root's box before the current seed.  Any triangle
+// found is a real hit of the ray, so its t (or FLT_MAX) bounds the ray's
+// closest hit from above.
+__global__ __launch_bounds__(kTraceBlock) void seed_treelets_kernel(
     const float4 *__restrict__ tri4, const float4 *__restrict__ pairs,
-    const uint32_t *__restrict__ flags, uint32_t root_word, const uint2 *__restrict__ ranges,
-    int n_tris, const float *__restrict__ ro, const float *__restrict__ rd,
-    const float *__restrict__ rmax, int n_rays, const int32_t *__restrict__ top,
-    float *__restrict__ t_seed, uint32_t *__restrict__ err) {
+    const uint32_t *__restrict__ flags, const rta_treelet *__restrict__ table,
+    const uint32_t *__restrict__ count, const float *__restrict__ ro,
+    const float *__restrict__ rd, const float *__restrict__ rmax, int n_rays,
+    const int32_t *__restrict__ top, float *__restrict__ t_seed, uint32_t *__restrict__ err) {
 	__shared__ uint32_t stack[kStack][kTraceBlock];
 	const int lt = threadIdx.x;
 	const int tid = blockIdx.x * kTraceBlock + lt;
 	if (tid >= n_rays) return;
-	uint32_t clo[kSeedTop], chi[kSeedTop];
-	for (int k = 0; k < kSeedTop; ++k) {
-		const uint32_t c = (uint32_t)top[(size_t)tid * kSeedTop + k] & (kSeedClusters - 1);
-		clo[k] = (uint32_t)(((uint64_t)c * (uint32_t)n_tris) / kSeedClusters);
-		chi[k] = (uint32_t)(((uint64_t)(c + 1) * (uint32_t)n_tris) / kSeedClusters);
-	}
-	auto wanted = [&](uint32_t word) {
-		uint32_t a, b;
-		if (word & 1u) { const uint2 r = ranges[word >> 1]; a = r.x; b = r.y; }
-		else { a = b = word >> 1; }
-		bool ok = false;
-		for (int k = 0; k < kSeedTop; ++k) ok = ok || (a < chi[k] && b >= clo[k]);
-		return ok;
-	};
+	const uint32_t K = *count;
 	const float ox = ro[3 * tid + 0], oy = ro[3 * tid + 1], oz = ro[3 * tid + 2];
 	const float dx = rd[3 * tid + 0], dy = rd[3 * tid + 1], dz = rd[3 * tid + 2];
 	const float tmax = rmax[tid];
 	const bool fast = ((*flags & 1u) == 0u) && ray_nz_ok(ox, oy, oz, dx, dy, dz);
 	const float yx = 1.0f / dx, yy = 1.0f / dy, yz = 1.0f / dz;
-	float ct = FLT_MAX;
-	uint32_t curr = root_word;
-	int sp = -1;
-	if (!wanted(curr)) {  // only a leaf root outside every candidate cluster
-		t_seed[tid] = FLT_MAX;
-		return;
-	}
-	for (uint32_t guard = 0;; ++guard) {
-		if (guard >= kMaxSteps) { atomicAdd(err, 1u); break; }
-		if (curr & 1u) {
-			const float4 *p = pairs + 4 * (size_t)(curr >> 1);
-			const float4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
-			float dl, dr;
-			bool hl, hr;
-			pair_boxes(q0, q1, q2, fast, ox, oy, oz, dx, dy, dz, yx, yy, yz, hl, dl, hr, dr);
-			const uint32_t wl = __float_as_uint(q3.x), wr = __float_as_uint(q3.y);
-			hl = hl && dl < ct && dl <= tmax && wanted(wl);
-			hr = hr && dr < ct && dr <= tmax && wanted(wr);
-			if (hl && hr) {
-				if (sp + 1 >= kStack) { atomicAdd(err, 1u); break; }
-				if (dl <= dr) { stack[++sp][lt] = wr; curr = wl; }
-				else          { stack[++sp][lt] = wl; curr = wr; }
-				continue;
-			}
-			if (hl) { curr = wl; continue; }
-			if (hr) { curr = wr; continue; }
-		} else {
-			uint32_t i = curr >> 1;
-			while (true) {
-				const float4 v0 = tri4[3 * (size_t)i + 0];
-				const float4 v1 = tri4[3 * (size_t)i + 1];
-				const float4 v2 = tri4[3 * (size_t)i + 2];
-				float t, be, ga;
-				if (tri_test(v0.x, v0.y, v0.z, v1.x, v1.y, v1.z, v2.x, v2.y, v2.z, ox, oy, oz, dx,
-				             dy, dz, t, be, ga) &&
-				    t < ct && t <= tmax)
-					ct = t;
-				if (__float_as_uint(v0.w) != 0u) break;
-				++i;
-			}
+	// the candidates' root boxes first; the treelets are then traversed in
+	// order of entry distance, so an early hit prunes the later ones.  A box
+	// entry distance is finite, so +inf marks a candidate that is skipped.
+	float dk[kSeedTop];
+	uint32_t wk[kSeedTop];
+	for (int k = 0; k < kSeedTop; ++k) {
+		const uint32_t c = (uint32_t)top[(size_t)tid * kSeedTop + k];
+		dk[k] = __builtin_inff();
+		wk[k] = 0u;
+		if (c >= K) continue;
+		const rta_treelet tr = table[c];
+		float dist;
+		if (box_test(tr.lo[0], tr.lo[1], tr.lo[2], tr.hi[0], tr.hi[1], tr.hi[2], ox, oy, oz, dx, dy,
+		             dz, dist) &&
+		    dist <= tmax) {
+			dk[k] = dist;
+			wk[k] = tr.word;
 		}
-		if (sp < 0) break;
-		curr = stack[sp--][lt];
+	}
+	auto order = [&](int a, int b) {
+		if (dk[b] < dk[a]) {
+			const float d = dk[a]; dk[a] = dk[b]; dk[b] = d;
+			const uint32_t w = wk[a]; wk[a] = wk[b]; wk[b] = w;
+		}
+	};
+	order(0, 1); order(2, 3); order(0, 2); order(1, 3); order(1, 2);
+	float ct = FLT_MAX;
+	uint32_t guard = 0;
+	for (int k = 0; k < kSeedTop; ++k) {
+		if (!(dk[k] < ct)) continue;     // skipped, or entered after the seed
+		uint32_t curr = wk[k];
+		int sp = -1;
+		for (;; ++guard) {
+			if (guard >= kMaxSteps) { atomicAdd(err, 1u); break; }
+			if (curr & 1u) {
+				const float4 *p = pairs + 4 * (size_t)(curr >> 1);
+				const float4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
+				float dl, dr;
+				bool hl, hr;
+				pair_boxes(q0, q1, q2, fast, ox, oy, oz, dx, dy, dz, yx, yy, yz, hl, dl, hr, dr);
+				const uint32_t wl = __float_as_uint(q3.x), wr = __float_as_uint(q3.y);
+				hl = hl && dl < ct && dl <= tmax;
+				hr = hr && dr < ct && dr <= tmax;
+				if (hl && hr) {
+					if (sp + 1 >= kStack) { atomicAdd(err, 1u); break; }
+					if (dl <= dr) { stack[++sp][lt] = wr; curr = wl; }
+					else          { stack[++sp][lt] = wl; curr = wr; }
+					continue;
+				}
+				if (hl) { curr = wl; continue; }
+				if (hr) { curr = wr; continue; }
+			} else {
+				uint32_t i = curr >> 1;
+				while (true) {
+					const float4 v0 = tri4[3 * (size_t)i + 0];
+					const float4 v1 = tri4[3 * (size_t)i + 1];
+					const float4 v2 = tri4[3 * (size_t)i + 2];
+					float t, be, ga;
+					if (tri_test(v0.x, v0.y, v0.z, v1.x, v1.y, v1.z, v2.x, v2.y, v2.z, ox, oy, oz, dx,
+					             dy, dz, t, be, ga) &&
+					    t < ct && t <= tmax)
+						ct = t;
+					if (__float_as_uint(v0.w) != 0u) break;
+					++i;
+				}
+			}
+			if (sp < 0) break;
+			curr = stack[sp--][lt];
+		}
 	}
 	t_seed[tid] = ct;
 }
@@ -1036,20 +1184,44 @@ int rta_trace_shadow_cis_persistent(const float *tri4, const rta_node_pair *pair
 	                              ray_count, hits, err, queue, cfg, as_stream(s));
 }
 
-int rta_trace_seed_clusters(const float *tri4, const rta_node_pair *pairs, const uint32_t *flags,
-                            uint32_t root_link, uint32_t root_index, const uint32_t *ranges,
-                            int n_tris, const float *ray_orig, const float *ray_dir,
-                            const float *max_t, int n_rays, const int32_t *top, float *t_seed,
-                            uint32_t *err, rta_stream s) {
-	if (!tri4 || !pairs || !flags || !ranges || !ray_orig || !ray_dir || !max_t || !top ||
-	    !t_seed || !err || n_rays <= 0 || n_tris <= 0)
-		return invalid("rta_trace_seed_clusters: bad arguments");
+int rta_treelet_cut(const rta_node_pair *pairs, uint32_t root_link, uint32_t root_index,
+                    const uint32_t *ranges, int n_tris, int max_treelets, rta_treelet *table,
+                    uint32_t *count, rta_stream s) {
+	if (!pairs || !ranges || !table || !count || n_tris <= 0 || max_treelets < 1 ||
+	    max_treelets > kMaxTreelets)
+		return invalid("rta_treelet_cut: bad arguments (max_treelets in 1..1024)");
 	const uint32_t root_word = link_inner(root_link) ? 1u : (root_index << 1);
-	hipLaunchKernelGGL(seed_clusters_kernel, dim3(grid_for(n_rays, kTraceBlock)),
+	hipLaunchKernelGGL(treelet_cut_kernel, dim3(1), dim3(kMaxTreelets), 0, as_stream(s),
+	                   (const float4 *)pairs, root_word, (const uint2 *)ranges, (uint32_t)n_tris,
+	                   max_treelets, table, count);
+	RTA_LAUNCH_CHECK("treelet_cut_kernel");
+	return RTA_OK;
+}
+
+int rta_treelet_labels(const rta_hit *hits, int n, const rta_treelet *table,
+                       const uint32_t *count, int32_t *label, float *target, rta_stream s) {
+	if (!hits || !table || !count || !label || !target || n < 0)
+		return invalid("rta_treelet_labels: bad arguments");
+	if (n == 0) return RTA_OK;
+	hipLaunchKernelGGL(treelet_labels_kernel, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(s),
+	                   hits, n, table, count, label, target);
+	RTA_LAUNCH_CHECK("treelet_labels_kernel");
+	return RTA_OK;
+}
+
+int rta_trace_seed_treelets(const float *tri4, const rta_node_pair *pairs, const uint32_t *flags,
+                            const rta_treelet *table, const uint32_t *count,
+                            const float *ray_orig, const float *ray_dir, const float *max_t,
+                            int n_rays, const int32_t *top, float *t_seed, uint32_t *err,
+                            rta_stream s) {
+	if (!tri4 || !pairs || !flags || !table || !count || !ray_orig || !ray_dir || !max_t ||
+	    !top || !t_seed || !err || n_rays <= 0)
+		return invalid("rta_trace_seed_treelets: bad arguments");
+	hipLaunchKernelGGL(seed_treelets_kernel, dim3(grid_for(n_rays, kTraceBlock)),
 	                   dim3(kTraceBlock), 0, as_stream(s), (const float4 *)tri4,
-	                   (const float4 *)pairs, flags, root_word, (const uint2 *)ranges, n_tris,
-	                   ray_orig, ray_dir, max_t, n_rays, top, t_seed, err);
-	RTA_LAUNCH_CHECK("seed_clusters_kernel");
+	                   (const float4 *)pairs, flags, table, count, ray_orig, ray_dir, max_t, n_rays,
+	                   top, t_seed, err);
+	RTA_LAUNCH_CHECK("seed_treelets_kernel");
 	return RTA_OK;
 }
 

@@ -1,13 +1,12 @@
 """The learned hit predictor (rta_pred_* in include/rta_hip.h) and the
 seeded traversal it feeds.  No counterpart in the reference.
 
-A ray's label is the leaf cluster of its closest hit: cluster
-c = sorted position * 1024 // n_tris (the LBVH's triangles are in Morton
-order, so a cluster is a compact region), and its distance target log(t).
-Inference proposes the 4 best clusters per ray; rta_trace_seed_clusters
-finds the closest hit among their triangles, whose t becomes the ray's
-max_t for the exact traversal.  Every seed is the t of a real hit, so the
-exact traversal sees the closest hit inside its range.
+The tree is cut into at most 1024 treelets (subtrees; rta_treelet_cut).  A
+ray's label is the treelet holding its closest hit and its distance target
+log(t).  Inference proposes the 4 best treelets per ray;
+rta_trace_seed_treelets finds the closest hit among their triangles, whose t
+becomes the ray's max_t for the exact traversal.  Every seed is the t of a
+real hit, so the exact traversal sees the closest hit inside its range.
 """
 from __future__ import annotations
 
@@ -16,7 +15,7 @@ import time
 
 import numpy as np
 
-from .abi import FLT_MAX, HIT_DTYPE, call, ptr
+from .abi import FLT_MAX, HIT_DTYPE, TREELET_DTYPE, call, ptr
 from .device import DeviceBuffer, Stream
 
 N_CLUSTERS = 1024
@@ -51,19 +50,44 @@ def unpack(params: np.ndarray) -> dict:
     return out
 
 
-def cluster_labels(hits: np.ndarray, n_tris: int):
-    """(label, target) for hit records: the cluster of the hit triangle's
-    sorted position and log(t); label -1 and target 0 for misses."""
-    hit = hits["t"] != FLT_MAX
-    label = np.full(len(hits), -1, np.int32)
-    label[hit] = (hits["ref"][hit].astype(np.int64) * N_CLUSTERS // n_tris).astype(np.int32)
-    target = np.zeros(len(hits), np.float32)
-    target[hit] = np.log(np.maximum(hits["t"][hit], np.float32(1e-30)))
-    return label, target
-
-
 def _at(buf: DeviceBuffer, offset_bytes: int):
     return C.c_void_p(buf.ptr + offset_bytes)
+
+
+class TreeletCut:
+    """A cut of a packed tree into at most max_treelets subtrees
+    (rta_treelet_cut): the device table and count, and labels from hit
+    records.  `packed` is a PackedBvh, `ranges` the build's Karras ranges
+    (LbvhConstructor.ranges())."""
+
+    def __init__(self, packed, ranges: DeviceBuffer, n_tris: int, max_treelets: int = N_CLUSTERS,
+                 stream=None):
+        self.max_treelets = max_treelets
+        self.table = DeviceBuffer(32 * max_treelets)
+        self.count = DeviceBuffer(4)
+        call("rta_treelet_cut", ptr(packed.pairs), packed.root_link, packed.root_index,
+             ptr(ranges), n_tris, max_treelets, ptr(self.table), ptr(self.count),
+             ptr(stream) if stream else None)
+
+    def host(self) -> np.ndarray:
+        """The table's entries (TREELET_DTYPE), ascending first position."""
+        k = int(self.count.download(np.uint32, 1)[0])
+        return self.table.download(TREELET_DTYPE, k)
+
+    def labels(self, hits: DeviceBuffer, n: int, label: DeviceBuffer, target: DeviceBuffer,
+               stream=None) -> None:
+        """label[i] = treelet of hits[i].ref, target[i] = log(t); -1 / 0 for
+        misses (rta_treelet_labels)."""
+        call("rta_treelet_labels", ptr(hits), n, ptr(self.table), ptr(self.count), ptr(label),
+             ptr(target), ptr(stream) if stream else None)
+
+    def labels_host(self, hits: DeviceBuffer, n: int, stream=None):
+        """labels() downloaded: (label int32[n], target float32[n])."""
+        lab, tgt = DeviceBuffer(4 * max(n, 1)), DeviceBuffer(4 * max(n, 1))
+        self.labels(hits, n, lab, tgt, stream)
+        if stream:
+            stream.synchronize()
+        return lab.download(np.int32, n), tgt.download(np.float32, n)
 
 
 class HitPredictor:
@@ -113,6 +137,14 @@ class HitPredictor:
              ptr(top) if top is not None else None,
              ptr(logdist) if logdist is not None else None, ptr(stream) if stream else None)
 
+    def infer(self, o, d, n, top=None, logdist=None, logits=None, stream=None, o_off=0):
+        """The fused inference kernel (any n): top-4 clusters by key, the
+        log distance and, for testing, all n x 1040 outputs."""
+        call("rta_pred_infer", self.h, _at(o, 12 * o_off), _at(d, 12 * o_off), n,
+             self.scene_min, self.scene_max, ptr(top) if top is not None else None,
+             ptr(logdist) if logdist is not None else None,
+             ptr(logits) if logits is not None else None, ptr(stream) if stream else None)
+
     def train(self, o, d, label, target, n, epochs=2, batch=8192, lr=2e-3, stream=None,
               seed=0):
         """Adam over `epochs` passes of the n rays in batches (a seeded
@@ -139,21 +171,17 @@ class HitPredictor:
         return losses, time.perf_counter() - t0
 
 
-def seed_max_t(pred: HitPredictor, packed, ranges: DeviceBuffer, n_tris: int, rays, n: int,
-               top: DeviceBuffer, t_seed: DeviceBuffer, maxt_out: DeviceBuffer,
-               err: DeviceBuffer, stream=None) -> None:
-    """Inference (top-4 clusters), the restricted traversal of those
-    clusters, and max_t' = min(max_t, seed) for the n rays.  `packed` is a
-    PackedBvh (pairs, tri4, flags, root words)."""
+def seed_max_t(pred: HitPredictor, packed, cut: TreeletCut, rays, n: int, top: DeviceBuffer,
+               t_seed: DeviceBuffer, maxt_out: DeviceBuffer, err: DeviceBuffer,
+               stream=None) -> None:
+    """Inference (top-4 treelets, fused kernel), the traversal of those
+    treelets, and max_t' = min(max_t, seed) for the n rays.  `packed` is a
+    PackedBvh (pairs, tri4, flags)."""
     sp = ptr(stream) if stream else None
-    for s0 in range(0, n, pred.max_batch):
-        m = min(pred.max_batch, n - s0)
-        call("rta_pred_forward", pred.h, _at(rays.gpu_origin, 12 * s0),
-             _at(rays.gpu_direction, 12 * s0), m, pred.scene_min, pred.scene_max, None,
-             _at(top, 16 * s0), None, sp)
-    call("rta_trace_seed_clusters", ptr(packed.tri4), ptr(packed.pairs), ptr(packed.flags),
-         packed.root_link, packed.root_index, ptr(ranges), n_tris, ptr(rays.gpu_origin),
-         ptr(rays.gpu_direction), ptr(rays.gpu_maxt), n, ptr(top), ptr(t_seed), ptr(err), sp)
+    pred.infer(rays.gpu_origin, rays.gpu_direction, n, top=top, stream=stream)
+    call("rta_trace_seed_treelets", ptr(packed.tri4), ptr(packed.pairs), ptr(packed.flags),
+         ptr(cut.table), ptr(cut.count), ptr(rays.gpu_origin), ptr(rays.gpu_direction),
+         ptr(rays.gpu_maxt), n, ptr(top), ptr(t_seed), ptr(err), sp)
     call("rta_seed_max_t", ptr(maxt_out), ptr(rays.gpu_maxt), ptr(t_seed), n, sp)
 
 
@@ -165,5 +193,5 @@ def top4_accuracy(top: np.ndarray, label: np.ndarray) -> float:
     return float((top[hit] == label[hit, None]).any(axis=1).mean())
 
 
-__all__ = ["HitPredictor", "cluster_labels", "seed_max_t", "top4_accuracy", "unpack",
-           "param_count", "PARAM_COUNT", "N_CLUSTERS", "TOP", "HIT_DTYPE"]
+__all__ = ["HitPredictor", "TreeletCut", "seed_max_t", "top4_accuracy", "unpack",
+           "param_count", "PARAM_COUNT", "N_CLUSTERS", "TOP", "HIT_DTYPE", "TREELET_DTYPE"]

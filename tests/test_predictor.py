@@ -126,6 +126,75 @@ def test_forward_matches_reference(gpu):
     assert np.array_equal(t4, exp)
 
 
+def keyed_top4(logits):
+    """The fused kernel's selection rule on the CPU: each cluster's key is its
+    logit with the low 10 mantissa bits replaced by 1023 - cluster; the 4
+    largest keys, largest first."""
+    bits = np.ascontiguousarray(logits[:, :1024], np.float32).view(np.uint32)
+    keys = ((bits & np.uint32(0xFFFFFC00)) | (np.uint32(1023) - np.arange(1024, dtype=np.uint32)))
+    return np.argsort(-keys.view(np.float32), axis=1, kind="stable")[:, :4].astype(np.int32)
+
+
+def test_keyed_top4_rule():
+    """The key rule on hand-made rows: the largest logits win, and logits
+    equal in their top 13 mantissa bits go to the lower cluster."""
+    rows = np.full((3, 1040), -5.0, np.float32)
+    rows[0, [7, 300, 2, 1023]] = [4.0, 3.0, 2.0, 1.0]
+    rows[1, [10, 20, 30, 40, 50]] = 1.0              # five equal logits
+    base = np.float32(1.5)
+    rows[2, 900] = base
+    rows[2, 100] = np.nextafter(base, np.float32(2))  # same top bits as 1.5
+    rows[2, 5] = np.float32(1.25)
+    rows[2, 6] = np.float32(-0.5)
+    got = keyed_top4(rows)
+    assert got[0].tolist() == [7, 300, 2, 1023]
+    assert got[1].tolist() == [10, 20, 30, 40]
+    assert got[2].tolist() == [100, 900, 5, 6]
+
+
+@pytest.mark.gpu
+def test_infer_matches_reference(gpu):
+    """The fused inference kernel: its outputs against the torch model and the
+    unfused forward pass, its top-4 against the key rule on its own logits,
+    with a ragged last tile (n not a multiple of 32)."""
+    torch = _torch()
+    n = 1001
+    o, d = _random_rays(n, 13)
+    pred = P.HitPredictor(1024, seed=17, scene_min=LO, scene_max=HI)
+    params = pred.params()
+    rng = np.random.default_rng(14)
+    for name in ("b1", "b2", "b3", "b4"):
+        off, rows, _ = P.LAYOUT[name]
+        params[off:off + rows] = rng.normal(scale=0.1, size=rows).astype(np.float32)
+    params[P.LAYOUT["b4"][0] + 1025:P.LAYOUT["b4"][0] + 1040] = 0.0
+    pred.set_params(params)
+    ob, db = DeviceBuffer.from_array(o), DeviceBuffer.from_array(d)
+    logits, top, ld = DeviceBuffer(4 * 1040 * n), DeviceBuffer(16 * n), DeviceBuffer(4 * n)
+    pred.infer(ob, db, n, top=top, logdist=ld, logits=logits)
+    out = DeviceBuffer(4 * 1040 * n)
+    pred.forward(ob, db, n, out=out)
+    top2, ld2 = DeviceBuffer(16 * n), DeviceBuffer(4 * n)
+    pred.infer(ob, db, n, top=top2, logdist=ld2)
+    rta_amd.synchronize()
+    got = logits.download(np.float32, 1040 * n).reshape(n, 1040)
+    unfused = out.download(np.float32, 1040 * n).reshape(n, 1040)
+    ref, _, _ = model_ref(torch, params, encode_ref(o, d, LO, HI))
+    ref = ref.detach().numpy()
+    err = np.linalg.norm(got[:, :1025] - ref[:, :1025]) / np.linalg.norm(ref[:, :1025])
+    err_u = np.linalg.norm(got - unfused) / np.linalg.norm(unfused)
+    say(f"fused inference: relative L2 error {err:.2e} vs the torch model, {err_u:.2e} vs the "
+        f"unfused forward")
+    assert err < 1e-2
+    assert err_u < 1e-4                       # same bf16 network, other summation order
+    assert np.all(got[:, 1025:] == 0.0)
+    t4 = top.download(np.int32, 4 * n).reshape(n, 4)
+    assert np.array_equal(t4, keyed_top4(got))
+    assert np.array_equal(ld.download(np.float32, n), got[:, 1024])
+    # the instance without the logits output gives the same results
+    assert np.array_equal(top2.download(np.int32, 4 * n).reshape(n, 4), t4)
+    assert np.array_equal(ld2.download(np.float32, n), got[:, 1024])
+
+
 @pytest.mark.gpu
 def test_gradients_match_reference(gpu):
     torch = _torch()
@@ -159,10 +228,110 @@ def test_gradients_match_reference(gpu):
     assert np.all(g["W1"][:, 102:] == 0.0) and np.all(g["W4"][1025:] == 0.0)
 
 
+# ------------------------------------------------------------- treelets
+
+def cut_restatement(nodes, ranges, n, max_treelets):
+    """The treelet cut on the host from the 32-B nodes and Karras ranges:
+    rounds that split the inner treelets holding more than half as many
+    triangles as the largest inner one (largest first, ties to the lower
+    first position), as many as fit; sorted by first position.  Entries are
+    (first, word, lo, hi)."""
+    big = float(F32MAX)
+
+    def child(c):
+        nd = nodes[c]
+        if nd["type_left_elems"] & 1:
+            a, b = ranges[c]
+            return [int(a), (c << 1) | 1, int(b) - int(a) + 1, tuple(nd["min"]), tuple(nd["max"])]
+        return [int(nd["right_tris"]), int(nd["right_tris"]) << 1, 1, tuple(nd["min"]),
+                tuple(nd["max"])]
+
+    r = nodes[0]
+    root_word = 1 if r["type_left_elems"] & 1 else int(r["right_tris"]) << 1
+    ent = [[0, root_word, n, (-big,) * 3, (big,) * 3]]
+    while len(ent) < max_treelets:
+        inner = [e for e in range(len(ent)) if ent[e][1] & 1]
+        largest = max((ent[e][2] for e in inner), default=0)
+        cand = sorted((e for e in inner if 2 * ent[e][2] > largest),
+                      key=lambda e: (-ent[e][2], ent[e][0]))
+        splits = min(len(cand), max_treelets - len(ent))
+        if splits == 0:
+            break
+        new = []
+        for e in cand[:splits]:
+            x = ent[e][1] >> 1
+            ent[e] = child(int(nodes[x]["type_left_elems"]) >> 1)
+            new.append(child(int(nodes[x]["right_tris"])))
+        ent.extend(new)
+    return sorted((e[0], e[1], e[3], e[4]) for e in ent)
+
+
+def _terrain_bvh(grid, n_random):
+    tris_h = scenes.terrain_scene(grid=grid, n_random=n_random)
+    n = len(tris_h)
+    ctor = rta_amd.LbvhConstructor()
+    bvh = ctor.build(DeviceBuffer.from_array(tris_h), n)
+    return tris_h, n, ctor, bvh
+
+
+@pytest.mark.gpu
+def test_treelet_cut_matches_restatement(gpu):
+    """The cut kernel's table equals the host restatement (words, first
+    positions and boxes bit for bit) for several treelet limits, and the
+    treelets partition the sorted triangle positions."""
+    tris_h, n, ctor, bvh = _terrain_bvh(64, 3_000)
+    ranges = ctor.ranges()
+    packed = bvh.packed()
+    rta_amd.synchronize()
+    nodes = bvh.node_data.download(rta_amd.abi.NODE_DTYPE, 2 * n - 1)
+    rng_h = ranges.download(np.uint32, 2 * (n - 1)).reshape(-1, 2)
+    for k in (1, 2, 7, 100, 1024):
+        cut = P.TreeletCut(packed, ranges, n, max_treelets=k)
+        got = cut.host()
+        exp = cut_restatement(nodes, rng_h, n, k)
+        say(f"{n} triangles, limit {k}: {len(got)} treelets, largest "
+            f"{np.max(np.diff(np.append(got['first'], n)))} triangles")
+        assert len(got) == len(exp) and len(got) <= k
+        assert got["first"][0] == 0 and np.all(np.diff(got["first"].astype(np.int64)) > 0)
+        for g, e in zip(got, exp):
+            assert int(g["first"]) == e[0] and int(g["word"]) == e[1]
+            assert bits(np.float32(g["lo"])).tobytes() == bits(np.float32(e[2])).tobytes()
+            assert bits(np.float32(g["hi"])).tobytes() == bits(np.float32(e[3])).tobytes()
+    assert len(got) == 1024                    # 11k triangles allow the full cut
+
+
+@pytest.mark.gpu
+def test_treelet_labels(gpu):
+    """Labels from hit records: the treelet holding each ref (a binary search
+    on the host) and log t, -1 / 0 for misses, including refs at treelet
+    boundaries."""
+    tris_h, n, ctor, bvh = _terrain_bvh(48, 1_000)
+    cut = P.TreeletCut(bvh.packed(), ctor.ranges(), n)
+    table = cut.host()
+    firsts = table["first"].astype(np.int64)
+    rng = np.random.default_rng(21)
+    m = 5000
+    hits = np.zeros(m, P.HIT_DTYPE)
+    refs = rng.integers(0, n, size=m)
+    refs[:len(firsts)] = firsts                      # every boundary
+    refs[len(firsts):2 * len(firsts)] = np.maximum(firsts - 1, 0)
+    hits["ref"] = refs
+    hits["t"] = rng.uniform(1e-3, 10.0, size=m).astype(np.float32)
+    miss = rng.uniform(size=m) < 0.2
+    hits["t"][miss] = F32MAX
+    lab, tgt = cut.labels_host(DeviceBuffer.from_array(hits), m)
+    exp = np.searchsorted(firsts, refs, side="right") - 1
+    exp[miss] = -1
+    assert np.array_equal(lab, exp)
+    assert np.all(tgt[miss] == 0.0)
+    np.testing.assert_allclose(tgt[~miss], np.log(hits["t"][~miss].astype(np.float64)),
+                               rtol=1e-6, atol=1e-6)
+
+
 @pytest.mark.gpu
 def test_training_gain_and_exact_seeded_traversal(gpu):
     """Trained on the diffuse bounce rays of four seeds of a 93k-triangle
-    heightfield scene and evaluated on a fifth: the top-4 cluster proposals
+    heightfield scene and evaluated on a fifth: the top-4 treelet proposals
     hold the closest hit far more often than the untrained network's, and
     the traversal with the seeded max_t gives the plain traversal's records
     on every held-out ray."""
@@ -174,6 +343,7 @@ def test_training_gain_and_exact_seeded_traversal(gpu):
     bvh = ctor.build(DeviceBuffer.from_array(tris_h), n)
     ranges = ctor.ranges()
     packed = bvh.packed()
+    cut = P.TreeletCut(packed, ranges, n)
     cam = scenes.terrain_camera(640, 360)
     prim = rta_amd.ShirleyRayGenerator(cam.w, cam.h)
     prim.setup(cam.pos, cam.view_dir, cam.up, cam.fovy)
@@ -195,23 +365,24 @@ def test_training_gain_and_exact_seeded_traversal(gpu):
     sets = [bounce_set(s) for s in (101, 102, 103, 104, 105)]
     o_tr = np.concatenate([s[0] for s in sets[:4]])
     d_tr = np.concatenate([s[1] for s in sets[:4]])
-    lab_tr, tgt_tr = P.cluster_labels(np.concatenate([s[3] for s in sets[:4]]), n)
+    h_tr = np.concatenate([s[3] for s in sets[:4]])
+    lab_tr, tgt_tr = cut.labels_host(DeviceBuffer.from_array(h_tr), len(h_tr))
     n_tr = len(lab_tr)
     o_ev, d_ev, t_ev, h_ev = sets[4]
-    lab_ev, _ = P.cluster_labels(h_ev, n)
+    lab_ev, _ = cut.labels_host(DeviceBuffer.from_array(h_ev), len(h_ev))
     n_ev = len(lab_ev)
-    say(f"{n} triangles, {n_tr} training rays, {n_ev} held-out rays, "
-        f"hit share {np.mean(lab_ev >= 0):.2f}")
+    say(f"{n} triangles, {len(cut.host())} treelets, {n_tr} training rays, {n_ev} held-out "
+        f"rays, hit share {np.mean(lab_ev >= 0):.2f}")
 
     def accuracy(pred):
         top = DeviceBuffer(16 * n_ev)
-        pred.forward(DeviceBuffer.from_array(o_ev), DeviceBuffer.from_array(d_ev), n_ev, top=top)
+        pred.infer(DeviceBuffer.from_array(o_ev), DeviceBuffer.from_array(d_ev), n_ev, top=top)
         rta_amd.synchronize()
         return P.top4_accuracy(top.download(np.int32, 4 * n_ev).reshape(-1, 4), lab_ev)
 
-    untrained = P.HitPredictor(max(n_ev, 8192), seed=3, scene_min=lo, scene_max=hi)
+    untrained = P.HitPredictor(8192, seed=3, scene_min=lo, scene_max=hi)
     acc0 = accuracy(untrained)
-    pred = P.HitPredictor(max(n_ev, 8192), seed=3, scene_min=lo, scene_max=hi)
+    pred = P.HitPredictor(8192, seed=3, scene_min=lo, scene_max=hi)
     losses, secs = pred.train(DeviceBuffer.from_array(o_tr), DeviceBuffer.from_array(d_tr),
                               DeviceBuffer.from_array(lab_tr), DeviceBuffer.from_array(tgt_tr),
                               n_tr, epochs=20, batch=8192, lr=2e-3)
@@ -228,7 +399,7 @@ def test_training_gain_and_exact_seeded_traversal(gpu):
     top, t_seed, maxt2 = DeviceBuffer(16 * n_ev), DeviceBuffer(4 * n_ev), DeviceBuffer(4 * n_ev)
     err = DeviceBuffer(4)
     err.fill_u32(0)
-    P.seed_max_t(pred, packed, ranges, n, rays, n_ev, top, t_seed, maxt2, err)
+    P.seed_max_t(pred, packed, cut, rays, n_ev, top, t_seed, maxt2, err)
     rta_amd.synchronize()
     assert int(err.download(np.uint32, 1)[0]) == 0
     seeds = t_seed.download(np.float32, n_ev)
@@ -237,6 +408,10 @@ def test_training_gain_and_exact_seeded_traversal(gpu):
     # a seed is a real hit, never before the closest one
     assert np.all(seeds[hit] >= h_ev["t"][hit])
     assert np.all(seeds[~hit] == F32MAX)
+    # a proposal that holds the closest hit yields it as the seed
+    t4 = top.download(np.int32, 4 * n_ev).reshape(-1, 4)
+    holds = (t4 == lab_ev[:, None]).any(axis=1) & hit
+    assert np.all(seeds[holds] == h_ev["t"][holds])
     say(f"rays with a seed {np.mean(seeds != F32MAX):.3f}, "
         f"seed equal to the closest hit {np.mean(seeds[hit] == h_ev['t'][hit]):.3f}")
     b2 = rta_amd.GpuBouncer(n_ev)
