@@ -420,71 +420,85 @@ def predictor_leg(args, tris_h, tris, nodes, packed, ranges, prim, hits_p, R, tc
             ev[1].record(s)
             ev[1].synchronize()
             plain_ms += ev[0].elapsed_ms(ev[1]) / reps
-        part = {"inference": 0.0, "seeding": 0.0, "traversal": 0.0}
-        total_ms = 0.0
-        for _ in range(reps):
-            call("rta_reset_intersections", ptr(e["pred"]), R, 1, sp)
-            ev[0].record(s)
-            pred.infer(rays.gpu_origin, rays.gpu_direction, m, top=top, stream=s)
-            ev[1].record(s)
-            call("rta_trace_seed_treelets", ptr(packed.tri4), ptr(packed.pairs),
-                 ptr(packed.flags), ptr(cut.table), ptr(cut.count), ptr(rays.gpu_origin),
-                 ptr(rays.gpu_direction), ptr(rays.gpu_maxt), m, ptr(top), ptr(t_seed), ptr(err),
-                 sp)
-            call("rta_seed_max_t", ptr(maxt2), ptr(rays.gpu_maxt), ptr(t_seed), m, sp)
-            ev[2].record(s)
-            persistent(rays, e["cnt"], maxt2, e["pred"])
-            ev[3].record(s)
-            ev[3].synchronize()
-            part["inference"] += ev[0].elapsed_ms(ev[1]) / reps
-            part["seeding"] += ev[1].elapsed_ms(ev[2]) / reps
-            part["traversal"] += ev[2].elapsed_ms(ev[3]) / reps
-            total_ms += ev[0].elapsed_ms(ev[3]) / reps
-        if int(err.download(np.uint32, 1)[0]):
-            raise SystemExit("hit-predicted leg: traversal reported stack/guard errors")
         hp = e["plain"].download(HIT_DTYPE, m)
-        hq = e["pred"].download(HIT_DTYPE, m)
         hit = hp["t"] != fmax
-        same = (hq["t"] != fmax) == hit
-        for f in ("t", "beta", "gamma"):
-            same &= (hq[f].view(np.uint32) == hp[f].view(np.uint32)) | ~hit
-        same &= (hq["ref"] == hp["ref"]) | ~hit
-        seeds = t_seed.download(np.float32, m)
-        tops = top.download(np.int32, 4 * m).reshape(m, 4)
         lab, _ = cut.labels_host(e["plain"], m, stream=s)
-        rec = {
-            "bounce_seed": e["seed"], "rays": m,
-            "plain_mrays_s": round(m / (plain_ms * 1e3), 3),
-            "hit_predicted_mrays_s": round(m / (total_ms * 1e3), 3),
-            "plain_ms": round(plain_ms, 4), "hit_predicted_ms": round(total_ms, 4),
-            "hit_predicted_breakdown_ms": {k: round(v, 4) for k, v in part.items()},
-            "exact_share": float(same.mean()),
-            "hit_share": round(float(hit.mean()), 4),
-            "top4_accuracy": round(P.top4_accuracy(tops, lab), 4),
-            "seeded_share": round(float((seeds != fmax).mean()), 4),
-            "seed_is_closest_share": round(float((seeds[hit] == hp["t"][hit]).mean()), 4),
-            "inference_tflop_s": round(PRED_FLOPS_PER_RAY * m / (part["inference"] * 1e-3) / 1e12,
-                                       3),
-        }
-        # inner-node visits per ray with and without the seed
         stats = DeviceBuffer(8 * m)
         scratch = DeviceBuffer(16 * m)
-        vis = []
-        for mt in (rays.gpu_maxt, maxt2):
+
+        def visits(mt):
+            """mean inner-node visits and triangle tests per ray with max_t mt"""
             call("rta_reset_intersections", ptr(scratch), m, 1, sp)
             call("rta_trace_cis", ptr(tris), n, ptr(nodes), ptr(rays.gpu_origin),
                  ptr(rays.gpu_direction), ptr(mt), m, 1, ptr(scratch), ptr(stats), ptr(err), sp)
             s.synchronize()
             vt = stats.download(np.uint32, 2 * m).reshape(-1, 2).astype(np.float64)
-            vis.append((float(vt[:, 0].mean()), float(vt[:, 1].mean())))
-        rec["V_plain"], rec["T_plain"] = round(vis[0][0], 3), round(vis[0][1], 3)
-        rec["V_seeded"], rec["T_seeded"] = round(vis[1][0], 3), round(vis[1][1], 3)
-        log(f"predictor seed {e['seed']}: plain {rec['plain_mrays_s']:.1f} Mrays/s, "
-            f"hit-predicted {rec['hit_predicted_mrays_s']:.1f} Mrays/s "
-            f"({rec['hit_predicted_breakdown_ms']}), top-4 {rec['top4_accuracy']:.3f}, "
-            f"exact {rec['exact_share']:.6f}, V {rec['V_plain']:.1f} -> {rec['V_seeded']:.1f}")
-        if rec["exact_share"] != 1.0:
-            raise SystemExit("hit-predicted leg: seeded records differ from the plain ones")
+            return round(float(vt[:, 0].mean()), 3), round(float(vt[:, 1].mean()), 3)
+
+        rec = {"bounce_seed": e["seed"], "rays": m,
+               "plain_mrays_s": round(m / (plain_ms * 1e3), 3), "plain_ms": round(plain_ms, 4),
+               "hit_share": round(float(hit.mean()), 4), "by_candidates": []}
+        rec["V_plain"], rec["T_plain"] = visits(rays.gpu_maxt)
+        # the seeding traverses the first 1, 2 or all 4 proposed treelets
+        for cand in (1, 2, 4):
+            part = {"inference": 0.0, "seeding": 0.0, "traversal": 0.0}
+            total_ms = 0.0
+            for _ in range(reps):
+                call("rta_reset_intersections", ptr(e["pred"]), R, 1, sp)
+                ev[0].record(s)
+                pred.infer(rays.gpu_origin, rays.gpu_direction, m, top=top, stream=s)
+                ev[1].record(s)
+                call("rta_trace_seed_treelets", ptr(packed.tri4), ptr(packed.pairs),
+                     ptr(packed.flags), ptr(cut.table), ptr(cut.count), ptr(rays.gpu_origin),
+                     ptr(rays.gpu_direction), ptr(rays.gpu_maxt), m, ptr(top), cand,
+                     ptr(t_seed), ptr(err), sp)
+                call("rta_seed_max_t", ptr(maxt2), ptr(rays.gpu_maxt), ptr(t_seed), m, sp)
+                ev[2].record(s)
+                persistent(rays, e["cnt"], maxt2, e["pred"])
+                ev[3].record(s)
+                ev[3].synchronize()
+                part["inference"] += ev[0].elapsed_ms(ev[1]) / reps
+                part["seeding"] += ev[1].elapsed_ms(ev[2]) / reps
+                part["traversal"] += ev[2].elapsed_ms(ev[3]) / reps
+                total_ms += ev[0].elapsed_ms(ev[3]) / reps
+            if int(err.download(np.uint32, 1)[0]):
+                raise SystemExit("hit-predicted leg: traversal reported stack/guard errors")
+            hq = e["pred"].download(HIT_DTYPE, m)
+            same = (hq["t"] != fmax) == hit
+            for f in ("t", "beta", "gamma"):
+                same &= (hq[f].view(np.uint32) == hp[f].view(np.uint32)) | ~hit
+            same &= (hq["ref"] == hp["ref"]) | ~hit
+            seeds = t_seed.download(np.float32, m)
+            tops = top.download(np.int32, 4 * m).reshape(m, 4)
+            v_s, t_s = visits(maxt2)
+            c = {"candidates": cand,
+                 "hit_predicted_mrays_s": round(m / (total_ms * 1e3), 3),
+                 "hit_predicted_ms": round(total_ms, 4),
+                 "breakdown_ms": {k: round(v, 4) for k, v in part.items()},
+                 "exact_share": float(same.mean()),
+                 "accuracy": round(P.top4_accuracy(tops[:, :cand], lab), 4),
+                 "seeded_share": round(float((seeds != fmax).mean()), 4),
+                 "seed_is_closest_share": round(float((seeds[hit] == hp["t"][hit]).mean()), 4),
+                 "V_seeded": v_s, "T_seeded": t_s}
+            rec["by_candidates"].append(c)
+            log(f"predictor seed {e['seed']}, {cand} candidate(s): plain "
+                f"{rec['plain_mrays_s']:.1f} Mrays/s, hit-predicted "
+                f"{c['hit_predicted_mrays_s']:.1f} Mrays/s ({c['breakdown_ms']}), accuracy "
+                f"{c['accuracy']:.3f}, exact {c['exact_share']:.6f}, V {rec['V_plain']:.1f} -> "
+                f"{v_s:.1f}")
+            if c["exact_share"] != 1.0:
+                raise SystemExit("hit-predicted leg: seeded records differ from the plain ones")
+            if cand == 4:
+                rec["top4_accuracy"] = c["accuracy"]
+                rec["inference_ms"] = round(part["inference"], 4)
+                rec["inference_tflop_s"] = round(
+                    PRED_FLOPS_PER_RAY * m / (part["inference"] * 1e-3) / 1e12, 3)
+        best = max(rec["by_candidates"], key=lambda c: c["hit_predicted_mrays_s"])
+        rec["hit_predicted_mrays_s"] = best["hit_predicted_mrays_s"]
+        rec["hit_predicted_best_candidates"] = best["candidates"]
+        rec["hit_predicted_vs_plain"] = round(best["hit_predicted_mrays_s"] /
+                                              rec["plain_mrays_s"], 4)
+        rec["exact_share"] = min(c["exact_share"] for c in rec["by_candidates"])
         out.append(rec)
     pred.close()
     return {

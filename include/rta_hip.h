@@ -494,16 +494,16 @@ int rta_treelet_labels(const rta_hit *hits, int n, const rta_treelet *table,
                        const uint32_t *count, int32_t *label, float *target, rta_stream s);
 
 /* Hit-distance seeds from predicted treelets: for each ray, the closest hit
- * among the triangles of its 4 treelets top[4*i..] (ids >= *count are
- * skipped), each traversed from its root after a test of its box;
- * t_seed[i] = that t, or FLT_MAX.  Every such t is the t of a real hit of
- * the ray (the same triangle test, t <= max_t), so the closest hit lies at
- * or before it. */
+ * among the triangles of its first `candidates` (1..4) treelets of
+ * top[4*i..] (ids >= *count are skipped), traversed from their roots in
+ * order of root-box entry distance; t_seed[i] = that t, or FLT_MAX.  Every
+ * such t is the t of a real hit of the ray (the same triangle test,
+ * t <= max_t), so the closest hit lies at or before it. */
 int rta_trace_seed_treelets(const float *tri4, const rta_node_pair *pairs, const uint32_t *flags,
                             const rta_treelet *table, const uint32_t *count,
                             const float *ray_orig, const float *ray_dir, const float *max_t,
-                            int n_rays, const int32_t *top, float *t_seed, uint32_t *err,
-                            rta_stream s);
+                            int n_rays, const int32_t *top, int candidates, float *t_seed,
+                            uint32_t *err, rta_stream s);
 /* out[i] = min(max_t[i], t_seed[i]): the max_t a seeded traversal runs with. */
 int rta_seed_max_t(float *out, const float *max_t, const float *t_seed, int n, rta_stream s);
 

@@ -156,7 +156,8 @@ HIP_SIGNATURES = {
     "rta_lbvh_copy_ranges": [_vp, _vp, _i, _vp],
     "rta_treelet_cut": [_vp, _u32, _u32, _vp, _i, _i, _vp, _vp, _vp],
     "rta_treelet_labels": [_vp, _i, _vp, _vp, _vp, _vp, _vp],
-    "rta_trace_seed_treelets": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp],
+    "rta_trace_seed_treelets": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp, _vp,
+                                _vp],
     "rta_seed_max_t": [_vp, _vp, _vp, _i, _vp],
 }
 

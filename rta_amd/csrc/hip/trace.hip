@@ -505,7 +505,8 @@ __global__ __launch_bounds__(kTraceBlock) void seed_treelets_kernel(
     const uint32_t *__restrict__ flags, const rta_treelet *__restrict__ table,
     const uint32_t *__restrict__ count, const float *__restrict__ ro,
     const float *__restrict__ rd, const float *__restrict__ rmax, int n_rays,
-    const int32_t *__restrict__ top, float *__restrict__ t_seed, uint32_t *__restrict__ err) {
+    const int32_t *__restrict__ top, int candidates, float *__restrict__ t_seed,
+    uint32_t *__restrict__ err) {
 	__shared__ uint32_t stack[kStack][kTraceBlock];
 	const int lt = threadIdx.x;
 	const int tid = blockIdx.x * kTraceBlock + lt;
@@ -525,7 +526,7 @@ __global__ __launch_bounds__(kTraceBlock) void seed_treelets_kernel(
 		const uint32_t c = (uint32_t)top[(size_t)tid * kSeedTop + k];
 		dk[k] = __builtin_inff();
 		wk[k] = 0u;
-		if (c >= K) continue;
+		if (k >= candidates || c >= K) continue;
 		const rta_treelet tr = table[c];
 		float dist;
 		if (box_test(tr.lo[0], tr.lo[1], tr.lo[2], tr.hi[0], tr.hi[1], tr.hi[2], ox, oy, oz, dx, dy,
@@ -1212,15 +1213,15 @@ int rta_treelet_labels(const rta_hit *hits, int n, const rta_treelet *table,
 int rta_trace_seed_treelets(const float *tri4, const rta_node_pair *pairs, const uint32_t *flags,
                             const rta_treelet *table, const uint32_t *count,
                             const float *ray_orig, const float *ray_dir, const float *max_t,
-                            int n_rays, const int32_t *top, float *t_seed, uint32_t *err,
-                            rta_stream s) {
+                            int n_rays, const int32_t *top, int candidates, float *t_seed,
+                            uint32_t *err, rta_stream s) {
 	if (!tri4 || !pairs || !flags || !table || !count || !ray_orig || !ray_dir || !max_t ||
-	    !top || !t_seed || !err || n_rays <= 0)
-		return invalid("rta_trace_seed_treelets: bad arguments");
+	    !top || !t_seed || !err || n_rays <= 0 || candidates < 1 || candidates > kSeedTop)
+		return invalid("rta_trace_seed_treelets: bad arguments (candidates in 1..4)");
 	hipLaunchKernelGGL(seed_treelets_kernel, dim3(grid_for(n_rays, kTraceBlock)),
 	                   dim3(kTraceBlock), 0, as_stream(s), (const float4 *)tri4,
 	                   (const float4 *)pairs, flags, table, count, ray_orig, ray_dir, max_t, n_rays,
-	                   top, t_seed, err);
+	                   top, candidates, t_seed, err);
 	RTA_LAUNCH_CHECK("seed_treelets_kernel");
 	return RTA_OK;
 }

@@ -173,15 +173,15 @@ class HitPredictor:
 
 def seed_max_t(pred: HitPredictor, packed, cut: TreeletCut, rays, n: int, top: DeviceBuffer,
                t_seed: DeviceBuffer, maxt_out: DeviceBuffer, err: DeviceBuffer,
-               stream=None) -> None:
-    """Inference (top-4 treelets, fused kernel), the traversal of those
-    treelets, and max_t' = min(max_t, seed) for the n rays.  `packed` is a
-    PackedBvh (pairs, tri4, flags)."""
+               stream=None, candidates: int = TOP) -> None:
+    """Inference (top-4 treelets, fused kernel), the traversal of the first
+    `candidates` of those treelets, and max_t' = min(max_t, seed) for the n
+    rays.  `packed` is a PackedBvh (pairs, tri4, flags)."""
     sp = ptr(stream) if stream else None
     pred.infer(rays.gpu_origin, rays.gpu_direction, n, top=top, stream=stream)
     call("rta_trace_seed_treelets", ptr(packed.tri4), ptr(packed.pairs), ptr(packed.flags),
          ptr(cut.table), ptr(cut.count), ptr(rays.gpu_origin), ptr(rays.gpu_direction),
-         ptr(rays.gpu_maxt), n, ptr(top), ptr(t_seed), ptr(err), sp)
+         ptr(rays.gpu_maxt), n, ptr(top), candidates, ptr(t_seed), ptr(err), sp)
     call("rta_seed_max_t", ptr(maxt_out), ptr(rays.gpu_maxt), ptr(t_seed), n, sp)
 
 
