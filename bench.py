@@ -242,6 +242,8 @@ def dynamic_main(args) -> None:
     fr.check_errors()
     counts_last = fr.ray_counts()
     rays = sum(counts_last.values())
+    del graph
+    finetune = None if args.no_predictor else dynamic_finetune(args, fr, v, n_eager)
     out = {
         "metric": METRIC,
         "value": round(rays / (frame_ms * 1e3), 3),
@@ -268,10 +270,68 @@ def dynamic_main(args) -> None:
         "build_and_pack_mtri_s": round(n / (stage_ms["build+pack"] * 1e3), 3),
         "eager_stage_ms": {k: round(x, 4) for k, x in stage_ms.items()},
         "eager_counts_first_frames": counts,
+        "online_finetune": finetune,
         "device": rta_amd.device_name(),
     }
     print(json.dumps(out), flush=True)
+
+
+def dynamic_finetune(args, fr, v, first_frame):
+    """The dynamic frame with a hit predictor fine-tuned online on each
+    frame's first-bounce hits (4 Adam steps on 8,192 sampled rays per frame,
+    from an untrained network): loss and top-4 accuracy of the predictions
+    made before each frame's steps over eager frames, the stage's time, and
+    the frame time of the captured frame with the stage inside."""
+    from rta_amd.device import Event
+    from rta_amd.frame import OnlineFineTune
+    from rta_amd.predictor import HitPredictor
+
+    pts = v.reshape(-1, 3)
+    pred = HitPredictor(8192, seed=1, scene_min=pts.min(axis=0) - 1.0,
+                        scene_max=pts.max(axis=0) + 1.0)
+    ft = OnlineFineTune(pred, fr.n, batch=8192, steps=4, lr=2e-3, level=1)
+    fr.attach_finetune(ft)
+    names = ["jitter", "build+pack", "primary", "shadow0", "bounce1", "finetune", "shadow1",
+             "bounce2", "shadow2"]
+    frames = max(40, args.warmup)
+    losses, accs, ft_ms = [], [], []
+    for f in range(frames):
+        fr.enqueue(frame=first_frame + f, timed=True)
+        fr.stream.synchronize()
+        ev = fr.events
+        ft_ms.append(ev[names.index("finetune")].elapsed_ms(ev[names.index("finetune") + 1]))
+        losses.append(ft.mean_loss())
+        accs.append(ft.accuracy())
+    fr.check_errors()
+    graph = fr.capture()
+    fr.set_frame_counter(first_frame + frames)
+    fr.stream.synchronize()
+    graph.launch(fr.stream)
+    fr.stream.synchronize()
+    a, b = Event(), Event()
+    a.record(fr.stream)
+    for _ in range(args.steps):
+        graph.launch(fr.stream)
+    b.record(fr.stream)
+    b.synchronize()
+    fr.check_errors()
+    frame_ms = a.elapsed_ms(b) / args.steps
+    fr.attach_finetune(None)
     del graph
+    log(f"online fine-tune: {np.median(ft_ms):.3f} ms per frame, loss {losses[0]:.3f} -> "
+        f"{losses[-1]:.3f}, top-4 accuracy {accs[0]:.3f} -> {accs[-1]:.3f}, frame with it "
+        f"{frame_ms:.3f} ms")
+    marks = sorted({0, 1, 2, 5, 10, 20, frames - 1})
+    return {
+        "batch": 8192, "steps_per_frame": 4, "lr": 2e-3, "level": "bounce1", "eager_frames": frames,
+        "stage_ms_median": round(float(np.median(ft_ms)), 4),
+        "frame_ms_with_finetune": round(frame_ms, 4),
+        "loss_by_frame": {str(k): round(losses[k], 4) for k in marks},
+        "top4_accuracy_by_frame": {str(k): round(accs[k], 4) for k in marks},
+        "note": ("accuracy of the predictions made before each frame's own steps, on that "
+                 "frame's first sample; the treelet cut is recomputed from every frame's "
+                 "rebuilt tree"),
+    }
 
 
 def cpu_baseline(tris_h, bo, bd, bt, n_bounce, w, max_rays):

@@ -447,6 +447,16 @@ int rta_pred_backward(rta_pred pred, const int32_t *label, const float *target, 
 /* One Adam step (beta 0.9 / 0.999, eps 1e-8) with the accumulated
  * gradients; step counts from 1. */
 int rta_pred_adam(rta_pred pred, float lr, int step, rta_stream s);
+/* The same with the step count in device memory: *step is incremented and
+ * the step then taken with it (for captured, replayed launch sequences). */
+int rta_pred_adam_device(rta_pred pred, float lr, uint32_t *step, rta_stream s);
+/* n_out rays with their hit records drawn from the first *count rays of a
+ * set (device arrays): entry i takes ray hash(seed, frame, i) mod *count,
+ * frame = *frame_word (or 0 when frame_word is NULL); an empty set gives
+ * miss records.  The training sample of an online fine-tune. */
+int rta_pred_sample(const float *ray_orig, const float *ray_dir, const rta_hit *hits,
+                    const uint32_t *count, int n_out, const uint32_t *frame_word, uint64_t seed,
+                    float *orig_out, float *dir_out, rta_hit *hits_out, rta_stream s);
 
 /* Fused inference for n rays (no batch limit): encoding, the four layers
  * and the top-4 selection in one kernel, the logits kept in registers.

@@ -289,13 +289,20 @@ __global__ __launch_bounds__(256) void topk_kernel(int32_t *__restrict__ top,
 }
 
 // Adam on the packed parameters; refreshes the bf16 weight copies.
+// step_word (optional): take the step from device memory (bias corrections
+// computed here) instead of c1, c2, so a captured launch advances with it.
 __global__ __launch_bounds__(256) void adam_kernel(float *__restrict__ p, float *__restrict__ mo,
                                                    float *__restrict__ ve,
                                                    const float *__restrict__ g,
                                                    uint16_t *__restrict__ wb, float lr, float c1,
-                                                   float c2) {
+                                                   float c2, const uint32_t *__restrict__ step_word) {
 	const int i = blockIdx.x * 256 + threadIdx.x;
 	if (i >= kParams) return;
+	if (step_word) {
+		const float st = (float)*step_word;
+		c1 = 1.0f / (1.0f - powf(0.9f, st));
+		c2 = 1.0f / (1.0f - powf(0.999f, st));
+	}
 	const float gi = g[i];
 	const float m1 = 0.9f * mo[i] + 0.1f * gi;
 	const float v1 = 0.999f * ve[i] + 0.001f * gi * gi;
@@ -304,6 +311,36 @@ __global__ __launch_bounds__(256) void adam_kernel(float *__restrict__ p, float 
 	const float pi = p[i] - lr * (m1 * c1) / (sqrtf(v1 * c2) + 1e-8f);
 	p[i] = pi;
 	wb[i] = to_bf16(pi);
+}
+
+// n_out rays (and their hit records) drawn from the first *count of a ray
+// set: entry i takes ray hash(seed, frame, i) mod *count, frame = *frame_word
+// (0 without one), so a captured launch draws a new sample every frame.  An
+// empty set gives misses.
+__global__ __launch_bounds__(256) void sample_kernel(
+    const float *__restrict__ o, const float *__restrict__ d, const rta_hit *__restrict__ hits,
+    const uint32_t *__restrict__ count, int n_out, const uint32_t *__restrict__ frame_word,
+    uint64_t seed, float *__restrict__ o_out, float *__restrict__ d_out,
+    rta_hit *__restrict__ hits_out) {
+	const int i = blockIdx.x * 256 + threadIdx.x;
+	if (i >= n_out) return;
+	const uint32_t c = *count;
+	if (c == 0u) {
+		for (int k = 0; k < 3; ++k) o_out[3 * (size_t)i + k] = d_out[3 * (size_t)i + k] = 0.0f;
+		hits_out[i] = rta_hit{FLT_MAX, 1.0f, 2.0f, 0u};
+		return;
+	}
+	uint64_t z = seed * 0x9e3779b97f4a7c15ull + ((uint64_t)(frame_word ? *frame_word : 0u) << 32) +
+	             (uint64_t)i;
+	z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+	z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+	z ^= z >> 31;
+	const size_t j = (size_t)(z % c);
+	for (int k = 0; k < 3; ++k) {
+		o_out[3 * (size_t)i + k] = o[3 * j + k];
+		d_out[3 * (size_t)i + k] = d[3 * j + k];
+	}
+	hits_out[i] = hits[j];
 }
 
 __global__ __launch_bounds__(256) void to_bf16_kernel(uint16_t *__restrict__ wb,
@@ -812,8 +849,30 @@ int rta_pred_adam(rta_pred h, float lr, int step, rta_stream s) {
 	const float c1 = 1.0f / (1.0f - powf(0.9f, (float)step));
 	const float c2 = 1.0f / (1.0f - powf(0.999f, (float)step));
 	hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(kParams, 256)), dim3(256), 0, as_stream(s),
-	                   p->params, p->mom, p->vel, p->grads, p->wb, lr, c1, c2);
+	                   p->params, p->mom, p->vel, p->grads, p->wb, lr, c1, c2, nullptr);
 	RTA_LAUNCH_CHECK("adam_kernel");
+	return RTA_OK;
+}
+
+int rta_pred_adam_device(rta_pred h, float lr, uint32_t *step, rta_stream s) {
+	State *p = reinterpret_cast<State *>(h);
+	if (!p || !step) return invalid("rta_pred_adam_device: NULL argument");
+	const int rc = rta_hip_add_u32(step, 1u, s);
+	if (rc != RTA_OK) return rc;
+	hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(kParams, 256)), dim3(256), 0, as_stream(s),
+	                   p->params, p->mom, p->vel, p->grads, p->wb, lr, 0.0f, 0.0f, step);
+	RTA_LAUNCH_CHECK("adam_kernel");
+	return RTA_OK;
+}
+
+int rta_pred_sample(const float *o, const float *d, const rta_hit *hits, const uint32_t *count,
+                    int n_out, const uint32_t *frame_word, uint64_t seed, float *o_out,
+                    float *d_out, rta_hit *hits_out, rta_stream s) {
+	if (!o || !d || !hits || !count || !o_out || !d_out || !hits_out || n_out <= 0)
+		return invalid("rta_pred_sample: bad arguments");
+	hipLaunchKernelGGL(sample_kernel, dim3(blocks_for(n_out, 256)), dim3(256), 0, as_stream(s), o,
+	                   d, hits, count, n_out, frame_word, seed, o_out, d_out, hits_out);
+	RTA_LAUNCH_CHECK("sample_kernel");
 	return RTA_OK;
 }
 

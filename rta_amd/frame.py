@@ -7,6 +7,8 @@ Stage order of one frame (every ray count stays on the device):
 
     jitter -> build -> pack -> primary rays -> trace (closest hit)
     for level 0..bounces:
+        [online fine-tune of a hit predictor on this level's hits, when
+         one is attached (OnlineFineTune)]
         shadow rays of level's hits -> trace (shadow)
         if level < bounces: diffuse bounce -> trace (closest hit)
     frame counter += 1
@@ -42,6 +44,78 @@ class Graph:
             except Exception:
                 pass
             self._h = None
+
+
+class OnlineFineTune:
+    """Per-frame fine-tune of a hit predictor (rta_amd.predictor.HitPredictor)
+    on the frame's own plain-traversal hits, enqueued inside the frame after
+    the trace of ray level `level` (1 = the first bounce):
+
+        Karras ranges -> treelet cut of this frame's tree (the classes)
+        repeat `steps` times:
+            sample `batch` rays of the level with their hits (a new hashed
+            sample per frame and step) -> labels -> inference (the
+            predictions before the step, kept for accuracy) -> forward ->
+            backward -> Adam (device step counter)
+
+    Only kernels and device copies: the stage is captured with the frame."""
+
+    def __init__(self, pred, n_tris: int, batch: int = 8192, steps: int = 4, lr: float = 1e-3,
+                 level: int = 1, seed: int = 7):
+        if batch > pred.max_batch:
+            raise ValueError("batch exceeds the predictor's max_batch")
+        self.pred, self.batch, self.steps, self.lr, self.level, self.seed = (
+            pred, batch, steps, lr, level, seed)
+        self.ranges = DeviceBuffer(8 * max(n_tris - 1, 1))
+        self.table = DeviceBuffer(32 * 1024)
+        self.count = DeviceBuffer(4)
+        self.o, self.d = DeviceBuffer(12 * batch), DeviceBuffer(12 * batch)
+        self.hits = DeviceBuffer(16 * batch)
+        self.label = [DeviceBuffer(4 * batch) for _ in range(steps)]
+        self.target = [DeviceBuffer(4 * batch) for _ in range(steps)]
+        self.top = DeviceBuffer(16 * batch)
+        self.loss = DeviceBuffer(4)
+        self.step = DeviceBuffer(4)
+        self.step.fill_u32(0)
+        self.frame_buf = DeviceBuffer(4)
+
+    def enqueue(self, frame, stream: Stream, frame_word: DeviceBuffer | None) -> None:
+        """frame: the enclosing DynamicFrame; frame_word: its device frame
+        counter when captured, None for an eager frame (then frame_buf holds
+        the frame number)."""
+        s = ptr(stream)
+        n = frame.n
+        call("rta_lbvh_copy_ranges", ptr(self.ranges), ptr(frame.ws), n, s)
+        call("rta_treelet_cut", ptr(frame.pairs), frame._root[0], frame._root[1],
+             ptr(self.ranges), n, 1024, ptr(self.table), ptr(self.count), s)
+        rays, hits, cnt = frame.rays[self.level], frame.hits[self.level], frame.counts[self.level]
+        fw = frame_word if frame_word is not None else self.frame_buf
+        call("rta_hip_fill_u32", ptr(self.loss), 0, 1, s)
+        for k in range(self.steps):
+            call("rta_pred_sample", ptr(rays.gpu_origin), ptr(rays.gpu_direction), ptr(hits),
+                 ptr(cnt), self.batch, ptr(fw), C.c_uint64(self.seed * 1000003 + k),
+                 ptr(self.o), ptr(self.d), ptr(self.hits), s)
+            call("rta_treelet_labels", ptr(self.hits), self.batch, ptr(self.table),
+                 ptr(self.count), ptr(self.label[k]), ptr(self.target[k]), s)
+            if k == 0:
+                self.pred.infer(self.o, self.d, self.batch, top=self.top, stream=stream)
+            call("rta_pred_zero_grads", self.pred.h, s)
+            self.pred.forward(self.o, self.d, self.batch, stream=stream)
+            call("rta_pred_backward", self.pred.h, ptr(self.label[k]), ptr(self.target[k]),
+                 self.batch, ptr(self.loss), s)
+            call("rta_pred_adam_device", self.pred.h, C.c_float(self.lr), ptr(self.step), s)
+
+    def accuracy(self) -> float:
+        """Top-4 accuracy, on the first sample of the last frame, of the
+        predictions made before that frame's steps."""
+        from .predictor import top4_accuracy
+
+        top = self.top.download(np.int32, 4 * self.batch).reshape(-1, 4)
+        return top4_accuracy(top, self.label[0].download(np.int32, self.batch))
+
+    def mean_loss(self) -> float:
+        """Mean of the last frame's per-step losses."""
+        return float(self.loss.download(np.float32, 1)[0]) / self.steps
 
 
 class DynamicFrame:
@@ -89,6 +163,7 @@ class DynamicFrame:
         self.counts[0].fill_u32(R)
         self.events: list[Event] = []
         self._root = None
+        self.finetune: OnlineFineTune | None = None
         # the fills above ran on the default stream; the frame's own stream
         # does not wait for it
         call("rta_hip_device_synchronize")
@@ -136,6 +211,12 @@ class DynamicFrame:
         self._trace(self.rays[0], self.counts[0], self.hits[0], shadow=False)
         self._mark(timed)
         for k in range(self.bounces + 1):
+            ft = self.finetune
+            if ft is not None and k == ft.level:
+                if frame is not None:
+                    call("rta_hip_fill_u32", ptr(ft.frame_buf), frame, 1, s)
+                ft.enqueue(self, self.stream, self.frame_word if frame is None else None)
+                self._mark(timed)
             if self.shadows:
                 lp = f3(self.light)
                 call("rta_shadow_rays", ptr(self.srays[k].gpu_origin),
@@ -157,6 +238,14 @@ class DynamicFrame:
                 self._mark(timed)
         if frame is None:
             call("rta_hip_add_u32", ptr(self.frame_word), 1, s)
+
+    def attach_finetune(self, ft: OnlineFineTune | None) -> None:
+        """Enqueue (or, with None, stop enqueueing) an online fine-tune stage
+        after the trace of ray level ft.level; graphs captured before keep
+        what they captured."""
+        if ft is not None and not 0 <= ft.level <= self.bounces:
+            raise ValueError("fine-tune level outside the frame's ray levels")
+        self.finetune = ft
 
     def prime(self) -> None:
         """One eager jitter + build to learn the root's link words (an

@@ -154,3 +154,47 @@ def test_dynamic_frame_graph_replays(gpu):
         # unrelated eager work between replays
         rta_amd.synchronize()
     del graph
+
+
+def test_dynamic_frame_online_finetune(gpu):
+    """A hit predictor fine-tuned inside the frame on each frame's first-
+    bounce hits: the frame's records stay the oracle's, the loss falls over
+    the frames, and the captured frame (fine-tune included) replays with its
+    own step and frame counters advancing."""
+    from rta_amd.frame import OnlineFineTune
+    from rta_amd.predictor import HitPredictor
+
+    say("online fine-tune")
+    fr, v, i, cam, light = _small_frame()
+    pts = v.reshape(-1, 3)
+    pred = HitPredictor(2048, seed=5, scene_min=pts.min(axis=0) - 1.0,
+                        scene_max=pts.max(axis=0) + 1.0)
+    ft = OnlineFineTune(pred, fr.n, batch=2048, steps=8, lr=2e-3, level=1)
+    fr.attach_finetune(ft)
+    losses, accs = [], []
+    for f in range(24):
+        fr.enqueue(frame=f)
+        fr.stream.synchronize()
+        losses.append(ft.mean_loss())
+        accs.append(ft.accuracy())
+        if f == 3:
+            fr.check_errors()
+            _compare(fr.results(), _oracle_frame(v, i, cam, light, 3))
+    say(f"loss {losses[0]:.3f} -> {losses[-1]:.3f}, top-4 accuracy before the frame's steps "
+        f"{accs[0]:.3f} -> {np.mean(accs[-4:]):.3f}")
+    assert int(ft.step.download(np.uint32, 1)[0]) == 24 * 8
+    assert np.mean(losses[-4:]) < 0.8 * losses[0]
+    assert np.mean(accs[-4:]) > 5 * max(accs[0], 4 / 1024)
+    graph = fr.capture()
+    fr.set_frame_counter(24)
+    fr.stream.synchronize()
+    before = pred.params()
+    for r in range(3):
+        graph.launch(fr.stream)
+        fr.stream.synchronize()
+        fr.check_errors()
+        assert int(fr.frame_word.download(np.uint32, 1)[0]) == 25 + r
+        assert int(ft.step.download(np.uint32, 1)[0]) == 24 * 8 + 8 * (r + 1)
+    assert not np.array_equal(pred.params(), before)
+    assert np.isfinite(ft.mean_loss())
+    del graph
