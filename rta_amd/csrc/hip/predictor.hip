@@ -207,17 +207,25 @@ __global__ __launch_bounds__(256) void bwd_weight_kernel(float *__restrict__ dw,
 		atomicAdd(&dw[(size_t)(n0 + rq + i) * K + kk], acc[i]);
 }
 
-// db += column sums of dY over kRowChunk-row chunks
+// db += column sums of dY: a block takes 64 columns of a kRowChunk-row
+// chunk, its four waves every fourth row each (one coalesced 128-B row
+// segment per wave and step), reduced in LDS; one atomic per column and
+// block.
 __global__ __launch_bounds__(256) void bwd_bias_kernel(float *__restrict__ db,
                                                        const uint16_t *__restrict__ dy, int lddy,
                                                        int N, int m) {
-	const int col = blockIdx.x * 256 + threadIdx.x;
-	if (col >= N) return;
+	__shared__ float part[4][kWave];
+	const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+	const int col = blockIdx.x * kWave + lane;
 	const int r0 = blockIdx.y * kRowChunk;
 	const int r1 = min(m, r0 + kRowChunk);
 	float s = 0.f;
-	for (int r = r0; r < r1; ++r) s += from_bf16(dy[(size_t)r * lddy + col]);
-	atomicAdd(&db[col], s);
+	if (col < N)
+		for (int r = r0 + w; r < r1; r += 4) s += from_bf16(dy[(size_t)r * lddy + col]);
+	part[w][lane] = s;
+	__syncthreads();
+	if (w == 0 && col < N)
+		atomicAdd(&db[col], (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]));
 }
 
 // --------------------------------------------------------- loss, top-k
@@ -242,18 +250,31 @@ __global__ __launch_bounds__(256) void loss_kernel(uint16_t *__restrict__ dout,
 		for (int c = lane; c < kOutP; c += kWave) g[c] = 0;
 		return;
 	}
+	// the lane's 16 logits stay in registers: max, then exp, then gradients
+	float v[kClusters / kWave];
 	float mx = -FLT_MAX;
-	for (int c = lane; c < kClusters; c += kWave) mx = fmaxf(mx, o[c]);
+#pragma unroll
+	for (int j = 0; j < kClusters / kWave; ++j) {
+		v[j] = o[lane + kWave * j];
+		mx = fmaxf(mx, v[j]);
+	}
 	for (int off = kWave / 2; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
 	float se = 0.f;
-	for (int c = lane; c < kClusters; c += kWave) se += expf(o[c] - mx);
+#pragma unroll
+	for (int j = 0; j < kClusters / kWave; ++j) {
+		v[j] = expf(v[j] - mx);
+		se += v[j];
+	}
 	for (int off = kWave / 2; off > 0; off >>= 1) se += __shfl_xor(se, off);
 	const float inv_m = 1.0f / (float)m;
-	for (int c = lane; c < kOutP; c += kWave) {
-		float gv = 0.f;
-		if (c < kClusters) gv = (expf(o[c] - mx) / se - (c == lab ? 1.f : 0.f)) * inv_m;
-		else if (c == kClusters) gv = 2.f * (o[c] - target[row]) * inv_m;
-		g[c] = to_bf16(gv);
+#pragma unroll
+	for (int j = 0; j < kClusters / kWave; ++j) {
+		const int c = lane + kWave * j;
+		g[c] = to_bf16((v[j] / se - (c == lab ? 1.f : 0.f)) * inv_m);
+	}
+	if (lane < kOutP - kClusters) {
+		const int c = kClusters + lane;
+		g[c] = to_bf16(c == kClusters ? 2.f * (o[c] - target[row]) * inv_m : 0.f);
 	}
 	if (lane == 0) {
 		const float d = o[kClusters] - target[row];
@@ -816,28 +837,28 @@ int rta_pred_backward(rta_pred h, const int32_t *label, const float *target, int
 	// layer 4: dW4, db4, dh3
 	hipLaunchKernelGGL(bwd_weight_kernel, dim3(blocks_for((long long)(kOutP / 16) * (kHid / 16) * chunks, 4)),
 	                   dim3(256), 0, st, g + oW4, p->dout, kOutP, p->h3, kHid, kHid, kOutP, n);
-	hipLaunchKernelGGL(bwd_bias_kernel, dim3(blocks_for(kOutP, 256), chunks), dim3(256), 0, st,
+	hipLaunchKernelGGL(bwd_bias_kernel, dim3(blocks_for(kOutP, kWave), chunks), dim3(256), 0, st,
 	                   g + oB4, p->dout, kOutP, kOutP, n);
 	hipLaunchKernelGGL(bwd_data_kernel, dim3(blocks_for(rt * (kHid / 16), 4)), dim3(256), 0, st,
 	                   p->dh3, p->dout, kOutP, p->wb + oW4, kHid, kOutP, p->h3, n);
 	// layer 3
 	hipLaunchKernelGGL(bwd_weight_kernel, dim3(blocks_for((long long)(kHid / 16) * (kHid / 16) * chunks, 4)),
 	                   dim3(256), 0, st, g + oW3, p->dh3, kHid, p->h2, kHid, kHid, kHid, n);
-	hipLaunchKernelGGL(bwd_bias_kernel, dim3(blocks_for(kHid, 256), chunks), dim3(256), 0, st,
+	hipLaunchKernelGGL(bwd_bias_kernel, dim3(blocks_for(kHid, kWave), chunks), dim3(256), 0, st,
 	                   g + oB3, p->dh3, kHid, kHid, n);
 	hipLaunchKernelGGL(bwd_data_kernel, dim3(blocks_for(rt * (kHid / 16), 4)), dim3(256), 0, st,
 	                   p->dh2, p->dh3, kHid, p->wb + oW3, kHid, kHid, p->h2, n);
 	// layer 2
 	hipLaunchKernelGGL(bwd_weight_kernel, dim3(blocks_for((long long)(kHid / 16) * (kHid / 16) * chunks, 4)),
 	                   dim3(256), 0, st, g + oW2, p->dh2, kHid, p->h1, kHid, kHid, kHid, n);
-	hipLaunchKernelGGL(bwd_bias_kernel, dim3(blocks_for(kHid, 256), chunks), dim3(256), 0, st,
+	hipLaunchKernelGGL(bwd_bias_kernel, dim3(blocks_for(kHid, kWave), chunks), dim3(256), 0, st,
 	                   g + oB2, p->dh2, kHid, kHid, n);
 	hipLaunchKernelGGL(bwd_data_kernel, dim3(blocks_for(rt * (kHid / 16), 4)), dim3(256), 0, st,
 	                   p->dh1, p->dh2, kHid, p->wb + oW2, kHid, kHid, p->h1, n);
 	// layer 1 (no input gradient)
 	hipLaunchKernelGGL(bwd_weight_kernel, dim3(blocks_for((long long)(kHid / 16) * (kInP / 16) * chunks, 4)),
 	                   dim3(256), 0, st, g + oW1, p->dh1, kHid, p->x, kInP, kInP, kHid, n);
-	hipLaunchKernelGGL(bwd_bias_kernel, dim3(blocks_for(kHid, 256), chunks), dim3(256), 0, st,
+	hipLaunchKernelGGL(bwd_bias_kernel, dim3(blocks_for(kHid, kWave), chunks), dim3(256), 0, st,
 	                   g + oB1, p->dh1, kHid, kHid, n);
 	RTA_LAUNCH_CHECK("backward kernels");
 	return RTA_OK;

@@ -195,6 +195,84 @@ def test_infer_matches_reference(gpu):
     assert np.array_equal(ld2.download(np.float32, n), got[:, 1024])
 
 
+def _sample_index(seed, frame, i, count):
+    """rta_pred_sample's draw on the host (splitmix64 finaliser of
+    seed * golden + (frame << 32) + i, modulo count)."""
+    m = (1 << 64) - 1
+    z = (seed * 0x9E3779B97F4A7C15 + (frame << 32) + i) & m
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
+    z ^= z >> 31
+    return z % count
+
+
+@pytest.mark.gpu
+def test_sample_draws_the_hashed_rays(gpu):
+    """rta_pred_sample copies ray i' = hash(seed, frame, i) mod count (origin,
+    direction and record) for every output, reads the frame from device
+    memory, and gives miss records for an empty set."""
+    rng = np.random.default_rng(31)
+    n, m, seed = 5000, 777, 12345
+    o = rng.normal(size=3 * n).astype(np.float32)
+    d = rng.normal(size=3 * n).astype(np.float32)
+    hits = np.zeros(n, P.HIT_DTYPE)
+    hits["t"] = rng.uniform(size=n).astype(np.float32)
+    hits["ref"] = np.arange(n)
+    ob, db, hb = DeviceBuffer.from_array(o), DeviceBuffer.from_array(d), DeviceBuffer.from_array(hits)
+    cnt, fw = DeviceBuffer(4), DeviceBuffer(4)
+    oo, do, ho = DeviceBuffer(12 * m), DeviceBuffer(12 * m), DeviceBuffer(16 * m)
+    for count, frame in ((n, 0), (3001, 9), (1, 4), (0, 2)):
+        cnt.upload(np.uint32([count]))
+        fw.upload(np.uint32([frame]))
+        call("rta_pred_sample", ptr(ob), ptr(db), ptr(hb), ptr(cnt), m, ptr(fw), C.c_uint64(seed),
+             ptr(oo), ptr(do), ptr(ho), None)
+        rta_amd.synchronize()
+        got_h = ho.download(P.HIT_DTYPE, m)
+        if count == 0:
+            assert np.all(got_h["t"] == F32MAX)
+            continue
+        idx = np.array([_sample_index(seed, frame, i, count) for i in range(m)])
+        assert np.array_equal(got_h["ref"], idx)
+        assert np.array_equal(oo.download(np.float32, 3 * m).reshape(-1, 3), o.reshape(-1, 3)[idx])
+        assert np.array_equal(do.download(np.float32, 3 * m).reshape(-1, 3), d.reshape(-1, 3)[idx])
+
+
+@pytest.mark.gpu
+def test_adam_device_step_matches_host_step(gpu):
+    """Adam with the step count in device memory follows Adam with host step
+    counts 1, 2, 3 on the same gradients (bias corrections from device powf
+    against host powf: relative 1e-5), and advances its counter.  256 rays
+    make one accumulation chunk, so both get bit-identical gradients."""
+    n = 256
+    o, d = _random_rays(n, 41)
+    rng = np.random.default_rng(42)
+    label = rng.integers(0, 1024, size=n).astype(np.int32)
+    target = rng.normal(size=n).astype(np.float32)
+    ob, db = DeviceBuffer.from_array(o), DeviceBuffer.from_array(d)
+    lb, tb = DeviceBuffer.from_array(label), DeviceBuffer.from_array(target)
+    loss, step = DeviceBuffer(4), DeviceBuffer(4)
+    step.fill_u32(0)
+    a = P.HitPredictor(1024, seed=3, scene_min=LO, scene_max=HI)
+    b = P.HitPredictor(1024, seed=3, scene_min=LO, scene_max=HI)
+    initial = a.params()
+    for k in (1, 2, 3):
+        for p, dev in ((a, False), (b, True)):
+            call("rta_pred_zero_grads", p.h, None)
+            p.forward(ob, db, n)
+            call("rta_pred_backward", p.h, ptr(lb), ptr(tb), n, ptr(loss), None)
+            if dev:
+                call("rta_pred_adam_device", p.h, C.c_float(1e-3), ptr(step), None)
+            else:
+                call("rta_pred_adam", p.h, C.c_float(1e-3), k, None)
+        rta_amd.synchronize()
+        pa, pb = a.params(), b.params()
+        assert int(step.download(np.uint32, 1)[0]) == k
+        assert np.allclose(pa, pb, rtol=1e-5, atol=1e-7), k
+        assert not np.array_equal(pa, initial)
+        if k == 1:   # same parameters so far, so the same (deterministic) gradients
+            assert np.array_equal(a.grads(), b.grads())
+
+
 @pytest.mark.gpu
 def test_gradients_match_reference(gpu):
     torch = _torch()
