@@ -708,22 +708,35 @@ def main():
         frame(root)
     s.synchronize()
 
-    seg = {"build": 0.0, "pack": 0.0, "primary": 0.0, "bounce_gen": 0.0, "bounce": 0.0}
+    seg = {"build": [], "pack": [], "primary": [], "bounce_gen": [], "bounce": []}
     log(f"timing {args.steps} steps")
     call("rta_hip_device_synchronize")
     t_start = time.perf_counter()
     for _ in range(args.steps):
         frame(root)
         ev[6].synchronize()
-        seg["build"] += ev[0].elapsed_ms(ev[1])
-        seg["pack"] += ev[1].elapsed_ms(ev[2])
-        seg["primary"] += ev[3].elapsed_ms(ev[4])
-        seg["bounce_gen"] += ev[4].elapsed_ms(ev[5])
-        seg["bounce"] += ev[5].elapsed_ms(ev[6])
+        seg["build"].append(ev[0].elapsed_ms(ev[1]))
+        seg["pack"].append(ev[1].elapsed_ms(ev[2]))
+        seg["primary"].append(ev[3].elapsed_ms(ev[4]))
+        seg["bounce_gen"].append(ev[4].elapsed_ms(ev[5]))
+        seg["bounce"].append(ev[5].elapsed_ms(ev[6]))
     call("rta_hip_device_synchronize")
     wall_ms = (time.perf_counter() - t_start) * 1e3 / args.steps
     K = args.steps
-    avg = {k: v / K for k, v in seg.items()}
+    avg = {k: float(np.mean(v)) for k, v in seg.items()}
+    med = {k: float(np.median(v)) for k, v in seg.items()}
+
+    # the reference's own convention for trace_rays (bbvh-cuda.h:221-229):
+    # host wall clock around launch + synchronisation, reset untimed
+    walls = []
+    for _ in range(max(K, 10)):
+        call("rta_reset_intersections", ptr(hits_b), R, 1, sp)
+        s.synchronize()
+        t0 = time.perf_counter()
+        trace(bnc, count, hits_b, root)
+        s.synchronize()
+        walls.append((time.perf_counter() - t0) * 1e3)
+    bounce_wall_ms = float(np.median(walls))
 
     e = int(err.download(np.uint32, 1)[0])
     if e:
@@ -828,6 +841,11 @@ def main():
         "bounce_gen_ms": round(avg["bounce_gen"], 4),
         "primary_trace_ms": round(avg["primary"], 4),
         "bounce_trace_ms": round(avg["bounce"], 4),
+        "median_ms": {k: round(v, 4) for k, v in med.items()},
+        "bounce_mrays_s_median": round(n_bounce / (med["bounce"] * 1e3), 3),
+        # reference-style trace time: launch + kernel + synchronisation
+        "bounce_trace_wall_ms": round(bounce_wall_ms, 4),
+        "bounce_mrays_s_wall": round(n_bounce / (bounce_wall_ms * 1e3), 3),
         "build_stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
         "lbvh_status": st,
         "roofline": {
