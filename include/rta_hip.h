@@ -416,7 +416,7 @@ int rta_shadow_rays(float *out_orig, float *out_dir, float *out_maxt,
  * max_t seed and the exact traversal decides every result.  Network: 102
  * encoded inputs
  * (origin mapped to [-1,1] by the scene box, unit direction, sin/cos of
- * 2^f*pi*v for f = 0..7), three ReLU layers of 64, 1024 cluster logits + 1
+ * 2^f*pi*v for f = 0..7), three ReLU layers of 64, 1024 treelet logits + 1
  * log distance; bf16 weights and activations, FP32 accumulation (MFMA),
  * FP32 master weights and Adam.  Parameters are one packed FP32 array of
  * rta_pred_param_count floats: W1[64][112], b1[64], W2[64][64], b2[64],
@@ -432,13 +432,13 @@ int rta_pred_set_params(rta_pred pred, const float *host);
 int rta_pred_get_grads(rta_pred pred, float *host);
 int rta_pred_zero_grads(rta_pred pred, rta_stream s);
 /* Forward pass of n rays (device arrays; scene_min/max: 3 host floats).
- * out (n x 1040 f32), top (n x 4 cluster ids, best first) and logdist (n)
+ * out (n x 1040 f32), top (n x 4 treelet ids, best first) and logdist (n)
  * are optional device outputs.  Keeps the activations for a backward pass. */
 int rta_pred_forward(rta_pred pred, const float *ray_orig, const float *ray_dir, int n,
                      const float *scene_min, const float *scene_max, float *out,
                      int32_t *top, float *logdist, rta_stream s);
 /* Gradients of the last forward pass (added to the gradient buffer):
- * softmax cross-entropy of the 1024 logits against label[i] (a cluster, or
+ * softmax cross-entropy of the 1024 logits against label[i] (a treelet, or
  * -1 for a ray without hit, which contributes nothing) plus the squared
  * error of the log distance against target[i], averaged over the n rays;
  * the loss is added to *loss (device float). */
@@ -460,9 +460,9 @@ int rta_pred_sample(const float *ray_orig, const float *ray_dir, const rta_hit *
 
 /* Fused inference for n rays (no batch limit): encoding, the four layers
  * and the top-4 selection in one kernel, the logits kept in registers.
- * top[4*i..] (optional): the 4 clusters with the largest keys, largest
- * first, where a cluster's key is its logit with the low 10 mantissa bits
- * replaced by 1023 - cluster; logdist[i] (optional): the log-distance
+ * top[4*i..] (optional): the 4 treelets with the largest keys, largest
+ * first, where a treelet's key is its logit with the low 10 mantissa bits
+ * replaced by 1023 - treelet; logdist[i] (optional): the log-distance
  * output; logits (optional, n x 1040 floats, for testing): all outputs.
  * The weights are those of the last rta_pred_set_params / rta_pred_adam on
  * the same stream.  Same network and encoding as rta_pred_forward; the

@@ -1,19 +1,21 @@
 // Learned hit predictor: a small MLP from an encoded ray to scores over the
-// LBVH's leaf clusters plus a log hit distance, with hand-written BF16 MFMA
-// kernels for inference and training (FP32 accumulate, FP32 master weights,
-// Adam).  The predictor has no counterpart in the reference; it only
-// proposes candidate clusters whose hit distance seeds a ray's max_t, and the
-// exact traversal decides every result.
+// tree's treelets (a fixed cut into at most 1024 subtrees) plus a log hit
+// distance, with hand-written BF16 MFMA kernels for training (FP32
+// accumulate, FP32 master weights, Adam) and a fused inference kernel.  The
+// predictor has no counterpart in the reference; it only proposes candidate
+// treelets whose hit distance seeds a ray's max_t, and the exact traversal
+// decides every result.
 //
-// Network (widths padded to multiples of 16 for the 16x16x16 MFMA tiles):
+// Network (widths padded to multiples of 16 for the MFMA tiles):
 //   x[112]  = 6 normalised inputs (origin mapped to [-1,1] by the scene box,
 //             unit direction) and sin/cos(2^f pi v) for f = 0..7 of each
 //             (102 features), zero padded
 //   h1..h3  = ReLU(W h + b), 64 wide, bf16
-//   out[1040] = W4 h3 + b4: 1024 cluster logits, the log hit distance at
+//   out[1040] = W4 h3 + b4: 1024 treelet logits, the log hit distance at
 //             column 1024, zero padding
 // Weights are stored out-major ([out][in], the input index contiguous), so
-// both MFMA operands of the forward pass are one 8-byte load per lane.
+// both MFMA operands of the training forward pass are one 8-byte load per
+// lane.  This file is built without NaN handling (finite inputs only).
 #include "common.hpp"
 
 #include <cstring>
@@ -230,8 +232,8 @@ __global__ __launch_bounds__(256) void bwd_bias_kernel(float *__restrict__ db,
 
 // --------------------------------------------------------- loss, top-k
 
-// One wave per row: softmax cross-entropy over the 1024 cluster logits with
-// the label's cluster and a squared error on the log distance, both for
+// One wave per row: softmax cross-entropy over the 1024 treelet logits with
+// the label's treelet and a squared error on the log distance, both for
 // rows with a hit (label >= 0), averaged over the m rows.  Writes dOut
 // (bf16, zero on padding columns and on rows without a hit) and adds the
 // loss to *loss.
@@ -282,7 +284,7 @@ __global__ __launch_bounds__(256) void loss_kernel(uint16_t *__restrict__ dout,
 	}
 }
 
-// One wave per row: the kTop highest logits (ties: lower cluster first) and
+// One wave per row: the kTop highest logits (ties: lower treelet first) and
 // the predicted log distance.
 __global__ __launch_bounds__(256) void topk_kernel(int32_t *__restrict__ top,
                                                    float *__restrict__ logdist,
@@ -372,7 +374,7 @@ __global__ __launch_bounds__(256) void to_bf16_kernel(uint16_t *__restrict__ wb,
 
 // ------------------------------------------------- fused inference
 //
-// One kernel from rays to top-4 clusters: activations stay in registers and
+// One kernel from rays to top-4 treelets: activations stay in registers and
 // the 1040 logits of a ray never reach memory.  The products are computed
 // transposed, D = W H (rows = output features, columns = rays), with the
 // 16x16x32 bf16 MFMA: lane l supplies weight row l%16 and ray l%16, and the
@@ -387,9 +389,9 @@ __global__ __launch_bounds__(256) void to_bf16_kernel(uint16_t *__restrict__ wb,
 // which a lane computes by itself.
 //
 // Top-4 per ray: each logit becomes a key whose low 10 mantissa bits are
-// replaced by 1023 - cluster (distinct keys; the order of the keys is the
+// replaced by 1023 - treelet (distinct keys; the order of the keys is the
 // order of the logits truncated to 13 mantissa bits, ties to the lower
-// cluster for positive logits), kept in a sorted register quadruple by one
+// treelet for positive logits), kept in a sorted register quadruple by one
 // max and three med3 per key, and the four lanes of a ray merged by
 // shuffles at the end.
 
@@ -535,7 +537,7 @@ __global__ __launch_bounds__(1024) void infer_kernel(const v4u *__restrict__ wp,
 		hidden_layer<4>(h1, x, wq + pW1, pq + oB1, lane);
 		hidden_layer<2>(h2, h1, wq + pW2, pq + oB2, lane);
 		hidden_layer<2>(h3, h2, wq + pW3, pq + oB3, lane);
-		// layer 4: 64 cluster tiles into the running top-4 of each lane,
+		// layer 4: 64 treelet tiles into the running top-4 of each lane,
 		// software-pipelined: the products of tile t+1 are issued and the
 		// operands of tile t+2 fetched before the keys of tile t are inserted;
 		// tile 64 (the log distance and padding) comes out of the last round
@@ -650,6 +652,35 @@ inline unsigned blocks_for(long long items, int per_block) {
 	return (unsigned)((items + per_block - 1) / per_block);
 }
 
+// He-uniform weights on the used entries, zero biases and padding, zero
+// Adam moments and gradients, the bf16 copy, and the device's CU count
+int init_state(State *p, uint64_t seed) {
+	std::vector<float> h(kParams, 0.0f);
+	auto fill = [&](int off, int rows, int cols_p, int cols, int fan_in) {
+		const float a = sqrtf(6.0f / (float)fan_in);
+		for (int r = 0; r < rows; ++r)
+			for (int c = 0; c < cols; ++c)
+				h[off + r * cols_p + c] = a * init_uniform(seed, (uint64_t)(off + r * cols_p + c));
+	};
+	fill(oW1, kHid, kInP, kIn, kIn);
+	fill(oW2, kHid, kHid, kHid, kHid);
+	fill(oW3, kHid, kHid, kHid, kHid);
+	fill(oW4, kOut, kHid, kHid, kHid);
+	RTA_HIP_TRY(hipMemcpy(p->params, h.data(), kParams * 4, hipMemcpyHostToDevice));
+	RTA_HIP_TRY(hipMemset(p->mom, 0, kParams * 4));
+	RTA_HIP_TRY(hipMemset(p->vel, 0, kParams * 4));
+	RTA_HIP_TRY(hipMemset(p->grads, 0, kParams * 4));
+	hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for(kParams, 256)), dim3(256), 0, 0, p->wb,
+	                   p->params);
+	RTA_LAUNCH_CHECK("to_bf16_kernel");
+	int dev = 0;
+	RTA_HIP_TRY(hipGetDevice(&dev));
+	RTA_HIP_TRY(hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, dev));
+	if (p->cus < 1) p->cus = 1;
+	RTA_HIP_TRY(hipDeviceSynchronize());
+	return RTA_OK;
+}
+
 }  // namespace pred
 }  // namespace rta_hip
 
@@ -692,30 +723,11 @@ int rta_pred_create(int max_batch, uint64_t seed, rta_pred *out) {
 		set_error("rta_pred_create: device allocation failed");
 		return RTA_E_INVALID;
 	}
-	// He-uniform weights on the used entries, zero biases and padding
-	std::vector<float> h(kParams, 0.0f);
-	auto fill = [&](int off, int rows, int cols_p, int cols, int fan_in) {
-		const float a = sqrtf(6.0f / (float)fan_in);
-		for (int r = 0; r < rows; ++r)
-			for (int c = 0; c < cols; ++c)
-				h[off + r * cols_p + c] = a * init_uniform(seed, (uint64_t)(off + r * cols_p + c));
-	};
-	fill(oW1, kHid, kInP, kIn, kIn);
-	fill(oW2, kHid, kHid, kHid, kHid);
-	fill(oW3, kHid, kHid, kHid, kHid);
-	fill(oW4, kOut, kHid, kHid, kHid);
-	RTA_HIP_TRY(hipMemcpy(p->params, h.data(), kParams * 4, hipMemcpyHostToDevice));
-	RTA_HIP_TRY(hipMemset(p->mom, 0, kParams * 4));
-	RTA_HIP_TRY(hipMemset(p->vel, 0, kParams * 4));
-	RTA_HIP_TRY(hipMemset(p->grads, 0, kParams * 4));
-	hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for(kParams, 256)), dim3(256), 0, 0, p->wb,
-	                   p->params);
-	RTA_LAUNCH_CHECK("to_bf16_kernel");
-	int dev = 0;
-	RTA_HIP_TRY(hipGetDevice(&dev));
-	RTA_HIP_TRY(hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, dev));
-	if (p->cus < 1) p->cus = 1;
-	RTA_HIP_TRY(hipDeviceSynchronize());
+	const int rc_init = init_state(p, seed);
+	if (rc_init != RTA_OK) {
+		delete p;
+		return rc_init;
+	}
 	*out = p;
 	return RTA_OK;
 }

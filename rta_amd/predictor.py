@@ -15,7 +15,7 @@ import time
 
 import numpy as np
 
-from .abi import FLT_MAX, HIT_DTYPE, TREELET_DTYPE, call, ptr
+from .abi import HIT_DTYPE, TREELET_DTYPE, call, ptr
 from .device import DeviceBuffer, Stream
 
 N_CLUSTERS = 1024
@@ -138,7 +138,7 @@ class HitPredictor:
              ptr(logdist) if logdist is not None else None, ptr(stream) if stream else None)
 
     def infer(self, o, d, n, top=None, logdist=None, logits=None, stream=None, o_off=0):
-        """The fused inference kernel (any n): top-4 clusters by key, the
+        """The fused inference kernel (any n): top-4 treelets by key, the
         log distance and, for testing, all n x 1040 outputs."""
         call("rta_pred_infer", self.h, _at(o, 12 * o_off), _at(d, 12 * o_off), n,
              self.scene_min, self.scene_max, ptr(top) if top is not None else None,
@@ -186,7 +186,7 @@ def seed_max_t(pred: HitPredictor, packed, cut: TreeletCut, rays, n: int, top: D
 
 
 def top4_accuracy(top: np.ndarray, label: np.ndarray) -> float:
-    """Share of hit rays whose cluster is among their 4 proposals."""
+    """Share of hit rays whose treelet is among their 4 proposals."""
     hit = label >= 0
     if not hit.any():
         return 0.0
