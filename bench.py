@@ -507,7 +507,8 @@ def predictor_leg(args, tris_h, tris, nodes, packed, ranges, prim, hits_p, R, tc
             for _ in range(reps):
                 call("rta_reset_intersections", ptr(e["pred"]), R, 1, sp)
                 ev[0].record(s)
-                pred.infer(rays.gpu_origin, rays.gpu_direction, m, top=top, stream=s)
+                pred.infer(rays.gpu_origin, rays.gpu_direction, m, top=top, stream=s,
+                           top_k=1 if cand == 1 else 4)
                 ev[1].record(s)
                 call("rta_trace_seed_treelets", ptr(packed.tri4), ptr(packed.pairs),
                      ptr(packed.flags), ptr(cut.table), ptr(cut.count), ptr(rays.gpu_origin),

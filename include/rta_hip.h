@@ -459,17 +459,17 @@ int rta_pred_sample(const float *ray_orig, const float *ray_dir, const rta_hit *
                     float *orig_out, float *dir_out, rta_hit *hits_out, rta_stream s);
 
 /* Fused inference for n rays (no batch limit): encoding, the four layers
- * and the top-4 selection in one kernel, the logits kept in registers.
- * top[4*i..] (optional): the 4 treelets with the largest keys, largest
- * first, where a treelet's key is its logit with the low 10 mantissa bits
- * replaced by 1023 - treelet; logdist[i] (optional): the log-distance
- * output; logits (optional, n x 1040 floats, for testing): all outputs.
- * The weights are those of the last rta_pred_set_params / rta_pred_adam on
- * the same stream.  Same network and encoding as rta_pred_forward; the
- * sums run in another order (16x16x32 tiles, bias first), so logits may
- * differ from it in the last bits. */
+ * and the top-k selection in one kernel, the logits kept in registers.
+ * top[4*i..] (optional): the top_k (4 or 1) treelets with the largest keys,
+ * largest first, where a treelet's key is its logit with the low 10
+ * mantissa bits replaced by 1023 - treelet (with top_k 1, entries 1..3 are
+ * -1); logdist[i] (optional): the log-distance output; logits (optional,
+ * n x 1040 floats, for testing): all outputs.  The weights are those of the
+ * last rta_pred_set_params / rta_pred_adam on the same stream.  Same network
+ * and encoding as rta_pred_forward; the sums run in another order (16x16x32
+ * tiles, bias first), so logits may differ from it in the last bits. */
 int rta_pred_infer(rta_pred pred, const float *ray_orig, const float *ray_dir, int n,
-                   const float *scene_min, const float *scene_max, int32_t *top,
+                   const float *scene_min, const float *scene_max, int32_t *top, int top_k,
                    float *logdist, float *logits, rta_stream s);
 
 /* Karras ranges (sorted positions [from, to], 2 words per inner node, n-1

@@ -150,7 +150,7 @@ HIP_SIGNATURES = {
     "rta_pred_get_grads": [_vp, _vp],
     "rta_pred_zero_grads": [_vp, _vp],
     "rta_pred_forward": [_vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp],
-    "rta_pred_infer": [_vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp],
+    "rta_pred_infer": [_vp, _vp, _vp, _i, _vp, _vp, _vp, _i, _vp, _vp, _vp],
     "rta_pred_backward": [_vp, _vp, _vp, _i, _vp, _vp],
     "rta_pred_adam": [_vp, _f, _i, _vp],
     "rta_pred_adam_device": [_vp, _f, _vp, _vp],

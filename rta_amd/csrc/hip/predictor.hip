@@ -480,8 +480,10 @@ __device__ inline int key_cluster(float k) { return 1023 - (int)(__float_as_uint
 // Persistent: each wave takes tiles of 32 rays (two sets of 16) from a
 // static stride; layer 4's permuted weights (133,120 B) and biases sit in
 // LDS, one block of 16 waves per CU.  LOGITS: also write every output (for
-// testing; a separate instance keeps the stores out of the fast loop).
-template <bool LOGITS>
+// testing; a separate instance keeps the stores out of the fast loop).  K:
+// 4, or 1 (top-1 only: one max per key; entries 1..3 of a ray's top are
+// written as -1).
+template <bool LOGITS, int K>
 __global__ __launch_bounds__(1024) void infer_kernel(const v4u *__restrict__ wp,
                                                      const float *__restrict__ params,
                                                      const float *__restrict__ ro,
@@ -569,7 +571,9 @@ __global__ __launch_bounds__(1024) void infer_kernel(const v4u *__restrict__ wp,
 				#pragma unroll
 				for (int i = 0; i < 4; ++i) {
 					const unsigned low = low0 - 16u * (unsigned)t - (unsigned)i;
-					insert4(tk[r], __uint_as_float((__float_as_uint(acc[r][i]) & 0xfffffc00u) | low));
+					const float key = __uint_as_float((__float_as_uint(acc[r][i]) & 0xfffffc00u) | low);
+					if (K == 1) tk[r][0] = fmaxf(tk[r][0], key);
+					else insert4(tk[r], key);
 				}
 			if (LOGITS)
 				#pragma unroll
@@ -601,6 +605,10 @@ __global__ __launch_bounds__(1024) void infer_kernel(const v4u *__restrict__ wp,
 		for (int r = 0; r < 2; ++r) {
 			#pragma unroll
 			for (int off = 16; off <= 32; off <<= 1) {
+				if (K == 1) {
+					tk[r][0] = fmaxf(tk[r][0], __shfl_xor(tk[r][0], off));
+					continue;
+				}
 				float o[4];
 				#pragma unroll
 				for (int i = 0; i < 4; ++i) o[i] = __shfl_xor(tk[r][i], off);
@@ -609,8 +617,10 @@ __global__ __launch_bounds__(1024) void infer_kernel(const v4u *__restrict__ wp,
 			}
 			const int ray = tile * 32 + r * 16 + col;
 			if (top && ray < n) {
-				const float k = g == 0 ? tk[r][0] : g == 1 ? tk[r][1] : g == 2 ? tk[r][2] : tk[r][3];
-				top[(size_t)ray * kTop + g] = key_cluster(k);
+				int c = -1;
+				if (K == 1) c = g == 0 ? key_cluster(tk[r][0]) : -1;
+				else c = key_cluster(g == 0 ? tk[r][0] : g == 1 ? tk[r][1] : g == 2 ? tk[r][2] : tk[r][3]);
+				top[(size_t)ray * kTop + g] = c;
 			}
 		}
 	}
@@ -804,12 +814,13 @@ int rta_pred_forward(rta_pred h, const float *o, const float *d, int n, const fl
 }
 
 int rta_pred_infer(rta_pred h, const float *o, const float *d, int n, const float *scene_min,
-                   const float *scene_max, int32_t *top, float *logdist, float *logits,
-                   rta_stream s) {
+                   const float *scene_max, int32_t *top, int top_k, float *logdist,
+                   float *logits, rta_stream s) {
 	State *p = reinterpret_cast<State *>(h);
 	if (!p || !o || !d || !scene_min || !scene_max) return invalid("rta_pred_infer: NULL pointer");
 	if (!top && !logdist && !logits) return invalid("rta_pred_infer: no output requested");
 	if (n < 0) return invalid("rta_pred_infer: n < 0");
+	if (top_k != 1 && top_k != kTop) return invalid("rta_pred_infer: top_k must be 1 or 4");
 	if (n == 0) return RTA_OK;
 	hipStream_t st = as_stream(s);
 	// the permuted weights follow the current parameters (one small launch)
@@ -822,14 +833,20 @@ int rta_pred_infer(rta_pred h, const float *o, const float *d, int n, const floa
 	const long long tiles = ((long long)n + 31) / 32;
 	const long long need = (tiles + kInferWaves - 1) / kInferWaves;
 	const unsigned blocks = (unsigned)(need < p->cus ? need : p->cus);
-	if (logits)
-		hipLaunchKernelGGL(infer_kernel<true>, dim3(blocks), dim3(kInferWaves * kWave), 0, st,
-		                   reinterpret_cast<const v4u *>(p->wpack), p->params, o, d, n, lo, sc, top,
-		                   logdist, logits);
+	const v4u *wp = reinterpret_cast<const v4u *>(p->wpack);
+	const dim3 grid(blocks), block(kInferWaves * kWave);
+	if (logits && top_k == 1)
+		hipLaunchKernelGGL((infer_kernel<true, 1>), grid, block, 0, st, wp, p->params, o, d, n, lo,
+		                   sc, top, logdist, logits);
+	else if (logits)
+		hipLaunchKernelGGL((infer_kernel<true, kTop>), grid, block, 0, st, wp, p->params, o, d, n,
+		                   lo, sc, top, logdist, logits);
+	else if (top_k == 1)
+		hipLaunchKernelGGL((infer_kernel<false, 1>), grid, block, 0, st, wp, p->params, o, d, n, lo,
+		                   sc, top, logdist, nullptr);
 	else
-		hipLaunchKernelGGL(infer_kernel<false>, dim3(blocks), dim3(kInferWaves * kWave), 0, st,
-		                   reinterpret_cast<const v4u *>(p->wpack), p->params, o, d, n, lo, sc, top,
-		                   logdist, nullptr);
+		hipLaunchKernelGGL((infer_kernel<false, kTop>), grid, block, 0, st, wp, p->params, o, d, n,
+		                   lo, sc, top, logdist, nullptr);
 	RTA_LAUNCH_CHECK("infer_kernel");
 	return RTA_OK;
 }

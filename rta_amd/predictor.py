@@ -137,11 +137,13 @@ class HitPredictor:
              ptr(top) if top is not None else None,
              ptr(logdist) if logdist is not None else None, ptr(stream) if stream else None)
 
-    def infer(self, o, d, n, top=None, logdist=None, logits=None, stream=None, o_off=0):
-        """The fused inference kernel (any n): top-4 treelets by key, the
-        log distance and, for testing, all n x 1040 outputs."""
+    def infer(self, o, d, n, top=None, logdist=None, logits=None, stream=None, o_off=0,
+              top_k=TOP):
+        """The fused inference kernel (any n): the top_k (4 or 1) treelets by
+        key (4 ids per ray; -1 beyond top_k), the log distance and, for
+        testing, all n x 1040 outputs."""
         call("rta_pred_infer", self.h, _at(o, 12 * o_off), _at(d, 12 * o_off), n,
-             self.scene_min, self.scene_max, ptr(top) if top is not None else None,
+             self.scene_min, self.scene_max, ptr(top) if top is not None else None, top_k,
              ptr(logdist) if logdist is not None else None,
              ptr(logits) if logits is not None else None, ptr(stream) if stream else None)
 
@@ -174,11 +176,13 @@ class HitPredictor:
 def seed_max_t(pred: HitPredictor, packed, cut: TreeletCut, rays, n: int, top: DeviceBuffer,
                t_seed: DeviceBuffer, maxt_out: DeviceBuffer, err: DeviceBuffer,
                stream=None, candidates: int = TOP) -> None:
-    """Inference (top-4 treelets, fused kernel), the traversal of the first
-    `candidates` of those treelets, and max_t' = min(max_t, seed) for the n
-    rays.  `packed` is a PackedBvh (pairs, tri4, flags)."""
+    """Inference (fused kernel; top-1 when one candidate is traversed, else
+    top-4), the traversal of the first `candidates` proposed treelets, and
+    max_t' = min(max_t, seed) for the n rays.  `packed` is a PackedBvh
+    (pairs, tri4, flags)."""
     sp = ptr(stream) if stream else None
-    pred.infer(rays.gpu_origin, rays.gpu_direction, n, top=top, stream=stream)
+    pred.infer(rays.gpu_origin, rays.gpu_direction, n, top=top, stream=stream,
+               top_k=1 if candidates == 1 else TOP)
     call("rta_trace_seed_treelets", ptr(packed.tri4), ptr(packed.pairs), ptr(packed.flags),
          ptr(cut.table), ptr(cut.count), ptr(rays.gpu_origin), ptr(rays.gpu_direction),
          ptr(rays.gpu_maxt), n, ptr(top), candidates, ptr(t_seed), ptr(err), sp)

@@ -193,6 +193,13 @@ def test_infer_matches_reference(gpu):
     # the instance without the logits output gives the same results
     assert np.array_equal(top2.download(np.int32, 4 * n).reshape(n, 4), t4)
     assert np.array_equal(ld2.download(np.float32, n), got[:, 1024])
+    # top-1: the best key, the other three entries -1
+    top1 = DeviceBuffer(16 * n)
+    pred.infer(ob, db, n, top=top1, top_k=1)
+    rta_amd.synchronize()
+    t1 = top1.download(np.int32, 4 * n).reshape(n, 4)
+    assert np.array_equal(t1[:, 0], t4[:, 0])
+    assert np.all(t1[:, 1:] == -1)
 
 
 def _sample_index(seed, frame, i, count):
