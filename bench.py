@@ -555,9 +555,46 @@ def predictor_leg(args, tris_h, tris, nodes, packed, ranges, prim, hits_p, R, tc
                 rec["inference_ms"] = round(part["inference"], 4)
                 rec["inference_tflop_s"] = round(
                     PRED_FLOPS_PER_RAY * m / (part["inference"] * 1e-3) / 1e12, 3)
+        # top-1 inference + the seeding fused into the persistent traversal
+        part = {"inference": 0.0, "seeded_traversal": 0.0}
+        total_ms = 0.0
+        for _ in range(reps):
+            call("rta_reset_intersections", ptr(e["pred"]), R, 1, sp)
+            ev[0].record(s)
+            pred.infer(rays.gpu_origin, rays.gpu_direction, m, top=top, stream=s, top_k=1)
+            ev[1].record(s)
+            call("rta_trace_cis_persistent_seeded", ptr(packed.tri4), ptr(packed.pairs),
+                 ptr(packed.flags), packed.root_link, packed.root_index, ptr(rays.gpu_origin),
+                 ptr(rays.gpu_direction), ptr(rays.gpu_maxt), R, ptr(e["cnt"]), ptr(cut.table),
+                 ptr(cut.count), ptr(top), ptr(e["pred"]), ptr(err), ptr(queue), C.byref(tcfg),
+                 sp)
+            ev[2].record(s)
+            ev[2].synchronize()
+            part["inference"] += ev[0].elapsed_ms(ev[1]) / reps
+            part["seeded_traversal"] += ev[1].elapsed_ms(ev[2]) / reps
+            total_ms += ev[0].elapsed_ms(ev[2]) / reps
+        if int(err.download(np.uint32, 1)[0]):
+            raise SystemExit("hit-predicted leg: traversal reported stack/guard errors")
+        hq = e["pred"].download(HIT_DTYPE, m)
+        same = (hq["t"] != fmax) == hit
+        for f in ("t", "beta", "gamma"):
+            same &= (hq[f].view(np.uint32) == hp[f].view(np.uint32)) | ~hit
+        same &= (hq["ref"] == hp["ref"]) | ~hit
+        c = {"candidates": 1, "fused": True,
+             "hit_predicted_mrays_s": round(m / (total_ms * 1e3), 3),
+             "hit_predicted_ms": round(total_ms, 4),
+             "breakdown_ms": {k: round(v, 4) for k, v in part.items()},
+             "exact_share": float(same.mean())}
+        rec["by_candidates"].append(c)
+        log(f"predictor seed {e['seed']}, fused (top-1 seeding inside the persistent "
+            f"traversal): hit-predicted {c['hit_predicted_mrays_s']:.1f} Mrays/s "
+            f"({c['breakdown_ms']}), exact {c['exact_share']:.6f}")
+        if c["exact_share"] != 1.0:
+            raise SystemExit("hit-predicted leg: fused seeded records differ from the plain ones")
         best = max(rec["by_candidates"], key=lambda c: c["hit_predicted_mrays_s"])
         rec["hit_predicted_mrays_s"] = best["hit_predicted_mrays_s"]
         rec["hit_predicted_best_candidates"] = best["candidates"]
+        rec["hit_predicted_best_fused"] = bool(best.get("fused", False))
         rec["hit_predicted_vs_plain"] = round(best["hit_predicted_mrays_s"] /
                                               rec["plain_mrays_s"], 4)
         rec["exact_share"] = min(c["exact_share"] for c in rec["by_candidates"])

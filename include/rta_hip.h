@@ -514,6 +514,21 @@ int rta_trace_seed_treelets(const float *tri4, const rta_node_pair *pairs, const
                             const float *ray_orig, const float *ray_dir, const float *max_t,
                             int n_rays, const int32_t *top, int candidates, float *t_seed,
                             uint32_t *err, rta_stream s);
+/* The seeding and the exact traversal in one persistent launch: ray i first
+ * traverses treelet top[4*i] (when it is < *count and the ray meets its root
+ * box within max_t[i]), then the whole tree with max_t' = min(max_t[i], the
+ * closest t of that phase) from a closest hit of FLT_MAX, as
+ * rta_trace_cis_persistent.  Records equal those of
+ * rta_trace_seed_treelets (one candidate) + rta_seed_max_t +
+ * rta_trace_cis_persistent, and so the plain traversal's. */
+int rta_trace_cis_persistent_seeded(const float *tri4, const rta_node_pair *pairs,
+                                    const uint32_t *flags, uint32_t root_link,
+                                    uint32_t root_index, const float *ray_orig,
+                                    const float *ray_dir, const float *max_t, int n_rays,
+                                    const uint32_t *ray_count, const rta_treelet *table,
+                                    const uint32_t *count, const int32_t *top, rta_hit *hits,
+                                    uint32_t *err, uint32_t *queue, const rta_trace_config *cfg,
+                                    rta_stream s);
 /* out[i] = min(max_t[i], t_seed[i]): the max_t a seeded traversal runs with. */
 int rta_seed_max_t(float *out, const float *max_t, const float *t_seed, int n, rta_stream s);
 

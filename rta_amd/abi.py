@@ -123,6 +123,8 @@ HIP_SIGNATURES = {
     "rta_trace_cis_packed": [_vp, _vp, _vp, _u32, _u32, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp],
     "rta_trace_cis_persistent": [_vp, _vp, _vp, _u32, _u32, _vp, _vp, _vp, _i, _vp, _vp, _vp,
                                  _vp, _vp, _vp],
+    "rta_trace_cis_persistent_seeded": [_vp, _vp, _vp, _u32, _u32, _vp, _vp, _vp, _i, _vp, _vp,
+                                        _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "rta_trace_shadow_cis_packed": [_vp, _vp, _vp, _u32, _u32, _vp, _vp, _vp, _i, _vp, _vp, _vp,
                                     _vp],
     "rta_trace_shadow_cis_persistent": [_vp, _vp, _vp, _u32, _u32, _vp, _vp, _vp, _i, _vp, _vp,

@@ -624,7 +624,20 @@ __global__ void fill_queue_kernel(uint32_t *queue) {
 // at pairs[node_off], its tri4 records at tri4[3 * tri_off], its flag word
 // is flags[mesh] and its root word root_words[mesh]; hit refs are local to
 // the mesh.
-template <int BLOCK, bool ANY_HIT, bool BATCHED>
+//
+// SEEDED (closest hit, one mesh): each ray first traverses the treelet
+// seed.top[4 * ray] proposes (seed.table entry, when its root box is met
+// within max_t), then the whole tree with max_t' = min(max_t, the closest t
+// of that phase) and its closest hit reset to FLT_MAX -- the same records
+// as rta_trace_seed_treelets + rta_seed_max_t + the plain traversal, in one
+// launch.  Only the second phase writes the record.
+struct SeedArgs {
+	const rta_treelet *table = nullptr;
+	const uint32_t *count = nullptr;
+	const int32_t *top = nullptr;
+};
+
+template <int BLOCK, bool ANY_HIT, bool BATCHED, bool SEEDED = false>
 __global__ __launch_bounds__(BLOCK) void trace_cis_persistent_kernel(
     const float4 *__restrict__ tri4, const float4 *__restrict__ pairs,
     const uint32_t *__restrict__ flags, uint32_t root_word, const float *__restrict__ ro,
@@ -632,7 +645,8 @@ __global__ __launch_bounds__(BLOCK) void trace_cis_persistent_kernel(
     const uint32_t *__restrict__ ray_count, rta_hit *__restrict__ hits,
     uint32_t *__restrict__ err, uint32_t *__restrict__ queue, int refill_min, int steps,
     int leaf_min, const BatchMesh *__restrict__ meshes, int n_meshes,
-    const uint32_t *__restrict__ root_words) {
+    const uint32_t *__restrict__ root_words, SeedArgs seed) {
+	static_assert(!SEEDED || (!ANY_HIT && !BATCHED), "seeding: closest hit on one mesh");
 	__shared__ uint32_t stack[kStack][BLOCK];
 	const int lt = threadIdx.x;
 	const int lane = __lane_id();
@@ -640,6 +654,8 @@ __global__ __launch_bounds__(BLOCK) void trace_cis_persistent_kernel(
 	uint32_t n = (uint32_t)n_rays;
 	if (ray_count) n = min(n, *ray_count);
 	const bool scene_fast = BATCHED ? false : (*flags & 1u) == 0u;
+	const uint32_t n_treelets = SEEDED ? *seed.count : 0u;
+	bool seeding = false;   // SEEDED: the lane is in its treelet phase
 
 	bool active = false, exhausted = false;
 	uint32_t ray = 0, curr = 0;
@@ -702,6 +718,20 @@ __global__ __launch_bounds__(BLOCK) void trace_cis_persistent_kernel(
 					} else {
 						curr = root_word;
 						fast = scene_fast && ray_nz_ok(ox, oy, oz, dx, dy, dz);
+					}
+					if (SEEDED) {
+						seeding = false;
+						const uint32_t c = (uint32_t)seed.top[4 * (size_t)ray];
+						if (c < n_treelets) {
+							const rta_treelet tr = seed.table[c];
+							float dist;
+							if (box_test(tr.lo[0], tr.lo[1], tr.lo[2], tr.hi[0], tr.hi[1], tr.hi[2], ox,
+							             oy, oz, dx, dy, dz, dist) &&
+							    dist <= tmax) {
+								curr = tr.word;
+								seeding = true;
+							}
+						}
 					}
 					yx = 1.0f / dx; yy = 1.0f / dy; yz = 1.0f / dz;
 					ct = FLT_MAX;
@@ -783,7 +813,17 @@ __global__ __launch_bounds__(BLOCK) void trace_cis_persistent_kernel(
 			}
 			if (done) {
 				if (overflow) atomicAdd(err, 1u);
-				finish();
+				if (SEEDED && seeding) {
+					// the treelet phase ends: its closest t (a real hit, or
+					// FLT_MAX) bounds the traversal of the whole tree
+					tmax = fminf(tmax, ct);
+					ct = FLT_MAX;
+					curr = root_word;
+					sp = -1;
+					seeding = false;
+				} else {
+					finish();
+				}
 			}
 		}
 	}
@@ -950,13 +990,13 @@ namespace rta_hip {
 // Grid of the persistent kernel: as many blocks as fit on every CU (or
 // blocks_per_cu), never more than the rays need.  The CU count and the
 // occupancy are queried once per device and kernel instance.
-template <int BLOCK, bool ANY_HIT, bool BATCHED>
+template <int BLOCK, bool ANY_HIT, bool BATCHED, bool SEEDED = false>
 int launch_persistent(const float *tri4, const rta_node_pair *pairs, const uint32_t *flags,
                       uint32_t root_word, const float *ro, const float *rd, const float *rmax,
                       int n_rays, const uint32_t *ray_count, rta_hit *hits, uint32_t *err,
                       uint32_t *queue, int refill_min, int steps, int blocks_per_cu,
                       int leaf_min, const BatchMesh *meshes, int n_meshes,
-                      const uint32_t *root_words, hipStream_t st) {
+                      const uint32_t *root_words, hipStream_t st, SeedArgs seed = SeedArgs{}) {
 	constexpr int kMaxDevices = 64;
 	static int cus[kMaxDevices] = {}, occ[kMaxDevices] = {};
 	int dev = 0;
@@ -966,7 +1006,7 @@ int launch_persistent(const float *tri4, const rta_node_pair *pairs, const uint3
 		int c = 0, o = 0;
 		RTA_HIP_TRY(hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev));
 		RTA_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-		    &o, trace_cis_persistent_kernel<BLOCK, ANY_HIT, BATCHED>, BLOCK, 0));
+		    &o, trace_cis_persistent_kernel<BLOCK, ANY_HIT, BATCHED, SEEDED>, BLOCK, 0));
 		occ[dev] = o < 1 ? 1 : o;
 		cus[dev] = c;
 	}
@@ -976,10 +1016,11 @@ int launch_persistent(const float *tri4, const rta_node_pair *pairs, const uint3
 	const unsigned blocks = (unsigned)(want < need ? want : need);
 	hipLaunchKernelGGL(fill_queue_kernel, dim3(1), dim3(64), 0, st, queue);
 	RTA_LAUNCH_CHECK("fill_queue_kernel");
-	hipLaunchKernelGGL((trace_cis_persistent_kernel<BLOCK, ANY_HIT, BATCHED>), dim3(blocks),
-	                   dim3(BLOCK), 0, st, (const float4 *)tri4, (const float4 *)pairs, flags,
-	                   root_word, ro, rd, rmax, n_rays, ray_count, hits, err, queue, refill_min,
-	                   steps, leaf_min, meshes, n_meshes, root_words);
+	hipLaunchKernelGGL((trace_cis_persistent_kernel<BLOCK, ANY_HIT, BATCHED, SEEDED>),
+	                   dim3(blocks), dim3(BLOCK), 0, st, (const float4 *)tri4,
+	                   (const float4 *)pairs, flags, root_word, ro, rd, rmax, n_rays, ray_count,
+	                   hits, err, queue, refill_min, steps, leaf_min, meshes, n_meshes,
+	                   root_words, seed);
 	RTA_LAUNCH_CHECK("trace_cis_persistent_kernel");
 	return RTA_OK;
 }
@@ -1172,6 +1213,43 @@ int rta_trace_cis_persistent(const float *tri4, const rta_node_pair *pairs,
 	return trace_persistent<false>("rta_trace_cis_persistent", tri4, pairs, flags, root_link,
 	                               root_index, ray_orig, ray_dir, max_t, n_rays, ray_count, hits,
 	                               err, queue, cfg, as_stream(s));
+}
+
+int rta_trace_cis_persistent_seeded(const float *tri4, const rta_node_pair *pairs,
+                                    const uint32_t *flags, uint32_t root_link,
+                                    uint32_t root_index, const float *ray_orig,
+                                    const float *ray_dir, const float *max_t, int n_rays,
+                                    const uint32_t *ray_count, const rta_treelet *table,
+                                    const uint32_t *count, const int32_t *top, rta_hit *hits,
+                                    uint32_t *err, uint32_t *queue, const rta_trace_config *cfg,
+                                    rta_stream s) {
+	const char *name = "rta_trace_cis_persistent_seeded";
+	if (!tri4 || !pairs || !flags || !ray_orig || !ray_dir || !max_t || !hits || !err ||
+	    !queue || !table || !count || !top || n_rays <= 0)
+		return invalid("rta_trace_cis_persistent_seeded: bad arguments");
+	rta_trace_config c;
+	const int rc = resolve_config(name, cfg, c);
+	if (rc != RTA_OK) return rc;
+	const uint32_t root_word = link_inner(root_link) ? 1u : (root_index << 1);
+	SeedArgs seed;
+	seed.table = table;
+	seed.count = count;
+	seed.top = top;
+	const hipStream_t st = as_stream(s);
+	switch (c.block) {
+	case 64:
+		return launch_persistent<64, false, false, true>(
+		    tri4, pairs, flags, root_word, ray_orig, ray_dir, max_t, n_rays, ray_count, hits, err,
+		    queue, c.refill_min, c.steps, c.blocks_per_cu, c.leaf_min, nullptr, 0, nullptr, st, seed);
+	case 128:
+		return launch_persistent<128, false, false, true>(
+		    tri4, pairs, flags, root_word, ray_orig, ray_dir, max_t, n_rays, ray_count, hits, err,
+		    queue, c.refill_min, c.steps, c.blocks_per_cu, c.leaf_min, nullptr, 0, nullptr, st, seed);
+	default:
+		return launch_persistent<256, false, false, true>(
+		    tri4, pairs, flags, root_word, ray_orig, ray_dir, max_t, n_rays, ray_count, hits, err,
+		    queue, c.refill_min, c.steps, c.blocks_per_cu, c.leaf_min, nullptr, 0, nullptr, st, seed);
+	}
 }
 
 int rta_trace_shadow_cis_persistent(const float *tri4, const rta_node_pair *pairs,

@@ -189,6 +189,26 @@ def seed_max_t(pred: HitPredictor, packed, cut: TreeletCut, rays, n: int, top: D
     call("rta_seed_max_t", ptr(maxt_out), ptr(rays.gpu_maxt), ptr(t_seed), n, sp)
 
 
+def trace_seeded(pred: HitPredictor, packed, cut: TreeletCut, rays, n: int, top: DeviceBuffer,
+                 hits: DeviceBuffer, err: DeviceBuffer, queue: DeviceBuffer, config=None,
+                 ray_count: DeviceBuffer | None = None, stream=None) -> None:
+    """Hit-predicted tracing in two launches: top-1 inference, then the
+    persistent traversal that first traverses the proposed treelet and then
+    the whole tree under the bound it found
+    (rta_trace_cis_persistent_seeded).  `hits` must hold the reset
+    sentinel, as for the plain traversal."""
+    from .abi import TraceConfig
+
+    cfg = config if config is not None else TraceConfig(64, 16, 32, 0, 10)
+    pred.infer(rays.gpu_origin, rays.gpu_direction, n, top=top, stream=stream, top_k=1)
+    call("rta_trace_cis_persistent_seeded", ptr(packed.tri4), ptr(packed.pairs),
+         ptr(packed.flags), packed.root_link, packed.root_index, ptr(rays.gpu_origin),
+         ptr(rays.gpu_direction), ptr(rays.gpu_maxt), n,
+         ptr(ray_count) if ray_count is not None else None, ptr(cut.table), ptr(cut.count),
+         ptr(top), ptr(hits), ptr(err), ptr(queue), C.byref(cfg),
+         ptr(stream) if stream else None)
+
+
 def top4_accuracy(top: np.ndarray, label: np.ndarray) -> float:
     """Share of hit rays whose treelet is among their 4 proposals."""
     hit = label >= 0
@@ -197,5 +217,5 @@ def top4_accuracy(top: np.ndarray, label: np.ndarray) -> float:
     return float((top[hit] == label[hit, None]).any(axis=1).mean())
 
 
-__all__ = ["HitPredictor", "TreeletCut", "seed_max_t", "top4_accuracy", "unpack",
+__all__ = ["HitPredictor", "TreeletCut", "seed_max_t", "trace_seeded", "top4_accuracy", "unpack",
            "param_count", "PARAM_COUNT", "N_CLUSTERS", "TOP", "HIT_DTYPE", "TREELET_DTYPE"]

@@ -507,3 +507,16 @@ def test_training_gain_and_exact_seeded_traversal(gpu):
     for f in ("t", "beta", "gamma"):
         assert np.array_equal(bits(got[f][hit]), bits(h_ev[f][hit])), f
     assert np.array_equal(got["ref"][hit], h_ev["ref"][hit])
+
+    # the seeding and the exact traversal fused into one persistent launch
+    hits_f, queue = DeviceBuffer(16 * n_ev), DeviceBuffer(4)
+    call("rta_reset_intersections", ptr(hits_f), n_ev, 1, None)
+    P.trace_seeded(pred, packed, cut, rays, n_ev, top, hits_f, err, queue)
+    rta_amd.synchronize()
+    assert int(err.download(np.uint32, 1)[0]) == 0
+    got = hits_f.download(P.HIT_DTYPE, n_ev)
+    assert np.array_equal(got["t"] != F32MAX, hit)
+    for f in ("t", "beta", "gamma"):
+        assert np.array_equal(bits(got[f][hit]), bits(h_ev[f][hit])), f
+    assert np.array_equal(got["ref"][hit], h_ev["ref"][hit])
+    say("fused seeded persistent traversal: records equal the plain traversal's")
