@@ -516,11 +516,10 @@ int rta_trace_seed_treelets(const float *tri4, const rta_node_pair *pairs, const
                             uint32_t *err, rta_stream s);
 /* The seeding and the exact traversal in one persistent launch: ray i first
  * traverses treelet top[4*i] (when it is < *count and the ray meets its root
- * box within max_t[i]), then the whole tree with max_t' = min(max_t[i], the
- * closest t of that phase) from a closest hit of FLT_MAX, as
- * rta_trace_cis_persistent.  Records equal those of
- * rta_trace_seed_treelets (one candidate) + rta_seed_max_t +
- * rta_trace_cis_persistent, and so the plain traversal's. */
+ * box within max_t[i]) up to the first leaf that holds a hit, then the whole
+ * tree with max_t' = min(max_t[i], the smallest t of that phase) from a
+ * closest hit of FLT_MAX, as rta_trace_cis_persistent.  Any hit of the ray
+ * bounds its closest one, so the records are the plain traversal's. */
 int rta_trace_cis_persistent_seeded(const float *tri4, const rta_node_pair *pairs,
                                     const uint32_t *flags, uint32_t root_link,
                                     uint32_t root_index, const float *ray_orig,

@@ -627,10 +627,11 @@ __global__ void fill_queue_kernel(uint32_t *queue) {
 //
 // SEEDED (closest hit, one mesh): each ray first traverses the treelet
 // seed.top[4 * ray] proposes (seed.table entry, when its root box is met
-// within max_t), then the whole tree with max_t' = min(max_t, the closest t
-// of that phase) and its closest hit reset to FLT_MAX -- the same records
-// as rta_trace_seed_treelets + rta_seed_max_t + the plain traversal, in one
-// launch.  Only the second phase writes the record.
+// within max_t) until the first leaf that holds a hit, then the whole tree
+// with max_t' = min(max_t, the smallest t of that phase) and its closest hit
+// reset to FLT_MAX.  Any hit of the ray bounds its closest one, so the
+// records are the plain traversal's.  Only the second phase writes the
+// record.
 struct SeedArgs {
 	const rta_treelet *table = nullptr;
 	const uint32_t *count = nullptr;
@@ -806,9 +807,11 @@ __global__ __launch_bounds__(BLOCK) void trace_cis_persistent_kernel(
 					if (__float_as_uint(v0.w) != 0u) break;
 					++i;
 				}
-				// pop, or finish (empty stack, or a shadow ray's first hit)
+				// pop, or finish (empty stack, a shadow ray's first hit, or the
+				// end of a treelet phase at its first hit leaf: any real hit
+				// is a valid bound)
 				curr = stack[sp >= 0 ? sp : 0][lt];
-				done = (ANY_HIT && ct != FLT_MAX) || sp < 0;
+				done = (ANY_HIT && ct != FLT_MAX) || (SEEDED && seeding && ct != FLT_MAX) || sp < 0;
 				--sp;
 			}
 			if (done) {
