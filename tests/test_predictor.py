@@ -487,6 +487,28 @@ def test_training_gain_and_exact_seeded_traversal(gpu):
     P.seed_max_t(pred, packed, cut, rays, n_ev, top, t_seed, maxt2, err)
     rta_amd.synchronize()
     assert int(err.download(np.uint32, 1)[0]) == 0
+
+    # what the trained network gains over the traversal without it (and
+    # over the untrained network): inner-node visits per ray, counted by the
+    # instrumented plain kernel under each max_t.  A tighter bound never
+    # adds a visit, so the comparison holds ray by ray.
+    maxt0 = DeviceBuffer(4 * n_ev)
+    P.seed_max_t(untrained, packed, cut, rays, n_ev, DeviceBuffer(16 * n_ev),
+                 DeviceBuffer(4 * n_ev), maxt0, err)
+
+    def visits(mt):
+        stats, scratch = DeviceBuffer(8 * n_ev), DeviceBuffer(16 * n_ev)
+        call("rta_reset_intersections", ptr(scratch), n_ev, 1, None)
+        call("rta_trace_cis", ptr(bvh.triangle_data), n, ptr(bvh.node_data), ptr(rays.gpu_origin),
+             ptr(rays.gpu_direction), ptr(mt), n_ev, 1, ptr(scratch), ptr(stats), ptr(err), None)
+        rta_amd.synchronize()
+        return stats.download(np.uint32, 2 * n_ev).reshape(-1, 2)[:, 0].astype(np.int64)
+
+    v_plain, v_untrained, v_trained = visits(rays.gpu_maxt), visits(maxt0), visits(maxt2)
+    say(f"inner-node visits per ray: plain {v_plain.mean():.2f}, untrained seeds "
+        f"{v_untrained.mean():.2f}, trained seeds {v_trained.mean():.2f}")
+    assert np.all(v_trained <= v_plain) and np.all(v_untrained <= v_plain)
+    assert v_trained.mean() < v_untrained.mean()
     seeds = t_seed.download(np.float32, n_ev)
     t2 = maxt2.download(np.float32, n_ev)
     hit = h_ev["t"] != F32MAX
