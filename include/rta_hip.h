@@ -96,6 +96,16 @@ enum {
 /* ---------------------------------------------------------------- runtime */
 
 const char *rta_hip_last_error(void);
+/* Error-path testing (no counterpart in the reference, whose CUDA failures
+ * print and exit, librta/cuda.h:70-78): the next call of `entry` returns
+ * `status` (non-zero) before doing any work, with the message "<entry>:
+ * injected fault"; one shot.  Entry points with this hook: rta_hip_malloc,
+ * rta_lbvh_build, rta_trace_cis_persistent, rta_pred_infer. */
+int rta_hip_inject_fault(const char *entry, int status);
+/* Named host-side ranges (roctx) for a profiler's timeline; the library
+ * marks rta_lbvh_build and the persistent traversals with them. */
+int rta_hip_range_push(const char *name);
+int rta_hip_range_pop(void);
 int rta_hip_device_count(int *count);
 int rta_hip_set_device(int device);
 int rta_hip_device_name(char *buf, int buflen);

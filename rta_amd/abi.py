@@ -79,6 +79,9 @@ _pi, _psz, _pf = C.POINTER(C.c_int), C.POINTER(C.c_size_t), C.POINTER(C.c_float)
 # name -> argtypes (every function returns int status)
 HIP_SIGNATURES = {
     "rta_hip_device_count": [_pi],
+    "rta_hip_inject_fault": [C.c_char_p, _i],
+    "rta_hip_range_push": [C.c_char_p],
+    "rta_hip_range_pop": [],
     "rta_hip_set_device": [_i],
     "rta_hip_device_name": [C.c_char_p, _i],
     "rta_hip_malloc": [C.POINTER(_vp), _sz],

@@ -44,6 +44,22 @@ int hip_status(hipError_t e, const char *what);
 
 int invalid(const char *msg);
 
+// a named profiler range (rta_hip_range_push/pop) for the enclosing scope
+struct ScopedRange {
+	explicit ScopedRange(const char *name) { rta_hip_range_push(name); }
+	~ScopedRange() { rta_hip_range_pop(); }
+	ScopedRange(const ScopedRange &) = delete;
+	ScopedRange &operator=(const ScopedRange &) = delete;
+};
+
+// true (and *status set) when a test armed a fault for this entry point
+bool take_fault(const char *entry, int *status);
+#define RTA_FAULT_POINT(entry)                                        \
+	do {                                                              \
+		int st__ = 0;                                                 \
+		if (::rta_hip::take_fault(entry, &st__)) return st__;         \
+	} while (0)
+
 inline hipStream_t as_stream(rta_stream s) { return reinterpret_cast<hipStream_t>(s); }
 
 inline unsigned grid_for(long long n, int block) {

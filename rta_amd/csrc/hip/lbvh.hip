@@ -269,6 +269,8 @@ int rta_lbvh_copy_ranges(uint32_t *ranges, const void *workspace, int n, rta_str
 int rta_lbvh_build(rta_triangle *tris, int n, rta_node *nodes, uint32_t *indices, void *workspace,
                    size_t workspace_bytes, uint32_t *status, rta_lbvh_times *times,
                    rta_stream s) {
+	RTA_FAULT_POINT("rta_lbvh_build");
+	const ScopedRange range("rta_lbvh_build");
 	// n = 0 is rejected, as the reference cannot build it (bbvh-cuda/lbvh.h:47)
 	if (n <= 0) return invalid("rta_lbvh_build: n must be > 0");
 	if (!tris || !nodes || !indices || !workspace) return invalid("rta_lbvh_build: NULL pointer");

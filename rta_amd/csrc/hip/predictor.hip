@@ -816,6 +816,7 @@ int rta_pred_forward(rta_pred h, const float *o, const float *d, int n, const fl
 int rta_pred_infer(rta_pred h, const float *o, const float *d, int n, const float *scene_min,
                    const float *scene_max, int32_t *top, int top_k, float *logdist,
                    float *logits, rta_stream s) {
+	RTA_FAULT_POINT("rta_pred_infer");
 	State *p = reinterpret_cast<State *>(h);
 	if (!p || !o || !d || !scene_min || !scene_max) return invalid("rta_pred_infer: NULL pointer");
 	if (!top && !logdist && !logits) return invalid("rta_pred_infer: no output requested");

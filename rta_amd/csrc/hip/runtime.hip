@@ -4,7 +4,11 @@
 // status code plus a per-thread message.
 #include "common.hpp"
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
+#include <atomic>
 #include <cstdio>
+#include <mutex>
 #include <string>
 
 namespace rta_hip {
@@ -21,6 +25,23 @@ int hip_status(hipError_t e, const char *what) {
 int invalid(const char *msg) {
 	g_last_error = msg;
 	return RTA_E_INVALID;
+}
+
+// One-shot fault injection (rta_hip_inject_fault): the armed entry point's
+// next call returns the chosen status before doing any work.
+static std::mutex g_fault_mutex;
+static std::atomic<int> g_fault_armed{0};
+static std::string g_fault_entry;
+static int g_fault_status = 0;
+
+bool take_fault(const char *entry, int *status) {
+	if (g_fault_armed.load(std::memory_order_relaxed) == 0) return false;
+	std::lock_guard<std::mutex> lock(g_fault_mutex);
+	if (g_fault_armed.load(std::memory_order_relaxed) == 0 || g_fault_entry != entry) return false;
+	g_fault_armed.store(0, std::memory_order_relaxed);
+	*status = g_fault_status;
+	g_last_error = std::string(entry) + ": injected fault";
+	return true;
 }
 
 __global__ void fill_u32_kernel(uint32_t *dst, uint32_t v, size_t n) {
@@ -47,6 +68,27 @@ extern "C" {
 
 const char *rta_hip_last_error(void) { return g_last_error.c_str(); }
 
+int rta_hip_range_push(const char *name) {
+	if (!name) return invalid("rta_hip_range_push: name is NULL");
+	roctxRangePushA(name);
+	return RTA_OK;
+}
+
+int rta_hip_range_pop(void) {
+	roctxRangePop();
+	return RTA_OK;
+}
+
+int rta_hip_inject_fault(const char *entry, int status) {
+	if (!entry || !*entry || status == RTA_OK)
+		return invalid("rta_hip_inject_fault: an entry point name and a non-zero status");
+	std::lock_guard<std::mutex> lock(g_fault_mutex);
+	g_fault_entry = entry;
+	g_fault_status = status;
+	g_fault_armed.store(1, std::memory_order_relaxed);
+	return RTA_OK;
+}
+
 int rta_hip_device_count(int *count) {
 	if (!count) return invalid("rta_hip_device_count: count is NULL");
 	RTA_HIP_TRY(hipGetDeviceCount(count));
@@ -70,6 +112,7 @@ int rta_hip_device_name(char *buf, int buflen) {
 }
 
 int rta_hip_malloc(void **ptr, size_t bytes) {
+	RTA_FAULT_POINT("rta_hip_malloc");
 	if (!ptr) return invalid("rta_hip_malloc: ptr is NULL");
 	*ptr = nullptr;
 	if (bytes == 0) return RTA_OK;

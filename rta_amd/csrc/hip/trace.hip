@@ -1092,6 +1092,7 @@ int trace_persistent(const char *name, const float *tri4, const rta_node_pair *p
                      const float *ray_orig, const float *ray_dir, const float *max_t, int n_rays,
                      const uint32_t *ray_count, rta_hit *hits, uint32_t *err, uint32_t *queue,
                      const rta_trace_config *cfg, hipStream_t st) {
+	const ScopedRange range(name);
 	if (!tri4 || !pairs || !flags || !ray_orig || !ray_dir || !max_t || !hits || !err ||
 	    !queue || n_rays <= 0) {
 		set_error(std::string(name) + ": bad arguments");
@@ -1213,6 +1214,7 @@ int rta_trace_cis_persistent(const float *tri4, const rta_node_pair *pairs,
                              const float *ray_orig, const float *ray_dir, const float *max_t,
                              int n_rays, const uint32_t *ray_count, rta_hit *hits, uint32_t *err,
                              uint32_t *queue, const rta_trace_config *cfg, rta_stream s) {
+	RTA_FAULT_POINT("rta_trace_cis_persistent");
 	return trace_persistent<false>("rta_trace_cis_persistent", tri4, pairs, flags, root_link,
 	                               root_index, ray_orig, ray_dir, max_t, n_rays, ray_count, hits,
 	                               err, queue, cfg, as_stream(s));

@@ -48,6 +48,31 @@ def test_argument_errors_need_no_gpu():
     assert isinstance(msg, bytes)
 
 
+def test_injected_fault_surfaces_as_an_error():
+    """The fault hook: the armed entry point's next call returns the injected
+    status before touching its arguments (so no GPU is needed), the Python
+    layer raises it with the message, other entry points are unaffected, and
+    the hook is one-shot."""
+    lib = rta_amd.hip()
+    fake = C.c_void_p(64)                         # never dereferenced on these paths
+    assert lib.rta_hip_inject_fault(b"rta_lbvh_build", 0) == abi.RTA_E_INVALID
+    assert lib.rta_hip_inject_fault(b"rta_lbvh_build", 701) == 0
+    # another entry point: its own argument check, not the fault
+    assert lib.rta_lbvh_workspace_bytes(0, None) == abi.RTA_E_INVALID
+    try:
+        abi.call("rta_lbvh_build", fake, 10, fake, fake, fake, C.c_size_t(1 << 20), None, None,
+                 None)
+    except abi.RtaError as e:
+        assert e.code == 701 and e.func == "rta_lbvh_build"
+        assert "injected fault" in str(e)
+    else:
+        raise AssertionError("the injected fault must raise")
+    # one shot: the next call runs its normal argument check (n = 0)
+    assert lib.rta_lbvh_build(fake, 0, fake, fake, fake, C.c_size_t(1 << 20), None, None,
+                              None) == abi.RTA_E_INVALID
+    assert b"n must be > 0" in lib.rta_hip_last_error()
+
+
 def test_library_is_built_for_gfx950():
     data = open(abi.HIP_LIB_PATH, "rb").read()
     assert b"gfx950" in data

@@ -57,6 +57,19 @@ def test_host_reports_missing_plugin():
     assert "cannot find plugin" in r.stderr
 
 
+def test_host_reports_plugin_missing_a_symbol(tmp_path):
+    """A library that loads but lacks the contract's symbols (the dlsym error
+    path of librta/plugins.cpp:20-28): the host names the failure and stops
+    before any GPU work."""
+    src = tmp_path / "partial.c"
+    src.write_text('char *description(void) { return "partial"; }\n')
+    lib = tmp_path / "partial.so"
+    subprocess.run(["gcc", "-shared", "-fPIC", "-o", str(lib), str(src)], check=True)
+    r = run_driver(str(lib), timeout=60)
+    assert r.returncode != 0
+    assert "initialize" in r.stderr or "parse_cmdline" in r.stderr or "create_rt_set" in r.stderr
+
+
 def test_host_rejects_bad_usage():
     r = run_driver("-r", "bogus", timeout=60)
     assert r.returncode == 2 and "usage" in r.stderr
