@@ -124,6 +124,32 @@ class HitPredictor:
         assert a.size == PARAM_COUNT
         call("rta_pred_set_params", self.h, ptr(a))
 
+    CHECKPOINT_FORMAT = "rta-hit-predictor-1"
+
+    def save(self, path: str) -> None:
+        """Weight checkpoint: the packed fp32 parameters, the scene box the
+        encoding uses and the Adam step count, as a numpy .npz (no pickled
+        objects).  Adam's moments are not kept: a loaded network infers
+        identically and fine-tunes from fresh moments."""
+        np.savez(path, format=np.array(self.CHECKPOINT_FORMAT), params=self.params(),
+                 scene_min=np.array(self.scene_min[:], np.float32),
+                 scene_max=np.array(self.scene_max[:], np.float32),
+                 steps=np.array(self.steps, np.int64))
+
+    @classmethod
+    def load(cls, path: str, max_batch: int) -> "HitPredictor":
+        """A predictor with the weights and scene box of a checkpoint."""
+        with np.load(path, allow_pickle=False) as z:
+            if str(z["format"]) != cls.CHECKPOINT_FORMAT:
+                raise ValueError(f"{path}: not a hit-predictor checkpoint")
+            params = np.asarray(z["params"], np.float32)
+            if params.size != PARAM_COUNT:
+                raise ValueError(f"{path}: {params.size} parameters, expected {PARAM_COUNT}")
+            p = cls(max_batch, scene_min=z["scene_min"], scene_max=z["scene_max"])
+            p.set_params(params)
+            p.steps = int(z["steps"])
+        return p
+
     def grads(self) -> np.ndarray:
         a = np.empty(PARAM_COUNT, np.float32)
         call("rta_pred_get_grads", self.h, ptr(a))
