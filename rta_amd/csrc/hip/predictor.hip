@@ -269,10 +269,11 @@ __global__ __launch_bounds__(256) void loss_kernel(uint16_t *__restrict__ dout,
 	}
 	for (int off = kWave / 2; off > 0; off >>= 1) se += __shfl_xor(se, off);
 	const float inv_m = 1.0f / (float)m;
+	const float inv_se = 1.0f / se;   // one division per row (softmax within 1 ulp)
 #pragma unroll
 	for (int j = 0; j < kClusters / kWave; ++j) {
 		const int c = lane + kWave * j;
-		g[c] = to_bf16((v[j] / se - (c == lab ? 1.f : 0.f)) * inv_m);
+		g[c] = to_bf16((v[j] * inv_se - (c == lab ? 1.f : 0.f)) * inv_m);
 	}
 	if (lane < kOutP - kClusters) {
 		const int c = kClusters + lane;
