@@ -450,26 +450,42 @@ __device__ inline void bottom_up_merge(rta_node *__restrict__ nodes, uint32_t *_
 		int g = 0;
 		while (prefix[g + 1] <= f) ++g;
 		uint32_t cur = in_slots[(size_t)g * kSlots + (f - prefix[g])];
+		if (cur == 0u) continue;
 		rta_aabb box = load_box(&nodes[cur]);
 		uint32_t code_cur = rep_code(codes, cur, leaf0);
-		uint32_t parent = cur != 0u ? parents[cur] : 0u;
-		// Three dependent rounds per level: (a) what earlier stages wrote
-		// and nothing changes before this thread does -- the parent's range,
-		// links and code and the grandparent; (b) the visit counter; (c)
-		// after its acquire, what the sibling subtree's thread wrote -- the
-		// sibling's box and the children's (possibly collapsed) links.
+		uint32_t parent = parents[cur];
+		// Per level: (a) what earlier stages wrote and nothing changes
+		// before this thread arrives -- a node's range, links, code and
+		// parent; (b) the visit counter; (c) after its acquire, what the
+		// sibling subtree's thread wrote -- the sibling's box and the
+		// children's (possibly collapsed) links.  (a) for the grandparent is
+		// issued before (b), so a level costs two dependent round trips.
+		// The grandparent's links change only when a thread arrives there
+		// second; this thread reads them before it could arrive, and uses
+		// them only if it does.
+		uint2 rg = ranges[parent];
+		uint2 pl = *reinterpret_cast<const uint2 *>(&nodes[parent]);
+		uint32_t grand = parent != 0u ? parents[parent] : 0u;
+		uint32_t code_parent = codes[parent];
 		for (int level = 0; cur != 0u && level < 128; ++level) {
-			const uint2 rg = ranges[parent];
-			const uint2 pl = *reinterpret_cast<const uint2 *>(&nodes[parent]);
-			const uint32_t grand = parent != 0u ? parents[parent] : 0u;
-			const uint32_t code_parent = codes[parent];
 			if (rg.x < lo || rg.y >= hi) {
 				const uint32_t slot = atomicAdd(&nfront, 1u);
 				if (slot < (uint32_t)kSlots) front[slot] = cur;
 				else atomicOr(status, 2u);
 				break;
 			}
-			const uint32_t old = __hip_atomic_fetch_add(&visit[parent], 1u, __ATOMIC_ACQ_REL,
+			uint2 rg_next = make_uint2(0u, 0u), pl_next = make_uint2(0u, 0u);
+			uint32_t grand_next = 0u, code_next = 0u;
+			if (parent != 0u) {
+				rg_next = ranges[grand];
+				pl_next = *reinterpret_cast<const uint2 *>(&nodes[grand]);
+				grand_next = grand != 0u ? parents[grand] : 0u;
+				code_next = codes[grand];
+			}
+			// acquire only: this thread's earlier box and link stores were
+			// released by the fence that ends the previous level (or by the
+			// launch boundary), so the prefetch above need not complete first
+			const uint32_t old = __hip_atomic_fetch_add(&visit[parent], 1u, __ATOMIC_ACQUIRE,
 			                                            __HIP_MEMORY_SCOPE_WORKGROUP);
 			if (old == 0u) break;
 			const uint32_t left = pl.x >> 1, right = pl.y;
@@ -490,6 +506,10 @@ __device__ inline void bottom_up_merge(rta_node *__restrict__ nodes, uint32_t *_
 			cur = parent;
 			parent = grand;
 			code_cur = code_parent;
+			rg = rg_next;
+			pl = pl_next;
+			grand = grand_next;
+			code_parent = code_next;
 		}
 	}
 	__syncthreads();
