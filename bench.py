@@ -893,6 +893,12 @@ def main():
             "traffic_source": traffic_src,
             "kernel": f"trace_cis_{args.trav}_kernel (bounce rays)",
             "bytes_per_ray": round(b_ray, 1), "V": round(V, 3), "T": round(T, 3),
+            # measured HBM bytes per launch over the same launch time
+            "traffic_gbs": (round(traffic / (avg["bounce"] * 1e-3) / 1e9, 1)
+                            if traffic else None),
+            "note": ("achieved counts every node and triangle fetch as HBM bytes; L2 and "
+                     "the last-level cache serve most re-fetches, so achieved can reach or "
+                     "pass the HBM peak while traffic_gbs is the rate HBM actually carries"),
         },
         "hit_predicted": hit_predicted,
         "cpu_baseline": cpu,

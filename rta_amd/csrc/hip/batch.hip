@@ -64,12 +64,15 @@ __global__ __launch_bounds__(256) void batch_karras_kernel(
     const BatchItem *__restrict__ items, const BatchMesh *__restrict__ meshes,
     rta_node *__restrict__ nodes, uint32_t *__restrict__ parents, uint32_t *__restrict__ visit,
     uint2 *__restrict__ ranges, const uint32_t *__restrict__ codes) {
+	__shared__ uint32_t window[kKarrasWindow];
 	const BatchItem it = items[blockIdx.x];
 	const BatchMesh m = meshes[it.mesh];
-	const int i = (int)it.block * 256 + (int)threadIdx.x;
+	const int base = (int)it.block * kKarrasBlock;
+	const CodeWindow w = stage_codes(window, codes + m.tri_off, (int)m.n, base);
+	const int i = base + (int)threadIdx.x;
 	if (i >= 2 * (int)m.n - 1) return;
-	karras_node(nodes + m.node_off, parents + m.node_off, visit + m.tri_off, ranges + m.tri_off,
-	            codes + m.tri_off, (int)m.n, i);
+	karras_node(nodes + m.node_off, parents + m.node_off, visit + m.tri_off, ranges + m.tri_off, w,
+	            (int)m.n, i);
 }
 
 __global__ __launch_bounds__(kChunk) void batch_chunk_kernel(
@@ -208,7 +211,7 @@ int rta_batch_create(int n_meshes, const int *n_tris, const int *n_rays, rta_bat
 	p->tri_blocks = items.size() - p->tri_blocks_off;
 	p->node_blocks_off = items.size();
 	for (int m = 0; m < n_meshes; ++m)
-		for (uint32_t b = 0; (long long)b * 256 < 2LL * p->meshes[m].n - 1; ++b)
+		for (uint32_t b = 0; (long long)b * kKarrasBlock < 2LL * p->meshes[m].n - 1; ++b)
 			items.push_back({(uint32_t)m, b, 0, 0});
 	p->node_blocks = items.size() - p->node_blocks_off;
 	p->chunk_items_off = items.size();
@@ -326,7 +329,7 @@ int rta_batch_build(rta_batch b, rta_triangle *tris, rta_node *nodes, uint32_t *
 	hipLaunchKernelGGL(batch_local_index_kernel, dim3(grid_for(N, 256)), dim3(256), 0, st,
 	                   indices, (long long)N);
 	RTA_LAUNCH_CHECK("batch_local_index_kernel");
-	hipLaunchKernelGGL(batch_karras_kernel, dim3((unsigned)p.node_blocks), dim3(256), 0, st,
+	hipLaunchKernelGGL(batch_karras_kernel, dim3((unsigned)p.node_blocks), dim3(kKarrasBlock), 0, st,
 	                   items + p.node_blocks_off, p.d_meshes, nodes, w.parents, w.visit, w.ranges,
 	                   w.codes);
 	RTA_LAUNCH_CHECK("batch_karras_kernel");

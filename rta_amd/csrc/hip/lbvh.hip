@@ -37,13 +37,16 @@ __global__ __launch_bounds__(256) void morton_kernel(
 	if (code == 0u && status) atomicOr(status, (uint32_t)RTA_LBVH_CODE_ZERO);
 }
 
-__global__ __launch_bounds__(256) void karras_kernel(
+__global__ __launch_bounds__(kKarrasBlock) void karras_kernel(
     rta_node *__restrict__ nodes, uint32_t *__restrict__ parents,
     uint32_t *__restrict__ visit, uint2 *__restrict__ ranges,
     const uint32_t *__restrict__ codes, int n) {
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	__shared__ uint32_t window[kKarrasWindow];
+	const int base = blockIdx.x * kKarrasBlock;
+	const CodeWindow w = stage_codes(window, codes, n, base);
+	const int i = base + threadIdx.x;
 	if (i >= 2 * n - 1) return;
-	karras_node(nodes, parents, visit, ranges, codes, n, i);
+	karras_node(nodes, parents, visit, ranges, w, n, i);
 }
 
 __global__ __launch_bounds__(kChunk) void bottom_up_chunk_kernel(
@@ -206,7 +209,7 @@ int rta_lbvh_build_tree(rta_node *nodes, uint32_t *parents, void *work, const ui
 	if (n <= 0) return invalid("rta_lbvh_build_tree: n must be > 0");
 	if (!nodes || !parents || !codes || !work) return invalid("rta_lbvh_build_tree: NULL pointer");
 	BoxesWork w = carve_boxes(work, n);
-	hipLaunchKernelGGL(karras_kernel, dim3(grid_for(2LL * n - 1, 256)), dim3(256), 0,
+	hipLaunchKernelGGL(karras_kernel, dim3(grid_for(2LL * n - 1, kKarrasBlock)), dim3(kKarrasBlock), 0,
 	                   as_stream(s), nodes, parents, w.visit, w.ranges, codes, n);
 	RTA_LAUNCH_CHECK("karras_kernel");
 	return RTA_OK;

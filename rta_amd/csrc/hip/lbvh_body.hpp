@@ -156,12 +156,43 @@ __device__ inline uint32_t morton_of(const rta_float3 *__restrict__ centers,
 
 // ----------------------------------------- stage 5: Karras hierarchy
 
+// The sorted codes of one mesh as the hierarchy searches read them:
+// positions [lo, hi) from a copy in LDS, the rest from global memory.  Most
+// probes of a node's searches fall within a few times its range length of
+// the node, so a block that stages its own positions plus a halo on each
+// side answers nearly all of them from LDS.
+constexpr int kKarrasBlock = 256;
+constexpr int kKarrasHalo = 256;   // 1024 measured slower: staging outweighs the probes saved
+constexpr int kKarrasWindow = kKarrasBlock + 2 * kKarrasHalo;
+struct CodeWindow {
+	const uint32_t *__restrict__ codes;
+	const uint32_t *lds;
+	int lo, hi;
+	__device__ uint32_t operator()(int j) const {
+		return (j >= lo && j < hi) ? lds[j - lo] : codes[j];
+	}
+};
+
+// Stages positions [base - halo, base + block + halo) of [0, n) in lds
+// (kKarrasWindow words) when the block holds inner nodes (base < n - 1);
+// every thread of the block must call it.
+__device__ inline CodeWindow stage_codes(uint32_t *lds, const uint32_t *__restrict__ codes, int n,
+                                         int base) {
+	CodeWindow w{codes, lds, 0, 0};
+	if (base >= n - 1) return w;
+	w.lo = base - kKarrasHalo > 0 ? base - kKarrasHalo : 0;
+	w.hi = base + kKarrasBlock + kKarrasHalo < n ? base + kKarrasBlock + kKarrasHalo : n;
+	for (int j = w.lo + (int)threadIdx.x; j < w.hi; j += kKarrasBlock) lds[j - w.lo] = codes[j];
+	__syncthreads();
+	return w;
+}
+
 // Common prefix length of the augmented keys (code, position):
 // clz(ci^cj), or 32+clz(i^j) for equal codes, -1 outside [0,n)
-// (__cpl, lbvh-kernels.cu:84-92).
-__device__ inline int cpl(int i, int j, const uint32_t *__restrict__ codes, int n) {
+// (__cpl, lbvh-kernels.cu:84-92); ci is the code of position i.
+__device__ inline int cpl(int i, uint32_t ci, int j, const CodeWindow &w, int n) {
 	if (j < 0 || j >= n) return -1;
-	const uint32_t ci = codes[i], cj = codes[j];
+	const uint32_t cj = w(j);
 	if (ci == cj) return __clz((int)((uint32_t)i ^ (uint32_t)j)) + 32;
 	return __clz((int)(ci ^ cj));
 }
@@ -173,28 +204,30 @@ __device__ inline int cpl(int i, int j, const uint32_t *__restrict__ codes, int 
 // counter and record their sorted range [from, to] for the bottom-up pass.
 __device__ inline void karras_node(rta_node *__restrict__ nodes, uint32_t *__restrict__ parents,
                                    uint32_t *__restrict__ visit, uint2 *__restrict__ ranges,
-                                   const uint32_t *__restrict__ codes, int n, int i) {
+                                   const CodeWindow &w, int n, int i) {
 	uint2 *links = reinterpret_cast<uint2 *>(&nodes[i]);
 	if (i >= n - 1) {
 		// leaf: tris = i-(n-1), elems = 1, leaf bit clear
 		*links = make_uint2(1u << 1, (uint32_t)(i - (n - 1)));
 		return;
 	}
-	int d = (cpl(i, i + 1, codes, n) - cpl(i, i - 1, codes, n)) >= 0 ? 1 : -1;
-	const int cpl_min = cpl(i, i - d, codes, n);
+	const uint32_t ci = w(i);
+	int d = (cpl(i, ci, i + 1, w, n) - cpl(i, ci, i - 1, w, n)) >= 0 ? 1 : -1;
+	const int cpl_min = cpl(i, ci, i - d, w, n);
 	int l_max = 2;
-	while (cpl(i, i + l_max * d, codes, n) > cpl_min) l_max *= 2;
+	while (cpl(i, ci, i + l_max * d, w, n) > cpl_min) l_max *= 2;
 	int l = 0;
 	for (int t = l_max / 2; t >= 1; t /= 2)
-		if (cpl(i, i + (l + t) * d, codes, n) > cpl_min) l += t;
+		if (cpl(i, ci, i + (l + t) * d, w, n) > cpl_min) l += t;
 	const int j = i + l * d;
 	const int from = j < i ? j : i;
 	const int to = j < i ? i : j;
 
-	const int cpl_node = cpl(from, to, codes, n);
+	const uint32_t cfrom = w(from);
+	const int cpl_node = cpl(from, cfrom, to, w, n);
 	int s = 0;
 	for (int t = (l / 2 + l % 2); t >= 1; t = (t / 2 + t % 2)) {
-		if (cpl(from, from + (s + t), codes, n) > cpl_node) s += t;
+		if (cpl(from, cfrom, from + (s + t), w, n) > cpl_node) s += t;
 		if (t <= 1) break;
 	}
 	const int mid = from + s;
