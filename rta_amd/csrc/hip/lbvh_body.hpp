@@ -41,7 +41,7 @@ __device__ inline void init_triangles_block(rta_triangle *__restrict__ scratch,
                                             rta_aabb *__restrict__ boxes,
                                             rta_float3 *__restrict__ centers,
                                             rta_aabb *__restrict__ partial, int n, int block) {
-	__shared__ uint2 words[kInitBlock * kTriWords];
+	__shared__ __align__(16) uint2 words[kInitBlock * kTriWords];
 	__shared__ float red[6][kInitBlock / kWave];
 
 	const int tid = threadIdx.x;
@@ -50,7 +50,24 @@ __device__ inline void init_triangles_block(rta_triangle *__restrict__ scratch,
 	const int nwords = count * kTriWords;
 	const uint2 *gsrc = reinterpret_cast<const uint2 *>(src + base);
 	uint2 *gdst = reinterpret_cast<uint2 *>(scratch + base);
-	for (int w = tid; w < nwords; w += kInitBlock) {
+	// 16-byte words when both ends are 16-byte aligned (a mesh of a batch
+	// may start at an odd triangle, 8 bytes off), then the odd last word
+	const bool wide = ((reinterpret_cast<uintptr_t>(gsrc) | reinterpret_cast<uintptr_t>(gdst)) &
+	                   15u) == 0u;
+	int w0 = 0;
+	if (wide) {
+		const uint4 *s4 = reinterpret_cast<const uint4 *>(gsrc);
+		uint4 *d4 = reinterpret_cast<uint4 *>(gdst);
+		uint4 *l4 = reinterpret_cast<uint4 *>(words);
+		const int n4 = nwords / 2;
+		for (int w = tid; w < n4; w += kInitBlock) {
+			uint4 v = s4[w];
+			l4[w] = v;
+			d4[w] = v;
+		}
+		w0 = 2 * n4;
+	}
+	for (int w = w0 + tid; w < nwords; w += kInitBlock) {
 		uint2 v = gsrc[w];
 		words[w] = v;
 		gdst[w] = v;
