@@ -17,7 +17,7 @@ constexpr int kBits = 8;
 constexpr int kBins = 1 << kBits;
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / kWave;
-constexpr int kItems = 16;
+constexpr int kItems = 8;
 constexpr int kTile = kThreads * kItems;
 constexpr int kPasses = 4;  // 32 bits: covers the 30-bit codes
 
