@@ -32,8 +32,8 @@ __device__ inline float wave_max(float v) {
 // ------------------------------------------------ stage 1: initTriangles
 
 // One block of kInitBlock threads handles triangles [block*256, +256) of a
-// mesh (26,624 contiguous bytes): a coalesced 8-byte-word copy src ->
-// scratch staged through LDS, then each thread forms its triangle's box
+// mesh (26,624 contiguous bytes): a coalesced copy src -> scratch staged
+// through LDS (16-byte words where aligned), then each thread forms its triangle's box
 // (lbvh-kernels.cu:141-153) and centroid ((a+b)+c)/3.0f (lbvh-kernels.cu:
 // 60-63) from LDS, and the block reduces its boxes into one partial box.
 __device__ inline void init_triangles_block(rta_triangle *__restrict__ scratch,
