@@ -527,9 +527,12 @@ int rta_trace_seed_treelets(const float *tri4, const rta_node_pair *pairs, const
 /* The seeding and the exact traversal in one persistent launch: ray i first
  * traverses treelet top[4*i] (when it is < *count and the ray meets its root
  * box within max_t[i]) up to the first leaf that holds a hit, then the whole
- * tree with max_t' = min(max_t[i], the smallest t of that phase) from a
- * closest hit of FLT_MAX, as rta_trace_cis_persistent.  Any hit of the ray
- * bounds its closest one, so the records are the plain traversal's. */
+ * tree with max_t' = min(max_t[i], s + s * 2^-10), s the smallest t of that
+ * phase, from a closest hit of FLT_MAX, as rta_trace_cis_persistent.  Any
+ * hit of the ray bounds its closest one; the 2^-10 margin (as in
+ * rta_seed_max_t) covers boxes whose computed entry distance exceeds the
+ * computed t of a triangle inside them, so the records are the plain
+ * traversal's. */
 int rta_trace_cis_persistent_seeded(const float *tri4, const rta_node_pair *pairs,
                                     const uint32_t *flags, uint32_t root_link,
                                     uint32_t root_index, const float *ray_orig,
@@ -538,7 +541,11 @@ int rta_trace_cis_persistent_seeded(const float *tri4, const rta_node_pair *pair
                                     const uint32_t *count, const int32_t *top, rta_hit *hits,
                                     uint32_t *err, uint32_t *queue, const rta_trace_config *cfg,
                                     rta_stream s);
-/* out[i] = min(max_t[i], t_seed[i]): the max_t a seeded traversal runs with. */
+/* out[i] = min(max_t[i], t_seed[i] + t_seed[i] * 2^-10): the max_t a seeded
+ * traversal runs with.  The seed (the t of a real hit) is widened by 2^-10
+ * relative because a box holding the closest triangle can have a computed
+ * entry distance slightly beyond that triangle's computed t; pruning at the
+ * seed itself could lose the closest hit. */
 int rta_seed_max_t(float *out, const float *max_t, const float *t_seed, int n, rta_stream s);
 
 #ifdef __cplusplus

@@ -50,7 +50,7 @@ def lib() -> C.CDLL:
             "oracle_median_build": (i, [vp, i, vp, vp, vp]),
             "oracle_trace_cis": (d, [vp, vp, vp, vp, vp, i, i, vp, vp, vp, i]),
             "oracle_trace_dis": (d, [vp, vp, vp, vp, vp, i, i, vp, i]),
-            "oracle_trace_kernel": (i, [vp, vp, vp, vp, vp, i, vp, i, vp, vp, i]),
+            "oracle_trace_kernel": (i, [vp, vp, vp, vp, vp, i, vp, i, vp, vp, vp, i]),
             "oracle_trace_brute": (None, [vp, i, vp, vp, i, vp]),
             "oracle_diffuse_bounce": (i, [vp, vp, vp, vp, i, u64, vp, vp, vp, vp]),
             "oracle_shadow_rays": (i, [vp, vp, vp, vp, i, vp, vp, vp, vp, vp]),
@@ -139,12 +139,13 @@ def trace_cis(tris, nodes, o, d, t, w, h=1, stats=False, threads=0, first=False)
 
 
 def trace_kernel(tris, nodes, o, d, t, any_hit=False, hits_in=None, threads=0, bound=None,
-                 stats=False):
+                 stats=False, depth=False):
     """The GPU tracers' semantics (kernels.cu:777-855, shadow :857-939).
     hits_in: the records the rays start from (default: the reset sentinel).
     bound: optional per-ray inclusive bound on child-box entry distances.
-    Returns (hits, rays that needed more than 32 stack entries) and, with
-    stats=True, per-ray (inner-node visits, triangle tests)."""
+    Returns (hits, rays that needed more than 32 stack entries), then with
+    stats=True per-ray (inner-node visits, triangle tests), then with
+    depth=True each ray's largest number of simultaneous stack entries."""
     n = len(t)
     if hits_in is None:
         hits = np.zeros(n, HIT_DTYPE)
@@ -155,11 +156,15 @@ def trace_kernel(tris, nodes, o, d, t, any_hit=False, hits_in=None, threads=0, b
     o, d, t = _f32(o), _f32(d), _f32(t)
     b = None if bound is None else _f32(bound)
     st = np.zeros(2 * n, np.uint32) if stats else None
+    dp = np.zeros(n, np.uint32) if depth else None
     deep = lib().oracle_trace_kernel(_p(tris), _p(nodes), _p(o), _p(d), _p(t), n, _p(hits),
-                                     int(bool(any_hit)), _p(b), _p(st), threads)
+                                     int(bool(any_hit)), _p(b), _p(st), _p(dp), threads)
+    out = (hits, int(deep))
     if stats:
-        return hits, int(deep), st.reshape(-1, 2)
-    return hits, int(deep)
+        out += (st.reshape(-1, 2),)
+    if depth:
+        out += (dp,)
+    return out
 
 
 def trace_dis(tris, nodes, o, d, t, w, h=1, threads=0):

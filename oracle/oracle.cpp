@@ -626,10 +626,12 @@ double oracle_trace_dis(const rta_triangle *tris, const rta_node *nodes, const f
 // the shadow form stops at the first kept triangle and then ends the ray
 // (:918-931).  The reference's stack has 32 entries and no check; here the
 // stack is larger and rays that would have needed more than 32 entries are
-// counted in the return value.  hits: in/out.
+// counted in the return value; depth (optional) receives each ray's largest
+// number of simultaneous stack entries.  hits: in/out.
 int oracle_trace_kernel(const rta_triangle *tris, const rta_node *nodes, const float *ro,
                         const float *rd, const float *rmax, int n_rays, rta_hit *hits,
-                        int any_hit, const float *bound, uint32_t *stats, int threads) {
+                        int any_hit, const float *bound, uint32_t *stats, uint32_t *depth,
+                        int threads) {
 	if (threads > 0) omp_set_num_threads(threads);
 	int overflow = 0;
 #pragma omp parallel for schedule(dynamic, 64) reduction(+ : overflow)
@@ -643,7 +645,7 @@ int oracle_trace_kernel(const rta_triangle *tris, const rta_node *nodes, const f
 		uint32_t n_inner = 0, n_tests = 0;
 		rta_hit closest = hits[id];
 		closest.t = FLT_MAX;
-		int sp = -1;
+		int sp = -1, max_sp = -1;
 		bool deep = false;
 		const rta_node *curr = &nodes[0];
 		while (true) {
@@ -667,6 +669,7 @@ int oracle_trace_kernel(const rta_triangle *tris, const rta_node *nodes, const f
 						stack[++sp] = l;
 						curr = &nodes[r];
 					}
+					if (sp > max_sp) max_sp = sp;
 				} else if (hit_l) {
 					curr = &nodes[l];
 				} else if (hit_r) {
@@ -703,6 +706,7 @@ int oracle_trace_kernel(const rta_triangle *tris, const rta_node *nodes, const f
 			stats[2 * (size_t)id] = n_inner;
 			stats[2 * (size_t)id + 1] = n_tests;
 		}
+		if (depth) depth[id] = (uint32_t)(max_sp + 1);
 		if (deep) ++overflow;
 	}
 	return overflow;

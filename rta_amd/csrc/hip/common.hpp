@@ -221,6 +221,17 @@ __device__ inline bool tri_test(float ax, float ay, float az,
 	return false;
 }
 
+// The bound on child-box entry distances that a seed imposes: a seed s is
+// the t of a real hit of the ray, so the closest hit lies at t <= s, but a
+// box holding that closest triangle can have a computed entry distance a
+// little beyond its computed t (the slab quotients and Cramer's rule round
+// differently; on planar triangles met at grazing angles the excess
+// reaches about 2^-16.3 relative).  Pruning at s itself would then lose the
+// closest hit; pruning beyond s + s * 2^-10 keeps a margin of 2^-10
+// relative.  s + s * 2^-10 >= s for every s > 0, and FLT_MAX widens to
+// +inf (no bound).
+__device__ __host__ inline float seed_bound(float s) { return s + s * 0x1p-10f; }
+
 // bbvh_node link word accessors (bbvh/bbvh-node.h:18-31)
 __device__ __host__ inline bool link_inner(uint32_t w) { return (w & 1u) == 1u; }
 __device__ __host__ inline uint32_t link_value(uint32_t w) { return w >> 1; }

@@ -593,7 +593,7 @@ __global__ __launch_bounds__(kTraceBlock) void seed_treelets_kernel(
 __global__ void seed_max_t_kernel(float *__restrict__ out, const float *__restrict__ max_t,
                                   const float *__restrict__ t_seed, int n) {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i < n) out[i] = fminf(max_t[i], t_seed[i]);
+	if (i < n) out[i] = fminf(max_t[i], seed_bound(t_seed[i]));
 }
 
 // ---------------------------- persistent packed CIS with ray refill
@@ -779,14 +779,18 @@ __global__ __launch_bounds__(BLOCK) void trace_cis_persistent_kernel(
 				// tie, and push the other; one kept -> enter it; none -> pop,
 				// or finish on an empty stack.  The far child is written one
 				// slot above the top on every step (after the top is read);
-				// only a push makes that slot part of the stack.  A push onto
-				// a full stack ends the ray with an error count.
+				// only a push makes that slot part of the stack.  On a full
+				// stack (kStack entries) there is no slot above the top: the
+				// top entry is written back with its own value, so a step that
+				// keeps one child or none leaves the stack intact, and a push
+				// ends the ray with an error count.
 				const bool both = hl && hr, any = hl || hr;
 				const bool enter_left = both ? (dl <= dr) : hl;
+				const bool full = sp + 1 >= kStack;
 				const uint32_t popped = stack[sp >= 0 ? sp : 0][lt];
-				stack[sp + 1 < kStack ? sp + 1 : kStack - 1][lt] = enter_left ? wr : wl;
+				stack[full ? kStack - 1 : sp + 1][lt] = full ? popped : (enter_left ? wr : wl);
 				curr = any ? (enter_left ? wl : wr) : popped;
-				overflow = both && sp + 1 >= kStack;
+				overflow = both && full;
 				done = overflow || (!any && sp < 0);
 				sp += both ? 1 : (any ? 0 : -1);
 			} else {
@@ -818,8 +822,9 @@ __global__ __launch_bounds__(BLOCK) void trace_cis_persistent_kernel(
 				if (overflow) atomicAdd(err, 1u);
 				if (SEEDED && seeding) {
 					// the treelet phase ends: its closest t (a real hit, or
-					// FLT_MAX) bounds the traversal of the whole tree
-					tmax = fminf(tmax, ct);
+					// FLT_MAX), widened by seed_bound's margin, bounds the
+					// traversal of the whole tree
+					tmax = fminf(tmax, seed_bound(ct));
 					ct = FLT_MAX;
 					curr = root_word;
 					sp = -1;

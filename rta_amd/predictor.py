@@ -5,9 +5,14 @@ The tree is cut into at most 1024 treelets (subtrees; rta_treelet_cut).  A
 ray's label is the treelet holding its closest hit and its distance target
 log(t).  Inference proposes the 4 best treelets per ray;
 rta_trace_seed_treelets finds the closest hit among their triangles, whose t
-becomes the ray's max_t for the exact traversal.  Every seed is the t of a
-real hit, so the exact traversal sees the closest hit inside its range.
+s, widened to s + s * 2^-10, becomes the ray's max_t for the exact
+traversal.  Every seed is the t of a real hit, so the closest hit lies at
+t <= s; the margin covers the boxes around it, whose computed entry
+distance can exceed its computed t by rounding (up to about 2^-16.3
+relative on planar triangles met at grazing angles), so the exact traversal
+still enters them.
 """
+
 from __future__ import annotations
 
 import ctypes as C
@@ -17,6 +22,17 @@ import numpy as np
 
 from .abi import HIT_DTYPE, TREELET_DTYPE, call, ptr
 from .device import DeviceBuffer, Stream
+
+SEED_MARGIN = np.float32(2.0 ** -10)
+
+
+def seed_bound(t_seed: np.ndarray) -> np.ndarray:
+    """The box-pruning bound a seed imposes, s + s * 2^-10 in float32 (the
+    kernels' seed_bound); FLT_MAX widens to +inf."""
+    s = np.asarray(t_seed, dtype=np.float32)
+    with np.errstate(over="ignore"):
+        return (s + s * SEED_MARGIN).astype(np.float32)
+
 
 N_CLUSTERS = 1024
 TOP = 4
@@ -204,7 +220,7 @@ def seed_max_t(pred: HitPredictor, packed, cut: TreeletCut, rays, n: int, top: D
                stream=None, candidates: int = TOP) -> None:
     """Inference (fused kernel; top-1 when one candidate is traversed, else
     top-4), the traversal of the first `candidates` proposed treelets, and
-    max_t' = min(max_t, seed) for the n rays.  `packed` is a PackedBvh
+    max_t' = min(max_t, seed + seed * 2^-10) for the n rays.  `packed` is a PackedBvh
     (pairs, tri4, flags)."""
     sp = ptr(stream) if stream else None
     pred.infer(rays.gpu_origin, rays.gpu_direction, n, top=top, stream=stream,
@@ -243,5 +259,6 @@ def top4_accuracy(top: np.ndarray, label: np.ndarray) -> float:
     return float((top[hit] == label[hit, None]).any(axis=1).mean())
 
 
-__all__ = ["HitPredictor", "TreeletCut", "seed_max_t", "trace_seeded", "top4_accuracy", "unpack",
+__all__ = ["HitPredictor", "TreeletCut", "seed_bound", "seed_max_t", "trace_seeded",
+           "top4_accuracy", "unpack",
            "param_count", "PARAM_COUNT", "N_CLUSTERS", "TOP", "HIT_DTYPE", "TREELET_DTYPE"]

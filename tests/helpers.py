@@ -135,3 +135,31 @@ def exact_cramer(a, b, c, o, d):
 def tri_boxes(tris):
     v = np.stack([tris["a"], tris["b"], tris["c"]], axis=1)
     return v.min(axis=1), v.max(axis=1)
+
+
+# ------------------------------------------------ planar scenes for seeding
+
+def planar_scene(n_random: int = 30_000, seed: int = 5, z: float = 0.3):
+    """Random small triangles (edges 1e-4..0.02) all lying in the plane
+    z = `z`, plus a flat 64x64 grid in the same plane: every leaf box has
+    zero thickness in z, so its entry distance comes from one slab quotient
+    while a triangle's t comes from Cramer's rule, and the two round
+    differently."""
+    from rta_amd import scenes
+    r = scenes.random_triangles(n_random, seed=seed, edge_min=1e-4, edge_max=0.02)
+    for k in ("a", "b", "c"):
+        r[k][:, 2] = np.float32(z)
+    g = scenes.heightfield(64, 64, z0=z, amplitude=0.0, seed=1)
+    return np.concatenate([r, g])
+
+
+def grazing_rays(n: int, seed: int = 1, z: float = 0.3):
+    """Rays from just above the plane z = `z`, heading down at slopes of
+    1e-4..1 (many of them nearly parallel to the plane)."""
+    rng = np.random.default_rng(seed)
+    o = np.column_stack([rng.uniform(0, 1, n), rng.uniform(0, 1, n),
+                         np.full(n, z) + rng.uniform(1e-4, 1e-2, n)]).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    d[:, 2] = -np.abs(d[:, 2]) * rng.choice([1e-4, 1e-3, 1e-2, 0.1, 1.0], n)
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    return o.reshape(-1), d.reshape(-1), np.full(n, np.finfo(np.float32).max, np.float32)

@@ -576,3 +576,79 @@ def test_training_gain_and_exact_seeded_traversal(gpu):
         assert np.array_equal(bits(got[f][hit]), bits(h_ev[f][hit])), f
     assert np.array_equal(got["ref"][hit], h_ev["ref"][hit])
     say("fused seeded persistent traversal: records equal the plain traversal's")
+
+
+@pytest.mark.gpu
+def test_seeded_traversal_exact_on_planar_scene(gpu):
+    """Planar triangles met at grazing angles, where a box's computed entry
+    distance can exceed the computed t of a triangle inside it: with
+    perfect proposals (each ray's own closest-hit treelet, the strongest
+    seeds there are) both seeded paths -- the seeding launch followed by the
+    traversal under rta_seed_max_t's bound, and the persistent seeded
+    traversal -- return the plain records, which equal the oracle's.  The
+    oracle test of the same scene shows that a bound at the seed itself
+    would lose closest hits there."""
+    import oracle
+    from helpers import grazing_rays, planar_scene
+
+    tris = planar_scene()
+    n = len(tris)
+    ob = oracle.lbvh_build(tris)
+    o, d, t = grazing_rays(200_000)
+    ref, _ = oracle.trace_kernel(ob["tris"], ob["nodes"], o, d, t)
+    ctor = rta_amd.LbvhConstructor()
+    bvh = ctor.build(DeviceBuffer.from_array(tris), n)
+    packed = bvh.packed()
+    cut = P.TreeletCut(packed, ctor.ranges(), n)
+    rays = rta_amd.RayBuffers.from_host(o, d, t)
+    m = len(t)
+
+    def plain(maxt):
+        b = rta_amd.GpuBouncer(m)
+        tr = rta_amd.CisTracer(rta_amd.RayBuffers.from_host(o, d, maxt), bvh, b)
+        tr.prepare_trace(); tr.launch(); rta_amd.synchronize(); tr.check_errors()
+        return b.intersections()
+
+    h = plain(t)
+    hit = ref["t"] != F32MAX
+    say(f"planar: {n} triangles, {m} grazing rays, {int(hit.sum())} hits")
+    for f in ("t", "beta", "gamma"):
+        assert np.array_equal(bits(h[f][hit]), bits(ref[f][hit])), f
+    assert np.array_equal(h["ref"][hit], ref["ref"][hit])
+    assert np.all(h["t"][~hit] == F32MAX)
+
+    lab, _ = cut.labels_host(DeviceBuffer.from_array(h), m)
+    top = np.full((m, 4), -1, np.int32)
+    top[:, 0] = lab                        # -1 (a miss) is never a treelet
+    top_d = DeviceBuffer.from_array(top)
+    err = DeviceBuffer(4)
+    err.fill_u32(0)
+    t_seed, maxt2 = DeviceBuffer(4 * m), DeviceBuffer(4 * m)
+    call("rta_trace_seed_treelets", ptr(packed.tri4), ptr(packed.pairs), ptr(packed.flags),
+         ptr(cut.table), ptr(cut.count), ptr(rays.gpu_origin), ptr(rays.gpu_direction),
+         ptr(rays.gpu_maxt), m, ptr(top_d), 1, ptr(t_seed), ptr(err), None)
+    call("rta_seed_max_t", ptr(maxt2), ptr(rays.gpu_maxt), ptr(t_seed), m, None)
+    rta_amd.synchronize()
+    assert int(err.download(np.uint32, 1)[0]) == 0
+    seeds = t_seed.download(np.float32, m)
+    # the perfect proposal yields the closest hit itself as the seed
+    assert np.array_equal(seeds[hit], h["t"][hit]) and np.all(seeds[~hit] == F32MAX)
+    mt = maxt2.download(np.float32, m)
+    assert np.array_equal(bits(mt), bits(np.minimum(t, P.seed_bound(seeds))))
+    h2 = plain(mt)
+    for f in ("t", "beta", "gamma", "ref"):
+        assert np.array_equal(bits(h2[f][hit]), bits(h[f][hit])), f
+
+    hits = DeviceBuffer(16 * m)
+    call("rta_reset_intersections", ptr(hits), m, 1, None)
+    queue = DeviceBuffer(4)
+    call("rta_trace_cis_persistent_seeded", ptr(packed.tri4), ptr(packed.pairs),
+         ptr(packed.flags), packed.root_link, packed.root_index, ptr(rays.gpu_origin),
+         ptr(rays.gpu_direction), ptr(rays.gpu_maxt), m, None, ptr(cut.table), ptr(cut.count),
+         ptr(top_d), ptr(hits), ptr(err), ptr(queue), None, None)
+    rta_amd.synchronize()
+    assert int(err.download(np.uint32, 1)[0]) == 0
+    h3 = hits.download(rta_amd.HIT_DTYPE, m)
+    for f in ("t", "beta", "gamma", "ref"):
+        assert np.array_equal(bits(h3[f][hit]), bits(h[f][hit])), f
+    assert np.all(h3["t"][~hit] == F32MAX)
