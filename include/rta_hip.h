@@ -88,9 +88,18 @@ enum {
 
 /* Build status bits written by rta_lbvh_build into *status (device word). */
 enum {
-	RTA_LBVH_CODE_ZERO = 1u /* some Morton code is 0: the reference's
-	                           bottom-up pass is racy there
-	                           (bbvh-cuda/lbvh-kernels.cu:288-291)        */
+	RTA_LBVH_CODE_ZERO = 1u, /* some Morton code is 0: the reference's
+	                            bottom-up pass is racy there
+	                            (bbvh-cuda/lbvh-kernels.cu:288-291)       */
+	RTA_LBVH_FRONTIER_OVERFLOW = 2u
+	                         /* a bottom-up frontier list (128 subtree
+	                            roots per 1024-leaf chunk) overflowed, so
+	                            some upper boxes were not merged.  Cannot
+	                            happen: each tree level lengthens the
+	                            common prefix of the 64-bit (code,
+	                            position) keys, so a tree has at most 64
+	                            inner levels and a chunk at most 2 frontier
+	                            roots per level.  Reported, not assumed.  */
 };
 
 /* ---------------------------------------------------------------- runtime */
@@ -227,6 +236,7 @@ int rta_lbvh_workspace_bytes(int n, size_t *bytes);
 int rta_lbvh_build(rta_triangle *tris, int n, rta_node *nodes,
                    uint32_t *indices, void *workspace, size_t workspace_bytes,
                    uint32_t *status, rta_lbvh_times *times, rta_stream s);
+
 
 /* --------------------------------------------------------------- tracing */
 

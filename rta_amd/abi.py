@@ -46,6 +46,7 @@ RTA_E_WORKSPACE = -2
 RTA_E_NOT_READY = -3
 RTA_E_STACK = -4
 RTA_LBVH_CODE_ZERO = 1
+RTA_LBVH_FRONTIER_OVERFLOW = 2
 
 
 class RtaTimes(C.Structure):

@@ -21,8 +21,8 @@ import time
 
 import numpy as np
 
-from .abi import (FLT_MAX, HIT_DTYPE, NODE_DTYPE, RTA_E_STACK, TRI_DTYPE, RtaError, RtaTimes,
-                  TraceConfig, call, ptr)
+from .abi import (FLT_MAX, HIT_DTYPE, NODE_DTYPE, RTA_E_INVALID, RTA_E_STACK,
+                  RTA_LBVH_FRONTIER_OVERFLOW, TRI_DTYPE, RtaError, RtaTimes, TraceConfig, call, ptr)
 from .device import DeviceBuffer
 
 
@@ -134,6 +134,9 @@ class LbvhConstructor:
         if timed:
             self.last_times = times.as_dict()
             self.last_status = int(self._status.download(np.uint32, 1)[0])
+            if self.last_status & RTA_LBVH_FRONTIER_OVERFLOW:
+                raise RtaError(RTA_E_INVALID, "rta_lbvh_build",
+                               "bottom-up frontier overflow: upper boxes not merged")
             if self.verbose:
                 print("lbvh stage times [ms]:", self.last_times)
         return BinaryLbvh(tris, n, self._nodes, self._idx)

@@ -170,6 +170,8 @@ public:
 		                     workspace_bytes, d_status, &last_times, 0),
 		      "rta_lbvh_build");
 		check(rta_hip_memcpy_dtoh(&last_status, d_status, 4), "status");
+		if (last_status & RTA_LBVH_FRONTIER_OVERFLOW)
+			throw std::runtime_error("rta_lbvh_build: bottom-up frontier overflow, upper boxes not merged");
 		if (verbose) {
 			std::cout << "LBVH build of " << n << " triangles [ms]: init " << last_times.init_triangles
 			          << ", bounds " << last_times.voxel_bounds << ", morton " << last_times.morton

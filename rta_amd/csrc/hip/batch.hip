@@ -307,8 +307,11 @@ int rta_batch_build(rta_batch b, rta_triangle *tris, rta_node *nodes, uint32_t *
 	hipStream_t st = as_stream(s);
 	const BatchItem *items = p.d_items;
 	const int N = (int)p.total_tris;
-	if (status) {
-		const int r = rta_hip_fill_u32(status, 0u, 1, s);
+	// the bottom-up stages OR RTA_LBVH_FRONTIER_OVERFLOW into the caller's
+	// word, or into the workspace's own when there is none
+	uint32_t *bu_status = status ? status : w.bu_status;
+	{
+		const int r = rta_hip_fill_u32(bu_status, 0u, 1, s);
 		if (r != RTA_OK) return r;
 	}
 	hipLaunchKernelGGL(batch_init_kernel, dim3((unsigned)p.tri_blocks), dim3(kInitBlock), 0, st,
@@ -335,7 +338,7 @@ int rta_batch_build(rta_batch b, rta_triangle *tris, rta_node *nodes, uint32_t *
 	RTA_LAUNCH_CHECK("batch_karras_kernel");
 	hipLaunchKernelGGL(batch_chunk_kernel, dim3((unsigned)p.chunk_items), dim3(kChunk), 0, st,
 	                   items + p.chunk_items_off, p.d_meshes, nodes, w.boxes, w.codes, indices,
-	                   w.parents, w.ranges, w.slots[0], w.counts[0], w.bu_status);
+	                   w.parents, w.ranges, w.slots[0], w.counts[0], bu_status);
 	RTA_LAUNCH_CHECK("batch_chunk_kernel");
 	long long width = kChunk;
 	for (size_t L = 0; L < p.merge_off.size(); ++L) {
@@ -344,7 +347,7 @@ int rta_batch_build(rta_batch b, rta_triangle *tris, rta_node *nodes, uint32_t *
 		hipLaunchKernelGGL(batch_merge_kernel, dim3((unsigned)p.merge_len[L]), dim3(1024), 0, st,
 		                   items + p.merge_off[L], p.d_meshes, nodes, w.visit, w.codes, w.parents,
 		                   w.ranges, w.slots[in], w.counts[in], w.slots[outb], w.counts[outb],
-		                   w.bu_status, width);
+		                   bu_status, width);
 		RTA_LAUNCH_CHECK("batch_merge_kernel");
 	}
 	hipLaunchKernelGGL(batch_permute_kernel, dim3((unsigned)p.tri_blocks), dim3(256), 0, st,

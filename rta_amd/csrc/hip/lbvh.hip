@@ -209,23 +209,28 @@ int rta_lbvh_build_tree(rta_node *nodes, uint32_t *parents, void *work, const ui
 	if (n <= 0) return invalid("rta_lbvh_build_tree: n must be > 0");
 	if (!nodes || !parents || !codes || !work) return invalid("rta_lbvh_build_tree: NULL pointer");
 	BoxesWork w = carve_boxes(work, n);
+	// the bottom-up stage ORs RTA_LBVH_FRONTIER_OVERFLOW into this word
+	const int rc = rta_hip_fill_u32(w.status, 0u, 1, s);
+	if (rc != RTA_OK) return rc;
 	hipLaunchKernelGGL(karras_kernel, dim3(grid_for(2LL * n - 1, kKarrasBlock)), dim3(kKarrasBlock), 0,
 	                   as_stream(s), nodes, parents, w.visit, w.ranges, codes, n);
 	RTA_LAUNCH_CHECK("karras_kernel");
 	return RTA_OK;
 }
 
-int rta_lbvh_build_boxes(rta_node *nodes, void *work, const rta_aabb *boxes,
-                         const uint32_t *codes, const uint32_t *indices,
-                         const uint32_t *parents, int n, rta_stream s) {
-	if (n <= 0) return invalid("rta_lbvh_build_boxes: n must be > 0");
-	if (!nodes || !boxes || !codes || !indices || !parents || !work)
-		return invalid("rta_lbvh_build_boxes: NULL pointer");
+}  // extern "C"
+
+// The bottom-up stage; a frontier overflow sets RTA_LBVH_FRONTIER_OVERFLOW
+// in *status.
+static int build_boxes(rta_node *nodes, void *work, const rta_aabb *boxes, const uint32_t *codes,
+                       const uint32_t *indices, const uint32_t *parents, int n, uint32_t *status,
+                       rta_stream s) {
 	BoxesWork w = carve_boxes(work, n);
+	if (!status) status = w.status;
 	hipStream_t st = as_stream(s);
 	int groups = (int)grid_for(n, kChunk);
 	hipLaunchKernelGGL(bottom_up_chunk_kernel, dim3(groups), dim3(kChunk), 0, st, nodes, boxes,
-	                   codes, indices, parents, w.ranges, w.slots[0], w.counts[0], w.status, n);
+	                   codes, indices, parents, w.ranges, w.slots[0], w.counts[0], status, n);
 	RTA_LAUNCH_CHECK("bottom_up_chunk_kernel");
 	long long chunk = kChunk;
 	int in = 0;
@@ -235,12 +240,23 @@ int rta_lbvh_build_boxes(rta_node *nodes, void *work, const rta_aabb *boxes,
 		hipLaunchKernelGGL(bottom_up_merge_kernel, dim3(blocks), dim3(1024), 0, st, nodes,
 		                   w.visit, codes, parents, w.ranges, w.slots[in], w.counts[in], groups,
 		                   blocks > 1 ? w.slots[1 - in] : nullptr,
-		                   blocks > 1 ? w.counts[1 - in] : nullptr, w.status, chunk, n);
+		                   blocks > 1 ? w.counts[1 - in] : nullptr, status, chunk, n);
 		RTA_LAUNCH_CHECK("bottom_up_merge_kernel");
 		groups = blocks;
 		in = 1 - in;
 	}
 	return RTA_OK;
+}
+
+extern "C" {
+
+int rta_lbvh_build_boxes(rta_node *nodes, void *work, const rta_aabb *boxes,
+                         const uint32_t *codes, const uint32_t *indices,
+                         const uint32_t *parents, int n, rta_stream s) {
+	if (n <= 0) return invalid("rta_lbvh_build_boxes: n must be > 0");
+	if (!nodes || !boxes || !codes || !indices || !parents || !work)
+		return invalid("rta_lbvh_build_boxes: NULL pointer");
+	return build_boxes(nodes, work, boxes, codes, indices, parents, n, nullptr, s);
 }
 
 int rta_lbvh_permute(rta_triangle *out, const rta_triangle *scratch, const uint32_t *indices,
@@ -305,7 +321,8 @@ int rta_lbvh_build(rta_triangle *tris, int n, rta_node *nodes, uint32_t *indices
 	mark(4);
 	if (rc == RTA_OK) rc = rta_lbvh_build_tree(nodes, w.parents, w.boxes_work, w.codes, n, s);
 	mark(5);
-	if (rc == RTA_OK) rc = rta_lbvh_build_boxes(nodes, w.boxes_work, w.boxes, w.codes, indices, w.parents, n, s);
+	if (rc == RTA_OK)
+		rc = build_boxes(nodes, w.boxes_work, w.boxes, w.codes, indices, w.parents, n, status, s);
 	mark(6);
 	if (rc == RTA_OK) rc = rta_lbvh_permute(tris, w.scratch, indices, n, s);
 	mark(7);

@@ -388,7 +388,7 @@ __device__ inline void bottom_up_chunk(rta_node *__restrict__ nodes,
 				const uint32_t g = (cur & kLocalLeaf) ? leaf0 + base + (cur & 0x3ffu) : base + cur;
 				const uint32_t slot = atomicAdd(&nfront, 1u);
 				if (slot < (uint32_t)kSlots) front[slot] = g;
-				else atomicOr(status, 2u);
+				else atomicOr(status, (uint32_t)RTA_LBVH_FRONTIER_OVERFLOW);
 				break;
 			}
 			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -521,7 +521,7 @@ __device__ inline void bottom_up_merge(rta_node *__restrict__ nodes, uint32_t *_
 			if (rg.x < lo || rg.y >= hi) {
 				const uint32_t slot = atomicAdd(&nfront, 1u);
 				if (slot < (uint32_t)kSlots) front[slot] = cur;
-				else atomicOr(status, 2u);
+				else atomicOr(status, (uint32_t)RTA_LBVH_FRONTIER_OVERFLOW);
 				break;
 			}
 			uint2 rg_next = make_uint2(0u, 0u), pl_next = make_uint2(0u, 0u);
