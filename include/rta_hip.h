@@ -237,6 +237,18 @@ int rta_lbvh_build(rta_triangle *tris, int n, rta_node *nodes,
                    uint32_t *indices, void *workspace, size_t workspace_bytes,
                    uint32_t *status, rta_lbvh_times *times, rta_stream s);
 
+/* rta_lbvh_build without a workspace argument, for callers that cannot
+ * allocate device memory (code under graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.
+ * Same stages, outputs and status bits as rta_lbvh_build; no stage timers.
+ * The workspace is a fixed 46 MiB device array inside the library, one per
+ * GPU, held until the process ends; the call allocates nothing.  n above
+ * RTA_LBVH_NOALLOC_MAX_TRIS returns RTA_E_WORKSPACE before any work.  All
+ * calls on one GPU share that GPU's array: calls that overlap on one GPU
+ * (other streams, other host threads) race; calls in one stream do not. */
+#define RTA_LBVH_NOALLOC_MAX_TRIS 262144
+int rta_lbvh_build_noalloc(rta_triangle *tris, int n, rta_node *nodes, uint32_t *indices,
+                           uint32_t *status, rta_stream s);
 
 /* --------------------------------------------------------------- tracing */
 

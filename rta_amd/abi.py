@@ -47,6 +47,7 @@ RTA_E_NOT_READY = -3
 RTA_E_STACK = -4
 RTA_LBVH_CODE_ZERO = 1
 RTA_LBVH_FRONTIER_OVERFLOW = 2
+RTA_LBVH_NOALLOC_MAX_TRIS = 262144
 
 
 class RtaTimes(C.Structure):
@@ -120,6 +121,7 @@ HIP_SIGNATURES = {
     "rta_lbvh_permute": [_vp, _vp, _vp, _i, _vp],
     "rta_lbvh_workspace_bytes": [_i, _psz],
     "rta_lbvh_build": [_vp, _i, _vp, _vp, _vp, _sz, _vp, C.POINTER(RtaTimes), _vp],
+    "rta_lbvh_build_noalloc": [_vp, _i, _vp, _vp, _vp, _vp],
     "rta_reset_intersections": [_vp, _i, _i, _vp],
     "rta_setup_shirley_rays": [_vp, _vp, _vp, _f, _f, _i, _i, _vp, _vp, _vp, _f, _vp],
     "rta_trace_cis": [_vp, _i, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp, _vp, _vp],
