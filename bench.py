@@ -375,8 +375,6 @@ def cpu_baseline(tris_h, bo, bd, bt, n_bounce, w, max_rays):
     }
 
 
-PRED_TRAIN_SEEDS = (100, 101, 102, 103)     # offsets from scenes.BOUNCE_SEED
-PRED_EVAL_SEEDS = (0, 1)                     # 0 = the headline bounce set
 PRED_FLOPS_PER_RAY = 2 * (102 * 64 + 64 * 64 + 64 * 64 + 64 * 1025)   # 160,640
 
 
@@ -436,7 +434,7 @@ def predictor_leg(args, tris_h, tris, nodes, packed, ranges, prim, hits_p, R, tc
     # training data: bounce rays of four seeds and their plain records
     tmp, tcnt, thits = rta_amd.RayBuffers(R), DeviceBuffer(4), DeviceBuffer(16 * R)
     o_tr, d_tr, lab_tr, tgt_tr = [], [], [], []
-    for k in PRED_TRAIN_SEEDS:
+    for k in scenes.PREDICTOR_TRAIN_SEEDS:
         m = bounce_set(scenes.BOUNCE_SEED + k, tmp, tcnt, thits)
         o, d, _ = tmp.download(m)
         lab, tgt = cut.labels_host(thits, m, stream=s)
@@ -447,7 +445,7 @@ def predictor_leg(args, tris_h, tris, nodes, packed, ranges, prim, hits_p, R, tc
     del tmp, thits
 
     evals = []
-    for k in PRED_EVAL_SEEDS:
+    for k in scenes.PREDICTOR_EVAL_SEEDS:
         e = {"seed": scenes.BOUNCE_SEED + k, "rays": rta_amd.RayBuffers(R), "cnt": DeviceBuffer(4),
              "plain": DeviceBuffer(16 * R), "pred": DeviceBuffer(16 * R)}
         e["m"] = bounce_set(e["seed"], e["rays"], e["cnt"], e["plain"])
@@ -487,19 +485,46 @@ def predictor_leg(args, tris_h, tris, nodes, packed, ranges, prim, hits_p, R, tc
         stats = DeviceBuffer(8 * m)
         scratch = DeviceBuffer(16 * m)
 
-        def visits(mt):
-            """mean inner-node visits and triangle tests per ray with max_t mt"""
+        def visit_counts(mt):
+            """per-ray inner-node visits and triangle tests with max_t mt"""
             call("rta_reset_intersections", ptr(scratch), m, 1, sp)
             call("rta_trace_cis", ptr(tris), n, ptr(nodes), ptr(rays.gpu_origin),
                  ptr(rays.gpu_direction), ptr(mt), m, 1, ptr(scratch), ptr(stats), ptr(err), sp)
             s.synchronize()
-            vt = stats.download(np.uint32, 2 * m).reshape(-1, 2).astype(np.float64)
+            return stats.download(np.uint32, 2 * m).reshape(-1, 2).astype(np.float64)
+
+        def visits(mt):
+            """mean inner-node visits and triangle tests per ray with max_t mt"""
+            vt = visit_counts(mt)
             return round(float(vt[:, 0].mean()), 3), round(float(vt[:, 1].mean()), 3)
 
         rec = {"bounce_seed": e["seed"], "rays": m,
                "plain_mrays_s": round(m / (plain_ms * 1e3), 3), "plain_ms": round(plain_ms, 4),
                "hit_share": round(float(hit.mean()), 4), "by_candidates": []}
-        rec["V_plain"], rec["T_plain"] = visits(rays.gpu_maxt)
+        vt_plain = visit_counts(rays.gpu_maxt)
+        rec["V_plain"] = round(float(vt_plain[:, 0].mean()), 3)
+        rec["T_plain"] = round(float(vt_plain[:, 1].mean()), 3)
+        # The ceiling of any bound seeding: the perfect seed, each hit ray's
+        # own closest t (widened by the kernels' seed margin), as max_t.  It
+        # changes no record; the visits it leaves bound what a predictor can
+        # save, and V_plain / V_perfect_seed bounds the speed-up of a
+        # traversal whose time follows its inner-node visits.
+        t_best = DeviceBuffer.from_array(np.where(hit, hp["t"], fmax).astype(np.float32))
+        maxt_c = DeviceBuffer(4 * m)
+        call("rta_seed_max_t", ptr(maxt_c), ptr(rays.gpu_maxt), ptr(t_best), m, sp)
+        vt_c = visit_counts(maxt_c)
+        rec["ceiling"] = {
+            "V_perfect_seed": round(float(vt_c[:, 0].mean()), 3),
+            "T_perfect_seed": round(float(vt_c[:, 1].mean()), 3),
+            "V_plain_hit_rays": round(float(vt_plain[hit, 0].mean()), 3),
+            "V_perfect_seed_hit_rays": round(float(vt_c[hit, 0].mean()), 3),
+            "visit_saving": round(1.0 - float(vt_c[:, 0].mean() / vt_plain[:, 0].mean()), 4),
+            "speedup_bound": round(float(vt_plain[:, 0].mean() / vt_c[:, 0].mean()), 4),
+        }
+        del t_best, maxt_c
+        log(f"predictor seed {e['seed']}: perfect-seed ceiling V {rec['V_plain']:.2f} -> "
+            f"{rec['ceiling']['V_perfect_seed']:.2f} (speed-up bound "
+            f"{rec['ceiling']['speedup_bound']:.3f})")
         # the seeding traverses the first 1, 2 or all 4 proposed treelets
         for cand in (1, 2, 4):
             part = {"inference": 0.0, "seeding": 0.0, "traversal": 0.0}
@@ -601,6 +626,7 @@ def predictor_leg(args, tris_h, tris, nodes, packed, ranges, prim, hits_p, R, tc
         out.append(rec)
     pred.close()
     return {
+        "ceiling": dict(out[0]["ceiling"], bounce_seed=out[0]["bounce_seed"]),
         "network": "102-64-64-64-1025 MLP, bf16 weights/activations, fp32 MFMA accumulation",
         "flops_per_ray": PRED_FLOPS_PER_RAY,
         "mfma_peak_tflop_s": 2500.0,
@@ -610,9 +636,46 @@ def predictor_leg(args, tris_h, tris, nodes, packed, ranges, prim, hits_p, R, tc
         "training": {"rays": n_tr, "epochs": args.predictor_epochs, "batch": 8192, "lr": 2e-3,
                      "seconds": round(train_s, 2),
                      "loss_first_last": [round(losses[0], 4), round(losses[-1], 4)],
-                     "bounce_seeds": [scenes.BOUNCE_SEED + k for k in PRED_TRAIN_SEEDS]},
+                     "bounce_seeds": [scenes.BOUNCE_SEED + k
+                                      for k in scenes.PREDICTOR_TRAIN_SEEDS]},
         "held_out": out,
     }
+
+
+def traversal_roofline(args, traffic, traffic_src, limiter, bounce_ms, b_ray, logical_gbs, V,
+                       T):
+    """The bounce launch against the HBM roofline.  achieved/frac use the
+    HBM bytes the launch really moves (rocprofv3 FETCH_SIZE and WRITE_SIZE
+    per dispatch, scaled by a copy kernel of known size, from the committed
+    profile of this workload) over this run's mean launch time, so frac <= 1.
+    The logical bytes of the per-ray model 28 + 16 + 64 V + 36 T count every
+    node and triangle fetch, most of which L2 and the last-level cache serve;
+    they are reported beside it and are not an HBM rate."""
+    gbs = traffic / (bounce_ms * 1e-3) / 1e9 if traffic else None
+    out = {
+        "bound": "hbm",
+        "achieved": round(gbs, 1) if gbs else None,
+        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(gbs / HBM_PEAK_GBS, 4) if gbs else None,
+        "traffic": traffic, "traffic_source": traffic_src,
+        "kernel": f"trace_cis_{args.trav}_kernel (bounce rays)",
+        "basis": ("measured HBM bytes per bounce launch (FETCH_SIZE x fetch factor + "
+                  "WRITE_SIZE) / this run's mean bounce launch time"),
+        "algorithmic": {
+            "bytes_per_ray": round(b_ray, 1), "V": round(V, 3), "T": round(T, 3),
+            "logical_gbs": round(logical_gbs, 1),
+            "logical_frac": round(logical_gbs / HBM_PEAK_GBS, 4),
+            "note": ("28 + 16 + 64 V + 36 T bytes per ray counts every node and triangle "
+                     "fetch; caches serve most of them, so this is not an HBM rate"),
+        },
+    }
+    if limiter:
+        out["limiter"] = dict(limiter)
+        out["limiter"]["statement"] = (
+            "dependent node-fetch latency and divergence, not HBM bandwidth: a ray's "
+            "next fetch needs the previous box test; lanes_per_valu of 64 shows the "
+            "divergence, wait_share the share of wave cycles spent waiting")
+    return out
 
 
 def main():
@@ -812,7 +875,7 @@ def main():
     # HBM traffic per bounce launch from the committed rocprofv3 --pmc
     # measurement of this same workload (FETCH_SIZE and WRITE_SIZE, each
     # scaled by what the counter reads for a copy kernel of known size)
-    traffic, traffic_src = None, None
+    traffic, traffic_src, limiter = None, None, None
     pmc_file = os.path.join(ROOT, "profiles", "trace_persistent_heightfield_2M_pmc.json")
     default_workload = (args.trav == "persistent" and args.grid == 1024 and
                         args.random == 200_000 and (args.width, args.height) == (1920, 1080))
@@ -823,6 +886,8 @@ def main():
         if rec:
             traffic = round(rec["fetch_bytes"] + rec["write_bytes"])
             traffic_src = os.path.relpath(pmc_file, ROOT)
+            limiter = {k: rec[k] for k in ("l2_hit_pct", "valu_issue_share", "lanes_per_valu",
+                                           "wait_share") if k in rec}
 
     hit_predicted = None
     if not args.no_predictor and default_workload:
@@ -887,19 +952,8 @@ def main():
         "bounce_mrays_s_wall": round(n_bounce / (bounce_wall_ms * 1e3), 3),
         "build_stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
         "lbvh_status": st,
-        "roofline": {
-            "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "traffic_source": traffic_src,
-            "kernel": f"trace_cis_{args.trav}_kernel (bounce rays)",
-            "bytes_per_ray": round(b_ray, 1), "V": round(V, 3), "T": round(T, 3),
-            # measured HBM bytes per launch over the same launch time
-            "traffic_gbs": (round(traffic / (avg["bounce"] * 1e-3) / 1e9, 1)
-                            if traffic else None),
-            "note": ("achieved counts every node and triangle fetch as HBM bytes; L2 and "
-                     "the last-level cache serve most re-fetches, so achieved can reach or "
-                     "pass the HBM peak while traffic_gbs is the rate HBM actually carries"),
-        },
+        "roofline": traversal_roofline(args, traffic, traffic_src, limiter, avg["bounce"], b_ray,
+                                       achieved, V, T),
         "hit_predicted": hit_predicted,
         "cpu_baseline": cpu,
         "device": dev_name,

@@ -320,16 +320,22 @@ int oracle_lbvh_build(const rta_triangle *tris_in, int n, rta_triangle *tris_out
 		centers[id].z = ((t.a.z + t.b.z) + t.c.z) / 3.0f;
 	}
 	// calculateVoxelBounds (lbvh-kernels.cu:518-535), reduce_max (:496-516)
+	// A NaN box coordinate is skipped: with a NaN operand MIN/MAX is neither
+	// associative nor commutative, so the reference's thrust::reduce result
+	// depends on its reduction order (parity unpinned there); without NaNs
+	// this is the reference's min/max.
 	rta_aabb vb;
 	vb.min = {100000000000.0f, 100000000000.0f, 100000000000.0f};
 	vb.max = {-100000000000.0f, -100000000000.0f, -100000000000.0f};
+	auto fold_min = [](float acc, float v) { return v == v ? MINF(acc, v) : acc; };
+	auto fold_max = [](float acc, float v) { return v == v ? MAXF(acc, v) : acc; };
 	for (int i = 0; i < n; ++i) {
-		vb.min.x = MINF(vb.min.x, boxes[i].min.x);
-		vb.min.y = MINF(vb.min.y, boxes[i].min.y);
-		vb.min.z = MINF(vb.min.z, boxes[i].min.z);
-		vb.max.x = MAXF(vb.max.x, boxes[i].max.x);
-		vb.max.y = MAXF(vb.max.y, boxes[i].max.y);
-		vb.max.z = MAXF(vb.max.z, boxes[i].max.z);
+		vb.min.x = fold_min(vb.min.x, boxes[i].min.x);
+		vb.min.y = fold_min(vb.min.y, boxes[i].min.y);
+		vb.min.z = fold_min(vb.min.z, boxes[i].min.z);
+		vb.max.x = fold_max(vb.max.x, boxes[i].max.x);
+		vb.max.y = fold_max(vb.max.y, boxes[i].max.y);
+		vb.max.z = fold_max(vb.max.z, boxes[i].max.z);
 	}
 	if (bounds) *bounds = vb;
 	// calculateMortonCodes2 (lbvh-kernels.cu:405-421) + morton_codes2 (:169-183)

@@ -84,8 +84,14 @@ __device__ inline void init_triangles_block(rta_triangle *__restrict__ scratch,
 			lo[c] = sel_min(sel_min(va, vb), vc);
 			hi[c] = sel_max(sel_max(va, vb), vc);
 			ce[c] = ((va + vb) + vc) / 3.0f;
-			mn[c] = lo[c];
-			mx[c] = hi[c];
+			// a NaN box coordinate does not enter the scene box: with a NaN
+			// operand the reference's MIN/MAX (reduce_max,
+			// lbvh-kernels.cu:496-516) is neither associative nor
+			// commutative, so its thrust::reduce result depends on the
+			// reduction order; skipping NaNs makes the bounds the min/max
+			// over the other coordinates in every order
+			if (lo[c] == lo[c]) mn[c] = lo[c];
+			if (hi[c] == hi[c]) mx[c] = hi[c];
 		}
 		rta_aabb b;
 		b.min = {lo[0], lo[1], lo[2]};

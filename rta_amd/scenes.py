@@ -108,6 +108,12 @@ def terrain_camera(w: int = 1920, h: int = 1080) -> Camera:
 
 
 BOUNCE_SEED = 0x5EED_B0B0
+# The hit-predictor workload on the terrain scene: the network trains on the
+# diffuse bounce rays of four seeds (about 8.26 M rays at full size) and is
+# evaluated on two held-out seeds (about 4.13 M rays; offset 0 is the
+# benchmark's own bounce set).  Offsets from BOUNCE_SEED.
+PREDICTOR_TRAIN_SEEDS = (100, 101, 102, 103)
+PREDICTOR_EVAL_SEEDS = (0, 1)
 
 
 def instances(n_inst: int = 64, g: int = 256, seed: int = 11):

@@ -1,9 +1,11 @@
-// Bit-for-bit check of rta_hip::div_cr (rta_amd/csrc/hip/exact.h) against
-// IEEE-754 binary32 division on the host.  Built by
-// tests/test_exact_division.py with g++ -O2 -mfma -ffp-contract=off, so both
-// fmaf and '/' are single correctly rounded operations.
+// Bit-for-bit check of rta_hip::div_cr and rta_hip::div_cr1
+// (rta_amd/csrc/hip/exact.h) against IEEE-754 binary32 division on the host.
+// Built by tests/test_exact_division.py with g++ -O2 -mfma
+// -ffp-contract=off, so both fmaf and '/' are single correctly rounded
+// operations.
 // Usage: exact_division_check <random_pairs> <seed>; prints the number of
-// checked pairs and mismatches, exit status 1 on any mismatch.
+// checked pairs and the mismatches of div_cr and of div_cr1, exit status 1
+// on any mismatch.
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -13,6 +15,7 @@
 #include "../rta_amd/csrc/hip/exact.h"
 
 using rta_hip::div_cr;
+using rta_hip::div_cr1;
 using rta_hip::dividend_fast_ok;
 using rta_hip::divisor_fast_ok;
 
@@ -37,19 +40,27 @@ static inline uint32_t bits_of(float f) {
 	return u;
 }
 
-static uint64_t checked = 0, bad = 0;
+static uint64_t checked = 0, bad = 0, bad1 = 0;
+
+static inline bool same_quotient(float q, float r) {
+	return bits_of(q) == bits_of(r) || (isnan(q) && isnan(r)) || (q == 0.0f && r == 0.0f);
+}
 
 static inline void check(float a, float b) {
 	if (b == 0.0f || !divisor_fast_ok(b) || !dividend_fast_ok(a)) return;
 	const float y = 1.0f / b;
 	const float q = div_cr(a, b, y);
+	const float q1 = div_cr1(a, b, y);
 	const float r = a / b;
 	++checked;
-	bool same = bits_of(q) == bits_of(r) || (isnan(q) && isnan(r)) || (q == 0.0f && r == 0.0f);
-	if (!same) {
-		if (bad < 10)
-			fprintf(stderr, "mismatch a=%a b=%a fast=%a ieee=%a\n", a, b, q, r);
+	if (!same_quotient(q, r)) {
+		if (bad < 10) fprintf(stderr, "div_cr mismatch a=%a b=%a fast=%a ieee=%a\n", a, b, q, r);
 		++bad;
+	}
+	if (!same_quotient(q1, r)) {
+		if (bad1 < 10)
+			fprintf(stderr, "div_cr1 mismatch a=%a b=%a fast=%a ieee=%a\n", a, b, q1, r);
+		++bad1;
 	}
 }
 
@@ -98,7 +109,20 @@ int main(int argc, char **argv) {
 		float b = make_float(0, 0, m);
 		for (float a : as) check(a, b);
 	}
-	printf("checked %llu mismatches %llu\n", (unsigned long long)checked,
-	       (unsigned long long)bad);
-	return bad ? 1 : 0;
+	// 5. the exponent-scaling argument behind div_cr1's enumeration: one
+	// significand pair (A, B) at every admitted exponent pair, at the
+	// extremes of the ranges included (|b| = 2^-60 .. 2^28, |a| = 2^-96 ..
+	// 2^60, so quotients from 2^-124 to 2^120 and residuals down to the
+	// subnormal range)
+	for (uint64_t i = 0; i < n / 64 + 1; ++i) {
+		const uint32_t ma = (uint32_t)next64(), mb = (uint32_t)next64();
+		for (int eb = -60; eb <= 27; eb += 1 + (int)(next64() % 8))
+			for (int ea = -96; ea <= 59; ea += 1 + (int)(next64() % 16))
+				check(make_float(0, ea, ma), make_float(0, eb, mb));
+		check(make_float(0, -96, ma), make_float(0, 27, mb));
+		check(make_float(0, 59, ma), make_float(0, -60, mb));
+	}
+	printf("checked %llu mismatches %llu mismatches1 %llu\n", (unsigned long long)checked,
+	       (unsigned long long)bad, (unsigned long long)bad1);
+	return (bad || bad1) ? 1 : 0;
 }

@@ -1,6 +1,11 @@
-"""The division-free slab test divides with div_cr (rta_amd/csrc/hip/exact.h):
-these tests check it bit-for-bit against IEEE-754 division, on the host with
-hardware FMA and on the GPU against the device's own division."""
+"""The division-free slab test divides with div_cr1, one correction step
+after a reciprocal (rta_amd/csrc/hip/exact.h); div_cr is the two-step form
+Markstein's theorem covers.  These tests check both bit for bit against
+IEEE-754 division: on the host with hardware FMA (random operands over the
+admitted ranges, slab-like operands, quotients next to representable values,
+and one significand pair at every admitted exponent pair, the case the
+scaling argument extends the enumeration to), and on the GPU against the
+device's own division, div_cr1 over all 2^46 significand pairs."""
 from __future__ import annotations
 
 import ctypes as C
@@ -21,12 +26,12 @@ def _build_checker(tmp_path):
     return exe
 
 
-def test_div_cr_matches_ieee_on_host(tmp_path):
+def test_div_cr_and_div_cr1_match_ieee_on_host(tmp_path):
     exe = _build_checker(tmp_path)
     out = subprocess.run([exe, "30000000", "11"], capture_output=True, text=True)
     assert out.returncode == 0, out.stderr + out.stdout
-    checked, bad = [int(x) for x in out.stdout.split()[1::2]]
-    assert checked > 50_000_000 and bad == 0
+    checked, bad, bad1 = [int(x) for x in out.stdout.split()[1::2]]
+    assert checked > 50_000_000 and bad == 0 and bad1 == 0
 
 
 def test_checker_detects_outside_range(tmp_path):
@@ -64,21 +69,17 @@ def _pairs(mb_begin, mb_count):
 
 @pytest.mark.gpu
 def test_div_cr1_significand_pairs_on_gpu(gpu):
-    """div_cr1 (one correction step) against IEEE division for every dividend
-    significand and three blocks of 2048 divisor significands (low, middle,
-    high end of [1, 2)).  RTA_DIVISION_ENUM_ALL=1 runs all 2^23 divisor
-    significands (2^46 pairs), which is what exact.h relies on."""
-    if os.environ.get("RTA_DIVISION_ENUM_ALL") == "1":
-        chunks = [(k << 17, 1 << 17) for k in range(64)]
-    else:
-        chunks = [(0, 2048), ((1 << 22) - 1024, 2048), ((1 << 23) - 2048, 2048)]
+    """div_cr1 (one correction step) against IEEE division for every pair of
+    significands A, B in [1, 2): all 2^23 divisor significands in 64 calls of
+    2^17 (2^46 pairs, about 40 s), which is what exact.h relies on."""
     total = 0
-    for mb0, cnt in chunks:
+    for k in range(64):
+        mb0, cnt = k << 17, 1 << 17
         bad, ex = _pairs(mb0, cnt)
         print(f"divisor significands [{mb0}, {mb0 + cnt}): {bad} mismatches", flush=True)
         assert bad == 0, f"mismatch A=0x{ex[0]:08x} B=0x{ex[1]:08x}"
         total += cnt
-    assert total >= 3 * 2048
+    assert total == 1 << 23
 
 
 @pytest.mark.gpu
