@@ -59,6 +59,20 @@ int rta_host_shirley_rays(float *dirs, float *origins, float *max_t, float fovy,
                           int w, int h, const float *view_dir, const float *pos, const float *up,
                           float maxt);
 
+/* Spatial-median binary BVH on the host (bbvh_constructor_using_median with
+ * spatial_median, bbvh/bbvh.h:110-279), the tree the bbvh-cuda plugin builds
+ * on the CPU and uploads for `-b median` (bbvh-cuda/main.cpp:158-163,
+ * 200-235).  nodes: room for rta_host_median_node_bound(n) = 2n-1 nodes,
+ * filled in pre-order (32-byte rta_node, leaves of at most 4 triangles);
+ * tris_out (optional, n): the triangles in leaf order, the array the tree's
+ * leaves index; order (optional, n): the input index of each of them.
+ * Returns the node count, or -1 for n <= 0 or a NULL tris/nodes.  Node boxes,
+ * split axes, split positions and the order among equal centroid
+ * components are the reference's (finite input). */
+int rta_host_median_node_bound(int n);
+int rta_host_median_build(const rta_triangle *tris, int n, rta_node *nodes, rta_triangle *tris_out,
+                          uint32_t *order);
+
 /* ---- the reference's measurement protocol (rta/main.cpp:200-462) ----
  * Functions below return 0 (or a count) on success and a negative value on
  * failure, with the message in rta_host_last_error(). */

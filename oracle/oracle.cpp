@@ -179,7 +179,7 @@ inline float cog_comp(const rta_triangle &t, int axis) {
 	return (float)((a[axis] + b[axis] + c[axis]) * 0.33333);
 }
 
-// merge / compute_aabb, librta/aabb.h:262-301
+// merge / compute_aabb, librta/aabb.h:10-28,40-49
 void merge_tri(rta_aabb &bb, const rta_triangle &t) {
 	const rta_float3 *v[3] = {&t.a, &t.b, &t.c};
 	for (int k = 0; k < 3; ++k) {

@@ -8,14 +8,15 @@ same ABI (ctypes, no framework types).
 from .abi import (AABB_DTYPE, FLT_MAX, HIT_DTYPE, NODE_DTYPE, TRI_DTYPE, RtaError, call,
                   declared_functions, hip, host)
 from .device import DeviceBuffer, Event, Stream, device_count, device_name, synchronize
-from .bvh import (BinaryLbvh, CisShadowTracer, CisTracer, GpuBouncer, LbvhConstructor, PackedBvh,
-                  RayBuffers, ShirleyRayGenerator, diffuse_bounce, shadow_rays)
+from .bvh import (BinaryLbvh, CisShadowTracer, CisTracer, GpuBouncer, LbvhConstructor,
+                  MedianConstructor, PackedBvh, RayBuffers, ShirleyRayGenerator, diffuse_bounce,
+                  shadow_rays)
 from . import scenes
 
 __all__ = [
     "AABB_DTYPE", "FLT_MAX", "HIT_DTYPE", "NODE_DTYPE", "TRI_DTYPE", "RtaError", "call",
     "declared_functions", "hip", "host", "DeviceBuffer", "Event", "Stream", "device_count",
     "device_name", "synchronize", "BinaryLbvh", "CisShadowTracer", "CisTracer", "GpuBouncer",
-    "LbvhConstructor", "PackedBvh", "RayBuffers", "ShirleyRayGenerator", "diffuse_bounce",
-    "shadow_rays", "scenes",
+    "LbvhConstructor", "MedianConstructor", "PackedBvh", "RayBuffers", "ShirleyRayGenerator",
+    "diffuse_bounce", "shadow_rays", "scenes",
 ]

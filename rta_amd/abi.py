@@ -183,6 +183,8 @@ HOST_SIGNATURES = {
     "rta_host_axial_positions": [_vp, _vp, _i, _vp],
     "rta_host_ply_vertices": [C.c_char_p, _vp, _i],
     "rta_host_load_obj": [C.c_char_p, _vp, _i],
+    "rta_host_median_node_bound": [_i],
+    "rta_host_median_build": [_vp, _i, _vp, _vp, _vp],
 }
 
 

@@ -22,24 +22,29 @@ import time
 import numpy as np
 
 from .abi import (FLT_MAX, HIT_DTYPE, NODE_DTYPE, RTA_E_INVALID, RTA_E_STACK,
-                  RTA_LBVH_FRONTIER_OVERFLOW, TRI_DTYPE, RtaError, RtaTimes, TraceConfig, call, ptr)
+                  RTA_LBVH_FRONTIER_OVERFLOW, TRI_DTYPE, RtaError, RtaTimes, TraceConfig, call,
+                  host, ptr)
 from .device import DeviceBuffer
 
 
 class BinaryLbvh:
-    """The acceleration structure: Morton-ordered triangles, 2n-1 nodes and
-    the index buffer (sorted position -> input triangle)."""
+    """The acceleration structure: triangles in the tree's order, the nodes
+    (2n-1 for an LBVH, fewer for a median tree) and the index buffer (tree
+    position -> input triangle)."""
 
     def __init__(self, triangle_data: DeviceBuffer, n: int, node_data: DeviceBuffer,
-                 index_buffer: DeviceBuffer):
+                 index_buffer: DeviceBuffer, n_nodes: int | None = None,
+                 name: str = "lbvh (Karras) on gfx950"):
         self.triangle_data = triangle_data
         self.n = n
         self.node_data = node_data
         self.index_buffer = index_buffer
+        self.n_nodes = 2 * n - 1 if n_nodes is None else n_nodes
+        self.name = name
         self._packed = None
 
     def identification(self) -> str:
-        return "lbvh (Karras) on gfx950"
+        return self.name
 
     def triangle_ptr(self) -> int:
         return self.triangle_data.ptr
@@ -48,7 +53,7 @@ class BinaryLbvh:
         return self.n
 
     def node_count(self) -> int:
-        return 2 * self.n - 1
+        return self.n_nodes
 
     def canonical_triangle_ptr(self) -> np.ndarray:
         """Host copy of the AS triangles (bbvh-cuda.h:65-67 downloads too)."""
@@ -147,6 +152,40 @@ class LbvhConstructor:
         out = DeviceBuffer(8 * max(self._n - 1, 1))
         call("rta_lbvh_copy_ranges", ptr(out), ptr(self._ws), self._n, ptr(stream))
         return out
+
+
+class MedianConstructor:
+    """The spatial-median tree built on the host and uploaded, the `-b
+    median` builder of the bbvh-cuda plugin (bbvh-cuda/main.cpp:158-163,
+    200-235; bbvh/bbvh.h:110-279): expects_host_triangles() is true, the
+    structure owns device copies of the nodes and of the triangles in leaf
+    order, and the index buffer maps leaf order to input triangles."""
+
+    def identification(self) -> str:
+        return "bbvh_constructor_using_median: SM (host, bbvh-hip)"
+
+    def expects_host_triangles(self) -> bool:
+        return True
+
+    def build_host(self, tris: np.ndarray):
+        """(nodes, triangles in leaf order, input index per leaf position)"""
+        n = len(tris)
+        if n <= 0:
+            raise RtaError(RTA_E_INVALID, "rta_host_median_build", "cannot build 0 triangles")
+        tris = np.ascontiguousarray(tris, dtype=TRI_DTYPE)
+        nodes = np.zeros(host().rta_host_median_node_bound(n), NODE_DTYPE)
+        out = np.zeros(n, TRI_DTYPE)
+        order = np.zeros(n, np.uint32)
+        k = host().rta_host_median_build(ptr(tris), n, ptr(nodes), ptr(out), ptr(order))
+        if k <= 0:
+            raise RtaError(RTA_E_INVALID, "rta_host_median_build", "bad arguments")
+        return nodes[:k].copy(), out, order
+
+    def build(self, tris: np.ndarray) -> BinaryLbvh:
+        nodes, out, order = self.build_host(tris)
+        return BinaryLbvh(DeviceBuffer.from_array(out), len(out), DeviceBuffer.from_array(nodes),
+                          DeviceBuffer.from_array(order), n_nodes=len(nodes),
+                          name="binary bvh (spatial median, host-built) on gfx950")
 
 
 class RayBuffers:

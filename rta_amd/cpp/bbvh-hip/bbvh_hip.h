@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../include/rta/hip_glue.h"
+#include "../../../include/rta_host.h"
 
 namespace rta {
 namespace hip {
@@ -23,33 +24,43 @@ inline double wall_ms() {
 	return (double)tv.tv_sec * 1000.0 + (double)tv.tv_usec / 1000.0;
 }
 
-//! Binary LBVH in device memory: Morton-ordered triangles (the array handed
-//! to the constructor, reordered in place), 2n-1 nodes, the index buffer
-//! (sorted position -> input triangle) and, on first use, the packed
-//! traversal layout.
+//! Binary BVH in device memory: the triangles in the tree's order, the
+//! nodes, the index buffer (tree position -> input triangle) and, on first
+//! use, the packed traversal layout.  An LBVH (Karras) has 2n-1 nodes, its
+//! triangle array is the one handed to the constructor (reordered in place)
+//! and the constructor owns nodes and index buffer; a host-built median tree
+//! has fewer nodes and owns device copies of all three (`owns_data`).
 template <typename box_t, typename tri_t>
 class binary_lbvh : public basic_acceleration_structure<box_t, tri_t> {
 public:
-	tri_t *triangles;        //!< not owned (the constructor's input array)
+	tri_t *triangles;
 	uint n;
-	rta_node *nodes;         //!< owned by the constructor
-	uint32_t *index_buffer;  //!< owned by the constructor
+	rta_node *nodes;
+	uint32_t *index_buffer;
+	uint n_nodes;
+	bool owns_data = false;
+	std::string name = "binary lbvh (Karras) on gfx950";
 	rta_node_pair *pairs = 0;
 	float *tri4 = 0;
 	uint32_t *flags = 0;
 	uint32_t root_link = 0, root_index = 0;
 
 	binary_lbvh(tri_t *tris, uint n, rta_node *nodes, uint32_t *index)
-	    : triangles(tris), n(n), nodes(nodes), index_buffer(index) {}
+	    : triangles(tris), n(n), nodes(nodes), index_buffer(index), n_nodes(2 * n - 1) {}
 	~binary_lbvh() override {
 		rta_hip_free(pairs);
 		rta_hip_free(tri4);
 		rta_hip_free(flags);
+		if (owns_data) {
+			rta_hip_free(triangles);
+			rta_hip_free(nodes);
+			rta_hip_free(index_buffer);
+		}
 	}
-	std::string identification() override { return "binary lbvh (Karras) on gfx950"; }
+	std::string identification() override { return name; }
 	tri_t *triangle_ptr() override { return triangles; }
 	int triangle_count() override { return (int)n; }
-	uint node_count() const { return 2 * n - 1; }
+	uint node_count() const { return n_nodes; }
 	tri_t *canonical_triangle_ptr() override {
 		tri_t *t = new tri_t[n];
 		check(rta_hip_memcpy_dtoh(t, triangles, sizeof(tri_t) * (size_t)n), "download triangles");
@@ -186,6 +197,50 @@ public:
 	}
 };
 
+//! Spatial-median constructor (bbvh_constructor_using_median with
+//! spatial_median, bbvh/bbvh.h:110-279), the `-b median` builder of the
+//! bbvh-cuda plugin (bbvh-cuda/main.cpp:158-163, 200-235): the tree is
+//! built on the host from host triangles (expects_host_triangles() is
+//! true) and uploaded with the triangles in leaf order; every build yields
+//! a structure that owns its device arrays.
+template <typename box_t, typename tri_t>
+class median_constructor : public basic_acceleration_structure_constructor<box_t, tri_t> {
+public:
+	bool verbose = false;
+	double last_build_ms = 0;
+
+	std::string identification() override { return "bbvh_constructor_using_median: SM (bbvh-hip)"; }
+	bool expects_host_triangles() override { return true; }
+
+	binary_lbvh<box_t, tri_t> *build(typename tri_t::input_flat_triangle_list_t *tris) override {
+		const int n = (int)tris->triangles;
+		if (n <= 0) throw std::runtime_error("median_constructor: cannot build 0 triangles");
+		const double t0 = wall_ms();
+		std::vector<rta_node> nodes((size_t)rta_host_median_node_bound(n));
+		std::vector<rta_triangle> out((size_t)n);
+		std::vector<uint32_t> order((size_t)n);
+		const int k = rta_host_median_build((const rta_triangle *)tris->triangle, n, nodes.data(),
+		                                    out.data(), order.data());
+		if (k <= 0) throw std::runtime_error("rta_host_median_build failed");
+		last_build_ms = wall_ms() - t0;
+		if (verbose)
+			std::cout << "median build of " << n << " triangles on the host: " << k << " nodes, "
+			          << last_build_ms << " ms" << std::endl;
+		void *pt = 0, *pn = 0, *pi = 0;
+		check(rta_hip_malloc(&pt, sizeof(rta_triangle) * (size_t)n), "triangles");
+		check(rta_hip_malloc(&pn, sizeof(rta_node) * (size_t)k), "nodes");
+		check(rta_hip_malloc(&pi, 4 * (size_t)n), "indices");
+		check(rta_hip_memcpy_htod(pt, out.data(), sizeof(rta_triangle) * (size_t)n), "upload triangles");
+		check(rta_hip_memcpy_htod(pn, nodes.data(), sizeof(rta_node) * (size_t)k), "upload nodes");
+		check(rta_hip_memcpy_htod(pi, order.data(), 4 * (size_t)n), "upload indices");
+		auto *bvh = new binary_lbvh<box_t, tri_t>((tri_t *)pt, (uint)n, (rta_node *)pn, (uint32_t *)pi);
+		bvh->n_nodes = (uint)k;
+		bvh->owns_data = true;
+		bvh->name = "binary bvh (spatial median, host-built) on gfx950";
+		return bvh;
+	}
+};
+
 enum class trace_layout { plain, packed, persistent };
 
 //! Closest-hit child-is tracer (bbvh_gpu_cis_tracer, bbvh-cuda.h:211-235).
@@ -218,7 +273,7 @@ public:
 	}
 	bbvh_gpu_cis_tracer(const bbvh_gpu_cis_tracer &o)
 	    : bbvh_gpu_cis_tracer(o.raygen, o.bvh, o.bouncer, o.layout, o.config, o.shadow) {}
-	~bbvh_gpu_cis_tracer() override { rta_hip_free(d_err); }
+	~bbvh_gpu_cis_tracer() { rta_hip_free(d_err); }
 
 	std::string identification() override {
 		return shadow ? "bbvh-hip child-is shadow tracer" : "bbvh-hip closest-hit child-is tracer";

@@ -3,7 +3,9 @@
 //
 // Options (after `--` on the host's command line):
 //   -v             verbose (per-stage build times)
-//   -b lbvh        GPU LBVH build (the only builder of this plugin)
+//   -b median|lbvh spatial-median tree built on the host and uploaded
+//                  (the default, as in bbvh-cuda/main.cpp:28; any value
+//                  other than lbvh selects it, :88) or the GPU LBVH build
 //   -t cis         child-is traversal (the only traversal of this plugin)
 //   -l cpu|2f4     accepted for compatibility; both byte images are one
 //   -s             shadow tracer: stop at the first accepted triangle
@@ -26,6 +28,7 @@ namespace {
 
 struct Cmdline {
 	bool verbose = false;
+	bool lbvh = false;   // -b lbvh; the median builder otherwise
 	hip::trace_layout layout = hip::trace_layout::persistent;
 	rta_trace_config config{64, 16, 32, 0, 0};
 	bool shadow = false;
@@ -35,7 +38,7 @@ char plugin_name[] = "bbvh-hip";
 
 int usage(const char *msg, const char *arg) {
 	std::cerr << "bbvh-hip: " << msg << (arg ? arg : "") << std::endl;
-	std::cerr << "usage: bbvh-hip [-v] [-s] [-b lbvh] [-t cis] [-l cpu|2f4] "
+	std::cerr << "usage: bbvh-hip [-v] [-s] [-b median|lbvh] [-t cis] [-l cpu|2f4] "
 	             "[--trav plain|packed|persistent] [--block N] [--refill N] [--steps N] [--leaf-min N]"
 	          << std::endl;
 	return 1;
@@ -62,8 +65,8 @@ int parse_cmdline(int argc, char **argv) {
 		if (a == "-v" || a == "--verbose") cmdline.verbose = true;
 		else if (a == "-b" || a == "--bvh") {
 			const char *v = value("bvh");
-			if (!v || std::string(v) != "lbvh")
-				return usage("only the GPU LBVH builder is available: -b lbvh, got ", v);
+			if (!v) return usage("missing value for ", a.c_str());
+			cmdline.lbvh = std::string(v) == "lbvh";
 		} else if (a == "-t" || a == "--trav") {
 			const char *v = value("trav");
 			if (!v) return usage("missing value for ", a.c_str());
@@ -100,21 +103,32 @@ int parse_cmdline(int argc, char **argv) {
 rt_set create_rt_set(basic_flat_triangle_list<simple_triangle> &triangle_lists, int w, int h) {
 	typedef simple_aabb box_t;
 	typedef simple_triangle tri_t;
-	// the build runs on device triangles (bbvh-cuda/main.cpp:466-470)
-	basic_flat_triangle_list<tri_t> ftl;
-	void *p = 0;
-	hip::check(rta_hip_malloc(&p, sizeof(tri_t) * (size_t)triangle_lists.triangles), "triangles");
-	hip::check(rta_hip_memcpy_htod(p, triangle_lists.triangle,
-	                               sizeof(tri_t) * (size_t)triangle_lists.triangles),
-	           "upload triangles");
-	ftl.triangle = (tri_t *)p;
-	ftl.triangles = triangle_lists.triangles;
-	auto *ctor = new hip::lbvh_constructor<box_t, tri_t>;
-	ctor->verbose = cmdline.verbose;
-	hip::binary_lbvh<box_t, tri_t> *bvh = ctor->build(&ftl);
 	rt_set set;
+	hip::binary_lbvh<box_t, tri_t> *bvh = 0;
+	if (!cmdline.lbvh) {
+		// the median tree is built from the host triangles and uploaded
+		// (bbvh-cuda/main.cpp:158-163, 200-235)
+		auto *ctor = new hip::median_constructor<box_t, tri_t>;
+		ctor->verbose = cmdline.verbose;
+		bvh = ctor->build(&triangle_lists);
+		set.ctor = ctor;
+	} else {
+		// the LBVH build runs on device triangles (bbvh-cuda/main.cpp:466-470)
+		basic_flat_triangle_list<tri_t> ftl;
+		void *p = 0;
+		hip::check(rta_hip_malloc(&p, sizeof(tri_t) * (size_t)triangle_lists.triangles),
+		           "triangles");
+		hip::check(rta_hip_memcpy_htod(p, triangle_lists.triangle,
+		                               sizeof(tri_t) * (size_t)triangle_lists.triangles),
+		           "upload triangles");
+		ftl.triangle = (tri_t *)p;
+		ftl.triangles = triangle_lists.triangles;
+		auto *ctor = new hip::lbvh_constructor<box_t, tri_t>;
+		ctor->verbose = cmdline.verbose;
+		bvh = ctor->build(&ftl);
+		set.ctor = ctor;
+	}
 	set.as = bvh;
-	set.ctor = ctor;
 	set.rt = new hip::bbvh_gpu_cis_tracer<box_t, tri_t>(0, bvh, 0, cmdline.layout, cmdline.config,
 	                                                   cmdline.shadow);
 	set.rgen = new hip::cam_ray_generator_shirley(w, h);

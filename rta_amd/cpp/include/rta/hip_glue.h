@@ -119,11 +119,17 @@ public:
 	uploaded_rays(uint w, uint h) : rta::ray_generator(w, h), gpu_ray_generator(w, h) {}
 	void generate_rays() override {
 		const size_t n = (size_t)w * h;
-		check(rta_hip_memcpy_htod(gpu_origin, origins.data(), 12 * n), "upload origins");
-		check(rta_hip_memcpy_htod(gpu_direction, directions.data(), 12 * n), "upload dirs");
-		check(rta_hip_memcpy_htod(gpu_maxt, ray_max_t.data(), 4 * n), "upload max_t");
+		std::vector<vec3f> o(n), d(n);
+		for (size_t i = 0; i < n; ++i) {
+			o[i] = raydata.data[2 * i];
+			d[i] = raydata.data[2 * i + 1];
+		}
+		check(rta_hip_memcpy_htod(gpu_origin, o.data(), 12 * n), "upload origins");
+		check(rta_hip_memcpy_htod(gpu_direction, d.data(), 12 * n), "upload dirs");
+		check(rta_hip_memcpy_htod(gpu_maxt, ray_max_t.data, 4 * n), "upload max_t");
 	}
 	std::string identification() override { return "uploaded host rays"; }
+	void dont_forget_to_initialize_max_t() override {}
 };
 
 template <typename box_t, typename tri_t> class gpu_ray_bouncer : virtual public rta::bouncer {

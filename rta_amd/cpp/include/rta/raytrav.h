@@ -3,10 +3,18 @@
 // structure, its constructor, bouncer, ray generator and ray tracer, with
 // basic_raytracer::trace() running the generate / (prepare, trace, bounce)*
 // loop whose trace_rays() timings the host reports.
+//
+// Objects cross the plugin boundary by pointer, so every class here keeps
+// the reference's virtual functions in the reference's declaration order
+// (the same vtable slots) and the reference's data members in its order
+// (the same offsets), including the reference's guard virtuals
+// (please_derive_from_*, dont_forget_to_initialize_max_t) and raytracer's
+// lack of a virtual destructor; tests/abi_layout_check.cpp checks both.
 #pragma once
 
 #include <math.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -14,10 +22,43 @@
 
 namespace rta {
 
+//! w x h pixels of C values of type T, row-major, channels interleaved (the
+//! layout of the reference's image<T, C>, librta/image.h:15-31; no PNG I/O).
+template <typename T, unsigned int C> class image {
+public:
+	image(unsigned int w, unsigned int h) : w(w), h(h), data(0) {
+		if (w > 0 && h > 0) data = new T[(size_t)w * h * C];
+	}
+	image(const image &o) : w(o.w), h(o.h), data(0) {
+		if (o.data) {
+			data = new T[(size_t)w * h * C];
+			std::copy(o.data, o.data + (size_t)w * h * C, data);
+		}
+	}
+	image &operator=(const image &) = delete;
+	~image() {
+		delete[] data;
+		w = h = 0;
+	}
+	unsigned int w, h;
+	T *data;  //!< 0: no pixels
+	T &pixel(unsigned int x, unsigned int y) { return data[((size_t)y * w + x) * C]; }
+	T &pixel(unsigned int x, unsigned int y, unsigned int c) {
+		return data[((size_t)y * w + x) * C + c];
+	}
+	const T &pixel(unsigned int x, unsigned int y) const { return data[((size_t)y * w + x) * C]; }
+	const T &pixel(unsigned int x, unsigned int y, unsigned int c) const {
+		return data[((size_t)y * w + x) * C + c];
+	}
+};
+
 class acceleration_structure {
 public:
 	virtual ~acceleration_structure() {}
 	virtual std::string identification() = 0;
+	//! guard of the reference (raytrav.h:48): implemented by
+	//! basic_acceleration_structure only
+	virtual void please_derive_from_basic_acceleration_structure() = 0;
 	virtual void dump_acceleration_structure(const std::string &) {}
 	virtual void dump_primitives(const std::string &) {}
 };
@@ -34,12 +75,15 @@ public:
 	//! free_canonical_triangles)
 	virtual tri_t *canonical_triangle_ptr() { return triangle_ptr(); }
 	virtual void free_canonical_triangles(tri_t *) {}
+	void please_derive_from_basic_acceleration_structure() override {}
 };
 
 class acceleration_structure_constructor {
 public:
 	virtual ~acceleration_structure_constructor() {}
 	virtual std::string identification() = 0;
+	//! guard of the reference (raytrav.h:89)
+	virtual void please_derive_from_basic_acceleration_structure_constructor() = 0;
 };
 
 template <typename box_t_, typename tri_t_>
@@ -51,6 +95,7 @@ public:
 	virtual basic_acceleration_structure<box_t, tri_t> *build(flat_triangle_list_t *tris) = 0;
 	//! whether build() expects triangles in host memory
 	virtual bool expects_host_triangles() { return true; }
+	void please_derive_from_basic_acceleration_structure_constructor() override {}
 };
 
 class bouncer {
@@ -62,25 +107,29 @@ public:
 	virtual std::string identification() = 0;
 };
 
-//! Host ray storage: origin and direction per pixel plus max_t.
+//! Host ray storage: per pixel its origin and direction (interleaved, two
+//! vec3f per pixel) and its max_t, as the reference stores them
+//! (raytrav.h:270-298).
 class ray_generator {
 public:
-	uint res_w, res_h;
-	std::vector<vec3f> origins, directions;
-	std::vector<float> ray_max_t;
-	ray_generator(uint res_x, uint res_y)
-	    : res_w(res_x), res_h(res_y), origins((size_t)res_x * res_y), directions((size_t)res_x * res_y),
-	      ray_max_t((size_t)res_x * res_y) {}
+	image<vec3f, 2> raydata;
+	image<float, 1> ray_max_t;
+	ray_generator(uint res_x, uint res_y) : raydata(res_x, res_y), ray_max_t(res_x, res_y) {}
 	virtual ~ray_generator() {}
 	virtual void generate_rays() = 0;
-	//! text dump of the rays (the .ray format, librta/raytrav.h:295,353-366)
-	virtual void dump_rays(const std::string &) {}
-	uint res_x() const { return res_w; }
-	uint res_y() const { return res_h; }
-	vec3f *origin(uint x, uint y) { return &origins[(size_t)y * res_w + x]; }
-	vec3f *direction(uint x, uint y) { return &directions[(size_t)y * res_w + x]; }
-	float &max_t(uint x, uint y) { return ray_max_t[(size_t)y * res_w + x]; }
+	uint res_x() const { return raydata.w; }
+	uint res_y() const { return raydata.h; }
+	vec3f *origin(uint x, uint y) { return &raydata.pixel(x, y, 0); }
+	const vec3f *origin(uint x, uint y) const { return &raydata.pixel(x, y, 0); }
+	vec3f *direction(uint x, uint y) { return &raydata.pixel(x, y, 1); }
+	const vec3f *direction(uint x, uint y) const { return &raydata.pixel(x, y, 1); }
+	float &max_t(uint x, uint y) { return ray_max_t.pixel(x, y, 0); }
+	const float &max_t(uint x, uint y) const { return ray_max_t.pixel(x, y, 0); }
 	virtual std::string identification() = 0;
+	//! text dump of the rays (the .ray format, librta/raytrav.h:353-366)
+	virtual void dump_rays(const std::string &) {}
+	//! guard of the reference (raytrav.h:297)
+	virtual void dont_forget_to_initialize_max_t() = 0;
 };
 
 //! Pinhole camera after Shirley (librta/raytrav.h:300-351).
@@ -102,6 +151,7 @@ public:
 	}
 	void generate_rays() override;  // host version, in librta
 	std::string identification() override { return "pinhole ray generator (Shirley)"; }
+	void dont_forget_to_initialize_max_t() override {}
 	float fov() const { return fovy; }
 	float aspect_ratio() const { return aspect; }
 	const vec3f &camera_position() const { return position; }
@@ -109,14 +159,19 @@ public:
 	const vec3f &camera_up() const { return up; }
 };
 
+//! No virtual destructor, as in the reference (raytrav.h:373-383): tracers
+//! are never deleted through this base, and a virtual destructor would add
+//! two vtable slots ahead of trace().
 class raytracer {
 public:
-	virtual ~raytracer() {}
 	virtual void trace() = 0;
 	virtual std::string identification() = 0;
 	virtual raytracer *copy() = 0;
 	//! whether the tracer honours the ray generator's max_t
 	virtual bool supports_max_t() = 0;
+
+protected:
+	~raytracer() {}
 };
 
 template <typename box_t_, typename tri_t_> class basic_raytracer : public raytracer {

@@ -83,9 +83,17 @@ bool load_plugin_functions(const std::string &plugin_name) {
 }
 
 void cam_ray_generator_shirley::generate_rays() {
-	const int rc = rta_host_shirley_rays(&directions[0].x, &origins[0].x, ray_max_t.data(), fovy,
-	                                     aspect, res_w, res_h, &dir.x, &position.x, &up.x, FLT_MAX);
+	// the generator writes separate origin / direction arrays; the rays are
+	// stored interleaved per pixel (origin, direction)
+	const size_t n = (size_t)res_x() * res_y();
+	std::vector<vec3f> o(n), d(n);
+	const int rc = rta_host_shirley_rays(&d[0].x, &o[0].x, ray_max_t.data, fovy, aspect, res_x(),
+	                                     res_y(), &dir.x, &position.x, &up.x, FLT_MAX);
 	if (rc != 0) throw std::runtime_error("rta_host_shirley_rays failed");
+	for (size_t i = 0; i < n; ++i) {
+		raydata.data[2 * i] = o[i];
+		raydata.data[2 * i + 1] = d[i];
+	}
 }
 
 namespace hip {

@@ -352,15 +352,22 @@ int main(int argc, char **argv) {
 		auto *cam = dynamic_cast<rta::cam_ray_generator_shirley *>(set.rgen);
 		if (!cam) throw std::runtime_error("plugin ray generator is not a pinhole camera");
 
-		// rebuild from the original triangles (rta/main.cpp:389-400)
+		// rebuild from the original triangles (rta/main.cpp:389-400): host
+		// triangles for a constructor that expects them, a device copy
+		// otherwise
 		auto rebuild = [&]() {
 			auto *ctor = set.basic_ctor<box_t, tri_t>();
-			void *p = 0;
-			hip::check(rta_hip_malloc(&p, sizeof(tri_t) * scene.size()), "triangles");
-			hip::check(rta_hip_memcpy_htod(p, scene.data(), sizeof(tri_t) * scene.size()), "upload");
 			basic_flat_triangle_list<tri_t> dl;
 			dl.triangles = (uint)scene.size();
-			dl.triangle = (tri_t *)p;
+			if (ctor->expects_host_triangles()) {
+				dl.triangle = scene.data();
+			} else {
+				void *p = 0;
+				hip::check(rta_hip_malloc(&p, sizeof(tri_t) * scene.size()), "triangles");
+				hip::check(rta_hip_memcpy_htod(p, scene.data(), sizeof(tri_t) * scene.size()),
+				           "upload");
+				dl.triangle = (tri_t *)p;
+			}
 			basic_acceleration_structure<box_t, tri_t> *as = ctor->build(&dl);
 			delete set.as;
 			set.as = as;

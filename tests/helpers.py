@@ -163,3 +163,37 @@ def grazing_rays(n: int, seed: int = 1, z: float = 0.3):
     d[:, 2] = -np.abs(d[:, 2]) * rng.choice([1e-4, 1e-3, 1e-2, 0.1, 1.0], n)
     d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
     return o.reshape(-1), d.reshape(-1), np.full(n, np.finfo(np.float32).max, np.float32)
+
+
+# ------------------------------------------ the reference's text dump formats
+
+def fmt_g(x) -> str:
+    """A float as `std::ostream << float` writes it by default (6 significant
+    digits, %g)."""
+    return "%g" % float(x)
+
+
+def bvh_dump_lines(nodes: np.ndarray) -> list[str]:
+    """The .bvh text of a node array (bbvh/bbvh.h:36-52): header, then per
+    node its index, I/L, the box, and split axis (0), left, right for an
+    inner node or the count and triangle indices for a leaf."""
+    out = ["bvh", str(len(nodes))]
+    for i, nd in enumerate(nodes):
+        w, r = int(nd["type_left_elems"]), int(nd["right_tris"])
+        box = " ".join(fmt_g(v) for v in (*nd["min"], *nd["max"]))
+        if w & 1:
+            out.append(f"{i} I {box} 0 {w >> 1} {r}")
+        else:
+            e = w >> 1
+            out.append(" ".join([f"{i} L {box} {e}"] + [str(r + j) for j in range(e)]))
+    return out
+
+
+def tri_dump_lines(tris: np.ndarray) -> list[str]:
+    """The .tri text of a triangle array (bbvh/bbvh.h:53-65): header, then
+    per triangle its index, vertices a, b, c and normals na, nb, nc."""
+    out = ["tri", str(len(tris))]
+    for i, t in enumerate(tris):
+        vals = (*t["a"], *t["b"], *t["c"], *t["na"], *t["nb"], *t["nc"])
+        out.append(" ".join([str(i)] + [fmt_g(v) for v in vals]))
+    return out

@@ -44,7 +44,9 @@ def test_plugin_option_parsing():
     assert parse("-b", "lbvh", "-t", "cis", "-l", "2f4", "-v") == 0
     assert parse("--trav", "packed", "--block", "64", "--refill", "8", "--steps", "4",
                  "--leaf-min", "12") == 0
-    assert parse("-b", "median") != 0        # no CPU-built tree in this plugin
+    assert parse("-b", "median") == 0        # the host-built median tree (the default)
+    assert parse("-b", "sah") == 0           # any value but lbvh selects it (main.cpp:88)
+    assert parse("-b") != 0                  # a value is required
     assert parse("-s") == 0                  # shadow tracer (persistent layout)
     assert parse("-s", "--trav", "plain") != 0   # no shadow form of the plain kernel
     assert parse("-A") != 0                  # Aila's box variant is not provided
@@ -99,7 +101,7 @@ def test_driver_frame_matches_oracle(gpu, tmp_path, trav):
 def test_driver_rebuild_and_dump(gpu, tmp_path):
     dump = str(tmp_path / "tree.bvh")
     r = run_driver("-r", "320x180", "--scene", "terrain:64:500", "--frames", "2", "--rebuild",
-                   "--dump-bvh", dump, "bbvh-hip", "--", "-v")
+                   "--dump-bvh", dump, "bbvh-hip", "--", "-b", "lbvh", "-v")
     assert r.returncode == 0, r.stdout + r.stderr
     assert "LBVH build of" in r.stdout
     lines = open(dump).read().splitlines()
@@ -122,6 +124,34 @@ def test_driver_shadow_tracer(gpu, tmp_path, trav):
     ref_b = oracle.lbvh_build(tris)
     o, d, t = scenes.sphere_camera(256, 256).host_rays()
     ref, _ = oracle.trace_kernel(ref_b["tris"], ref_b["nodes"], o, d, t, any_hit=True)
+    hit = ref["t"] != F32MAX
+    assert hit.mean() > 0.2
+    assert np.array_equal(got["t"] != F32MAX, hit)
+    for f in ("t", "beta", "gamma", "ref"):
+        assert np.array_equal(got[f][hit], ref[f][hit])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("builder", [(), ("-b", "median")])
+def test_driver_median_builder(gpu, tmp_path, builder):
+    """The plugin's default builder, as the reference's (bbvh-cuda/main.cpp:
+    28, 158-163): the spatial-median tree built on the host and uploaded.
+    Two frames with a rebuild from host triangles between them; the dumped
+    .bvh/.tri bodies are the oracle's median tree and leaf-ordered triangles
+    in the reference's text format, the hits the CPU child-is tracer's."""
+    from helpers import bvh_dump_lines, tri_dump_lines
+    hits, prefix = str(tmp_path / "hits.bin"), str(tmp_path / "m")
+    r = run_driver("-r", "256x256", "--frames", "2", "--rebuild", "-M", prefix, "--dump-hits",
+                   hits, "bbvh-hip", "--", *builder, "-v")
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "median build of" in r.stdout and "LBVH build of" not in r.stdout
+    tris = scenes.sphere_scene()
+    mb = oracle.median_build(tris)
+    assert open(prefix + ".bvh").read().splitlines() == bvh_dump_lines(mb["nodes"])
+    assert open(prefix + ".tri").read().splitlines() == tri_dump_lines(mb["tris"])
+    got = np.fromfile(hits, dtype=abi.HIT_DTYPE)
+    o, d, t = scenes.sphere_camera(256, 256).host_rays()
+    ref, _, _ = oracle.trace_cis(mb["tris"], mb["nodes"], o, d, t, 256, 256)
     hit = ref["t"] != F32MAX
     assert hit.mean() > 0.2
     assert np.array_equal(got["t"] != F32MAX, hit)

@@ -22,6 +22,7 @@ import pytest
 import oracle
 from rta_amd import abi
 from rta_amd.abi import TRI_DTYPE, host, ptr
+from helpers import bvh_dump_lines, fmt_g, tri_dump_lines
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "rta_amd", "bin", "rta-hip")
@@ -263,14 +264,20 @@ def test_driver_sphere_series(gpu, tmp_path):
     aspect = float(np.float32(w) / np.float32(h))
     host().rta_host_shirley_rays(ptr(dd), ptr(o), ptr(t), 45.0, aspect, w, h, ptr(d), ptr(pos),
                                  ptr(up), float(F32MAX))
+    # the .ray body (librta/raytrav.h:353-366): id, 2, origin, direction,
+    # 0, max_t, each float as the stream writes it (%g)
     ray = open(f"{prefix}005.ray").read().splitlines()
     assert ray[:2] == ["ray", f"{w} {h}"]
-    vals = np.array([l.split() for l in ray[2:]], np.float64)
-    assert np.array_equal(vals[:, 0], np.arange(R)) and np.all(vals[:, 1] == 2)
-    assert np.allclose(vals[:, 2:5], o.reshape(-1, 3), rtol=1e-5, atol=1e-6)
-    assert np.allclose(vals[:, 5:8], dd.reshape(-1, 3), rtol=1e-5, atol=1e-6)
+    o3, d3 = o.reshape(-1, 3), dd.reshape(-1, 3)
+    exp = [" ".join([str(i), "2"] + [fmt_g(v) for v in (*o3[i], *d3[i])] + ["0", fmt_g(t[i])])
+           for i in range(R)]
+    assert ray[2:] == exp
     got = np.fromfile(dump, dtype=abi.HIT_DTYPE)
     ob = oracle.lbvh_build(tris)
+    # the .bvh/.tri bodies (bbvh/bbvh.h:36-65): the oracle's LBVH nodes and
+    # Morton-ordered triangles in the reference's text format
+    assert bvh == bvh_dump_lines(ob["nodes"])
+    assert tri == tri_dump_lines(ob["tris"])
     ref, _, _ = oracle.trace_cis(ob["tris"], ob["nodes"], o, dd, t, w, h)
     hit = ref["t"] != F32MAX
     assert hit.mean() > 0.05
