@@ -11,9 +11,13 @@ the start of each step (the build reorders it in place, as the reference).
 
 `value` is the plain closest-hit Mrays/s on the bounce rays (kernel time from
 HIP events on the stream the kernels run on).  Primary-ray Mrays/s and LBVH
-build Mtri/s ride along in the same line.  The CPU baseline is the oracle's
-restatement of the reference CPU bbvh plugin (spatial-median tree, child-is
-tracer, OpenMP over rows) on the same bounce rays, timed on this host.
+build Mtri/s ride along in the same line, with the hit-predicted leg on the
+same scene (`hit_predicted`), the 256-mesh batch (`batch`) and the 8.4 M
+triangle dynamic frame (`dynamic`), each measured in this run
+(`--workload batch|dynamic` prints either as a line of its own).  The CPU
+baseline is the oracle's restatement of the reference CPU bbvh plugin
+(spatial-median tree, child-is tracer, OpenMP over rows) on the same bounce
+rays, timed on this host.
 """
 from __future__ import annotations
 
@@ -53,6 +57,8 @@ def parse():
     p.add_argument("--steps-per-refill", dest="steps_per_refill", type=int, default=32)
     p.add_argument("--leaf-min", dest="leaf_min", type=int, default=10)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    p.add_argument("--no-legs", dest="no_legs", action="store_true",
+                   help="skip the many-mesh and dynamic-frame legs of the default run")
     p.add_argument("--no-predictor", action="store_true",
                    help="skip the hit-predicted leg (training + seeded traversal)")
     p.add_argument("--predictor-epochs", dest="predictor_epochs", type=int, default=8)
@@ -71,7 +77,7 @@ def parse():
     return p.parse_args()
 
 
-def batch_main(args) -> None:
+def batch_leg(args) -> dict:
     """256 meshes (1k-200k triangles, 4k-256k rays each): all builds in one
     launch per stage, all rays in one autotuned persistent launch; beside
     it the same work as one build and one trace launch per mesh."""
@@ -156,26 +162,17 @@ def batch_main(args) -> None:
     each_build_ms = timed(build_each, 3)
     each_trace_ms = timed(trace_each, 3)
     R, N = b.total_rays, b.total_tris
-    out = {
-        "metric": METRIC,
-        "value": round(R / (trace_ms * 1e3), 3),
-        "unit": "Mrays/s",
-        "n_gpus": 1,
+    log(f"batch: build {build_ms:.3f} ms, trace {trace_ms:.3f} ms ({R / (trace_ms * 1e3):.1f} "
+        f"Mrays/s); one launch per mesh: build {each_build_ms:.3f} ms, trace "
+        f"{each_trace_ms:.3f} ms")
+    return {
+        "workload": ("256 independent meshes (1k-200k triangles, jittered icospheres and "
+                     "heightfield patches) with 4k-256k rays each (half camera, half "
+                     "random): all LBVH builds in one launch per stage, all rays in one "
+                     "persistent launch with an autotuned shape"),
+        "meshes": len(meshes), "n_tris": N, "rays": R,
+        "batch_mrays_s": round(R / (trace_ms * 1e3), 3),
         "steps": max(args.steps, 3),
-        "warmup": 1,
-        "ms_per_step": round(build_ms + trace_ms, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {
-            "workload": ("256 independent meshes (1k-200k triangles, jittered icospheres and "
-                         "heightfield patches) with 4k-256k rays each (half camera, half "
-                         "random): all LBVH builds in one launch per stage, all rays in one "
-                         "persistent launch with an autotuned shape; value = batched Mrays/s"),
-            "meshes": len(meshes), "n_tris": N, "rays": R,
-        },
         "batch_build_ms": round(build_ms, 4),
         "batch_build_mtri_s": round(N / (build_ms * 1e3), 3),
         "batch_trace_ms": round(trace_ms, 4),
@@ -185,12 +182,36 @@ def batch_main(args) -> None:
         "per_mesh_launches_trace_ms": round(each_trace_ms, 4),
         "batching_speedup_build": round(each_build_ms / build_ms, 3),
         "batching_speedup_trace": round(each_trace_ms / trace_ms, 3),
+    }
+
+
+def batch_main(args) -> None:
+    import rta_amd
+
+    leg = batch_leg(args)
+    out = {
+        "metric": METRIC,
+        "value": leg["batch_mrays_s"],
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": leg["steps"],
+        "warmup": 1,
+        "ms_per_step": round(leg["batch_build_ms"] + leg["batch_trace_ms"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": leg["workload"] + "; value = batched Mrays/s",
+                   "meshes": leg["meshes"], "n_tris": leg["n_tris"], "rays": leg["rays"]},
+        **{k: v for k, v in leg.items() if k not in ("workload", "meshes", "n_tris", "rays",
+                                                     "batch_mrays_s", "steps")},
         "device": rta_amd.device_name(),
     }
     print(json.dumps(out), flush=True)
 
 
-def dynamic_main(args) -> None:
+def dynamic_leg(args) -> dict:
     """The dynamic frame: per-frame jitter, full rebuild, primary rays, two
     diffuse bounces and a shadow ray per hit, replayed from a graph."""
     import rta_amd
@@ -243,34 +264,51 @@ def dynamic_main(args) -> None:
     counts_last = fr.ray_counts()
     rays = sum(counts_last.values())
     del graph
+    log(f"dynamic: {frame_ms:.3f} ms per captured frame, {rays / (frame_ms * 1e3):.1f} Mrays/s "
+        f"over the frame's {rays} rays")
     finetune = None if args.no_predictor else dynamic_finetune(args, fr, v, n_eager)
-    out = {
-        "metric": METRIC,
-        "value": round(rays / (frame_ms * 1e3), 3),
-        "unit": "Mrays/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": n_eager,
-        "ms_per_step": round(wall_ms, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {
-            "workload": (f"dynamic scene: {args.instances} instances x "
-                         f"{2 * args.instance_grid ** 2} triangles = {n}, per-vertex jitter "
-                         f"sigma 1e-3 and full LBVH rebuild every frame, {w}x{h} primary rays, "
-                         "2 diffuse bounces and 1 shadow ray per hit; value = all rays traced "
-                         "per frame / frame time, frames replayed from one captured graph"),
-            "n_tris": n, "rays_per_frame": counts_last,
-        },
+    return {
+        "workload": (f"dynamic scene: {args.instances} instances x "
+                     f"{2 * args.instance_grid ** 2} triangles = {n}, per-vertex jitter "
+                     f"sigma 1e-3 and full LBVH rebuild every frame, {w}x{h} primary rays, "
+                     "2 diffuse bounces and 1 shadow ray per hit, frames replayed from one "
+                     "captured graph"),
+        "n_tris": n, "rays_per_frame": counts_last,
+        "frame_mrays_s": round(rays / (frame_ms * 1e3), 3),
+        "steps": args.steps, "warmup": n_eager,
+        "frame_wall_ms": round(wall_ms, 4),
         "frame_ms": round(frame_ms, 4),
         "frames_per_s": round(1e3 / frame_ms, 2),
         "build_and_pack_mtri_s": round(n / (stage_ms["build+pack"] * 1e3), 3),
         "eager_stage_ms": {k: round(x, 4) for k, x in stage_ms.items()},
         "eager_counts_first_frames": counts,
         "online_finetune": finetune,
+    }
+
+
+def dynamic_main(args) -> None:
+    import rta_amd
+
+    leg = dynamic_leg(args)
+    out = {
+        "metric": METRIC,
+        "value": leg["frame_mrays_s"],
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": leg["steps"],
+        "warmup": leg["warmup"],
+        "ms_per_step": leg["frame_wall_ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": leg["workload"] + "; value = all rays traced per frame / frame "
+                                                 "time",
+                   "n_tris": leg["n_tris"], "rays_per_frame": leg["rays_per_frame"]},
+        **{k: v for k, v in leg.items() if k not in ("workload", "n_tris", "rays_per_frame",
+                                                     "frame_mrays_s", "steps", "warmup",
+                                                     "frame_wall_ms")},
         "device": rta_amd.device_name(),
     }
     print(json.dumps(out), flush=True)
@@ -759,14 +797,16 @@ def main():
 
     # the root's link words are fixed by the scene (always an inner node
     # here); read them once after a first build
+    # the build writes the traversal layout in its own passes
+    # (rta_lbvh_build_packed), so build_ms covers everything the traversal
+    # needs; the unfused pair (rta_lbvh_build, then rta_bvh_pack) is timed
+    # once beside it
     def build_and_pack():
         call("rta_hip_memcpy_dtod", ptr(tris), ptr(pristine), C.c_size_t(104 * n), sp)
         ev[0].record(s)
-        call("rta_lbvh_build", ptr(tris), n, ptr(nodes), ptr(idx), ptr(ws), ws_bytes, ptr(status),
-             None, sp)
+        call("rta_lbvh_build_packed", ptr(tris), n, ptr(nodes), ptr(idx), ptr(pairs), ptr(tri4),
+             ptr(pflags), ptr(ws), ws_bytes, ptr(status), None, sp)
         ev[1].record(s)
-        call("rta_bvh_pack", ptr(pairs), ptr(tri4), ptr(pflags), ptr(nodes), 2 * n - 1, ptr(tris),
-             n, sp)
         ev[2].record(s)
 
     queue = DeviceBuffer(4)
@@ -810,7 +850,7 @@ def main():
         frame(root)
     s.synchronize()
 
-    seg = {"build": [], "pack": [], "primary": [], "bounce_gen": [], "bounce": []}
+    seg = {"build": [], "primary": [], "bounce_gen": [], "bounce": []}
     log(f"timing {args.steps} steps")
     call("rta_hip_device_synchronize")
     t_start = time.perf_counter()
@@ -818,7 +858,6 @@ def main():
         frame(root)
         ev[6].synchronize()
         seg["build"].append(ev[0].elapsed_ms(ev[1]))
-        seg["pack"].append(ev[1].elapsed_ms(ev[2]))
         seg["primary"].append(ev[3].elapsed_ms(ev[4]))
         seg["bounce_gen"].append(ev[4].elapsed_ms(ev[5]))
         seg["bounce"].append(ev[5].elapsed_ms(ev[6]))
@@ -849,9 +888,25 @@ def main():
     # per-stage build breakdown (one extra, synchronising build)
     times = rta_amd.abi.RtaTimes()
     call("rta_hip_memcpy_dtod", ptr(tris), ptr(pristine), C.c_size_t(104 * n), sp)
-    call("rta_lbvh_build", ptr(tris), n, ptr(nodes), ptr(idx), ptr(ws), ws_bytes, ptr(status),
-         C.byref(times), sp)
+    call("rta_lbvh_build_packed", ptr(tris), n, ptr(nodes), ptr(idx), ptr(pairs), ptr(tri4),
+         ptr(pflags), ptr(ws), ws_bytes, ptr(status), C.byref(times), sp)
     stage_ms = times.as_dict()
+    # the unfused pair: the seven stages, then the layout derived from the
+    # finished nodes (medians of 10)
+    unfused = {"build": [], "pack": []}
+    for _ in range(10):
+        call("rta_hip_memcpy_dtod", ptr(tris), ptr(pristine), C.c_size_t(104 * n), sp)
+        ev[0].record(s)
+        call("rta_lbvh_build", ptr(tris), n, ptr(nodes), ptr(idx), ptr(ws), ws_bytes,
+             ptr(status), None, sp)
+        ev[1].record(s)
+        call("rta_bvh_pack", ptr(pairs), ptr(tri4), ptr(pflags), ptr(nodes), 2 * n - 1,
+             ptr(tris), n, sp)
+        ev[2].record(s)
+        ev[2].synchronize()
+        unfused["build"].append(ev[0].elapsed_ms(ev[1]))
+        unfused["pack"].append(ev[1].elapsed_ms(ev[2]))
+    unfused = {k: float(np.median(v)) for k, v in unfused.items()}
 
     # per-ray inner-node visits V and triangle tests T on the bounce rays,
     # from the instrumented plain kernel (equal to the oracle's counts, see
@@ -900,6 +955,12 @@ def main():
                                       tcfg, s)
         del ranges
 
+    # the many-mesh and dynamic-frame workloads, timed in the same run
+    batch = dynamic = None
+    if default_workload and not args.no_legs:
+        batch = batch_leg(args)
+        dynamic = dynamic_leg(args)
+
     cpu = None
     if not args.no_cpu:
         log("CPU baseline (oracle restatement of the reference CPU plugin)")
@@ -941,7 +1002,9 @@ def main():
             "frac": round(276 * n / (avg["build"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         },
         "build_ms": round(avg["build"], 4),
-        "pack_ms": round(avg["pack"], 4),
+        "build_includes_traversal_layout": True,
+        "unfused_build_ms": round(unfused["build"], 4),
+        "unfused_pack_ms": round(unfused["pack"], 4),
         "bounce_gen_ms": round(avg["bounce_gen"], 4),
         "primary_trace_ms": round(avg["primary"], 4),
         "bounce_trace_ms": round(avg["bounce"], 4),
@@ -955,6 +1018,8 @@ def main():
         "roofline": traversal_roofline(args, traffic, traffic_src, limiter, avg["bounce"], b_ray,
                                        achieved, V, T),
         "hit_predicted": hit_predicted,
+        "batch": batch,
+        "dynamic": dynamic,
         "cpu_baseline": cpu,
         "device": dev_name,
     }

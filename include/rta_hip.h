@@ -62,8 +62,9 @@ typedef struct rta_hit { float t, beta, gamma; uint32_t ref; } rta_hit;
 typedef struct __attribute__((aligned(64))) rta_node_pair {
 	float lo_x[2], lo_y[2], lo_z[2];
 	float hi_x[2], hi_y[2], hi_z[2];
-	uint32_t link[2];   /* child's type_left_elems word                 */
-	uint32_t index[2];  /* child's right_tris word (leaf: first tri)    */
+	uint32_t link[2];   /* child word: (id << 1) | 1 for an inner child,
+	                       first triangle << 1 for a leaf              */
+	uint32_t index[2];  /* child node ids                               */
 } rta_node_pair;
 
 /* Per-stage build times in milliseconds (hipEvent based), in the stage
@@ -109,7 +110,8 @@ const char *rta_hip_last_error(void);
  * print and exit, librta/cuda.h:70-78): the next call of `entry` returns
  * `status` (non-zero) before doing any work, with the message "<entry>:
  * injected fault"; one shot.  Entry points with this hook: rta_hip_malloc,
- * rta_lbvh_build, rta_trace_cis_persistent, rta_pred_infer. */
+ * rta_lbvh_build, rta_lbvh_build_packed, rta_trace_cis_persistent,
+ * rta_pred_infer. */
 int rta_hip_inject_fault(const char *entry, int status);
 /* Named host-side ranges (roctx) for a profiler's timeline; the library
  * marks rta_lbvh_build and the persistent traversals with them. */
@@ -236,6 +238,18 @@ int rta_lbvh_workspace_bytes(int n, size_t *bytes);
 int rta_lbvh_build(rta_triangle *tris, int n, rta_node *nodes,
                    uint32_t *indices, void *workspace, size_t workspace_bytes,
                    uint32_t *status, rta_lbvh_times *times, rta_stream s);
+
+/* rta_lbvh_build that also writes the traversal layout rta_bvh_pack derives
+ * (pairs: 2n-1 records, of which those of inner nodes are written; tri4:
+ * 3n float4; flags: one word), with the same bytes, inside the build's own
+ * passes: the permutation writes tri4 and runs before the bottom-up pass,
+ * which writes each inner node's pair record as it completes the node.
+ * times then holds the permutation before the boxes.  No counterpart in the
+ * reference (which traces its 2f4 node array directly). */
+int rta_lbvh_build_packed(rta_triangle *tris, int n, rta_node *nodes, uint32_t *indices,
+                          rta_node_pair *pairs, float *tri4, uint32_t *flags, void *workspace,
+                          size_t workspace_bytes, uint32_t *status, rta_lbvh_times *times,
+                          rta_stream s);
 
 /* rta_lbvh_build without a workspace argument, for callers that cannot
  * allocate device memory (code under graph capture, serving loops,

@@ -122,6 +122,8 @@ HIP_SIGNATURES = {
     "rta_lbvh_workspace_bytes": [_i, _psz],
     "rta_lbvh_build": [_vp, _i, _vp, _vp, _vp, _sz, _vp, C.POINTER(RtaTimes), _vp],
     "rta_lbvh_build_noalloc": [_vp, _i, _vp, _vp, _vp, _vp],
+    "rta_lbvh_build_packed": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp,
+                              C.POINTER(RtaTimes), _vp],
     "rta_reset_intersections": [_vp, _i, _i, _vp],
     "rta_setup_shirley_rays": [_vp, _vp, _vp, _f, _f, _i, _i, _vp, _vp, _vp, _f, _vp],
     "rta_trace_cis": [_vp, _i, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp, _vp, _vp],

@@ -39,6 +39,7 @@ public:
 	uint32_t *index_buffer;
 	uint n_nodes;
 	bool owns_data = false;
+	bool owns_layout = true;   //!< false: the constructor owns pairs/tri4/flags
 	std::string name = "binary lbvh (Karras) on gfx950";
 	rta_node_pair *pairs = 0;
 	float *tri4 = 0;
@@ -48,9 +49,11 @@ public:
 	binary_lbvh(tri_t *tris, uint n, rta_node *nodes, uint32_t *index)
 	    : triangles(tris), n(n), nodes(nodes), index_buffer(index), n_nodes(2 * n - 1) {}
 	~binary_lbvh() override {
-		rta_hip_free(pairs);
-		rta_hip_free(tri4);
-		rta_hip_free(flags);
+		if (owns_layout) {
+			rta_hip_free(pairs);
+			rta_hip_free(tri4);
+			rta_hip_free(flags);
+		}
 		if (owns_data) {
 			rta_hip_free(triangles);
 			rta_hip_free(nodes);
@@ -140,16 +143,25 @@ class lbvh_constructor : public basic_acceleration_structure_constructor<box_t, 
 	rta_node *d_nodes = 0;
 	uint32_t *d_indices = 0;
 	uint32_t *d_status = 0;
+	rta_node_pair *d_pairs = 0;
+	float *d_tri4 = 0;
+	uint32_t *d_flags = 0;
 
 	void free_data() {
 		rta_hip_free(workspace);
 		rta_hip_free(d_nodes);
 		rta_hip_free(d_indices);
 		rta_hip_free(d_status);
+		rta_hip_free(d_pairs);
+		rta_hip_free(d_tri4);
+		rta_hip_free(d_flags);
 		workspace = 0;
 		d_nodes = 0;
 		d_indices = 0;
 		d_status = 0;
+		d_pairs = 0;
+		d_tri4 = 0;
+		d_flags = 0;
 	}
 
 public:
@@ -175,25 +187,42 @@ public:
 			d_indices = (uint32_t *)p;
 			check(rta_hip_malloc(&p, 4), "status");
 			d_status = (uint32_t *)p;
+			check(rta_hip_malloc(&p, sizeof(rta_node_pair) * (2 * (size_t)n - 1)), "pairs");
+			d_pairs = (rta_node_pair *)p;
+			check(rta_hip_malloc(&p, 48 * (size_t)n), "tri4");
+			d_tri4 = (float *)p;
+			check(rta_hip_malloc(&p, 4), "flags");
+			d_flags = (uint32_t *)p;
 		}
 		last_n = n;
-		check(rta_lbvh_build((rta_triangle *)tris->triangle, (int)n, d_nodes, d_indices, workspace,
-		                     workspace_bytes, d_status, &last_times, 0),
-		      "rta_lbvh_build");
+		// the build writes the packed traversal layout in its own passes
+		check(rta_lbvh_build_packed((rta_triangle *)tris->triangle, (int)n, d_nodes, d_indices,
+		                            d_pairs, d_tri4, d_flags, workspace, workspace_bytes,
+		                            d_status, &last_times, 0),
+		      "rta_lbvh_build_packed");
 		check(rta_hip_memcpy_dtoh(&last_status, d_status, 4), "status");
 		if (last_status & RTA_LBVH_FRONTIER_OVERFLOW)
 			throw std::runtime_error("rta_lbvh_build: bottom-up frontier overflow, upper boxes not merged");
 		if (verbose) {
 			std::cout << "LBVH build of " << n << " triangles [ms]: init " << last_times.init_triangles
 			          << ", bounds " << last_times.voxel_bounds << ", morton " << last_times.morton
-			          << ", sort " << last_times.sort << ", tree " << last_times.tree << ", boxes "
-			          << last_times.boxes << ", permute " << last_times.permute << ", total "
-			          << last_times.total << std::endl;
+			          << ", sort " << last_times.sort << ", tree " << last_times.tree
+			          << ", permute " << last_times.permute << ", boxes " << last_times.boxes
+			          << ", total (with the traversal layout) " << last_times.total << std::endl;
 			if (last_status & RTA_LBVH_CODE_ZERO)
 				std::cout << "note: a Morton code is 0 (the reference's bottom-up pass is racy there)"
 				          << std::endl;
 		}
-		return new binary_lbvh<box_t, tri_t>(tris->triangle, n, d_nodes, d_indices);
+		auto *bvh = new binary_lbvh<box_t, tri_t>(tris->triangle, n, d_nodes, d_indices);
+		bvh->pairs = d_pairs;
+		bvh->tri4 = d_tri4;
+		bvh->flags = d_flags;
+		bvh->owns_layout = false;
+		rta_node root;
+		check(rta_hip_memcpy_dtoh(&root, d_nodes, sizeof root), "root");
+		bvh->root_link = root.type_left_elems;
+		bvh->root_index = root.right_tris;
+		return bvh;
 	}
 };
 

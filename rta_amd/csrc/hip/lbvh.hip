@@ -54,10 +54,10 @@ __global__ __launch_bounds__(kChunk) void bottom_up_chunk_kernel(
     const uint32_t *__restrict__ codes, const uint32_t *__restrict__ indices,
     const uint32_t *__restrict__ parents, const uint2 *__restrict__ ranges,
     uint32_t *__restrict__ slots, uint32_t *__restrict__ counts, uint32_t *__restrict__ status,
-    int n) {
+    int n, PackOut pk) {
 	bottom_up_chunk(nodes, boxes, codes, indices, parents, ranges,
 	                slots + (size_t)blockIdx.x * kSlots, counts + blockIdx.x, status, n,
-	                (int)blockIdx.x);
+	                (int)blockIdx.x, pk);
 }
 
 // One block merges the frontier slots of up to kMergeFan chunks of the
@@ -67,7 +67,8 @@ __global__ __launch_bounds__(1024) void bottom_up_merge_kernel(
     const uint32_t *__restrict__ codes, const uint32_t *__restrict__ parents,
     const uint2 *__restrict__ ranges, const uint32_t *__restrict__ in_slots,
     const uint32_t *__restrict__ in_counts, int n_in, uint32_t *__restrict__ out_slots,
-    uint32_t *__restrict__ out_counts, uint32_t *__restrict__ status, long long chunk, int n) {
+    uint32_t *__restrict__ out_counts, uint32_t *__restrict__ status, long long chunk, int n,
+    PackOut pk) {
 	const int g0 = blockIdx.x * kMergeFan;
 	const int ng = min(kMergeFan, n_in - g0);
 	const uint32_t lo = (uint32_t)(blockIdx.x * chunk);
@@ -75,7 +76,14 @@ __global__ __launch_bounds__(1024) void bottom_up_merge_kernel(
 	bottom_up_merge(nodes, visit, codes, parents, ranges, in_slots + (size_t)g0 * kSlots,
 	                in_counts + g0, ng,
 	                out_slots ? out_slots + (size_t)blockIdx.x * kSlots : nullptr,
-	                out_counts ? out_counts + blockIdx.x : nullptr, status, lo, hi, n);
+	                out_counts ? out_counts + blockIdx.x : nullptr, status, lo, hi, n, pk);
+}
+
+__global__ __launch_bounds__(256) void pack_boundary_pairs_kernel(
+    rta_node_pair *__restrict__ pairs, uint32_t *__restrict__ flags,
+    const rta_node *__restrict__ nodes, const uint2 *__restrict__ ranges, int n_inner) {
+	const int x = blockIdx.x * blockDim.x + threadIdx.x;
+	if (x < n_inner) pack_boundary_pair(pairs, flags, nodes, ranges, (uint32_t)x);
 }
 
 struct BoxesWork {
@@ -111,8 +119,8 @@ static BoxesWork carve_boxes(void *base, int n) {
 
 __global__ __launch_bounds__(kPermuteBlock) void permute_kernel(
     rta_triangle *__restrict__ out, const rta_triangle *__restrict__ scratch,
-    const uint32_t *__restrict__ indices, int n) {
-	permute_block(out, scratch, indices, n, (int)blockIdx.x);
+    const uint32_t *__restrict__ indices, int n, float4 *__restrict__ tri4) {
+	permute_block(out, scratch, indices, n, (int)blockIdx.x, tri4);
 }
 
 // --------------------------------------------------------- workspace
@@ -224,13 +232,13 @@ int rta_lbvh_build_tree(rta_node *nodes, uint32_t *parents, void *work, const ui
 // in *status.
 static int build_boxes(rta_node *nodes, void *work, const rta_aabb *boxes, const uint32_t *codes,
                        const uint32_t *indices, const uint32_t *parents, int n, uint32_t *status,
-                       rta_stream s) {
+                       rta_stream s, PackOut pk = PackOut{}) {
 	BoxesWork w = carve_boxes(work, n);
 	if (!status) status = w.status;
 	hipStream_t st = as_stream(s);
 	int groups = (int)grid_for(n, kChunk);
 	hipLaunchKernelGGL(bottom_up_chunk_kernel, dim3(groups), dim3(kChunk), 0, st, nodes, boxes,
-	                   codes, indices, parents, w.ranges, w.slots[0], w.counts[0], status, n);
+	                   codes, indices, parents, w.ranges, w.slots[0], w.counts[0], status, n, pk);
 	RTA_LAUNCH_CHECK("bottom_up_chunk_kernel");
 	long long chunk = kChunk;
 	int in = 0;
@@ -240,10 +248,15 @@ static int build_boxes(rta_node *nodes, void *work, const rta_aabb *boxes, const
 		hipLaunchKernelGGL(bottom_up_merge_kernel, dim3(blocks), dim3(1024), 0, st, nodes,
 		                   w.visit, codes, parents, w.ranges, w.slots[in], w.counts[in], groups,
 		                   blocks > 1 ? w.slots[1 - in] : nullptr,
-		                   blocks > 1 ? w.counts[1 - in] : nullptr, status, chunk, n);
+		                   blocks > 1 ? w.counts[1 - in] : nullptr, status, chunk, n, pk);
 		RTA_LAUNCH_CHECK("bottom_up_merge_kernel");
 		groups = blocks;
 		in = 1 - in;
+	}
+	if (pk.pairs && n > kChunk) {
+		hipLaunchKernelGGL(pack_boundary_pairs_kernel, dim3(grid_for(n - 1, 256)), dim3(256), 0, st,
+		                   pk.pairs, pk.flags, nodes, w.ranges, n - 1);
+		RTA_LAUNCH_CHECK("pack_boundary_pairs_kernel");
 	}
 	return RTA_OK;
 }
@@ -264,7 +277,7 @@ int rta_lbvh_permute(rta_triangle *out, const rta_triangle *scratch, const uint3
 	if (n <= 0) return invalid("rta_lbvh_permute: n must be > 0");
 	if (!out || !scratch || !indices) return invalid("rta_lbvh_permute: NULL pointer");
 	hipLaunchKernelGGL(permute_kernel, dim3(grid_for(n, kPermuteBlock)), dim3(kPermuteBlock), 0,
-	                   as_stream(s), out, scratch, indices, n);
+	                   as_stream(s), out, scratch, indices, n, (float4 *)nullptr);
 	RTA_LAUNCH_CHECK("permute_kernel");
 	return RTA_OK;
 }
@@ -285,17 +298,24 @@ int rta_lbvh_copy_ranges(uint32_t *ranges, const void *workspace, int n, rta_str
 	return RTA_OK;
 }
 
-int rta_lbvh_build(rta_triangle *tris, int n, rta_node *nodes, uint32_t *indices, void *workspace,
-                   size_t workspace_bytes, uint32_t *status, rta_lbvh_times *times,
-                   rta_stream s) {
-	RTA_FAULT_POINT("rta_lbvh_build");
-	const ScopedRange range("rta_lbvh_build");
+}  // extern "C"
+
+// The seven stages (lbvh_constructor::buildCUDA, bbvh-cuda/lbvh.h:153-211).
+// With pk set, the permutation also writes tri4 and runs before the
+// bottom-up pass, which writes the pair records (and clears the leaf-end
+// marks that collapses make interior), so the traversal layout needs no
+// pass of its own; the times then follow that order.
+static int build_impl(const char *name, rta_triangle *tris, int n, rta_node *nodes,
+                      uint32_t *indices, void *workspace, size_t workspace_bytes,
+                      uint32_t *status, rta_lbvh_times *times, rta_stream s, PackOut pk) {
+	const ScopedRange range(name);
 	// n = 0 is rejected, as the reference cannot build it (bbvh-cuda/lbvh.h:47)
-	if (n <= 0) return invalid("rta_lbvh_build: n must be > 0");
-	if (!tris || !nodes || !indices || !workspace) return invalid("rta_lbvh_build: NULL pointer");
+	if (n <= 0) return invalid((std::string(name) + ": n must be > 0").c_str());
+	if (!tris || !nodes || !indices || !workspace)
+		return invalid((std::string(name) + ": NULL pointer").c_str());
 	LbvhWorkspace w = carve(workspace, n);
 	if (workspace_bytes < w.total) {
-		set_error("rta_lbvh_build: workspace too small");
+		set_error(std::string(name) + ": workspace too small");
 		return RTA_E_WORKSPACE;
 	}
 	hipEvent_t ev[8] = {};
@@ -310,7 +330,16 @@ int rta_lbvh_build(rta_triangle *tris, int n, rta_node *nodes, uint32_t *indices
 			if (e != hipSuccess) rc = hip_status(e, "hipEventRecord");
 		}
 	};
+	const bool packed = pk.pairs != nullptr;
+	auto permute = [&]() {
+		if (rc != RTA_OK) return;
+		hipLaunchKernelGGL(permute_kernel, dim3(grid_for(n, kPermuteBlock)), dim3(kPermuteBlock),
+		                   0, as_stream(s), tris, w.scratch, indices, n, pk.tri4);
+		const hipError_t e = hipGetLastError();
+		if (e != hipSuccess) rc = hip_status(e, "permute_kernel");
+	};
 	if (status) rc = rta_hip_fill_u32(status, 0u, 1, s);
+	if (rc == RTA_OK && packed) rc = rta_hip_fill_u32(pk.flags, 0u, 1, s);
 	if (rc == RTA_OK) rc = rta_lbvh_init_triangles(w.scratch, tris, w.boxes, w.centers, w.partials, n, s);
 	mark(1);
 	if (rc == RTA_OK) rc = rta_lbvh_voxel_bounds(w.bounds, w.partials, n, s);
@@ -321,11 +350,21 @@ int rta_lbvh_build(rta_triangle *tris, int n, rta_node *nodes, uint32_t *indices
 	mark(4);
 	if (rc == RTA_OK) rc = rta_lbvh_build_tree(nodes, w.parents, w.boxes_work, w.codes, n, s);
 	mark(5);
-	if (rc == RTA_OK)
-		rc = build_boxes(nodes, w.boxes_work, w.boxes, w.codes, indices, w.parents, n, status, s);
-	mark(6);
-	if (rc == RTA_OK) rc = rta_lbvh_permute(tris, w.scratch, indices, n, s);
-	mark(7);
+	if (packed) {
+		permute();
+		mark(6);
+		if (rc == RTA_OK)
+			rc = build_boxes(nodes, w.boxes_work, w.boxes, w.codes, indices, w.parents, n, status,
+			                 s, pk);
+		mark(7);
+	} else {
+		if (rc == RTA_OK)
+			rc = build_boxes(nodes, w.boxes_work, w.boxes, w.codes, indices, w.parents, n, status,
+			                 s);
+		mark(6);
+		permute();
+		mark(7);
+	}
 	if (times) {
 		if (rc == RTA_OK) {
 			hipError_t e = hipEventSynchronize(ev[7]);
@@ -333,13 +372,39 @@ int rta_lbvh_build(rta_triangle *tris, int n, rta_node *nodes, uint32_t *indices
 		}
 		if (rc == RTA_OK) {
 			float *slots[7] = {&times->init_triangles, &times->voxel_bounds, &times->morton,
-			                   &times->sort, &times->tree, &times->boxes, &times->permute};
+			                   &times->sort, &times->tree,
+			                   packed ? &times->permute : &times->boxes,
+			                   packed ? &times->boxes : &times->permute};
 			for (int k = 0; k < 7; ++k) (void)hipEventElapsedTime(slots[k], ev[k], ev[k + 1]);
 			(void)hipEventElapsedTime(&times->total, ev[0], ev[7]);
 		}
 		for (int i = 0; i < 8; ++i) (void)hipEventDestroy(ev[i]);
 	}
 	return rc;
+}
+
+extern "C" {
+
+int rta_lbvh_build(rta_triangle *tris, int n, rta_node *nodes, uint32_t *indices, void *workspace,
+                   size_t workspace_bytes, uint32_t *status, rta_lbvh_times *times,
+                   rta_stream s) {
+	RTA_FAULT_POINT("rta_lbvh_build");
+	return build_impl("rta_lbvh_build", tris, n, nodes, indices, workspace, workspace_bytes,
+	                  status, times, s, PackOut{});
+}
+
+int rta_lbvh_build_packed(rta_triangle *tris, int n, rta_node *nodes, uint32_t *indices,
+                          rta_node_pair *pairs, float *tri4, uint32_t *flags, void *workspace,
+                          size_t workspace_bytes, uint32_t *status, rta_lbvh_times *times,
+                          rta_stream s) {
+	RTA_FAULT_POINT("rta_lbvh_build_packed");
+	if (!pairs || !tri4 || !flags) return invalid("rta_lbvh_build_packed: NULL layout pointer");
+	PackOut pk;
+	pk.pairs = pairs;
+	pk.tri4 = reinterpret_cast<float4 *>(tri4);
+	pk.flags = flags;
+	return build_impl("rta_lbvh_build_packed", tris, n, nodes, indices, workspace,
+	                  workspace_bytes, status, times, s, pk);
 }
 
 }  // extern "C"

@@ -305,6 +305,49 @@ __device__ inline rta_aabb union_lr(const rta_aabb &lb, const rta_aabb &rb) {
 	return u;
 }
 
+// The traversal layout written by the build itself (rta_lbvh_build_packed):
+// per inner node x a 64-byte child-pair record pairs[x] (both children's
+// boxes SoA, their words -- (x << 1) | 1 for an inner child, first
+// triangle << 1 for a leaf -- and their node ids), per triangle three
+// float4 of tri4 (a, a - b, a - c; the w of the first is nonzero on the last
+// triangle of a leaf), and flags bit 0 when some box coordinate leaves the
+// exact-division range.  The same bytes as rta_bvh_pack over the finished
+// nodes.  All pointers null: the plain build.
+struct PackOut {
+	rta_node_pair *pairs = nullptr;
+	float4 *tri4 = nullptr;
+	uint32_t *flags = nullptr;
+};
+
+__device__ inline bool box_fast(const rta_aabb &b) {
+	return coord_fast_ok(b.min.x) && coord_fast_ok(b.min.y) && coord_fast_ok(b.min.z) &&
+	       coord_fast_ok(b.max.x) && coord_fast_ok(b.max.y) && coord_fast_ok(b.max.z);
+}
+
+// pairs[x] from its children's boxes, words and ids; returns false when a
+// box coordinate leaves the exact-division range
+__device__ inline bool write_pair(rta_node_pair *pairs, uint32_t x, const rta_aabb &lb,
+                                  const rta_aabb &rb, uint32_t lword, uint32_t rword, uint32_t lid,
+                                  uint32_t rid) {
+	rta_node_pair p;
+	p.lo_x[0] = lb.min.x; p.lo_x[1] = rb.min.x;
+	p.lo_y[0] = lb.min.y; p.lo_y[1] = rb.min.y;
+	p.lo_z[0] = lb.min.z; p.lo_z[1] = rb.min.z;
+	p.hi_x[0] = lb.max.x; p.hi_x[1] = rb.max.x;
+	p.hi_y[0] = lb.max.y; p.hi_y[1] = rb.max.y;
+	p.hi_z[0] = lb.max.z; p.hi_z[1] = rb.max.z;
+	p.link[0] = lword; p.link[1] = rword;
+	p.index[0] = lid; p.index[1] = rid;
+	pairs[x] = p;
+	return box_fast(lb) && box_fast(rb);
+}
+
+// a collapse merges two leaves: the left one's last triangle no longer ends
+// a leaf (the permute wrote every triangle as a leaf end)
+__device__ inline void clear_leaf_end(float4 *tri4, uint32_t pos) {
+	tri4[3 * (size_t)pos].w = 0.0f;
+}
+
 __device__ inline uint32_t rep_code(const uint32_t *codes, uint32_t node, uint32_t leaf0) {
 	return node >= leaf0 ? codes[node - leaf0] : codes[node];
 }
@@ -331,7 +374,8 @@ __device__ inline void bottom_up_chunk(rta_node *__restrict__ nodes,
                                        const uint32_t *__restrict__ parents,
                                        const uint2 *__restrict__ ranges,
                                        uint32_t *__restrict__ slots, uint32_t *__restrict__ count,
-                                       uint32_t *__restrict__ status, int n, int chunk) {
+                                       uint32_t *__restrict__ status, int n, int chunk,
+                                       PackOut pk = PackOut{}) {
 	__shared__ float lb_leaf[6][kChunk];
 	__shared__ float lb_int[6][kChunk];
 	__shared__ uint32_t lvisit[kChunk];
@@ -447,13 +491,42 @@ __device__ inline void bottom_up_chunk(rta_node *__restrict__ nodes,
 		f[0] = lb_leaf[0][t]; f[1] = lb_leaf[1][t]; f[2] = lb_leaf[2][t];
 		f[3] = lb_leaf[3][t]; f[4] = lb_leaf[4][t]; f[5] = lb_leaf[5][t];
 	}
+	bool slow = false;
 	if (k < leaf0 && lelems[t] != kStateNone) {
 		float *f = reinterpret_cast<float *>(&nodes[k]) + 2;
 		f[0] = lb_int[0][t]; f[1] = lb_int[1][t]; f[2] = lb_int[2][t];
 		f[3] = lb_int[3][t]; f[4] = lb_int[4][t]; f[5] = lb_int[5][t];
-		if (lelems[t] != kStateInner)
+		if (lelems[t] != kStateInner) {
 			*reinterpret_cast<uint2 *>(&nodes[k]) = make_uint2(lelems[t] << 1, ltris[t]);
+			// the collapse merged two leaves that meet at the node's split,
+			// the left child's last position: base + its local reference
+			// (a leaf's position, or an inner left child's index, which is
+			// the end of its range)
+			if (pk.tri4) clear_leaf_end(pk.tri4, base + (lkids[t] & 0x3ffu));
+		} else if (pk.pairs) {
+			// a completed inner node: its pair record from the children's
+			// final state (an original leaf, a collapsed node or an inner one)
+			const uint32_t kids = lkids[t];
+			rta_aabb cb[2];
+			uint32_t word[2], id[2];
+			for (int c = 0; c < 2; ++c) {
+				const uint32_t ref = c == 0 ? (kids & 0xffffu) : (kids >> 16);
+				const uint32_t q = ref & 0x3ffu;
+				const float(*src)[kChunk] = (ref & kLocalLeaf) ? lb_leaf : lb_int;
+				cb[c].min = {src[0][q], src[1][q], src[2][q]};
+				cb[c].max = {src[3][q], src[4][q], src[5][q]};
+				if (ref & kLocalLeaf) {
+					word[c] = (base + q) << 1;
+					id[c] = leaf0 + base + q;
+				} else {
+					word[c] = lelems[q] == kStateInner ? ((base + q) << 1) | 1u : ltris[q] << 1;
+					id[c] = base + q;
+				}
+			}
+			slow = !write_pair(pk.pairs, k, cb[0], cb[1], word[0], word[1], id[0], id[1]);
+		}
 	}
+	if (pk.flags && __syncthreads_or(slow) && t == 0) atomicOr(pk.flags, 1u);
 	const uint32_t nf = min(nfront, (uint32_t)kSlots);
 	if ((uint32_t)t < nf) slots[t] = front[t];
 	if (t == 0) *count = nf;
@@ -485,7 +558,7 @@ __device__ inline void bottom_up_merge(rta_node *__restrict__ nodes, uint32_t *_
                                        uint32_t *__restrict__ out_slots,
                                        uint32_t *__restrict__ out_count,
                                        uint32_t *__restrict__ status, uint32_t lo, uint32_t hi,
-                                       int n) {
+                                       int n, PackOut pk = PackOut{}) {
 	__shared__ uint32_t prefix[kMergeFan + 1];
 	__shared__ uint32_t front[kSlots];
 	__shared__ uint32_t nfront;
@@ -551,14 +624,21 @@ __device__ inline void bottom_up_merge(rta_node *__restrict__ nodes, uint32_t *_
 			const uint32_t code_sib = rep_code(codes, sib, leaf0);
 			const uint2 ll = *reinterpret_cast<const uint2 *>(&nodes[left]);
 			const uint2 rl = *reinterpret_cast<const uint2 *>(&nodes[right]);
-			box = cur_is_left ? union_lr(box, sbox) : union_lr(sbox, box);
+			const rta_aabb lbox = cur_is_left ? box : sbox, rbox = cur_is_left ? sbox : box;
+			box = union_lr(lbox, rbox);
 			store_box(&nodes[parent], box);
-			if (code_cur == code_sib && !link_inner(ll.x) && !link_inner(rl.x)) {
+			const bool collapse = code_cur == code_sib && !link_inner(ll.x) && !link_inner(rl.x);
+			if (collapse) {
 				const uint32_t tris = ll.y < rl.y ? ll.y : rl.y;
 				const uint32_t elems = (rl.x >> 1) + (ll.x >> 1);
 				*reinterpret_cast<uint2 *>(&nodes[parent]) = make_uint2(elems << 1, tris);
 			}
 			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+			// the leaf-end mark is read only after the build: its store
+			// follows the fence, which waits for the box and links only.
+			// (Pair records of the nodes completed here are written by
+			// pack_boundary_pairs after the last level.)
+			if (collapse && pk.tri4) clear_leaf_end(pk.tri4, ll.y + (ll.x >> 1) - 1u);
 			cur = parent;
 			parent = grand;
 			code_cur = code_parent;
@@ -574,6 +654,25 @@ __device__ inline void bottom_up_merge(rta_node *__restrict__ nodes, uint32_t *_
 	if (out_count && t == 0) *out_count = nf;
 }
 
+// Pair records of the inner nodes the chunk level cannot complete: those
+// whose range [from, to] spans more than one kChunk-leaf chunk, finished by
+// the merge levels.  Node x is one of them iff from / kChunk != to / kChunk;
+// its record comes from the finished nodes, as rta_bvh_pack forms it.
+__device__ inline void pack_boundary_pair(rta_node_pair *__restrict__ pairs,
+                                          uint32_t *__restrict__ flags,
+                                          const rta_node *__restrict__ nodes,
+                                          const uint2 *__restrict__ ranges, uint32_t x) {
+	const uint2 rg = ranges[x];
+	if (rg.x / (uint32_t)kChunk == rg.y / (uint32_t)kChunk) return;
+	const uint2 links = *reinterpret_cast<const uint2 *>(&nodes[x]);
+	if (!link_inner(links.x)) return;   // collapsed into a leaf
+	const uint32_t L = links.x >> 1, R = links.y;
+	const rta_node nl = nodes[L], nr = nodes[R];
+	const uint32_t lw = link_inner(nl.type_left_elems) ? (L << 1) | 1u : nl.right_tris << 1;
+	const uint32_t rw = link_inner(nr.type_left_elems) ? (R << 1) | 1u : nr.right_tris << 1;
+	if (!write_pair(pairs, x, nl.box, nr.box, lw, rw, L, R)) atomicOr(flags, 1u);
+}
+
 // -------------------------------------------------- stage 7: permute
 
 // out[i] = scratch[indices[i]] (invert_triangle_permutation_cuda,
@@ -583,10 +682,13 @@ __device__ inline void bottom_up_merge(rta_node *__restrict__ nodes, uint32_t *_
 // store of a wave is contiguous and each scattered source record is read
 // by 13 neighbouring lanes.  All threads of the block must call it.
 constexpr int kPermuteBlock = 256;
+constexpr int kVertexWords = 5;   // the 36 bytes of a, b, c lie in words 0..4
 __device__ inline void permute_block(rta_triangle *__restrict__ out,
                                      const rta_triangle *__restrict__ scratch,
-                                     const uint32_t *__restrict__ indices, int n, int block) {
+                                     const uint32_t *__restrict__ indices, int n, int block,
+                                     float4 *__restrict__ tri4 = nullptr) {
 	__shared__ uint32_t src[kPermuteBlock];
+	__shared__ uint2 verts[kPermuteBlock][kVertexWords];
 	const int t = threadIdx.x;
 	const int base = block * kPermuteBlock;
 	const int count = min(kPermuteBlock, n - base);
@@ -608,6 +710,29 @@ __device__ inline void permute_block(rta_triangle *__restrict__ out,
 	for (int k = 0; k < kTriWords; ++k) {
 		const int w = t + k * kPermuteBlock;
 		if (w < nwords) d[w] = v[k];
+	}
+	if (tri4) {
+		// the vertex words of each triangle, gathered in LDS, give its three
+		// tri4 records (a, a - b, a - c; every triangle first marked as the
+		// end of a leaf, the bottom-up pass clears the marks that collapses
+		// make interior)
+#pragma unroll
+		for (int k = 0; k < kTriWords; ++k) {
+			const int w = t + k * kPermuteBlock;
+			if (w < nwords) {
+				const int r = w / kTriWords, c = w - r * kTriWords;
+				if (c < kVertexWords) verts[r][c] = v[k];
+			}
+		}
+		__syncthreads();
+		if (t < count) {
+			const float *f = reinterpret_cast<const float *>(&verts[t][0]);
+			const float ax = f[0], ay = f[1], az = f[2];
+			float4 *o = tri4 + 3 * (size_t)(base + t);
+			o[0] = make_float4(ax, ay, az, __uint_as_float(1u));
+			o[1] = make_float4(ax - f[3], ay - f[4], az - f[5], 0.0f);
+			o[2] = make_float4(ax - f[6], ay - f[7], az - f[8], 0.0f);
+		}
 	}
 }
 

@@ -199,10 +199,10 @@ class DynamicFrame:
         call("rta_jitter_triangles", ptr(self.tris), ptr(self.verts), ptr(self.idx), self.n,
              self.sigma, C.c_uint64(self.seed), 0 if frame is None else frame, fw, s)
         self._mark(timed)
-        call("rta_lbvh_build", ptr(self.tris), self.n, ptr(self.nodes), ptr(self.index),
-             ptr(self.ws), C.c_size_t(self.ws_bytes), ptr(self.status), None, s)
-        call("rta_bvh_pack", ptr(self.pairs), ptr(self.tri4), ptr(self.flags), ptr(self.nodes),
-             2 * self.n - 1, ptr(self.tris), self.n, s)
+        # the build writes the traversal layout in its own passes
+        call("rta_lbvh_build_packed", ptr(self.tris), self.n, ptr(self.nodes), ptr(self.index),
+             ptr(self.pairs), ptr(self.tri4), ptr(self.flags), ptr(self.ws),
+             C.c_size_t(self.ws_bytes), ptr(self.status), None, s)
         self._mark(timed)
         cam, p = self.cam, self.rays[0]
         call("rta_setup_shirley_rays", ptr(p.gpu_direction), ptr(p.gpu_origin), ptr(p.gpu_maxt),

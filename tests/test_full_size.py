@@ -67,6 +67,22 @@ def test_terrain_benchmark_workload_bit_exact(gpu):
     assert int(flags.download(np.uint32, 1)[0]) == 0      # the exact-division path
     root = (int(got_nodes[0]["type_left_elems"]), int(got_nodes[0]["right_tris"]))
 
+    say("packed build (the layout written by the build's own passes)")
+    t2 = DeviceBuffer.from_array(tris_h)
+    nodes2, idx2 = DeviceBuffer(32 * (2 * n - 1)), DeviceBuffer(4 * n)
+    pairs2, tri42, flags2 = DeviceBuffer(64 * (2 * n - 1)), DeviceBuffer(48 * n), DeviceBuffer(4)
+    call("rta_lbvh_build_packed", ptr(t2), n, ptr(nodes2), ptr(idx2), ptr(pairs2), ptr(tri42),
+         ptr(flags2), ptr(ws), C.c_size_t(sz.value), ptr(status), None, None)
+    rta_amd.synchronize()
+    assert nodes2.download(NODE_DTYPE, 2 * n - 1).tobytes() == got_nodes.tobytes()
+    assert np.array_equal(idx2.download(np.uint32, n), idx.download(np.uint32, n))
+    assert np.array_equal(tri42.download(np.uint8, 48 * n), tri4.download(np.uint8, 48 * n))
+    inner = (got_nodes["type_left_elems"] & 1) == 1
+    assert np.array_equal(pairs2.download(np.uint8, 64 * (2 * n - 1)).reshape(-1, 64)[inner],
+                          pairs.download(np.uint8, 64 * (2 * n - 1)).reshape(-1, 64)[inner])
+    assert int(flags2.download(np.uint32, 1)[0]) == 0
+    del t2, nodes2, idx2, pairs2, tri42, flags2
+
     say("primary rays")
     prim = rta_amd.RayBuffers(R)
     f3 = lambda v: (C.c_float * 3)(*[float(x) for x in v])

@@ -486,3 +486,74 @@ def test_two_bounce_chain(gpu):
         cur_rays, cur_o, cur_d = b_rays, go, gd
         cur_exp = np.concatenate([exp, got[count:]])
         cur_exp["t"][count:] = F32MAX
+
+
+def _build_then_pack(tris):
+    n = len(tris)
+    t = DeviceBuffer.from_array(tris)
+    nodes, idx, st = DeviceBuffer(32 * (2 * n - 1)), DeviceBuffer(4 * n), DeviceBuffer(4)
+    sz = C.c_size_t(0)
+    call("rta_lbvh_workspace_bytes", n, C.byref(sz))
+    ws = DeviceBuffer(sz.value)
+    call("rta_lbvh_build", ptr(t), n, ptr(nodes), ptr(idx), ptr(ws), C.c_size_t(sz.value),
+         ptr(st), None, None)
+    pairs, tri4, flags = DeviceBuffer(64 * (2 * n - 1)), DeviceBuffer(48 * n), DeviceBuffer(4)
+    call("rta_bvh_pack", ptr(pairs), ptr(tri4), ptr(flags), ptr(nodes), 2 * n - 1, ptr(t), n,
+         None)
+    rta_amd.synchronize()
+    return {"nodes": nodes.download(NODE_DTYPE, 2 * n - 1), "idx": idx.download(np.uint32, n),
+            "tris": t.download(TRI_DTYPE, n), "status": int(st.download(np.uint32, 1)[0]),
+            "pairs": pairs.download(np.uint8, 64 * (2 * n - 1)).reshape(-1, 64),
+            "tri4": tri4.download(np.uint8, 48 * n), "flags": int(flags.download(np.uint32, 1)[0])}
+
+
+def _build_packed(tris):
+    n = len(tris)
+    t = DeviceBuffer.from_array(tris)
+    nodes, idx, st = DeviceBuffer(32 * (2 * n - 1)), DeviceBuffer(4 * n), DeviceBuffer(4)
+    sz = C.c_size_t(0)
+    call("rta_lbvh_workspace_bytes", n, C.byref(sz))
+    ws = DeviceBuffer(sz.value)
+    pairs, tri4, flags = DeviceBuffer(64 * (2 * n - 1)), DeviceBuffer(48 * n), DeviceBuffer(4)
+    flags.fill_u32(0xffffffff)               # the build must clear it
+    tri4.fill_u32(0x7fc00000)                # and write every tri4 word
+    call("rta_lbvh_build_packed", ptr(t), n, ptr(nodes), ptr(idx), ptr(pairs), ptr(tri4),
+         ptr(flags), ptr(ws), C.c_size_t(sz.value), ptr(st), None, None)
+    rta_amd.synchronize()
+    return {"nodes": nodes.download(NODE_DTYPE, 2 * n - 1), "idx": idx.download(np.uint32, n),
+            "tris": t.download(TRI_DTYPE, n), "status": int(st.download(np.uint32, 1)[0]),
+            "pairs": pairs.download(np.uint8, 64 * (2 * n - 1)).reshape(-1, 64),
+            "tri4": tri4.download(np.uint8, 48 * n), "flags": int(flags.download(np.uint32, 1)[0])}
+
+
+PACKED_SCENES = dict(SCENES, **{
+    "one": lambda: scenes.random_triangles(1, seed=1),
+    "two": lambda: scenes.random_triangles(2, seed=2),
+    "dup_heavy": lambda: clustered(70_000, 5, seed=8),        # long collapsed runs
+    "two_merge_levels": lambda: scenes.random_triangles(100_003, seed=12),
+    "outside_fast_range": lambda: scenes.random_triangles(3000, seed=13, lo=0.0, hi=1e30),
+})
+
+
+@pytest.mark.parametrize("name", list(PACKED_SCENES))
+def test_lbvh_build_packed_equals_build_then_pack(gpu, name):
+    """rta_lbvh_build_packed writes the traversal layout inside the build's
+    own passes; its nodes, indices, triangles and status equal
+    rta_lbvh_build's, its tri4 and flags rta_bvh_pack's byte for byte, and
+    so does the pair record of every inner node."""
+    tris = PACKED_SCENES[name]()
+    say(f"packed build scene={name} n={len(tris)}")
+    a, b = _build_then_pack(tris), _build_packed(tris)
+    assert a["nodes"].tobytes() == b["nodes"].tobytes()
+    assert np.array_equal(a["idx"], b["idx"])
+    assert a["tris"].tobytes() == b["tris"].tobytes()
+    assert a["status"] == b["status"]
+    assert np.array_equal(a["tri4"], b["tri4"])
+    assert a["flags"] == b["flags"]
+    inner = (a["nodes"]["type_left_elems"] & 1) == 1
+    assert np.array_equal(a["pairs"][inner], b["pairs"][inner])
+    if name == "outside_fast_range":
+        assert b["flags"] == 1
+    if name == "dup_heavy":
+        n = len(tris)
+        assert inner.sum() < n - 1               # collapses happened
