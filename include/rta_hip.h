@@ -111,7 +111,7 @@ const char *rta_hip_last_error(void);
  * `status` (non-zero) before doing any work, with the message "<entry>:
  * injected fault"; one shot.  Entry points with this hook: rta_hip_malloc,
  * rta_lbvh_build, rta_lbvh_build_packed, rta_trace_cis_persistent,
- * rta_pred_infer. */
+ * rta_pred_infer, rta_pred_train_step, rta_pred_train_batches. */
 int rta_hip_inject_fault(const char *entry, int status);
 /* Named host-side ranges (roctx) for a profiler's timeline; the library
  * marks rta_lbvh_build and the persistent traversals with them. */
@@ -496,6 +496,29 @@ int rta_pred_adam(rta_pred pred, float lr, int step, rta_stream s);
 /* The same with the step count in device memory: *step is incremented and
  * the step then taken with it (for captured, replayed launch sequences). */
 int rta_pred_adam_device(rta_pred pred, float lr, uint32_t *step, rta_stream s);
+/* One fused training step on rays [0, n) (n in 1..max_batch): forward pass,
+ * the loss of rta_pred_backward, gradients and an Adam step in three
+ * launches, without atomics, so the result does not depend on scheduling.
+ * The 1040 outputs of a ray are never stored.  The gradient buffer
+ * (rta_pred_get_grads) receives this step's gradient, replacing its
+ * contents; *step (device word) is incremented and the Adam step taken with
+ * it, as rta_pred_adam_device does; the loss is added to *loss (device
+ * float).  Bias sums of the 1024 logits' softmax and of every gradient run
+ * in another order than rta_pred_forward/backward, so results agree with
+ * that path to float rounding, not bit for bit. */
+int rta_pred_train_step(rta_pred pred, const float *ray_orig, const float *ray_dir,
+                        const int32_t *label, const float *target, int n,
+                        const float *scene_min, const float *scene_max, float lr,
+                        uint32_t *step, float *loss, rta_stream s);
+/* rta_pred_train_step over n_batches batches of the n rays: batch b covers
+ * rays [starts[b], min(starts[b] + batch, n)) (starts: host array, each in
+ * [0, n); batch in 1..max_batch).  The launches of all batches are queued
+ * by one call. */
+int rta_pred_train_batches(rta_pred pred, const float *ray_orig, const float *ray_dir,
+                           const int32_t *label, const float *target, int n,
+                           const int32_t *starts, int n_batches, int batch,
+                           const float *scene_min, const float *scene_max, float lr,
+                           uint32_t *step, float *loss, rta_stream s);
 /* n_out rays with their hit records drawn from the first *count rays of a
  * set (device arrays): entry i takes ray hash(seed, frame, i) mod *count,
  * frame = *frame_word (or 0 when frame_word is NULL); an empty set gives
@@ -511,7 +534,7 @@ int rta_pred_sample(const float *ray_orig, const float *ray_dir, const rta_hit *
  * mantissa bits replaced by 1023 - treelet (with top_k 1, entries 1..3 are
  * -1); logdist[i] (optional): the log-distance output; logits (optional,
  * n x 1040 floats, for testing): all outputs.  The weights are those of the
- * last rta_pred_set_params / rta_pred_adam on the same stream.  Same network
+ * last rta_pred_set_params / Adam step on the same stream.  Same network
  * and encoding as rta_pred_forward; the sums run in another order (16x16x32
  * tiles, bias first), so logits may differ from it in the last bits. */
 int rta_pred_infer(rta_pred pred, const float *ray_orig, const float *ray_dir, int n,

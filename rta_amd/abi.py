@@ -165,6 +165,9 @@ HIP_SIGNATURES = {
     "rta_pred_adam": [_vp, _f, _i, _vp],
     "rta_pred_adam_device": [_vp, _f, _vp, _vp],
     "rta_pred_sample": [_vp, _vp, _vp, _vp, _i, _vp, _u64, _vp, _vp, _vp, _vp],
+    "rta_pred_train_step": [_vp, _vp, _vp, _vp, _vp, _i, _vp, _vp, _f, _vp, _vp, _vp],
+    "rta_pred_train_batches": [_vp, _vp, _vp, _vp, _vp, _i, _vp, _i, _i, _vp, _vp, _f, _vp, _vp,
+                               _vp],
     "rta_lbvh_copy_ranges": [_vp, _vp, _i, _vp],
     "rta_treelet_cut": [_vp, _u32, _u32, _vp, _i, _i, _vp, _vp, _vp],
     "rta_treelet_labels": [_vp, _i, _vp, _vp, _vp, _vp, _vp],

@@ -41,6 +41,82 @@ constexpr int kParams = oB4 + kOutP;  // 83,152 with padding (81,537 used)
 
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef unsigned v4u __attribute__((ext_vector_type(4)));
+typedef __bf16 v8b __attribute__((ext_vector_type(8)));
+typedef float v2f __attribute__((ext_vector_type(2)));
+typedef __bf16 v2b __attribute__((ext_vector_type(2)));
+
+// ------------------------------------------------- MFMA weight packs
+//
+// The fused kernels compute every product transposed, D = W H (rows =
+// output features, columns = rays), with the 16x16x32 bf16 MFMA: lane l
+// supplies A row l%16 and B column (ray) l%16, and the 8 k-slots of lane
+// group g = l/16.  Lane l then holds D[4g+i][l%16], which is already the
+// next product's B operand once the k-slots of a 32-wide k-step s are
+//   slot j<4  -> feature 32s + 4g + j        (tile 2s)
+//   slot j>=4 -> feature 32s + 16 + 4g + j-4 (tile 2s+1)
+// The first layer's slots are the encoded inputs rotated by 6 (k' =
+// feature - 6 for the 96 sin/cos features, then the 6 raw inputs at k' =
+// 96..101, zeros from 102) and assigned k' = 32s + 8g + j: slot (s, g, j) is
+// octave block 4s+g, which a lane computes by itself.
+//
+// Forward pack: W as A operands, one 16-B record per (16-row tile, k-step,
+// lane).  Backward pack: W^T as A operands of dH = W^T dZ (rows = input
+// features k, k-steps over output features n, same slot rule), for layers
+// 4, 3, 2.  Every writer of the parameters (initialisation, set_params,
+// Adam) rewrites the bf16 copy and both packs, so they always match.
+constexpr int kRec1 = 4 * 4 * 64;                     // layer 1: 4 tiles x 4 k-steps
+constexpr int kRecH = 4 * 2 * 64;                     // layers 2, 3
+constexpr int kRec4 = (kOutP / 16) * 2 * 64;          // layer 4: 65 tiles x 2 k-steps
+constexpr int pW1 = 0, pW2 = pW1 + kRec1, pW3 = pW2 + kRecH, pW4 = pW3 + kRecH;
+constexpr int kPackRecords = pW4 + kRec4;             // 16-B records (165,888 B)
+constexpr int kStepsT4 = (kOutP + 31) / 32;           // 33 k-steps over W4's rows
+constexpr int qW4 = 0, qW3 = qW4 + 4 * kStepsT4 * 64, qW2 = qW3 + kRecH;
+constexpr int kPackTRecords = qW2 + kRecH;            // 9,472 records (151,552 B)
+
+// group and slot of feature f within its 32-wide k-step (hidden layers)
+__device__ __host__ inline void slot_of(int f, int &g, int &j) {
+	const int r = f & 15;
+	g = r >> 2;
+	j = 4 * ((f >> 4) & 1) + (r & 3);
+}
+
+// element (bf16 index) of the forward pack holding W_L[n][k]; -1 for W1's
+// padding columns, which are not packed
+__device__ __host__ inline int fwd_elem(int L, int n, int k) {
+	if (L == 1) {
+		if (k >= kIn) return -1;
+		const int kp = k >= 6 ? k - 6 : k + 96;
+		return 8 * (pW1 + ((n >> 4) * 4 + (kp >> 5)) * 64 + 16 * ((kp & 31) >> 3) + (n & 15)) +
+		       (kp & 7);
+	}
+	const int base = L == 2 ? pW2 : (L == 3 ? pW3 : pW4);
+	int g, j;
+	slot_of(k, g, j);
+	return 8 * (base + ((n >> 4) * 2 + (k >> 5)) * 64 + 16 * g + (n & 15)) + j;
+}
+
+// element of the backward pack holding W_L[n][k], L = 2..4
+__device__ __host__ inline int bwd_elem(int L, int n, int k) {
+	const int base = L == 4 ? qW4 : (L == 3 ? qW3 : qW2);
+	const int steps = L == 4 ? kStepsT4 : 2;
+	int g, j;
+	slot_of(n, g, j);
+	return 8 * (base + ((k >> 4) * steps + (n >> 5)) * 64 + 16 * g + (k & 15)) + j;
+}
+
+// layer, row and column of parameter i; false for a bias
+__device__ __host__ inline bool weight_at(int i, int &L, int &n, int &k) {
+	int off, ld;
+	if (i < oB1) { L = 1; off = oW1; ld = kInP; }
+	else if (i >= oW2 && i < oB2) { L = 2; off = oW2; ld = kHid; }
+	else if (i >= oW3 && i < oB3) { L = 3; off = oW3; ld = kHid; }
+	else if (i >= oW4 && i < oB4) { L = 4; off = oW4; ld = kHid; }
+	else return false;
+	n = (i - off) / ld;
+	k = (i - off) % ld;
+	return true;
+}
 
 // float -> bf16, round to nearest even (finite inputs)
 __device__ __host__ inline uint16_t to_bf16(float f) {
@@ -312,29 +388,52 @@ __global__ __launch_bounds__(256) void topk_kernel(int32_t *__restrict__ top,
 	if (logdist && lane == 0) logdist[row] = o[kClusters];
 }
 
-// Adam on the packed parameters; refreshes the bf16 weight copies.
-// step_word (optional): take the step from device memory (bias corrections
-// computed here) instead of c1, c2, so a captured launch advances with it.
-__global__ __launch_bounds__(256) void adam_kernel(float *__restrict__ p, float *__restrict__ mo,
-                                                   float *__restrict__ ve,
-                                                   const float *__restrict__ g,
-                                                   uint16_t *__restrict__ wb, float lr, float c1,
-                                                   float c2, const uint32_t *__restrict__ step_word) {
-	const int i = blockIdx.x * 256 + threadIdx.x;
-	if (i >= kParams) return;
-	if (step_word) {
-		const float st = (float)*step_word;
-		c1 = 1.0f / (1.0f - powf(0.9f, st));
-		c2 = 1.0f / (1.0f - powf(0.999f, st));
-	}
-	const float gi = g[i];
+// The bf16 copy and both MFMA packs of parameter i (biases: the copy only).
+__device__ inline void store_param(int i, float v, uint16_t *__restrict__ wb,
+                                   uint16_t *__restrict__ wp, uint16_t *__restrict__ wpt) {
+	const uint16_t b = to_bf16(v);
+	wb[i] = b;
+	int L, n, k;
+	if (!weight_at(i, L, n, k)) return;
+	const int e = fwd_elem(L, n, k);
+	if (e >= 0) wp[e] = b;
+	if (L >= 2) wpt[bwd_elem(L, n, k)] = b;
+}
+
+// One Adam step of parameter i with gradient gi (beta 0.9 / 0.999, eps
+// 1e-8; c1, c2: the bias corrections); returns the new parameter.
+__device__ inline float adam_step(float *__restrict__ p, float *__restrict__ mo,
+                                  float *__restrict__ ve, int i, float gi, float lr, float c1,
+                                  float c2) {
 	const float m1 = 0.9f * mo[i] + 0.1f * gi;
 	const float v1 = 0.999f * ve[i] + 0.001f * gi * gi;
 	mo[i] = m1;
 	ve[i] = v1;
 	const float pi = p[i] - lr * (m1 * c1) / (sqrtf(v1 * c2) + 1e-8f);
 	p[i] = pi;
-	wb[i] = to_bf16(pi);
+	return pi;
+}
+
+__device__ inline void bias_corrections(const uint32_t *step_word, float &c1, float &c2) {
+	const float st = (float)*step_word;
+	c1 = 1.0f / (1.0f - powf(0.9f, st));
+	c2 = 1.0f / (1.0f - powf(0.999f, st));
+}
+
+// Adam on the packed parameters with the accumulated gradients g.
+// step_word (optional): take the step from device memory (bias corrections
+// computed here) instead of c1, c2, so a captured launch advances with it.
+__global__ __launch_bounds__(256) void adam_kernel(float *__restrict__ p, float *__restrict__ mo,
+                                                   float *__restrict__ ve,
+                                                   const float *__restrict__ g,
+                                                   uint16_t *__restrict__ wb,
+                                                   uint16_t *__restrict__ wp,
+                                                   uint16_t *__restrict__ wpt, float lr, float c1,
+                                                   float c2, const uint32_t *__restrict__ step_word) {
+	const int i = blockIdx.x * 256 + threadIdx.x;
+	if (i >= kParams) return;
+	if (step_word) bias_corrections(step_word, c1, c2);
+	store_param(i, adam_step(p, mo, ve, i, g[i], lr, c1, c2), wb, wp, wpt);
 }
 
 // n_out rays (and their hit records) drawn from the first *count of a ray
@@ -367,27 +466,21 @@ __global__ __launch_bounds__(256) void sample_kernel(
 	hits_out[i] = hits[j];
 }
 
-__global__ __launch_bounds__(256) void to_bf16_kernel(uint16_t *__restrict__ wb,
-                                                      const float *__restrict__ p) {
+// the bf16 copy and packs of every parameter (after initialisation or
+// set_params)
+__global__ __launch_bounds__(256) void store_params_kernel(uint16_t *__restrict__ wb,
+                                                           uint16_t *__restrict__ wp,
+                                                           uint16_t *__restrict__ wpt,
+                                                           const float *__restrict__ p) {
 	const int i = blockIdx.x * 256 + threadIdx.x;
-	if (i < kParams) wb[i] = to_bf16(p[i]);
+	if (i < kParams) store_param(i, p[i], wb, wp, wpt);
 }
 
 // ------------------------------------------------- fused inference
 //
 // One kernel from rays to top-4 treelets: activations stay in registers and
-// the 1040 logits of a ray never reach memory.  The products are computed
-// transposed, D = W H (rows = output features, columns = rays), with the
-// 16x16x32 bf16 MFMA: lane l supplies weight row l%16 and ray l%16, and the
-// 8 k-slots of lane group g = l/16.  Lane l then holds D[4g+i][l%16], which
-// is already the next layer's B operand once the k-slots are assigned as
-//   k-step s, group g, slot j<4  -> feature 32s + 4g + j       (tile 2s)
-//                      slot j>=4 -> feature 32s + 16 + 4g + j-4 (tile 2s+1)
-// so the weights are stored permuted to match (pack_infer_kernel), one 16-B
-// record per (tile, k-step, lane).  The first layer's slots are the encoded
-// inputs rotated by 6 (k' = feature - 6 for the 96 sin/cos features, then
-// the 6 raw inputs at k' = 96..101): slot (s, g, j) is octave block 4s+g,
-// which a lane computes by itself.
+// the 1040 logits of a ray never reach memory (operand layout and weight
+// packs: see the MFMA weight packs above).
 //
 // Top-4 per ray: each logit becomes a key whose low 10 mantissa bits are
 // replaced by 1023 - treelet (distinct keys; the order of the keys is the
@@ -396,17 +489,7 @@ __global__ __launch_bounds__(256) void to_bf16_kernel(uint16_t *__restrict__ wb,
 // max and three med3 per key, and the four lanes of a ray merged by
 // shuffles at the end.
 
-typedef unsigned v4u __attribute__((ext_vector_type(4)));
-typedef __bf16 v8b __attribute__((ext_vector_type(8)));
-typedef float v2f __attribute__((ext_vector_type(2)));
-typedef __bf16 v2b __attribute__((ext_vector_type(2)));
-
 constexpr int kInferWaves = 16;                       // 1024 threads per block
-constexpr int kRec1 = 4 * 4 * 64;                     // layer 1: 4 tiles x 4 k-steps
-constexpr int kRecH = 4 * 2 * 64;                     // layers 2, 3
-constexpr int kRec4 = (kOutP / 16) * 2 * 64;          // layer 4: 65 tiles x 2 k-steps
-constexpr int pW1 = 0, pW2 = pW1 + kRec1, pW3 = pW2 + kRecH, pW4 = pW3 + kRecH;
-constexpr int kPackRecords = pW4 + kRec4;             // 16-B records (165,888 B)
 
 __device__ inline unsigned bf16x2(float a, float b) {
 	return __builtin_bit_cast(unsigned, __builtin_convertvector((v2f){a, b}, v2b));
@@ -417,54 +500,62 @@ __device__ inline v4f mfma32(v4u a, v4u b, v4f c) {
 	                                               __builtin_bit_cast(v8b, b), c, 0, 0, 0);
 }
 
-// wp[e] for element e = 8 * record + slot of the permuted bf16 weights
-__global__ __launch_bounds__(256) void pack_infer_kernel(uint16_t *__restrict__ wp,
-                                                         const uint16_t *__restrict__ wb) {
-	const int e = blockIdx.x * 256 + threadIdx.x;
-	if (e >= kPackRecords * 8) return;
-	const int rec = e >> 3, j = e & 7, lane = rec & 63, g = lane >> 4;
-	uint16_t val = 0;
-	if (rec < pW2) {
-		const int r = rec - pW1, t = r / (4 * 64), s = (r / 64) % 4;
-		const int kp = 32 * s + 8 * g + j;
-		const int col = kp < 96 ? kp + 6 : (kp < 102 ? kp - 96 : -1);
-		if (col >= 0) val = wb[oW1 + (16 * t + (lane & 15)) * kInP + col];
-	} else {
-		const int base = rec < pW3 ? pW2 : (rec < pW4 ? pW3 : pW4);
-		const int off = rec < pW3 ? oW2 : (rec < pW4 ? oW3 : oW4);
-		const int r = rec - base, t = r / (2 * 64), s = (r / 64) % 2;
-		const int k = j < 4 ? 32 * s + 4 * g + j : 32 * s + 16 + 4 * g + (j - 4);
-		val = wb[off + (16 * t + (lane & 15)) * kHid + k];
-	}
-	wp[e] = val;
-}
-
-// one 64-wide ReLU layer for two sets of 16 rays; in: KS k-steps of B
+// one 64-wide ReLU layer for NS sets of 16 rays; in: KS k-steps of B
 // operands per set, out: the next layer's 2 k-steps per set
-template <int KS>
-__device__ inline void hidden_layer(v4u out[2][2], const v4u in[2][KS], const v4u *__restrict__ wp,
-                                    const float *__restrict__ bias, int lane) {
-	unsigned d[2][4][2];
+template <int KS, int NS>
+__device__ inline void hidden_layer(v4u out[NS][2], const v4u in[NS][KS],
+                                    const v4u *__restrict__ wp, const float *__restrict__ bias,
+                                    int lane) {
+	unsigned d[NS][4][2];
 	#pragma unroll
 	for (int t = 0; t < 4; ++t) {
 		const float4 b = *reinterpret_cast<const float4 *>(bias + 16 * t + 4 * (lane >> 4));
-		v4f a0 = {b.x, b.y, b.z, b.w}, a1 = a0;
+		v4f a[NS];
+		#pragma unroll
+		for (int r = 0; r < NS; ++r) a[r] = (v4f){b.x, b.y, b.z, b.w};
 		#pragma unroll
 		for (int s = 0; s < KS; ++s) {
 			const v4u w = wp[(t * KS + s) * 64 + lane];
-			a0 = mfma32(w, in[0][s], a0);
-			a1 = mfma32(w, in[1][s], a1);
+			#pragma unroll
+			for (int r = 0; r < NS; ++r) a[r] = mfma32(w, in[r][s], a[r]);
 		}
-		d[0][t][0] = bf16x2(fmaxf(a0[0], 0.f), fmaxf(a0[1], 0.f));
-		d[0][t][1] = bf16x2(fmaxf(a0[2], 0.f), fmaxf(a0[3], 0.f));
-		d[1][t][0] = bf16x2(fmaxf(a1[0], 0.f), fmaxf(a1[1], 0.f));
-		d[1][t][1] = bf16x2(fmaxf(a1[2], 0.f), fmaxf(a1[3], 0.f));
+		#pragma unroll
+		for (int r = 0; r < NS; ++r) {
+			d[r][t][0] = bf16x2(fmaxf(a[r][0], 0.f), fmaxf(a[r][1], 0.f));
+			d[r][t][1] = bf16x2(fmaxf(a[r][2], 0.f), fmaxf(a[r][3], 0.f));
+		}
 	}
 	#pragma unroll
-	for (int r = 0; r < 2; ++r)
+	for (int r = 0; r < NS; ++r)
 		#pragma unroll
 		for (int s = 0; s < 2; ++s)
 			out[r][s] = (v4u){d[r][2 * s][0], d[r][2 * s][1], d[r][2 * s + 1][0], d[r][2 * s + 1][1]};
+}
+
+// layer-1 B operands of the ray in lane column col (zero beyond n): octave
+// blocks 4s+g for s = 0..2 and the raw inputs (group 0) at s = 3
+__device__ inline void encode_operands(v4u x[4], const float *__restrict__ ro,
+                                       const float *__restrict__ rd, int ray, int n, float3 lo,
+                                       float3 scale, int g) {
+	float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f;
+	if (ray < n) {
+		v0 = (ro[3 * (size_t)ray + 0] - lo.x) * scale.x - 1.0f;
+		v1 = (ro[3 * (size_t)ray + 1] - lo.y) * scale.y - 1.0f;
+		v2 = (ro[3 * (size_t)ray + 2] - lo.z) * scale.z - 1.0f;
+		v3 = rd[3 * (size_t)ray + 0];
+		v4 = rd[3 * (size_t)ray + 1];
+		v5 = rd[3 * (size_t)ray + 2];
+	}
+	// block 4s+g encodes input 2s + g/2, octaves 4*(g%2)..+3
+	const float in[3] = {(g & 2) ? v1 : v0, (g & 2) ? v3 : v2, (g & 2) ? v5 : v4};
+	#pragma unroll
+	for (int s = 0; s < 3; ++s) {
+		float e[8];
+		encode_octaves(in[s], (g & 1) ? 4 : 0, e);
+		x[s] = (v4u){bf16x2(e[0], e[1]), bf16x2(e[2], e[3]), bf16x2(e[4], e[5]),
+		             bf16x2(e[6], e[7])};
+	}
+	x[3] = g == 0 ? (v4u){bf16x2(v0, v1), bf16x2(v2, v3), bf16x2(v4, v5), 0u} : (v4u){0u, 0u, 0u, 0u};
 }
 
 // sorted (t[0] > t[1] > t[2] > t[3]) insertion of a key
@@ -503,43 +594,21 @@ __global__ __launch_bounds__(1024) void infer_kernel(const v4u *__restrict__ wp,
 	const int n_tiles = (n + 31) / 32;
 	for (int tile = blockIdx.x * kInferWaves + (int)(threadIdx.x / kWave); tile < n_tiles;
 	     tile += gridDim.x * kInferWaves) {
-		// layer 1 operands: octave blocks 4s+g of the lane's ray, s = 0..2,
-		// and the raw inputs (group 0) at s = 3
 		v4u x[2][4];
 		#pragma unroll
-		for (int r = 0; r < 2; ++r) {
-			const int ray = tile * 32 + r * 16 + col;
-			float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f;
-			if (ray < n) {
-				v0 = (ro[3 * (size_t)ray + 0] - lo.x) * scale.x - 1.0f;
-				v1 = (ro[3 * (size_t)ray + 1] - lo.y) * scale.y - 1.0f;
-				v2 = (ro[3 * (size_t)ray + 2] - lo.z) * scale.z - 1.0f;
-				v3 = rd[3 * (size_t)ray + 0];
-				v4 = rd[3 * (size_t)ray + 1];
-				v5 = rd[3 * (size_t)ray + 2];
-			}
-			// block 4s+g encodes input 2s + g/2, octaves 4*(g%2)..+3
-			const float in[3] = {(g & 2) ? v1 : v0, (g & 2) ? v3 : v2, (g & 2) ? v5 : v4};
-			#pragma unroll
-			for (int s = 0; s < 3; ++s) {
-				float e[8];
-				encode_octaves(in[s], (g & 1) ? 4 : 0, e);
-				x[r][s] = (v4u){bf16x2(e[0], e[1]), bf16x2(e[2], e[3]), bf16x2(e[4], e[5]),
-				                bf16x2(e[6], e[7])};
-			}
-			x[r][3] = g == 0 ? (v4u){bf16x2(v0, v1), bf16x2(v2, v3), bf16x2(v4, v5), 0u}
-			                 : (v4u){0u, 0u, 0u, 0u};
-		}
+		for (int r = 0; r < 2; ++r) encode_operands(x[r], ro, rd, tile * 32 + r * 16 + col, n, lo, scale, g);
 		// the hidden layers' weights and biases are loaded per tile (from the
 		// caches): the empty asm hides their loop invariance, which would
-		// otherwise hoist all of them into registers and spill
-		const v4u *wq = wp;
-		const float *pq = params;
-		asm volatile("" : "+s"(wq), "+s"(pq));
+		// otherwise hoist all of them into registers and spill (an opaque
+		// offset, so that the loads stay global rather than flat)
+		int z = 0;
+		asm volatile("" : "+s"(z));
+		const v4u *wq = wp + z;
+		const float *pq = params + z;
 		v4u h1[2][2], h2[2][2], h3[2][2];
-		hidden_layer<4>(h1, x, wq + pW1, pq + oB1, lane);
-		hidden_layer<2>(h2, h1, wq + pW2, pq + oB2, lane);
-		hidden_layer<2>(h3, h2, wq + pW3, pq + oB3, lane);
+		hidden_layer<4, 2>(h1, x, wq + pW1, pq + oB1, lane);
+		hidden_layer<2, 2>(h2, h1, wq + pW2, pq + oB2, lane);
+		hidden_layer<2, 2>(h3, h2, wq + pW3, pq + oB3, lane);
 		// layer 4: 64 treelet tiles into the running top-4 of each lane,
 		// software-pipelined: the products of tile t+1 are issued and the
 		// operands of tile t+2 fetched before the keys of tile t are inserted;
@@ -627,6 +696,418 @@ __global__ __launch_bounds__(1024) void infer_kernel(const v4u *__restrict__ wp,
 	}
 }
 
+// --------------------------------------------- fused training step
+//
+// Three launches per batch of m rays, without atomics (the results do not
+// depend on scheduling):
+//  1. train_rows_kernel, per set of 16 rays and wave: the forward pass as in
+//     the fused inference; the 1040 outputs from the LDS-resident W4 pack
+//     twice, never stored -- first for the running max and sum of exp of
+//     the 1024 treelet logits, then tile by tile for dOut = (softmax -
+//     onehot(label)) / m and 2 (logdist - target) / m, which is at once the
+//     B operand of dH3 = W4^T dOut; then dZ3 = dH3 [h3 > 0], dZ2, dZ1
+//     through the backward pack.  It stores what the weight gradients need,
+//     feature-major bf16 rows ([feature][ray], ld rays per row): the layer
+//     inputs x (rows in operand order k'), h1, h2, h3 and the output
+//     gradients dOut, dZ3, dZ2, dZ1; and one partial loss per set.
+//  2. grad_partials_kernel: dW = dZ X^T and db = dZ 1 of every layer, MFMA
+//     products over the rays split into kSplits ranges whose sums are
+//     stored apart.
+//  3. train_adam_kernel: per parameter the sum of its kSplits partials (in
+//     split order), Adam, the bf16 copy and packs; block 0 adds the sets'
+//     partial losses (in a fixed order) to the loss word.
+
+constexpr int kTrainWaves = 2;             // waves (16-ray sets) per block of train_rows_kernel
+constexpr int kSplits = 16;                // ray ranges of the weight-gradient products
+
+struct TrainRows {
+	uint16_t *x, *h1, *h2, *h3;            // rows 112 (operand order k'), 64, 64, 64
+	uint16_t *dout, *dz3, *dz2, *dz1;      // rows 1040, 64, 64, 64
+	float *loss_part;                      // one per 16-ray set
+	int ld;                                // rays per row (a multiple of 32)
+};
+
+__device__ inline uint16_t slot_bits(v4u v, int j) {
+	const unsigned w = v[j >> 1];
+	return (uint16_t)((j & 1) ? (w >> 16) : (w & 0xffffu));
+}
+
+// Feature-major stores go through a per-wave LDS tile of 32 rows x 16 rays
+// (bf16, row stride kTRow): each lane writes its 8 slots, then reads 8
+// consecutive rays of one row and stores them with one 16-B store.  Rows
+// are 48 B: 16-B aligned, and the four lane groups of a hidden-layer write
+// (rows 4g + j) fall in distinct banks.  A wave's LDS accesses execute in
+// order, so no barrier is needed.
+constexpr int kTRow = 24;
+constexpr int kTTile = 32 * kTRow;          // elements per wave
+
+// one 32-feature k-step operand v of the set's rays (slots: layer1 ? k' =
+// 32s + 8g + j : the hidden-layer rule) as rows f0..f0+31 of dst (rows >=
+// nrows skipped), rays ray0..ray0+15
+__device__ inline void store_kstep(uint16_t *__restrict__ dst, int ld, int ray0, int f0,
+                                   int nrows, v4u v, bool layer1, uint16_t *t, int lane) {
+	const int g = lane >> 4, c = lane & 15;
+	#pragma unroll
+	for (int j = 0; j < 8; ++j) {
+		const int fl = layer1 ? 8 * g + j : (j < 4 ? 4 * g + j : 16 + 4 * g + (j - 4));
+		t[fl * kTRow + c] = slot_bits(v, j);
+	}
+	const int fr = lane >> 1, hh = lane & 1;
+	const v4u row = *reinterpret_cast<const v4u *>(t + fr * kTRow + 8 * hh);
+	if (f0 + fr < nrows)
+		*reinterpret_cast<v4u *>(dst + (size_t)(f0 + fr) * ld + ray0 + 8 * hh) = row;
+}
+
+// the 2 hidden-layer k-steps as rows 0..63
+__device__ inline void store_hidden(uint16_t *__restrict__ dst, int ld, int ray0, const v4u h[2],
+                                    uint16_t *t, int lane) {
+	store_kstep(dst, ld, ray0, 0, kHid, h[0], false, t, lane);
+	store_kstep(dst, ld, ray0, 32, kHid, h[1], false, t, lane);
+}
+
+// dZ = dH [h > 0] as B operands: dh[kt] holds features 16kt + 4g + i, h is
+// the layer's output in operand layout (a bf16 is > 0 iff, read as a signed
+// 16-bit integer, it is > 0)
+__device__ inline void relu_back(v4u dz[2], const v4f dh[4], const v4u h[2]) {
+	#pragma unroll
+	for (int s = 0; s < 2; ++s) {
+		unsigned w[4];
+		#pragma unroll
+		for (int half = 0; half < 2; ++half) {
+			const v4f a = dh[2 * s + half];
+			#pragma unroll
+			for (int q = 0; q < 2; ++q) {
+				const unsigned hw = h[s][2 * half + q];
+				const float d0 = (short)(hw & 0xffffu) > 0 ? a[2 * q] : 0.f;
+				const float d1 = (short)(hw >> 16) > 0 ? a[2 * q + 1] : 0.f;
+				w[2 * half + q] = bf16x2(d0, d1);
+			}
+		}
+		dz[s] = (v4u){w[0], w[1], w[2], w[3]};
+	}
+}
+
+// dH = W^T dZ of a 64-wide layer from the lane's 8 backward-pack records
+__device__ inline void data_back(v4f dh[4], const v4u dz[2], const v4u w[8]) {
+	#pragma unroll
+	for (int kt = 0; kt < 4; ++kt) {
+		v4f a = {0.f, 0.f, 0.f, 0.f};
+		#pragma unroll
+		for (int s = 0; s < 2; ++s) a = mfma32(w[kt * 2 + s], dz[s], a);
+		dh[kt] = a;
+	}
+}
+
+// log2(e): exp(a - m) = exp2(fma(a, kLog2e, -m kLog2e)) on the hardware exp2
+constexpr float kLog2e = 1.44269504088896341f;
+
+__device__ inline float exp2_hw(float x) { return __builtin_amdgcn_exp2f(x); }
+
+template <int WAVES>
+__global__ __launch_bounds__(WAVES * kWave) void train_rows_kernel(
+    const v4u *__restrict__ wp, const v4u *__restrict__ wpt, const float *__restrict__ params,
+    const float *__restrict__ ro, const float *__restrict__ rd, const int32_t *__restrict__ label,
+    const float *__restrict__ target, int n, int n_sets, float3 lo, float3 scale, float inv_m,
+    TrainRows out, uint32_t *__restrict__ step) {
+	// LDS: the W4 pack and bias, the W2 and W3 packs, the hidden biases and
+	// each wave's transpose tile (157,504 B)
+	__shared__ v4u w4s[kRec4];
+	__shared__ v4f b4s[kOutP / 4];
+	__shared__ v4u w23s[2 * kRecH];
+	__shared__ __attribute__((aligned(16))) float bhs[3 * kHid];
+	__shared__ __attribute__((aligned(16))) uint16_t tr[WAVES][kTTile];
+	const int lane = __lane_id(), g = lane >> 4, col = lane & 15;
+	const int wave = (int)(threadIdx.x / kWave);
+	// the packs of W4, W2 and W3 go to LDS by direct global->LDS loads (64
+	// records of 16 B per instruction, lane l's record at base + 16 l), all
+	// in flight at once; W1's records are read from the caches per set
+	for (int i = wave; i < kRec4 / 64; i += WAVES)
+		__builtin_amdgcn_global_load_lds(
+		    (const __attribute__((address_space(1))) void *)(wp + pW4 + 64 * i + lane),
+		    (__attribute__((address_space(3))) void *)(w4s + 64 * i), 16, 0, 0);
+	for (int i = wave; i < 2 * kRecH / 64; i += WAVES)
+		__builtin_amdgcn_global_load_lds(
+		    (const __attribute__((address_space(1))) void *)(wp + pW2 + 64 * i + lane),
+		    (__attribute__((address_space(3))) void *)(w23s + 64 * i), 16, 0, 0);
+	for (int i = threadIdx.x; i < kOutP / 4; i += WAVES * kWave)
+		b4s[i] = reinterpret_cast<const v4f *>(params + oB4)[i];
+	for (int i = threadIdx.x; i < 3 * kHid; i += WAVES * kWave)
+		bhs[i] = params[i < kHid ? oB1 + i : (i < 2 * kHid ? oB2 + i - kHid : oB3 + i - 2 * kHid)];
+	if (blockIdx.x == 0 && threadIdx.x == 0) *step += 1u;
+	__builtin_amdgcn_s_waitcnt(0);
+	__syncthreads();
+	uint16_t *t = tr[wave];
+	const int ld = out.ld;
+	for (int set = blockIdx.x * WAVES + wave; set < n_sets; set += gridDim.x * WAVES) {
+		const int ray0 = set * 16, ray = ray0 + col;
+		const int lab = ray < n ? label[ray] : -1;       // >= 0: a hit, in the loss
+		const float tgt = ray < n ? target[ray] : 0.f;
+		v4u x[4];
+		encode_operands(x, ro, rd, ray, n, lo, scale, g);
+		// per-set reads of W1's records and the backward records (an opaque
+		// zero offset keeps them from being hoisted out of the set loop and
+		// held in registers throughout)
+		int z = 0;
+		asm volatile("" : "+s"(z));
+		const v4u *wq = wp + z;
+		const v4u *wqt = wpt + z;
+		v4u xs[1][4] = {{x[0], x[1], x[2], x[3]}};
+		v4u h1[1][2], h2[1][2], h3[1][2];
+		hidden_layer<4, 1>(h1, xs, wq + pW1, bhs, lane);
+		hidden_layer<2, 1>(h2, h1, w23s, bhs + kHid, lane);
+		hidden_layer<2, 1>(h3, h2, w23s + kRecH, bhs + 2 * kHid, lane);
+		#pragma unroll
+		for (int s = 0; s < 4; ++s) store_kstep(out.x, ld, ray0, 32 * s, kInP, x[s], true, t, lane);
+		store_hidden(out.h1, ld, ray0, h1[0], t, lane);
+		store_hidden(out.h2, ld, ray0, h2[0], t, lane);
+		store_hidden(out.h3, ld, ray0, h3[0], t, lane);
+
+		// pass 1: running max and sum of exp over the lane's treelet
+		// logits.  Software-pipelined: the products of tile u+1 are in
+		// flight while tile u's are reduced, and the records of tile u+2 are
+		// read meanwhile; two steps per iteration with the roles of the
+		// register sets swapped, so no value is copied across iterations
+		// (a copy would wait for its load or product).
+		float mrun = -__builtin_inff(), srun = 0.f;
+		auto p1_step = [&](int u, const v4f &cur, v4f &nxt, const v4u &ua, const v4u &ub,
+		                   const v4f &ubias, v4u &la, v4u &lb, v4f &lbias) {
+			nxt = mfma32(ub, h3[0][1], mfma32(ua, h3[0][0], ubias));
+			const int q = u + 2 <= kClusters / 16 ? u + 2 : kClusters / 16;
+			la = w4s[(2 * q) * 64 + lane];
+			lb = w4s[(2 * q + 1) * 64 + lane];
+			lbias = b4s[4 * q + g];
+			const float nm = fmaxf(mrun, fmaxf(fmaxf(cur[0], cur[1]), fmaxf(cur[2], cur[3])));
+			const float nml = nm * kLog2e;
+			const float e = ((exp2_hw(__builtin_fmaf(cur[0], kLog2e, -nml)) +
+			                  exp2_hw(__builtin_fmaf(cur[1], kLog2e, -nml))) +
+			                 (exp2_hw(__builtin_fmaf(cur[2], kLog2e, -nml)) +
+			                  exp2_hw(__builtin_fmaf(cur[3], kLog2e, -nml))));
+			srun = __builtin_fmaf(srun, exp2_hw(__builtin_fmaf(mrun, kLog2e, -nml)), e);
+			mrun = nm;
+		};
+		v4f k0 = mfma32(w4s[64 + lane], h3[0][1], mfma32(w4s[lane], h3[0][0], b4s[g])), k1;
+		v4u a0 = w4s[2 * 64 + lane], b0 = w4s[3 * 64 + lane], a1, b1;
+		v4f i0 = b4s[4 + g], i1;
+		for (int u = 0; u < kClusters / 16; u += 2) {
+			p1_step(u, k0, k1, a0, b0, i0, a1, b1, i1);
+			p1_step(u + 1, k1, k0, a1, b1, i1, a0, b0, i0);
+		}
+		const float dist = k0[0];                          // tile 64: feature 1024 in group 0
+		float mx = fmaxf(mrun, __shfl_xor(mrun, 16));
+		mx = fmaxf(mx, __shfl_xor(mx, 32));
+		const float mxl = mx * kLog2e;
+		float se = srun * exp2_hw(__builtin_fmaf(mrun, kLog2e, -mxl));
+		se += __shfl_xor(se, 16);
+		se += __shfl_xor(se, 32);
+		const float log_z = __logf(se) + mx;
+		// dOut = e * scale - [f == label] / m (zero for rays without a hit)
+		const float scale_g = lab >= 0 ? inv_m / se : 0.f;
+
+		// pass 2: dOut per pair of tiles (k-step s), stored and multiplied
+		// into dH3 = W4^T dOut; products one k-step ahead, LDS records two,
+		// backward records one; pipelined as pass 1
+		v4f dh[4];
+		#pragma unroll
+		for (int kt = 0; kt < 4; ++kt) dh[kt] = (v4f){0.f, 0.f, 0.f, 0.f};
+		float vlab = 0.f;
+		struct Rec { v4u r0, r1, r2, r3; v4f c0, c1; };
+		auto p2_step = [&](int s, const v4f &pa, const v4f &pb, v4f &na, v4f &nb, const Rec &ru,
+		                   Rec &rl, const v4u bu[4], v4u bl[4]) {
+			na = mfma32(ru.r1, h3[0][1], mfma32(ru.r0, h3[0][0], ru.c0));
+			nb = mfma32(ru.r3, h3[0][1], mfma32(ru.r2, h3[0][0], ru.c1));
+			const int q = s + 2 < kClusters / 32 ? s + 2 : kClusters / 32 - 1;
+			rl.r0 = w4s[(4 * q) * 64 + lane];
+			rl.r1 = w4s[(4 * q + 1) * 64 + lane];
+			rl.r2 = w4s[(4 * q + 2) * 64 + lane];
+			rl.r3 = w4s[(4 * q + 3) * 64 + lane];
+			rl.c0 = b4s[8 * q + g];
+			rl.c1 = b4s[8 * q + 4 + g];
+			#pragma unroll
+			for (int kt = 0; kt < 4; ++kt) bl[kt] = wqt[qW4 + (kt * kStepsT4 + s + 1) * 64 + lane];
+			const int rel = lab - 32 * s - 4 * g;          // label's slot offset in this k-step
+			float gv[8];
+			#pragma unroll
+			for (int i = 0; i < 4; ++i) {
+				const bool la = rel == i, lb = rel == 16 + i;
+				gv[i] = __builtin_fmaf(exp2_hw(__builtin_fmaf(pa[i], kLog2e, -mxl)), scale_g,
+				                       la ? -inv_m : 0.f);
+				gv[4 + i] = __builtin_fmaf(exp2_hw(__builtin_fmaf(pb[i], kLog2e, -mxl)), scale_g,
+				                           lb ? -inv_m : 0.f);
+				vlab += (la ? pa[i] : 0.f) + (lb ? pb[i] : 0.f);   // one match per lane at most
+			}
+			const v4u db = {bf16x2(gv[0], gv[1]), bf16x2(gv[2], gv[3]), bf16x2(gv[4], gv[5]),
+			                bf16x2(gv[6], gv[7])};
+			store_kstep(out.dout, ld, ray0, 32 * s, kOutP, db, false, t, lane);
+			#pragma unroll
+			for (int kt = 0; kt < 4; ++kt) dh[kt] = mfma32(bu[kt], db, dh[kt]);
+		};
+		v4f pa0 = mfma32(w4s[64 + lane], h3[0][1], mfma32(w4s[lane], h3[0][0], b4s[g]));
+		v4f pb0 = mfma32(w4s[192 + lane], h3[0][1], mfma32(w4s[128 + lane], h3[0][0], b4s[4 + g]));
+		v4f pa1, pb1;
+		Rec q0{w4s[256 + lane], w4s[320 + lane], w4s[384 + lane], w4s[448 + lane], b4s[8 + g],
+		       b4s[12 + g]}, q1;
+		v4u bt0[4], bt1[4];
+		#pragma unroll
+		for (int kt = 0; kt < 4; ++kt) bt0[kt] = wqt[qW4 + (kt * kStepsT4) * 64 + lane];
+		for (int s = 0; s < kClusters / 32; s += 2) {
+			p2_step(s, pa0, pb0, pa1, pb1, q0, q1, bt0, bt1);
+			p2_step(s + 1, pa1, pb1, pa0, pb0, q1, q0, bt1, bt0);
+		}
+		const v4u *bt = bt0;                               // k-step 32's records
+		// k-step 32: tile 64 (the log distance at feature 1024) and padding
+		float lossc = 0.f;
+		{
+			const bool dl = g == 0 && lab >= 0;
+			const float e = dist - tgt;
+			const v4u db = {bf16x2(dl ? 2.0f * e * inv_m : 0.f, 0.f), 0u, 0u, 0u};
+			store_kstep(out.dout, ld, ray0, 32 * (kClusters / 32), kOutP, db, false, t, lane);
+			#pragma unroll
+			for (int kt = 0; kt < 4; ++kt) dh[kt] = mfma32(bt[kt], db, dh[kt]);
+			if (dl) lossc += e * e;
+			if (lab >= 0 && g == ((lab >> 2) & 3)) lossc += log_z - vlab;
+		}
+
+		// the hidden layers' data gradients
+		v4u b3[8], b2[8];
+		#pragma unroll
+		for (int r = 0; r < 8; ++r) {
+			b3[r] = wqt[qW3 + 64 * r + lane];
+			b2[r] = wqt[qW2 + 64 * r + lane];
+		}
+		v4u dz[2];
+		relu_back(dz, dh, h3[0]);
+		store_hidden(out.dz3, ld, ray0, dz, t, lane);
+		data_back(dh, dz, b3);
+		relu_back(dz, dh, h2[0]);
+		store_hidden(out.dz2, ld, ray0, dz, t, lane);
+		data_back(dh, dz, b2);
+		relu_back(dz, dh, h1[0]);
+		store_hidden(out.dz1, ld, ray0, dz, t, lane);
+
+		#pragma unroll
+		for (int off = kWave / 2; off > 0; off >>= 1) lossc += __shfl_xor(lossc, off);
+		if (lane == 0) out.loss_part[set] = lossc * inv_m;
+	}
+}
+
+// dW and db of one layer for NT row tiles of 16 (rows n0.., tiles beyond
+// nt skipped) and KT tiles of 16 inputs, over rays [r0, r1) (multiples of
+// 32): D[n][k] = sum_r dY[n][r] X[k][r] and db as the product with a ones
+// column, written to the split's partial array.  The X operands of a k-step
+// serve all NT row tiles.
+template <int KT, int NT>
+__device__ inline void grad_tiles(float *__restrict__ part, const uint16_t *__restrict__ dy,
+                                  const uint16_t *__restrict__ x, int ld, int n0, int nt, int r0,
+                                  int r1, int ow, int ldw, int ob, bool layer1, int lane) {
+	const int g = lane >> 4, c = lane & 15;
+	v4f acc[NT][KT + 1];
+	#pragma unroll
+	for (int u = 0; u < NT; ++u)
+		#pragma unroll
+		for (int t = 0; t <= KT; ++t) acc[u][t] = (v4f){0.f, 0.f, 0.f, 0.f};
+	const unsigned one = c == 0 ? 0x3f803f80u : 0u;   // bf16 1.0 in column 0
+	const v4u ones = {one, one, one, one};
+	const uint16_t *arow[NT];
+	#pragma unroll
+	for (int u = 0; u < NT; ++u) arow[u] = dy + (size_t)(n0 + 16 * (u < nt ? u : 0) + c) * ld + 8 * g;
+	const uint16_t *brow = x + (size_t)c * ld + 8 * g;
+#pragma unroll 2
+	for (int r = r0; r < r1; r += 32) {
+		v4u a[NT], b[KT];
+		#pragma unroll
+		for (int u = 0; u < NT; ++u) a[u] = *reinterpret_cast<const v4u *>(arow[u] + r);
+		#pragma unroll
+		for (int t = 0; t < KT; ++t) b[t] = *reinterpret_cast<const v4u *>(brow + (size_t)16 * t * ld + r);
+		#pragma unroll
+		for (int u = 0; u < NT; ++u) {
+			#pragma unroll
+			for (int t = 0; t < KT; ++t) acc[u][t] = mfma32(a[u], b[t], acc[u][t]);
+			acc[u][KT] = mfma32(a[u], ones, acc[u][KT]);
+		}
+	}
+	#pragma unroll
+	for (int u = 0; u < NT; ++u) {
+		if (u >= nt) break;
+		const int nb = n0 + 16 * u + 4 * g;
+		#pragma unroll
+		for (int t = 0; t < KT; ++t) {
+			const int k = 16 * t + c;
+			const int colw = layer1 ? (k < 96 ? k + 6 : (k < kIn ? k - 96 : k)) : k;
+			#pragma unroll
+			for (int i = 0; i < 4; ++i) part[ow + (nb + i) * ldw + colw] = acc[u][t][i];
+		}
+		if (c == 0)
+			#pragma unroll
+			for (int i = 0; i < 4; ++i) part[ob + nb + i] = acc[u][KT][i];
+	}
+}
+
+// job = group * kSplits + split; groups of kGradNT row tiles: 0..32 W4
+// (the last one tile), then two each of W3, W2, W1; split: rays
+// [split * per_split, +per_split)
+constexpr int kGradNT = 2;
+constexpr int kGroups4 = (kOutP / 16 + kGradNT - 1) / kGradNT;     // 33
+constexpr int kGroupsH = kHid / 16 / kGradNT;                      // 2
+constexpr int kGradGroups = kGroups4 + 3 * kGroupsH;               // 39
+__global__ __launch_bounds__(256) void grad_partials_kernel(float *__restrict__ part,
+                                                            TrainRows in, int n_pad,
+                                                            int per_split) {
+	const int job = blockIdx.x * 4 + (int)(threadIdx.x / kWave);
+	if (job >= kGradGroups * kSplits) return;
+	const int grp = job / kSplits, split = job % kSplits;
+	const int rb = split * per_split;
+	const int r0 = rb < n_pad ? rb : n_pad, r1 = rb + per_split < n_pad ? rb + per_split : n_pad;
+	float *pp = part + (size_t)split * kParams;
+	const int lane = __lane_id();
+	if (grp < kGroups4) {
+		const int t0 = kGradNT * grp, nt = kOutP / 16 - t0 < kGradNT ? kOutP / 16 - t0 : kGradNT;
+		grad_tiles<kHid / 16, kGradNT>(pp, in.dout, in.h3, in.ld, 16 * t0, nt, r0, r1, oW4, kHid,
+		                               oB4, false, lane);
+		return;
+	}
+	const int h = grp - kGroups4, layer = 3 - h / kGroupsH, n0 = 16 * kGradNT * (h % kGroupsH);
+	if (layer == 3)
+		grad_tiles<kHid / 16, kGradNT>(pp, in.dz3, in.h2, in.ld, n0, kGradNT, r0, r1, oW3, kHid,
+		                               oB3, false, lane);
+	else if (layer == 2)
+		grad_tiles<kHid / 16, kGradNT>(pp, in.dz2, in.h1, in.ld, n0, kGradNT, r0, r1, oW2, kHid,
+		                               oB2, false, lane);
+	else
+		grad_tiles<kInP / 16, kGradNT>(pp, in.dz1, in.x, in.ld, n0, kGradNT, r0, r1, oW1, kInP,
+		                               oB1, true, lane);
+}
+
+// gradient = sum of the kSplits partials, Adam with the step word, the bf16
+// copy and packs; block 0 also adds the sets' losses to *loss
+__global__ __launch_bounds__(256) void train_adam_kernel(
+    float *__restrict__ p, float *__restrict__ mo, float *__restrict__ ve,
+    float *__restrict__ grads, uint16_t *__restrict__ wb, uint16_t *__restrict__ wp,
+    uint16_t *__restrict__ wpt, const float *__restrict__ part, float lr,
+    const uint32_t *__restrict__ step_word, const float *__restrict__ loss_part, int n_sets,
+    float *__restrict__ loss) {
+	if (blockIdx.x == 0) {
+		__shared__ float red[256];
+		float sum = 0.f;
+		for (int k = threadIdx.x; k < n_sets; k += 256) sum += loss_part[k];
+		red[threadIdx.x] = sum;
+		__syncthreads();
+		for (int w = 128; w > 0; w >>= 1) {
+			if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+			__syncthreads();
+		}
+		if (threadIdx.x == 0) *loss += red[0];
+	}
+	const int i = blockIdx.x * 256 + threadIdx.x;
+	if (i >= kParams) return;
+	float gi = 0.f;
+	#pragma unroll
+	for (int s = 0; s < kSplits; ++s) gi += part[(size_t)s * kParams + i];
+	grads[i] = gi;
+	float c1, c2;
+	bias_corrections(step_word, c1, c2);
+	store_param(i, adam_step(p, mo, ve, i, gi, lr, c1, c2), wb, wp, wpt);
+}
+
 // counter-based uniform in [-1, 1)
 __host__ inline float init_uniform(uint64_t seed, uint64_t i) {
 	uint64_t z = seed * 0x9e3779b97f4a7c15ull + i + 1;
@@ -643,7 +1124,10 @@ struct State {
 	uint16_t *x = nullptr, *h1 = nullptr, *h2 = nullptr, *h3 = nullptr;
 	float *out = nullptr;
 	uint16_t *dout = nullptr, *dh3 = nullptr, *dh2 = nullptr, *dh1 = nullptr;
-	uint16_t *wpack = nullptr;   // permuted bf16 weights of the fused inference kernel
+	uint16_t *wpack = nullptr;   // forward MFMA pack of the weights
+	uint16_t *wpackt = nullptr;  // backward MFMA pack (W4, W3, W2 transposed)
+	TrainRows rows{};            // the fused training step's feature-major rows
+	float *part = nullptr;       // kSplits partial gradients
 	int cus = 0;                 // compute units of the device the state lives on
 	int n_fwd = 0;               // rows of the last forward pass
 	std::vector<void *> allocs;
@@ -681,14 +1165,46 @@ int init_state(State *p, uint64_t seed) {
 	RTA_HIP_TRY(hipMemset(p->mom, 0, kParams * 4));
 	RTA_HIP_TRY(hipMemset(p->vel, 0, kParams * 4));
 	RTA_HIP_TRY(hipMemset(p->grads, 0, kParams * 4));
-	hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for(kParams, 256)), dim3(256), 0, 0, p->wb,
-	                   p->params);
-	RTA_LAUNCH_CHECK("to_bf16_kernel");
+	// pack slots without a weight (W1's padding columns, W4^T beyond row
+	// 1039) stay zero
+	RTA_HIP_TRY(hipMemset(p->wpack, 0, (size_t)kPackRecords * 16));
+	RTA_HIP_TRY(hipMemset(p->wpackt, 0, (size_t)kPackTRecords * 16));
+	hipLaunchKernelGGL(store_params_kernel, dim3(blocks_for(kParams, 256)), dim3(256), 0, 0, p->wb,
+	                   p->wpack, p->wpackt, p->params);
+	RTA_LAUNCH_CHECK("store_params_kernel");
 	int dev = 0;
 	RTA_HIP_TRY(hipGetDevice(&dev));
 	RTA_HIP_TRY(hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, dev));
 	if (p->cus < 1) p->cus = 1;
 	RTA_HIP_TRY(hipDeviceSynchronize());
+	return RTA_OK;
+}
+
+inline float3 box_scale(const float *mn, const float *mx) {
+	return make_float3(2.0f / (mx[0] - mn[0]), 2.0f / (mx[1] - mn[1]), 2.0f / (mx[2] - mn[2]));
+}
+
+// the three launches of one fused training step on rays [0, n)
+int train_step(State *p, const float *o, const float *d, const int32_t *label,
+               const float *target, int n, float3 lo, float3 sc, float lr, uint32_t *step,
+               float *loss, hipStream_t st) {
+	const int n_pad = (n + 31) / 32 * 32;
+	const int n_sets = n_pad / 16;
+	const long long need = (n_sets + kTrainWaves - 1) / kTrainWaves;
+	const unsigned blocks = (unsigned)(need < p->cus ? need : p->cus);
+	hipLaunchKernelGGL(train_rows_kernel<kTrainWaves>, dim3(blocks), dim3(kTrainWaves * kWave), 0,
+	                   st, reinterpret_cast<const v4u *>(p->wpack),
+	                   reinterpret_cast<const v4u *>(p->wpackt), p->params, o, d, label, target, n,
+	                   n_sets, lo, sc, 1.0f / (float)n, p->rows, step);
+	RTA_LAUNCH_CHECK("train_rows_kernel");
+	const int per_split = ((n_pad + kSplits - 1) / kSplits + 31) / 32 * 32;
+	hipLaunchKernelGGL(grad_partials_kernel, dim3(blocks_for((long long)kGradGroups * kSplits, 4)),
+	                   dim3(256), 0, st, p->part, p->rows, n_pad, per_split);
+	RTA_LAUNCH_CHECK("grad_partials_kernel");
+	hipLaunchKernelGGL(train_adam_kernel, dim3(blocks_for(kParams, 256)), dim3(256), 0, st,
+	                   p->params, p->mom, p->vel, p->grads, p->wb, p->wpack, p->wpackt, p->part, lr,
+	                   step, p->rows.loss_part, n_sets, loss);
+	RTA_LAUNCH_CHECK("train_adam_kernel");
 	return RTA_OK;
 }
 
@@ -729,6 +1245,19 @@ int rta_pred_create(int max_batch, uint64_t seed, rta_pred *out) {
 	rc |= p->alloc(p->dh2, B * kHid);
 	rc |= p->alloc(p->dh1, B * kHid);
 	rc |= p->alloc(p->wpack, (size_t)kPackRecords * 8);
+	rc |= p->alloc(p->wpackt, (size_t)kPackTRecords * 8);
+	const size_t ld = ((size_t)max_batch + 31) / 32 * 32;
+	p->rows.ld = (int)ld;
+	rc |= p->alloc(p->rows.x, ld * kInP);
+	rc |= p->alloc(p->rows.h1, ld * kHid);
+	rc |= p->alloc(p->rows.h2, ld * kHid);
+	rc |= p->alloc(p->rows.h3, ld * kHid);
+	rc |= p->alloc(p->rows.dout, ld * kOutP);
+	rc |= p->alloc(p->rows.dz3, ld * kHid);
+	rc |= p->alloc(p->rows.dz2, ld * kHid);
+	rc |= p->alloc(p->rows.dz1, ld * kHid);
+	rc |= p->alloc(p->rows.loss_part, ld / 16);
+	rc |= p->alloc(p->part, (size_t)kSplits * kParams);
 	if (rc != RTA_OK) {
 		delete p;
 		set_error("rta_pred_create: device allocation failed");
@@ -759,9 +1288,9 @@ int rta_pred_set_params(rta_pred h, const float *host) {
 	State *p = reinterpret_cast<State *>(h);
 	if (!p || !host) return invalid("rta_pred_set_params: NULL pointer");
 	RTA_HIP_TRY(hipMemcpy(p->params, host, kParams * 4, hipMemcpyHostToDevice));
-	hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for(kParams, 256)), dim3(256), 0, 0, p->wb,
-	                   p->params);
-	RTA_LAUNCH_CHECK("to_bf16_kernel");
+	hipLaunchKernelGGL(store_params_kernel, dim3(blocks_for(kParams, 256)), dim3(256), 0, 0, p->wb,
+	                   p->wpack, p->wpackt, p->params);
+	RTA_LAUNCH_CHECK("store_params_kernel");
 	RTA_HIP_TRY(hipDeviceSynchronize());
 	return RTA_OK;
 }
@@ -825,10 +1354,6 @@ int rta_pred_infer(rta_pred h, const float *o, const float *d, int n, const floa
 	if (top_k != 1 && top_k != kTop) return invalid("rta_pred_infer: top_k must be 1 or 4");
 	if (n == 0) return RTA_OK;
 	hipStream_t st = as_stream(s);
-	// the permuted weights follow the current parameters (one small launch)
-	hipLaunchKernelGGL(pack_infer_kernel, dim3(blocks_for((long long)kPackRecords * 8, 256)),
-	                   dim3(256), 0, st, p->wpack, p->wb);
-	RTA_LAUNCH_CHECK("pack_infer_kernel");
 	float3 lo = make_float3(scene_min[0], scene_min[1], scene_min[2]);
 	float3 sc = make_float3(2.0f / (scene_max[0] - scene_min[0]), 2.0f / (scene_max[1] - scene_min[1]),
 	                        2.0f / (scene_max[2] - scene_min[2]));
@@ -901,7 +1426,8 @@ int rta_pred_adam(rta_pred h, float lr, int step, rta_stream s) {
 	const float c1 = 1.0f / (1.0f - powf(0.9f, (float)step));
 	const float c2 = 1.0f / (1.0f - powf(0.999f, (float)step));
 	hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(kParams, 256)), dim3(256), 0, as_stream(s),
-	                   p->params, p->mom, p->vel, p->grads, p->wb, lr, c1, c2, nullptr);
+	                   p->params, p->mom, p->vel, p->grads, p->wb, p->wpack, p->wpackt, lr, c1, c2,
+	                   nullptr);
 	RTA_LAUNCH_CHECK("adam_kernel");
 	return RTA_OK;
 }
@@ -912,7 +1438,8 @@ int rta_pred_adam_device(rta_pred h, float lr, uint32_t *step, rta_stream s) {
 	const int rc = rta_hip_add_u32(step, 1u, s);
 	if (rc != RTA_OK) return rc;
 	hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(kParams, 256)), dim3(256), 0, as_stream(s),
-	                   p->params, p->mom, p->vel, p->grads, p->wb, lr, 0.0f, 0.0f, step);
+	                   p->params, p->mom, p->vel, p->grads, p->wb, p->wpack, p->wpackt, lr, 0.0f,
+	                   0.0f, step);
 	RTA_LAUNCH_CHECK("adam_kernel");
 	return RTA_OK;
 }
@@ -925,6 +1452,46 @@ int rta_pred_sample(const float *o, const float *d, const rta_hit *hits, const u
 	hipLaunchKernelGGL(sample_kernel, dim3(blocks_for(n_out, 256)), dim3(256), 0, as_stream(s), o,
 	                   d, hits, count, n_out, frame_word, seed, o_out, d_out, hits_out);
 	RTA_LAUNCH_CHECK("sample_kernel");
+	return RTA_OK;
+}
+
+int rta_pred_train_step(rta_pred h, const float *o, const float *d, const int32_t *label,
+                        const float *target, int n, const float *scene_min,
+                        const float *scene_max, float lr, uint32_t *step, float *loss,
+                        rta_stream s) {
+	RTA_FAULT_POINT("rta_pred_train_step");
+	State *p = reinterpret_cast<State *>(h);
+	if (!p || !o || !d || !label || !target || !scene_min || !scene_max || !step || !loss)
+		return invalid("rta_pred_train_step: NULL pointer");
+	if (n <= 0 || n > p->max_batch) return invalid("rta_pred_train_step: n must be in 1..max_batch");
+	return train_step(p, o, d, label, target, n, make_float3(scene_min[0], scene_min[1], scene_min[2]),
+	                  box_scale(scene_min, scene_max), lr, step, loss, as_stream(s));
+}
+
+int rta_pred_train_batches(rta_pred h, const float *o, const float *d, const int32_t *label,
+                           const float *target, int n, const int32_t *starts, int n_batches,
+                           int batch, const float *scene_min, const float *scene_max, float lr,
+                           uint32_t *step, float *loss, rta_stream s) {
+	RTA_FAULT_POINT("rta_pred_train_batches");
+	State *p = reinterpret_cast<State *>(h);
+	if (!p || !o || !d || !label || !target || !scene_min || !scene_max || !step || !loss ||
+	    (!starts && n_batches > 0))
+		return invalid("rta_pred_train_batches: NULL pointer");
+	if (batch <= 0 || batch > p->max_batch || n < 0 || n_batches < 0)
+		return invalid("rta_pred_train_batches: batch must be in 1..max_batch, n and n_batches >= 0");
+	for (int b = 0; b < n_batches; ++b)
+		if (starts[b] < 0 || starts[b] >= n)
+			return invalid("rta_pred_train_batches: a batch start outside [0, n)");
+	const float3 lo = make_float3(scene_min[0], scene_min[1], scene_min[2]);
+	const float3 sc = box_scale(scene_min, scene_max);
+	hipStream_t st = as_stream(s);
+	for (int b = 0; b < n_batches; ++b) {
+		const size_t r = (size_t)starts[b];
+		const int m = n - starts[b] < batch ? n - starts[b] : batch;
+		const int rc = train_step(p, o + 3 * r, d + 3 * r, label + r, target + r, m, lo, sc, lr, step,
+		                          loss, st);
+		if (rc != RTA_OK) return rc;
+	}
 	return RTA_OK;
 }
 

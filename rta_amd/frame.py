@@ -55,8 +55,9 @@ class OnlineFineTune:
         repeat `steps` times:
             sample `batch` rays of the level with their hits (a new hashed
             sample per frame and step) -> labels -> inference (the
-            predictions before the step, kept for accuracy) -> forward ->
-            backward -> Adam (device step counter)
+            predictions before the step, kept for accuracy) -> fused
+            training step (rta_pred_train_step: forward, loss, gradients and
+            Adam with the device step counter, three launches)
 
     Only kernels and device copies: the stage is captured with the frame."""
 
@@ -99,11 +100,9 @@ class OnlineFineTune:
                  ptr(self.count), ptr(self.label[k]), ptr(self.target[k]), s)
             if k == 0:
                 self.pred.infer(self.o, self.d, self.batch, top=self.top, stream=stream)
-            call("rta_pred_zero_grads", self.pred.h, s)
-            self.pred.forward(self.o, self.d, self.batch, stream=stream)
-            call("rta_pred_backward", self.pred.h, ptr(self.label[k]), ptr(self.target[k]),
-                 self.batch, ptr(self.loss), s)
-            call("rta_pred_adam_device", self.pred.h, C.c_float(self.lr), ptr(self.step), s)
+            call("rta_pred_train_step", self.pred.h, ptr(self.o), ptr(self.d),
+                 ptr(self.label[k]), ptr(self.target[k]), self.batch, self.pred.scene_min,
+                 self.pred.scene_max, C.c_float(self.lr), ptr(self.step), ptr(self.loss), s)
 
     def accuracy(self) -> float:
         """Top-4 accuracy, on the first sample of the last frame, of the

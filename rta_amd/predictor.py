@@ -118,6 +118,7 @@ class HitPredictor:
         self.scene_max = (C.c_float * 3)(*[float(v) for v in scene_max])
         self.steps = 0
         self.loss = DeviceBuffer(4)
+        self.step_word = DeviceBuffer(4)      # the fused step's device step counter
 
     def close(self):
         if self.h:
@@ -189,11 +190,22 @@ class HitPredictor:
              ptr(logdist) if logdist is not None else None,
              ptr(logits) if logits is not None else None, ptr(stream) if stream else None)
 
+    def train_step(self, o, d, label, target, n, lr=2e-3, stream=None, o_off=0):
+        """One fused step (rta_pred_train_step) on rays [o_off, o_off + n);
+        the loss is added to self.loss."""
+        self.step_word.fill_u32(self.steps, stream=stream)
+        call("rta_pred_train_step", self.h, _at(o, 12 * o_off), _at(d, 12 * o_off),
+             _at(label, 4 * o_off), _at(target, 4 * o_off), n, self.scene_min, self.scene_max,
+             C.c_float(lr), ptr(self.step_word), ptr(self.loss), ptr(stream) if stream else None)
+        self.steps += 1
+
     def train(self, o, d, label, target, n, epochs=2, batch=8192, lr=2e-3, stream=None,
-              seed=0):
+              seed=0, fused=True):
         """Adam over `epochs` passes of the n rays in batches (a seeded
         permutation of batch starts per epoch); returns the mean loss per
-        epoch and the wall time."""
+        epoch and the wall time.  fused: the three-launch fused step
+        (rta_pred_train_batches, one call per epoch); otherwise forward,
+        backward and Adam per batch."""
         rng = np.random.default_rng(seed)
         starts = np.arange(0, n, batch)
         sp = ptr(stream) if stream else None
@@ -201,14 +213,23 @@ class HitPredictor:
         t0 = time.perf_counter()
         for _ in range(epochs):
             self.loss.fill_u32(0, stream=stream)
-            for s0 in rng.permutation(starts):
-                m = int(min(batch, n - s0))
-                call("rta_pred_zero_grads", self.h, sp)
-                self.forward(o, d, m, stream=stream, o_off=int(s0))
-                call("rta_pred_backward", self.h, _at(label, 4 * int(s0)),
-                     _at(target, 4 * int(s0)), m, ptr(self.loss), sp)
-                self.steps += 1
-                call("rta_pred_adam", self.h, C.c_float(lr), self.steps, sp)
+            order = rng.permutation(starts)
+            if fused:
+                self.step_word.fill_u32(self.steps, stream=stream)
+                first = np.ascontiguousarray(order, dtype=np.int32)
+                call("rta_pred_train_batches", self.h, ptr(o), ptr(d), ptr(label), ptr(target), n,
+                     ptr(first), len(first), batch, self.scene_min, self.scene_max,
+                     C.c_float(lr), ptr(self.step_word), ptr(self.loss), sp)
+                self.steps += len(first)
+            else:
+                for s0 in order:
+                    m = int(min(batch, n - s0))
+                    call("rta_pred_zero_grads", self.h, sp)
+                    self.forward(o, d, m, stream=stream, o_off=int(s0))
+                    call("rta_pred_backward", self.h, _at(label, 4 * int(s0)),
+                         _at(target, 4 * int(s0)), m, ptr(self.loss), sp)
+                    self.steps += 1
+                    call("rta_pred_adam", self.h, C.c_float(lr), self.steps, sp)
             if stream:
                 stream.synchronize()
             losses.append(float(self.loss.download(np.float32, 1)[0]) / len(starts))
