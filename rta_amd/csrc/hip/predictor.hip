@@ -711,39 +711,49 @@ __global__ __launch_bounds__(1024) void infer_kernel(const v4u *__restrict__ wp,
 //     inputs x (rows in operand order k'), h1, h2, h3 and the output
 //     gradients dOut, dZ3, dZ2, dZ1; and one partial loss per set.
 //  2. grad_partials_kernel: dW = dZ X^T and db = dZ 1 of every layer, MFMA
-//     products over the rays split into kSplits ranges whose sums are
-//     stored apart.
-//  3. train_adam_kernel: per parameter the sum of its kSplits partials (in
-//     split order), Adam, the bf16 copy and packs; block 0 adds the sets'
-//     partial losses (in a fixed order) to the loss word.
+//     products over the rays in splits of 256 (rows padded to whole splits
+//     by rays without a hit); the kGradWaves splits of a block are added in
+//     LDS (in wave order) and stored as one partial.
+//  3. train_adam_kernel: per parameter the sum of its partials (in order),
+//     Adam, the bf16 copy and packs; block 0 adds the sets' partial losses
+//     (in a fixed order) to the loss word.
 
 constexpr int kTrainWaves = 2;             // waves (16-ray sets) per block of train_rows_kernel
-constexpr int kSplits = 16;                // ray ranges of the weight-gradient products
 
+// The rows the weight gradients read are stored in MFMA operand order: for
+// each tile of 16 features and each k-step of 32 rays, the 16 x 32 bf16
+// block as one 1-KiB record, lane l's 8 values (feature l%16, rays 8(l/16)
+// ..+7 of the k-step) at 16 l.  Element (f, r) of an array with ld rays per
+// row (ld a multiple of 32) is at rec_offset(f, r, ld).
 struct TrainRows {
-	uint16_t *x, *h1, *h2, *h3;            // rows 112 (operand order k'), 64, 64, 64
-	uint16_t *dout, *dz3, *dz2, *dz1;      // rows 1040, 64, 64, 64
+	uint16_t *x, *h1, *h2, *h3;            // features 112 (operand order k'), 64, 64, 64
+	uint16_t *dout, *dz3, *dz2, *dz1;      // features 1040, 64, 64, 64
 	float *loss_part;                      // one per 16-ray set
-	int ld;                                // rays per row (a multiple of 32)
+	int ld;                                // rays per feature (a multiple of 256)
 };
+
+__device__ __host__ inline size_t rec_offset(int f, int r, int ld) {
+	return ((size_t)(f >> 4) * (ld >> 5) + (r >> 5)) * 512 +
+	       (size_t)(16 * ((r & 31) >> 3) + (f & 15)) * 8 + (r & 7);
+}
 
 __device__ inline uint16_t slot_bits(v4u v, int j) {
 	const unsigned w = v[j >> 1];
 	return (uint16_t)((j & 1) ? (w >> 16) : (w & 0xffffu));
 }
 
-// Feature-major stores go through a per-wave LDS tile of 32 rows x 16 rays
-// (bf16, row stride kTRow): each lane writes its 8 slots, then reads 8
-// consecutive rays of one row and stores them with one 16-B store.  Rows
-// are 48 B: 16-B aligned, and the four lane groups of a hidden-layer write
-// (rows 4g + j) fall in distinct banks.  A wave's LDS accesses execute in
-// order, so no barrier is needed.
+// Stores go through a per-wave LDS tile of 32 features x 16 rays (bf16,
+// row stride kTRow): each lane writes its 8 slots, then reads 8 consecutive
+// rays of one feature, which are 16 contiguous bytes of a record; the 64
+// lanes write two 512-B runs.  LDS rows are 48 B: 16-B aligned, and the four
+// lane groups of a hidden-layer write (rows 4g + j) fall in distinct banks.
+// A wave's LDS accesses execute in order, so no barrier is needed.
 constexpr int kTRow = 24;
 constexpr int kTTile = 32 * kTRow;          // elements per wave
 
 // one 32-feature k-step operand v of the set's rays (slots: layer1 ? k' =
-// 32s + 8g + j : the hidden-layer rule) as rows f0..f0+31 of dst (rows >=
-// nrows skipped), rays ray0..ray0+15
+// 32s + 8g + j : the hidden-layer rule) as features f0..f0+31 of dst
+// (features >= nrows skipped), rays ray0..ray0+15 (ray0 a multiple of 16)
 __device__ inline void store_kstep(uint16_t *__restrict__ dst, int ld, int ray0, int f0,
                                    int nrows, v4u v, bool layer1, uint16_t *t, int lane) {
 	const int g = lane >> 4, c = lane & 15;
@@ -755,10 +765,10 @@ __device__ inline void store_kstep(uint16_t *__restrict__ dst, int ld, int ray0,
 	const int fr = lane >> 1, hh = lane & 1;
 	const v4u row = *reinterpret_cast<const v4u *>(t + fr * kTRow + 8 * hh);
 	if (f0 + fr < nrows)
-		*reinterpret_cast<v4u *>(dst + (size_t)(f0 + fr) * ld + ray0 + 8 * hh) = row;
+		*reinterpret_cast<v4u *>(dst + rec_offset(f0 + fr, ray0 + 8 * hh, ld)) = row;
 }
 
-// the 2 hidden-layer k-steps as rows 0..63
+// the 2 hidden-layer k-steps as features 0..63
 __device__ inline void store_hidden(uint16_t *__restrict__ dst, int ld, int ray0, const v4u h[2],
                                     uint16_t *t, int lane) {
 	store_kstep(dst, ld, ray0, 0, kHid, h[0], false, t, lane);
@@ -990,99 +1000,148 @@ __global__ __launch_bounds__(WAVES * kWave) void train_rows_kernel(
 	}
 }
 
-// dW and db of one layer for NT row tiles of 16 (rows n0.., tiles beyond
-// nt skipped) and KT tiles of 16 inputs, over rays [r0, r1) (multiples of
-// 32): D[n][k] = sum_r dY[n][r] X[k][r] and db as the product with a ones
-// column, written to the split's partial array.  The X operands of a k-step
-// serve all NT row tiles.
-template <int KT, int NT>
-__device__ inline void grad_tiles(float *__restrict__ part, const uint16_t *__restrict__ dy,
+// dW and db of one layer for kGradNT row tiles of 16 (rows n0.., tiles at
+// and beyond nt left at zero) and KT tiles of 16 inputs over the kSplitRays
+// rays from r0: D[n][k] = sum_r dY[n][r] X[k][r], and db as the product
+// with a ones column.  The k-steps are unrolled so that all operand loads
+// can be in flight at once; the X operands of a k-step serve both row
+// tiles.  acc[u][t]: row tile u, input tile t (t = KT: the bias column).
+constexpr int kSplitRays = 256;             // rays per split of the gradient products
+constexpr int kGradNT = 2;
+constexpr int kGradWaves = 4;               // splits (waves) per block, added in LDS
+template <int KT>
+__device__ inline void grad_tiles(v4f acc[kGradNT][KT + 1], const uint16_t *__restrict__ dy,
                                   const uint16_t *__restrict__ x, int ld, int n0, int nt, int r0,
-                                  int r1, int ow, int ldw, int ob, bool layer1, int lane) {
+                                  int lane) {
 	const int g = lane >> 4, c = lane & 15;
-	v4f acc[NT][KT + 1];
 	#pragma unroll
-	for (int u = 0; u < NT; ++u)
+	for (int u = 0; u < kGradNT; ++u)
 		#pragma unroll
 		for (int t = 0; t <= KT; ++t) acc[u][t] = (v4f){0.f, 0.f, 0.f, 0.f};
 	const unsigned one = c == 0 ? 0x3f803f80u : 0u;   // bf16 1.0 in column 0
 	const v4u ones = {one, one, one, one};
-	const uint16_t *arow[NT];
+	// operand records: 1 KiB each, lane l's 16 B at 16 l
+	const size_t tstride = (size_t)(ld >> 5) * 512;               // one feature tile
+	const uint16_t *arow[kGradNT];
 	#pragma unroll
-	for (int u = 0; u < NT; ++u) arow[u] = dy + (size_t)(n0 + 16 * (u < nt ? u : 0) + c) * ld + 8 * g;
-	const uint16_t *brow = x + (size_t)c * ld + 8 * g;
-#pragma unroll 2
-	for (int r = r0; r < r1; r += 32) {
-		v4u a[NT], b[KT];
+	for (int u = 0; u < kGradNT; ++u)
+		arow[u] = dy + rec_offset(n0 + 16 * (u < nt ? u : 0), r0, ld) + 8 * lane;
+	const uint16_t *brow = x + rec_offset(0, r0, ld) + 8 * lane;
+	// every operand load first (one memory latency for the whole split),
+	// then the products
+	constexpr int kSteps = kSplitRays / 32;
+	v4u a[kSteps][kGradNT], b[kSteps][KT];
+	#pragma unroll
+	for (int q = 0; q < kSteps; ++q) {
 		#pragma unroll
-		for (int u = 0; u < NT; ++u) a[u] = *reinterpret_cast<const v4u *>(arow[u] + r);
+		for (int u = 0; u < kGradNT; ++u) a[q][u] = *reinterpret_cast<const v4u *>(arow[u] + 512 * q);
 		#pragma unroll
-		for (int t = 0; t < KT; ++t) b[t] = *reinterpret_cast<const v4u *>(brow + (size_t)16 * t * ld + r);
-		#pragma unroll
-		for (int u = 0; u < NT; ++u) {
-			#pragma unroll
-			for (int t = 0; t < KT; ++t) acc[u][t] = mfma32(a[u], b[t], acc[u][t]);
-			acc[u][KT] = mfma32(a[u], ones, acc[u][KT]);
-		}
+		for (int t = 0; t < KT; ++t)
+			b[q][t] = *reinterpret_cast<const v4u *>(brow + t * tstride + 512 * q);
 	}
+	__builtin_amdgcn_sched_barrier(0);   // keep the scheduler from interleaving them again
 	#pragma unroll
-	for (int u = 0; u < NT; ++u) {
-		if (u >= nt) break;
-		const int nb = n0 + 16 * u + 4 * g;
+	for (int q = 0; q < kSteps; ++q)
 		#pragma unroll
-		for (int t = 0; t < KT; ++t) {
+		for (int u = 0; u < kGradNT; ++u) {
+			#pragma unroll
+			for (int t = 0; t < KT; ++t) acc[u][t] = mfma32(a[q][u], b[q][t], acc[u][t]);
+			acc[u][KT] = mfma32(a[q][u], ones, acc[u][KT]);
+		}
+}
+
+// The block's kGradWaves sums (LDS red: [wave][entry][lane]) added in
+// wave order and stored as the block's partial gradient; wave w stores
+// every kGradWaves-th entry from w.
+template <int KT>
+__device__ inline void grad_store(float *__restrict__ part, const float *red, int n0, int nt,
+                                  int ow, int ldw, int ob, bool layer1, int wave, int lane) {
+	constexpr int kEntries = kGradNT * (KT + 1) * 4;
+	const int g = lane >> 4, c = lane & 15;
+	for (int e = wave; e < kEntries; e += kGradWaves) {
+		float v = red[e * kWave + lane];
+		#pragma unroll
+		for (int w = 1; w < kGradWaves; ++w) v += red[(w * kEntries + e) * kWave + lane];
+		const int u = e / ((KT + 1) * 4), t = (e / 4) % (KT + 1), i = e % 4;
+		if (u >= nt) continue;
+		const int n = n0 + 16 * u + 4 * g + i;
+		if (t < KT) {
 			const int k = 16 * t + c;
 			const int colw = layer1 ? (k < 96 ? k + 6 : (k < kIn ? k - 96 : k)) : k;
-			#pragma unroll
-			for (int i = 0; i < 4; ++i) part[ow + (nb + i) * ldw + colw] = acc[u][t][i];
+			part[ow + n * ldw + colw] = v;
+		} else if (c == 0) {
+			part[ob + n] = v;
 		}
-		if (c == 0)
-			#pragma unroll
-			for (int i = 0; i < 4; ++i) part[ob + nb + i] = acc[u][KT][i];
 	}
 }
 
-// job = group * kSplits + split; groups of kGradNT row tiles: 0..32 W4
-// (the last one tile), then two each of W3, W2, W1; split: rays
-// [split * per_split, +per_split)
-constexpr int kGradNT = 2;
+template <int KT>
+__device__ inline void grad_group(float *__restrict__ part, float *red, const uint16_t *dy,
+                                  const uint16_t *x, int ld, int n0, int nt, int r0, bool active,
+                                  int ow, int ldw, int ob, bool layer1, int wave, int lane) {
+	constexpr int kEntries = kGradNT * (KT + 1) * 4;
+	v4f acc[kGradNT][KT + 1];
+	if (active) {
+		grad_tiles<KT>(acc, dy, x, ld, n0, nt, r0, lane);
+	} else {
+		#pragma unroll
+		for (int u = 0; u < kGradNT; ++u)
+			#pragma unroll
+			for (int t = 0; t <= KT; ++t) acc[u][t] = (v4f){0.f, 0.f, 0.f, 0.f};
+	}
+	#pragma unroll
+	for (int u = 0; u < kGradNT; ++u)
+		#pragma unroll
+		for (int t = 0; t <= KT; ++t)
+			#pragma unroll
+			for (int i = 0; i < 4; ++i)
+				red[((size_t)wave * kEntries + (u * (KT + 1) + t) * 4 + i) * kWave + lane] = acc[u][t][i];
+	__syncthreads();
+	grad_store<KT>(part, red, n0, nt, ow, ldw, ob, layer1, wave, lane);
+}
+
+// Block = (group of kGradNT row tiles, kGradWaves consecutive splits);
+// wave w takes split kGradWaves * blk + w (zero beyond the last split).
+// Groups 0..32: W4 (the last one tile), then two each of W3, W2, W1.  The
+// block's partial goes to part[blk].
 constexpr int kGroups4 = (kOutP / 16 + kGradNT - 1) / kGradNT;     // 33
 constexpr int kGroupsH = kHid / 16 / kGradNT;                      // 2
 constexpr int kGradGroups = kGroups4 + 3 * kGroupsH;               // 39
-__global__ __launch_bounds__(256) void grad_partials_kernel(float *__restrict__ part,
-                                                            TrainRows in, int n_pad,
-                                                            int per_split) {
-	const int job = blockIdx.x * 4 + (int)(threadIdx.x / kWave);
-	if (job >= kGradGroups * kSplits) return;
-	const int grp = job / kSplits, split = job % kSplits;
-	const int rb = split * per_split;
-	const int r0 = rb < n_pad ? rb : n_pad, r1 = rb + per_split < n_pad ? rb + per_split : n_pad;
-	float *pp = part + (size_t)split * kParams;
-	const int lane = __lane_id();
+constexpr int kRedFloats = kGradWaves * kGradNT * (kInP / 16 + 1) * 4 * kWave;  // 64 KiB
+__global__ __launch_bounds__(kGradWaves * kWave) void grad_partials_kernel(
+    float *__restrict__ part, TrainRows in, int splits) {
+	__shared__ float red[kRedFloats];
+	const int blks = (splits + kGradWaves - 1) / kGradWaves;
+	const int grp = blockIdx.x / blks, blk = blockIdx.x % blks;
+	const int wave = (int)(threadIdx.x / kWave), lane = __lane_id();
+	const int split = kGradWaves * blk + wave;
+	const bool active = split < splits;
+	const int r0 = split * kSplitRays;
+	float *pp = part + (size_t)blk * kParams;
 	if (grp < kGroups4) {
 		const int t0 = kGradNT * grp, nt = kOutP / 16 - t0 < kGradNT ? kOutP / 16 - t0 : kGradNT;
-		grad_tiles<kHid / 16, kGradNT>(pp, in.dout, in.h3, in.ld, 16 * t0, nt, r0, r1, oW4, kHid,
-		                               oB4, false, lane);
+		grad_group<kHid / 16>(pp, red, in.dout, in.h3, in.ld, 16 * t0, nt, r0, active, oW4, kHid,
+		                      oB4, false, wave, lane);
 		return;
 	}
 	const int h = grp - kGroups4, layer = 3 - h / kGroupsH, n0 = 16 * kGradNT * (h % kGroupsH);
 	if (layer == 3)
-		grad_tiles<kHid / 16, kGradNT>(pp, in.dz3, in.h2, in.ld, n0, kGradNT, r0, r1, oW3, kHid,
-		                               oB3, false, lane);
+		grad_group<kHid / 16>(pp, red, in.dz3, in.h2, in.ld, n0, kGradNT, r0, active, oW3, kHid,
+		                      oB3, false, wave, lane);
 	else if (layer == 2)
-		grad_tiles<kHid / 16, kGradNT>(pp, in.dz2, in.h1, in.ld, n0, kGradNT, r0, r1, oW2, kHid,
-		                               oB2, false, lane);
+		grad_group<kHid / 16>(pp, red, in.dz2, in.h1, in.ld, n0, kGradNT, r0, active, oW2, kHid,
+		                      oB2, false, wave, lane);
 	else
-		grad_tiles<kInP / 16, kGradNT>(pp, in.dz1, in.x, in.ld, n0, kGradNT, r0, r1, oW1, kInP,
-		                               oB1, true, lane);
+		grad_group<kInP / 16>(pp, red, in.dz1, in.x, in.ld, n0, kGradNT, r0, active, oW1, kInP,
+		                      oB1, true, wave, lane);
 }
 
-// gradient = sum of the kSplits partials, Adam with the step word, the bf16
+// gradient = sum of the n_part partials (in order), Adam with the step word, the bf16
 // copy and packs; block 0 also adds the sets' losses to *loss
 __global__ __launch_bounds__(256) void train_adam_kernel(
     float *__restrict__ p, float *__restrict__ mo, float *__restrict__ ve,
     float *__restrict__ grads, uint16_t *__restrict__ wb, uint16_t *__restrict__ wp,
-    uint16_t *__restrict__ wpt, const float *__restrict__ part, float lr,
+    uint16_t *__restrict__ wpt, const float *__restrict__ part, int n_part, float lr,
     const uint32_t *__restrict__ step_word, const float *__restrict__ loss_part, int n_sets,
     float *__restrict__ loss) {
 	if (blockIdx.x == 0) {
@@ -1099,9 +1158,15 @@ __global__ __launch_bounds__(256) void train_adam_kernel(
 	}
 	const int i = blockIdx.x * 256 + threadIdx.x;
 	if (i >= kParams) return;
+	// the partials 8 at a time, all 8 loads in flight, added in order
 	float gi = 0.f;
-	#pragma unroll
-	for (int s = 0; s < kSplits; ++s) gi += part[(size_t)s * kParams + i];
+	for (int s0 = 0; s0 < n_part; s0 += 8) {
+		float v[8];
+		#pragma unroll
+		for (int j = 0; j < 8; ++j) v[j] = s0 + j < n_part ? part[(size_t)(s0 + j) * kParams + i] : 0.f;
+		#pragma unroll
+		for (int j = 0; j < 8; ++j) gi += v[j];
+	}
 	grads[i] = gi;
 	float c1, c2;
 	bias_corrections(step_word, c1, c2);
@@ -1127,7 +1192,7 @@ struct State {
 	uint16_t *wpack = nullptr;   // forward MFMA pack of the weights
 	uint16_t *wpackt = nullptr;  // backward MFMA pack (W4, W3, W2 transposed)
 	TrainRows rows{};            // the fused training step's feature-major rows
-	float *part = nullptr;       // kSplits partial gradients
+	float *part = nullptr;       // partial gradients, one per kGradWaves ray splits
 	int cus = 0;                 // compute units of the device the state lives on
 	int n_fwd = 0;               // rows of the last forward pass
 	std::vector<void *> allocs;
@@ -1188,7 +1253,9 @@ inline float3 box_scale(const float *mn, const float *mx) {
 int train_step(State *p, const float *o, const float *d, const int32_t *label,
                const float *target, int n, float3 lo, float3 sc, float lr, uint32_t *step,
                float *loss, hipStream_t st) {
-	const int n_pad = (n + 31) / 32 * 32;
+	// rows padded to whole splits: the padding rays are rays without a hit
+	// (zero output gradients), so every split is a full kSplitRays
+	const int n_pad = (n + kSplitRays - 1) / kSplitRays * kSplitRays;
 	const int n_sets = n_pad / 16;
 	const long long need = (n_sets + kTrainWaves - 1) / kTrainWaves;
 	const unsigned blocks = (unsigned)(need < p->cus ? need : p->cus);
@@ -1197,13 +1264,13 @@ int train_step(State *p, const float *o, const float *d, const int32_t *label,
 	                   reinterpret_cast<const v4u *>(p->wpackt), p->params, o, d, label, target, n,
 	                   n_sets, lo, sc, 1.0f / (float)n, p->rows, step);
 	RTA_LAUNCH_CHECK("train_rows_kernel");
-	const int per_split = ((n_pad + kSplits - 1) / kSplits + 31) / 32 * 32;
-	hipLaunchKernelGGL(grad_partials_kernel, dim3(blocks_for((long long)kGradGroups * kSplits, 4)),
-	                   dim3(256), 0, st, p->part, p->rows, n_pad, per_split);
+	const int splits = n_pad / kSplitRays, blks = (splits + kGradWaves - 1) / kGradWaves;
+	hipLaunchKernelGGL(grad_partials_kernel, dim3(kGradGroups * blks), dim3(kGradWaves * kWave), 0,
+	                   st, p->part, p->rows, splits);
 	RTA_LAUNCH_CHECK("grad_partials_kernel");
 	hipLaunchKernelGGL(train_adam_kernel, dim3(blocks_for(kParams, 256)), dim3(256), 0, st,
-	                   p->params, p->mom, p->vel, p->grads, p->wb, p->wpack, p->wpackt, p->part, lr,
-	                   step, p->rows.loss_part, n_sets, loss);
+	                   p->params, p->mom, p->vel, p->grads, p->wb, p->wpack, p->wpackt, p->part,
+	                   blks, lr, step, p->rows.loss_part, n_sets, loss);
 	RTA_LAUNCH_CHECK("train_adam_kernel");
 	return RTA_OK;
 }
@@ -1246,7 +1313,7 @@ int rta_pred_create(int max_batch, uint64_t seed, rta_pred *out) {
 	rc |= p->alloc(p->dh1, B * kHid);
 	rc |= p->alloc(p->wpack, (size_t)kPackRecords * 8);
 	rc |= p->alloc(p->wpackt, (size_t)kPackTRecords * 8);
-	const size_t ld = ((size_t)max_batch + 31) / 32 * 32;
+	const size_t ld = ((size_t)max_batch + kSplitRays - 1) / kSplitRays * kSplitRays;
 	p->rows.ld = (int)ld;
 	rc |= p->alloc(p->rows.x, ld * kInP);
 	rc |= p->alloc(p->rows.h1, ld * kHid);
@@ -1257,7 +1324,7 @@ int rta_pred_create(int max_batch, uint64_t seed, rta_pred *out) {
 	rc |= p->alloc(p->rows.dz2, ld * kHid);
 	rc |= p->alloc(p->rows.dz1, ld * kHid);
 	rc |= p->alloc(p->rows.loss_part, ld / 16);
-	rc |= p->alloc(p->part, (size_t)kSplits * kParams);
+	rc |= p->alloc(p->part, (ld / kSplitRays + kGradWaves - 1) / kGradWaves * kParams);
 	if (rc != RTA_OK) {
 		delete p;
 		set_error("rta_pred_create: device allocation failed");
