@@ -718,7 +718,7 @@ __global__ __launch_bounds__(1024) void infer_kernel(const v4u *__restrict__ wp,
 //     Adam, the bf16 copy and packs; block 0 adds the sets' partial losses
 //     (in a fixed order) to the loss word.
 
-constexpr int kTrainWaves = 2;             // waves (16-ray sets) per block of train_rows_kernel
+constexpr int kTrainSets = 2;              // 16-ray sets per block of train_rows_kernel (2 waves each)
 
 // The rows the weight gradients read are stored in MFMA operand order: for
 // each tile of 16 features and each k-step of 32 rays, the 16 x 32 bf16
@@ -728,7 +728,7 @@ constexpr int kTrainWaves = 2;             // waves (16-ray sets) per block of t
 struct TrainRows {
 	uint16_t *x, *h1, *h2, *h3;            // features 112 (operand order k'), 64, 64, 64
 	uint16_t *dout, *dz3, *dz2, *dz1;      // features 1040, 64, 64, 64
-	float *loss_part;                      // one per 16-ray set
+	float *loss_part;                      // two per 16-ray set (one per half)
 	int ld;                                // rays per feature (a multiple of 256)
 };
 
@@ -813,71 +813,71 @@ constexpr float kLog2e = 1.44269504088896341f;
 
 __device__ inline float exp2_hw(float x) { return __builtin_amdgcn_exp2f(x); }
 
-template <int WAVES>
-__global__ __launch_bounds__(WAVES * kWave) void train_rows_kernel(
+// Two waves per set of 16 rays: both run the forward pass; wave half h
+// takes logit tiles 32h..32h+31 (k-steps 16h..16h+15) of the softmax passes;
+// the halves' (max, sum) per ray and the second half's dH3 are exchanged in
+// LDS.  Half 0 then runs the hidden layers' data gradients, half 1 stores
+// the layer inputs and the log distance's k-step.  SETS sets per block.
+template <int SETS>
+__global__ __launch_bounds__(2 * SETS * kWave) void train_rows_kernel(
     const v4u *__restrict__ wp, const v4u *__restrict__ wpt, const float *__restrict__ params,
     const float *__restrict__ ro, const float *__restrict__ rd, const int32_t *__restrict__ label,
     const float *__restrict__ target, int n, int n_sets, float3 lo, float3 scale, float inv_m,
     TrainRows out, uint32_t *__restrict__ step) {
-	// LDS: the W4 pack and bias, the W2 and W3 packs, the hidden biases and
-	// each wave's transpose tile (157,504 B)
+	// LDS: the W4 pack and bias, the hidden biases, each wave's transpose
+	// tile and the halves' exchange (152,896 B with SETS = 2)
 	__shared__ v4u w4s[kRec4];
 	__shared__ v4f b4s[kOutP / 4];
-	__shared__ v4u w23s[2 * kRecH];
 	__shared__ __attribute__((aligned(16))) float bhs[3 * kHid];
-	__shared__ __attribute__((aligned(16))) uint16_t tr[WAVES][kTTile];
+	__shared__ __attribute__((aligned(16))) uint16_t tr[2 * SETS][kTTile];
+	__shared__ v4f xch[SETS][4][kWave];
+	__shared__ float2 stat[SETS][2][16];
 	const int lane = __lane_id(), g = lane >> 4, col = lane & 15;
-	const int wave = (int)(threadIdx.x / kWave);
-	// the packs of W4, W2 and W3 go to LDS by direct global->LDS loads (64
-	// records of 16 B per instruction, lane l's record at base + 16 l), all
-	// in flight at once; W1's records are read from the caches per set
-	for (int i = wave; i < kRec4 / 64; i += WAVES)
+	const int wave = (int)(threadIdx.x / kWave), sl = wave >> 1, half = wave & 1;
+	// W4's pack goes to LDS by direct global->LDS loads (64 records of 16 B
+	// per instruction, lane l's record at base + 16 l), all in flight at once
+	for (int i = wave; i < kRec4 / 64; i += 2 * SETS)
 		__builtin_amdgcn_global_load_lds(
 		    (const __attribute__((address_space(1))) void *)(wp + pW4 + 64 * i + lane),
 		    (__attribute__((address_space(3))) void *)(w4s + 64 * i), 16, 0, 0);
-	for (int i = wave; i < 2 * kRecH / 64; i += WAVES)
-		__builtin_amdgcn_global_load_lds(
-		    (const __attribute__((address_space(1))) void *)(wp + pW2 + 64 * i + lane),
-		    (__attribute__((address_space(3))) void *)(w23s + 64 * i), 16, 0, 0);
-	for (int i = threadIdx.x; i < kOutP / 4; i += WAVES * kWave)
+	for (int i = threadIdx.x; i < kOutP / 4; i += 2 * SETS * kWave)
 		b4s[i] = reinterpret_cast<const v4f *>(params + oB4)[i];
-	for (int i = threadIdx.x; i < 3 * kHid; i += WAVES * kWave)
+	for (int i = threadIdx.x; i < 3 * kHid; i += 2 * SETS * kWave)
 		bhs[i] = params[i < kHid ? oB1 + i : (i < 2 * kHid ? oB2 + i - kHid : oB3 + i - 2 * kHid)];
 	if (blockIdx.x == 0 && threadIdx.x == 0) *step += 1u;
 	__builtin_amdgcn_s_waitcnt(0);
 	__syncthreads();
 	uint16_t *t = tr[wave];
 	const int ld = out.ld;
-	for (int set = blockIdx.x * WAVES + wave; set < n_sets; set += gridDim.x * WAVES) {
+	// the loop runs the same number of times in every wave of the block
+	// (its barriers are reached by all of them)
+	for (int base = blockIdx.x * SETS; base < n_sets; base += gridDim.x * SETS) {
+		const int set = base + sl;
+		const bool act = set < n_sets;
 		const int ray0 = set * 16, ray = ray0 + col;
-		const int lab = ray < n ? label[ray] : -1;       // >= 0: a hit, in the loss
-		const float tgt = ray < n ? target[ray] : 0.f;
-		v4u x[4];
-		encode_operands(x, ro, rd, ray, n, lo, scale, g);
-		// per-set reads of W1's records and the backward records (an opaque
-		// zero offset keeps them from being hoisted out of the set loop and
-		// held in registers throughout)
+		const int lab = act && ray < n ? label[ray] : -1;      // >= 0: a hit, in the loss
+		const float tgt = act && ray < n ? target[ray] : 0.f;
+		// per-set reads of the hidden layers' records (an opaque zero offset
+		// keeps them from being hoisted out of the set loop and held in
+		// registers throughout)
 		int z = 0;
 		asm volatile("" : "+s"(z));
 		const v4u *wq = wp + z;
 		const v4u *wqt = wpt + z;
-		v4u xs[1][4] = {{x[0], x[1], x[2], x[3]}};
+		v4u xs[1][4];
+		encode_operands(xs[0], ro, rd, act ? ray : n, n, lo, scale, g);
 		v4u h1[1][2], h2[1][2], h3[1][2];
 		hidden_layer<4, 1>(h1, xs, wq + pW1, bhs, lane);
-		hidden_layer<2, 1>(h2, h1, w23s, bhs + kHid, lane);
-		hidden_layer<2, 1>(h3, h2, w23s + kRecH, bhs + 2 * kHid, lane);
-		#pragma unroll
-		for (int s = 0; s < 4; ++s) store_kstep(out.x, ld, ray0, 32 * s, kInP, x[s], true, t, lane);
-		store_hidden(out.h1, ld, ray0, h1[0], t, lane);
-		store_hidden(out.h2, ld, ray0, h2[0], t, lane);
-		store_hidden(out.h3, ld, ray0, h3[0], t, lane);
+		hidden_layer<2, 1>(h2, h1, wq + pW2, bhs + kHid, lane);
+		hidden_layer<2, 1>(h3, h2, wq + pW3, bhs + 2 * kHid, lane);
 
-		// pass 1: running max and sum of exp over the lane's treelet
-		// logits.  Software-pipelined: the products of tile u+1 are in
-		// flight while tile u's are reduced, and the records of tile u+2 are
-		// read meanwhile; two steps per iteration with the roles of the
-		// register sets swapped, so no value is copied across iterations
-		// (a copy would wait for its load or product).
+		// pass 1 over the half's tiles u0..u0+31: running max and sum of
+		// exp.  Software-pipelined: the products of tile u+1 are in flight
+		// while tile u's are reduced, and the records of tile u+2 are read
+		// meanwhile; two steps per iteration with the roles of the register
+		// sets swapped, so no value is copied across iterations (a copy
+		// would wait for its load or product).
+		const int u0 = 32 * half;
 		float mrun = -__builtin_inff(), srun = 0.f;
 		auto p1_step = [&](int u, const v4f &cur, v4f &nxt, const v4u &ua, const v4u &ub,
 		                   const v4f &ubias, v4u &la, v4u &lb, v4f &lbias) {
@@ -895,27 +895,36 @@ __global__ __launch_bounds__(WAVES * kWave) void train_rows_kernel(
 			srun = __builtin_fmaf(srun, exp2_hw(__builtin_fmaf(mrun, kLog2e, -nml)), e);
 			mrun = nm;
 		};
-		v4f k0 = mfma32(w4s[64 + lane], h3[0][1], mfma32(w4s[lane], h3[0][0], b4s[g])), k1;
-		v4u a0 = w4s[2 * 64 + lane], b0 = w4s[3 * 64 + lane], a1, b1;
-		v4f i0 = b4s[4 + g], i1;
-		for (int u = 0; u < kClusters / 16; u += 2) {
+		v4f k0 = mfma32(w4s[(2 * u0 + 1) * 64 + lane], h3[0][1],
+		                mfma32(w4s[(2 * u0) * 64 + lane], h3[0][0], b4s[4 * u0 + g])), k1;
+		v4u a0 = w4s[(2 * u0 + 2) * 64 + lane], b0 = w4s[(2 * u0 + 3) * 64 + lane], a1, b1;
+		v4f i0 = b4s[4 * u0 + 4 + g], i1;
+		for (int u = u0; u < u0 + kClusters / 32; u += 2) {
 			p1_step(u, k0, k1, a0, b0, i0, a1, b1, i1);
 			p1_step(u + 1, k1, k0, a1, b1, i1, a0, b0, i0);
 		}
-		const float dist = k0[0];                          // tile 64: feature 1024 in group 0
-		float mx = fmaxf(mrun, __shfl_xor(mrun, 16));
-		mx = fmaxf(mx, __shfl_xor(mx, 32));
+		const float dist = k0[0];           // half 1: tile 64, feature 1024 in group 0
+		float mh = fmaxf(mrun, __shfl_xor(mrun, 16));
+		mh = fmaxf(mh, __shfl_xor(mh, 32));
+		float sh = srun * exp2_hw(__builtin_fmaf(mrun, kLog2e, -mh * kLog2e));
+		sh += __shfl_xor(sh, 16);
+		sh += __shfl_xor(sh, 32);
+		if (g == 0) stat[sl][half][col] = make_float2(mh, sh);
+		__syncthreads();
+		const float2 st0 = stat[sl][0][col], st1 = stat[sl][1][col];
+		const float mx = fmaxf(st0.x, st1.x);
 		const float mxl = mx * kLog2e;
-		float se = srun * exp2_hw(__builtin_fmaf(mrun, kLog2e, -mxl));
-		se += __shfl_xor(se, 16);
-		se += __shfl_xor(se, 32);
+		const float se = st0.y * exp2_hw(__builtin_fmaf(st0.x, kLog2e, -mxl)) +
+		                 st1.y * exp2_hw(__builtin_fmaf(st1.x, kLog2e, -mxl));
 		const float log_z = __logf(se) + mx;
 		// dOut = e * scale - [f == label] / m (zero for rays without a hit)
 		const float scale_g = lab >= 0 ? inv_m / se : 0.f;
 
-		// pass 2: dOut per pair of tiles (k-step s), stored and multiplied
-		// into dH3 = W4^T dOut; products one k-step ahead, LDS records two,
-		// backward records one; pipelined as pass 1
+		// pass 2 over the half's k-steps s0..s0+15: dOut per pair of tiles,
+		// stored and multiplied into (the half's part of) dH3 = W4^T dOut;
+		// products one k-step ahead, LDS records two, backward records one;
+		// pipelined as pass 1
+		const int s0 = 16 * half;
 		v4f dh[4];
 		#pragma unroll
 		for (int kt = 0; kt < 4; ++kt) dh[kt] = (v4f){0.f, 0.f, 0.f, 0.f};
@@ -947,56 +956,72 @@ __global__ __launch_bounds__(WAVES * kWave) void train_rows_kernel(
 			}
 			const v4u db = {bf16x2(gv[0], gv[1]), bf16x2(gv[2], gv[3]), bf16x2(gv[4], gv[5]),
 			                bf16x2(gv[6], gv[7])};
-			store_kstep(out.dout, ld, ray0, 32 * s, kOutP, db, false, t, lane);
+			if (act) store_kstep(out.dout, ld, ray0, 32 * s, kOutP, db, false, t, lane);
 			#pragma unroll
 			for (int kt = 0; kt < 4; ++kt) dh[kt] = mfma32(bu[kt], db, dh[kt]);
 		};
-		v4f pa0 = mfma32(w4s[64 + lane], h3[0][1], mfma32(w4s[lane], h3[0][0], b4s[g]));
-		v4f pb0 = mfma32(w4s[192 + lane], h3[0][1], mfma32(w4s[128 + lane], h3[0][0], b4s[4 + g]));
+		v4f pa0 = mfma32(w4s[(4 * s0 + 1) * 64 + lane], h3[0][1],
+		                 mfma32(w4s[(4 * s0) * 64 + lane], h3[0][0], b4s[8 * s0 + g]));
+		v4f pb0 = mfma32(w4s[(4 * s0 + 3) * 64 + lane], h3[0][1],
+		                 mfma32(w4s[(4 * s0 + 2) * 64 + lane], h3[0][0], b4s[8 * s0 + 4 + g]));
 		v4f pa1, pb1;
-		Rec q0{w4s[256 + lane], w4s[320 + lane], w4s[384 + lane], w4s[448 + lane], b4s[8 + g],
-		       b4s[12 + g]}, q1;
+		Rec q0{w4s[(4 * s0 + 4) * 64 + lane], w4s[(4 * s0 + 5) * 64 + lane],
+		       w4s[(4 * s0 + 6) * 64 + lane], w4s[(4 * s0 + 7) * 64 + lane], b4s[8 * s0 + 8 + g],
+		       b4s[8 * s0 + 12 + g]}, q1;
 		v4u bt0[4], bt1[4];
 		#pragma unroll
-		for (int kt = 0; kt < 4; ++kt) bt0[kt] = wqt[qW4 + (kt * kStepsT4) * 64 + lane];
-		for (int s = 0; s < kClusters / 32; s += 2) {
+		for (int kt = 0; kt < 4; ++kt) bt0[kt] = wqt[qW4 + (kt * kStepsT4 + s0) * 64 + lane];
+		for (int s = s0; s < s0 + kClusters / 64; s += 2) {
 			p2_step(s, pa0, pb0, pa1, pb1, q0, q1, bt0, bt1);
 			p2_step(s + 1, pa1, pb1, pa0, pb0, q1, q0, bt1, bt0);
 		}
-		const v4u *bt = bt0;                               // k-step 32's records
-		// k-step 32: tile 64 (the log distance at feature 1024) and padding
 		float lossc = 0.f;
-		{
+		if (lab >= 0 && (lab >> 9) == half && g == ((lab >> 2) & 3)) lossc += log_z - vlab;
+		if (half == 1) {
+			// k-step 32: tile 64 (the log distance at feature 1024) and padding
 			const bool dl = g == 0 && lab >= 0;
 			const float e = dist - tgt;
 			const v4u db = {bf16x2(dl ? 2.0f * e * inv_m : 0.f, 0.f), 0u, 0u, 0u};
-			store_kstep(out.dout, ld, ray0, 32 * (kClusters / 32), kOutP, db, false, t, lane);
+			if (act) store_kstep(out.dout, ld, ray0, 32 * (kClusters / 32), kOutP, db, false, t, lane);
 			#pragma unroll
-			for (int kt = 0; kt < 4; ++kt) dh[kt] = mfma32(bt[kt], db, dh[kt]);
+			for (int kt = 0; kt < 4; ++kt) dh[kt] = mfma32(bt0[kt], db, dh[kt]);
 			if (dl) lossc += e * e;
-			if (lab >= 0 && g == ((lab >> 2) & 3)) lossc += log_z - vlab;
+			#pragma unroll
+			for (int kt = 0; kt < 4; ++kt) xch[sl][kt][lane] = dh[kt];
 		}
-
-		// the hidden layers' data gradients
-		v4u b3[8], b2[8];
-		#pragma unroll
-		for (int r = 0; r < 8; ++r) {
-			b3[r] = wqt[qW3 + 64 * r + lane];
-			b2[r] = wqt[qW2 + 64 * r + lane];
-		}
-		v4u dz[2];
-		relu_back(dz, dh, h3[0]);
-		store_hidden(out.dz3, ld, ray0, dz, t, lane);
-		data_back(dh, dz, b3);
-		relu_back(dz, dh, h2[0]);
-		store_hidden(out.dz2, ld, ray0, dz, t, lane);
-		data_back(dh, dz, b2);
-		relu_back(dz, dh, h1[0]);
-		store_hidden(out.dz1, ld, ray0, dz, t, lane);
-
 		#pragma unroll
 		for (int off = kWave / 2; off > 0; off >>= 1) lossc += __shfl_xor(lossc, off);
-		if (lane == 0) out.loss_part[set] = lossc * inv_m;
+		if (act && lane == 0) out.loss_part[2 * set + half] = lossc * inv_m;
+		__syncthreads();
+		if (!act) continue;
+		if (half == 0) {
+			// the hidden layers' data gradients
+			#pragma unroll
+			for (int kt = 0; kt < 4; ++kt) dh[kt] += xch[sl][kt][lane];
+			v4u b3[8], b2[8];
+			#pragma unroll
+			for (int r = 0; r < 8; ++r) {
+				b3[r] = wqt[qW3 + 64 * r + lane];
+				b2[r] = wqt[qW2 + 64 * r + lane];
+			}
+			v4u dz[2];
+			relu_back(dz, dh, h3[0]);
+			store_hidden(out.dz3, ld, ray0, dz, t, lane);
+			data_back(dh, dz, b3);
+			relu_back(dz, dh, h2[0]);
+			store_hidden(out.dz2, ld, ray0, dz, t, lane);
+			data_back(dh, dz, b2);
+			relu_back(dz, dh, h1[0]);
+			store_hidden(out.dz1, ld, ray0, dz, t, lane);
+		} else {
+			// the layer inputs
+			#pragma unroll
+			for (int s = 0; s < 4; ++s)
+				store_kstep(out.x, ld, ray0, 32 * s, kInP, xs[0][s], true, t, lane);
+			store_hidden(out.h1, ld, ray0, h1[0], t, lane);
+			store_hidden(out.h2, ld, ray0, h2[0], t, lane);
+			store_hidden(out.h3, ld, ray0, h3[0], t, lane);
+		}
 	}
 }
 
@@ -1257,9 +1282,9 @@ int train_step(State *p, const float *o, const float *d, const int32_t *label,
 	// (zero output gradients), so every split is a full kSplitRays
 	const int n_pad = (n + kSplitRays - 1) / kSplitRays * kSplitRays;
 	const int n_sets = n_pad / 16;
-	const long long need = (n_sets + kTrainWaves - 1) / kTrainWaves;
+	const long long need = (n_sets + kTrainSets - 1) / kTrainSets;
 	const unsigned blocks = (unsigned)(need < p->cus ? need : p->cus);
-	hipLaunchKernelGGL(train_rows_kernel<kTrainWaves>, dim3(blocks), dim3(kTrainWaves * kWave), 0,
+	hipLaunchKernelGGL(train_rows_kernel<kTrainSets>, dim3(blocks), dim3(2 * kTrainSets * kWave), 0,
 	                   st, reinterpret_cast<const v4u *>(p->wpack),
 	                   reinterpret_cast<const v4u *>(p->wpackt), p->params, o, d, label, target, n,
 	                   n_sets, lo, sc, 1.0f / (float)n, p->rows, step);
@@ -1270,7 +1295,7 @@ int train_step(State *p, const float *o, const float *d, const int32_t *label,
 	RTA_LAUNCH_CHECK("grad_partials_kernel");
 	hipLaunchKernelGGL(train_adam_kernel, dim3(blocks_for(kParams, 256)), dim3(256), 0, st,
 	                   p->params, p->mom, p->vel, p->grads, p->wb, p->wpack, p->wpackt, p->part,
-	                   blks, lr, step, p->rows.loss_part, n_sets, loss);
+	                   blks, lr, step, p->rows.loss_part, 2 * n_sets, loss);
 	RTA_LAUNCH_CHECK("train_adam_kernel");
 	return RTA_OK;
 }
@@ -1323,7 +1348,7 @@ int rta_pred_create(int max_batch, uint64_t seed, rta_pred *out) {
 	rc |= p->alloc(p->rows.dz3, ld * kHid);
 	rc |= p->alloc(p->rows.dz2, ld * kHid);
 	rc |= p->alloc(p->rows.dz1, ld * kHid);
-	rc |= p->alloc(p->rows.loss_part, ld / 16);
+	rc |= p->alloc(p->rows.loss_part, ld / 8);
 	rc |= p->alloc(p->part, (ld / kSplitRays + kGradWaves - 1) / kGradWaves * kParams);
 	if (rc != RTA_OK) {
 		delete p;
