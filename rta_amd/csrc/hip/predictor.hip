@@ -754,6 +754,7 @@ constexpr int kTTile = 32 * kTRow;          // elements per wave
 // one 32-feature k-step operand v of the set's rays (slots: layer1 ? k' =
 // 32s + 8g + j : the hidden-layer rule) as features f0..f0+31 of dst
 // (features >= nrows skipped), rays ray0..ray0+15 (ray0 a multiple of 16)
+template <bool STREAM = false>
 __device__ inline void store_kstep(uint16_t *__restrict__ dst, int ld, int ray0, int f0,
                                    int nrows, v4u v, bool layer1, uint16_t *t, int lane) {
 	const int g = lane >> 4, c = lane & 15;
@@ -764,8 +765,11 @@ __device__ inline void store_kstep(uint16_t *__restrict__ dst, int ld, int ray0,
 	}
 	const int fr = lane >> 1, hh = lane & 1;
 	const v4u row = *reinterpret_cast<const v4u *>(t + fr * kTRow + 8 * hh);
-	if (f0 + fr < nrows)
-		*reinterpret_cast<v4u *>(dst + rec_offset(f0 + fr, ray0 + 8 * hh, ld)) = row;
+	if (f0 + fr < nrows) {
+		v4u *q = reinterpret_cast<v4u *>(dst + rec_offset(f0 + fr, ray0 + 8 * hh, ld));
+		if (STREAM) __builtin_nontemporal_store(row, q);   // read once, by the next kernel
+		else *q = row;
+	}
 }
 
 // the 2 hidden-layer k-steps as features 0..63
@@ -824,33 +828,29 @@ __global__ __launch_bounds__(2 * SETS * kWave) void train_rows_kernel(
     const float *__restrict__ ro, const float *__restrict__ rd, const int32_t *__restrict__ label,
     const float *__restrict__ target, int n, int n_sets, float3 lo, float3 scale, float inv_m,
     TrainRows out, uint32_t *__restrict__ step) {
-	// LDS: the W4 pack and bias, the hidden biases, each wave's transpose
-	// tile and the halves' exchange (152,896 B with SETS = 2)
+	// LDS: the W4 pack and bias, each wave's transpose tile and the halves'
+	// exchange (152,128 B with SETS = 2)
 	__shared__ v4u w4s[kRec4];
 	__shared__ v4f b4s[kOutP / 4];
-	__shared__ __attribute__((aligned(16))) float bhs[3 * kHid];
 	__shared__ __attribute__((aligned(16))) uint16_t tr[2 * SETS][kTTile];
 	__shared__ v4f xch[SETS][4][kWave];
 	__shared__ float2 stat[SETS][2][16];
 	const int lane = __lane_id(), g = lane >> 4, col = lane & 15;
 	const int wave = (int)(threadIdx.x / kWave), sl = wave >> 1, half = wave & 1;
 	// W4's pack goes to LDS by direct global->LDS loads (64 records of 16 B
-	// per instruction, lane l's record at base + 16 l), all in flight at once
+	// per instruction, lane l's record at base + 16 l), all in flight while
+	// the first set's forward pass runs; waited for before its pass 1
 	for (int i = wave; i < kRec4 / 64; i += 2 * SETS)
 		__builtin_amdgcn_global_load_lds(
 		    (const __attribute__((address_space(1))) void *)(wp + pW4 + 64 * i + lane),
 		    (__attribute__((address_space(3))) void *)(w4s + 64 * i), 16, 0, 0);
 	for (int i = threadIdx.x; i < kOutP / 4; i += 2 * SETS * kWave)
 		b4s[i] = reinterpret_cast<const v4f *>(params + oB4)[i];
-	for (int i = threadIdx.x; i < 3 * kHid; i += 2 * SETS * kWave)
-		bhs[i] = params[i < kHid ? oB1 + i : (i < 2 * kHid ? oB2 + i - kHid : oB3 + i - 2 * kHid)];
 	if (blockIdx.x == 0 && threadIdx.x == 0) *step += 1u;
-	__builtin_amdgcn_s_waitcnt(0);
-	__syncthreads();
 	uint16_t *t = tr[wave];
 	const int ld = out.ld;
-	// the loop runs the same number of times in every wave of the block
-	// (its barriers are reached by all of them)
+	// the loop runs the same number of times in every wave of the block, at
+	// least once (its barriers are reached by all of them)
 	for (int base = blockIdx.x * SETS; base < n_sets; base += gridDim.x * SETS) {
 		const int set = base + sl;
 		const bool act = set < n_sets;
@@ -866,10 +866,15 @@ __global__ __launch_bounds__(2 * SETS * kWave) void train_rows_kernel(
 		const v4u *wqt = wpt + z;
 		v4u xs[1][4];
 		encode_operands(xs[0], ro, rd, act ? ray : n, n, lo, scale, g);
+		const float *pq = params + z;
 		v4u h1[1][2], h2[1][2], h3[1][2];
-		hidden_layer<4, 1>(h1, xs, wq + pW1, bhs, lane);
-		hidden_layer<2, 1>(h2, h1, wq + pW2, bhs + kHid, lane);
-		hidden_layer<2, 1>(h3, h2, wq + pW3, bhs + 2 * kHid, lane);
+		hidden_layer<4, 1>(h1, xs, wq + pW1, pq + oB1, lane);
+		hidden_layer<2, 1>(h2, h1, wq + pW2, pq + oB2, lane);
+		hidden_layer<2, 1>(h3, h2, wq + pW3, pq + oB3, lane);
+		if (base == blockIdx.x * SETS) {                  // W4 and its bias in LDS
+			__builtin_amdgcn_s_waitcnt(0);
+			__syncthreads();
+		}
 
 		// pass 1 over the half's tiles u0..u0+31: running max and sum of
 		// exp.  Software-pipelined: the products of tile u+1 are in flight
@@ -956,7 +961,7 @@ __global__ __launch_bounds__(2 * SETS * kWave) void train_rows_kernel(
 			}
 			const v4u db = {bf16x2(gv[0], gv[1]), bf16x2(gv[2], gv[3]), bf16x2(gv[4], gv[5]),
 			                bf16x2(gv[6], gv[7])};
-			if (act) store_kstep(out.dout, ld, ray0, 32 * s, kOutP, db, false, t, lane);
+			if (act) store_kstep<true>(out.dout, ld, ray0, 32 * s, kOutP, db, false, t, lane);
 			#pragma unroll
 			for (int kt = 0; kt < 4; ++kt) dh[kt] = mfma32(bu[kt], db, dh[kt]);
 		};
@@ -982,7 +987,7 @@ __global__ __launch_bounds__(2 * SETS * kWave) void train_rows_kernel(
 			const bool dl = g == 0 && lab >= 0;
 			const float e = dist - tgt;
 			const v4u db = {bf16x2(dl ? 2.0f * e * inv_m : 0.f, 0.f), 0u, 0u, 0u};
-			if (act) store_kstep(out.dout, ld, ray0, 32 * (kClusters / 32), kOutP, db, false, t, lane);
+			if (act) store_kstep<true>(out.dout, ld, ray0, 32 * (kClusters / 32), kOutP, db, false, t, lane);
 			#pragma unroll
 			for (int kt = 0; kt < 4; ++kt) dh[kt] = mfma32(bt0[kt], db, dh[kt]);
 			if (dl) lossc += e * e;
@@ -1032,34 +1037,34 @@ __global__ __launch_bounds__(2 * SETS * kWave) void train_rows_kernel(
 // can be in flight at once; the X operands of a k-step serve both row
 // tiles.  acc[u][t]: row tile u, input tile t (t = KT: the bias column).
 constexpr int kSplitRays = 256;             // rays per split of the gradient products
-constexpr int kGradNT = 2;
 constexpr int kGradWaves = 4;               // splits (waves) per block, added in LDS
-template <int KT>
-__device__ inline void grad_tiles(v4f acc[kGradNT][KT + 1], const uint16_t *__restrict__ dy,
+
+// dW and db of one layer for NT row tiles of 16 (rows n0.., tiles at and
+// beyond nt read tile 0 and are not stored) and KT tiles of 16 inputs over
+// the kSplitRays rays from r0: D[n][k] = sum_r dY[n][r] X[k][r], and db as
+// the product with a ones column.  Every operand load is issued before the
+// products (one memory latency for the whole split); the X operands of a
+// k-step serve all NT row tiles.  acc[u][t]: row tile u, input tile t (t =
+// KT: the bias column).
+template <int KT, int NT>
+__device__ inline void grad_tiles(v4f acc[NT][KT + 1], const uint16_t *__restrict__ dy,
                                   const uint16_t *__restrict__ x, int ld, int n0, int nt, int r0,
                                   int lane) {
-	const int g = lane >> 4, c = lane & 15;
-	#pragma unroll
-	for (int u = 0; u < kGradNT; ++u)
-		#pragma unroll
-		for (int t = 0; t <= KT; ++t) acc[u][t] = (v4f){0.f, 0.f, 0.f, 0.f};
+	const int c = lane & 15;
 	const unsigned one = c == 0 ? 0x3f803f80u : 0u;   // bf16 1.0 in column 0
 	const v4u ones = {one, one, one, one};
 	// operand records: 1 KiB each, lane l's 16 B at 16 l
 	const size_t tstride = (size_t)(ld >> 5) * 512;               // one feature tile
-	const uint16_t *arow[kGradNT];
+	const uint16_t *arow[NT];
 	#pragma unroll
-	for (int u = 0; u < kGradNT; ++u)
-		arow[u] = dy + rec_offset(n0 + 16 * (u < nt ? u : 0), r0, ld) + 8 * lane;
+	for (int u = 0; u < NT; ++u) arow[u] = dy + rec_offset(n0 + 16 * (u < nt ? u : 0), r0, ld) + 8 * lane;
 	const uint16_t *brow = x + rec_offset(0, r0, ld) + 8 * lane;
-	// every operand load first (one memory latency for the whole split),
-	// then the products
 	constexpr int kSteps = kSplitRays / 32;
-	v4u a[kSteps][kGradNT], b[kSteps][KT];
+	v4u a[kSteps][NT], b[kSteps][KT];
 	#pragma unroll
 	for (int q = 0; q < kSteps; ++q) {
 		#pragma unroll
-		for (int u = 0; u < kGradNT; ++u) a[q][u] = *reinterpret_cast<const v4u *>(arow[u] + 512 * q);
+		for (int u = 0; u < NT; ++u) a[q][u] = *reinterpret_cast<const v4u *>(arow[u] + 512 * q);
 		#pragma unroll
 		for (int t = 0; t < KT; ++t)
 			b[q][t] = *reinterpret_cast<const v4u *>(brow + t * tstride + 512 * q);
@@ -1068,20 +1073,35 @@ __device__ inline void grad_tiles(v4f acc[kGradNT][KT + 1], const uint16_t *__re
 	#pragma unroll
 	for (int q = 0; q < kSteps; ++q)
 		#pragma unroll
-		for (int u = 0; u < kGradNT; ++u) {
+		for (int u = 0; u < NT; ++u) {
 			#pragma unroll
 			for (int t = 0; t < KT; ++t) acc[u][t] = mfma32(a[q][u], b[q][t], acc[u][t]);
 			acc[u][KT] = mfma32(a[q][u], ones, acc[u][KT]);
 		}
 }
 
-// The block's kGradWaves sums (LDS red: [wave][entry][lane]) added in
-// wave order and stored as the block's partial gradient; wave w stores
-// every kGradWaves-th entry from w.
-template <int KT>
-__device__ inline void grad_store(float *__restrict__ part, const float *red, int n0, int nt,
+// One block's job: the waves' sums over their splits, added in LDS (red:
+// [wave][entry][lane]) in wave order and stored as the block's partial
+// gradient (wave w stores every kGradWaves-th entry from w).
+template <int KT, int NT>
+__device__ inline void grad_group(float *__restrict__ part, float *red, const uint16_t *dy,
+                                  const uint16_t *x, int ld, int n0, int nt, int r0, int r1,
                                   int ow, int ldw, int ob, bool layer1, int wave, int lane) {
-	constexpr int kEntries = kGradNT * (KT + 1) * 4;
+	constexpr int kEntries = NT * (KT + 1) * 4;
+	v4f acc[NT][KT + 1];
+	#pragma unroll
+	for (int u = 0; u < NT; ++u)
+		#pragma unroll
+		for (int t = 0; t <= KT; ++t) acc[u][t] = (v4f){0.f, 0.f, 0.f, 0.f};
+	for (int r = r0; r < r1; r += kSplitRays) grad_tiles<KT, NT>(acc, dy, x, ld, n0, nt, r, lane);
+	#pragma unroll
+	for (int u = 0; u < NT; ++u)
+		#pragma unroll
+		for (int t = 0; t <= KT; ++t)
+			#pragma unroll
+			for (int i = 0; i < 4; ++i)
+				red[((size_t)wave * kEntries + (u * (KT + 1) + t) * 4 + i) * kWave + lane] = acc[u][t][i];
+	__syncthreads();
 	const int g = lane >> 4, c = lane & 15;
 	for (int e = wave; e < kEntries; e += kGradWaves) {
 		float v = red[e * kWave + lane];
@@ -1100,65 +1120,40 @@ __device__ inline void grad_store(float *__restrict__ part, const float *red, in
 	}
 }
 
-template <int KT>
-__device__ inline void grad_group(float *__restrict__ part, float *red, const uint16_t *dy,
-                                  const uint16_t *x, int ld, int n0, int nt, int r0, bool active,
-                                  int ow, int ldw, int ob, bool layer1, int wave, int lane) {
-	constexpr int kEntries = kGradNT * (KT + 1) * 4;
-	v4f acc[kGradNT][KT + 1];
-	if (active) {
-		grad_tiles<KT>(acc, dy, x, ld, n0, nt, r0, lane);
-	} else {
-		#pragma unroll
-		for (int u = 0; u < kGradNT; ++u)
-			#pragma unroll
-			for (int t = 0; t <= KT; ++t) acc[u][t] = (v4f){0.f, 0.f, 0.f, 0.f};
-	}
-	#pragma unroll
-	for (int u = 0; u < kGradNT; ++u)
-		#pragma unroll
-		for (int t = 0; t <= KT; ++t)
-			#pragma unroll
-			for (int i = 0; i < 4; ++i)
-				red[((size_t)wave * kEntries + (u * (KT + 1) + t) * 4 + i) * kWave + lane] = acc[u][t][i];
-	__syncthreads();
-	grad_store<KT>(part, red, n0, nt, ow, ldw, ob, layer1, wave, lane);
-}
-
-// Block = (group of kGradNT row tiles, kGradWaves consecutive splits);
-// wave w takes split kGradWaves * blk + w (zero beyond the last split).
-// Groups 0..32: W4 (the last one tile), then two each of W3, W2, W1.  The
-// block's partial goes to part[blk].
-constexpr int kGroups4 = (kOutP / 16 + kGradNT - 1) / kGradNT;     // 33
-constexpr int kGroupsH = kHid / 16 / kGradNT;                      // 2
-constexpr int kGradGroups = kGroups4 + 3 * kGroupsH;               // 39
-constexpr int kRedFloats = kGradWaves * kGradNT * (kInP / 16 + 1) * 4 * kWave;  // 64 KiB
+// Block = (group of row tiles, kGradWaves consecutive wave ranges of
+// kWaveRays rays); wave w takes range kGradWaves * blk + w (empty beyond the
+// rows).  Groups: W4 in pairs of row tiles (0..32, the last one tile), then
+// pairs of W3 (33, 34) and W2 (35, 36), then single row tiles of W1
+// (37..40), whose 7 input tiles would not fit a pair in the register
+// budget.  The block's partial goes to part[blk].
+constexpr int kWaveRays = 2 * kSplitRays;   // 512 rays per wave: one block per CU at 8,192 rays
+constexpr int kGroups4 = (kOutP / 16 + 1) / 2;                     // 33
+constexpr int kGradGroups = kGroups4 + 2 + 2 + 4;                  // 41
+constexpr int kRedFloats = kGradWaves * 2 * (kHid / 16 + 1) * 4 * kWave;  // 40 KiB
 __global__ __launch_bounds__(kGradWaves * kWave) void grad_partials_kernel(
     float *__restrict__ part, TrainRows in, int splits) {
 	__shared__ float red[kRedFloats];
-	const int blks = (splits + kGradWaves - 1) / kGradWaves;
+	const int n_pad = splits * kSplitRays;
+	const int blks = (n_pad + kGradWaves * kWaveRays - 1) / (kGradWaves * kWaveRays);
 	const int grp = blockIdx.x / blks, blk = blockIdx.x % blks;
 	const int wave = (int)(threadIdx.x / kWave), lane = __lane_id();
-	const int split = kGradWaves * blk + wave;
-	const bool active = split < splits;
-	const int r0 = split * kSplitRays;
+	const int rb = (kGradWaves * blk + wave) * kWaveRays;
+	const int r0 = rb < n_pad ? rb : n_pad, r1 = rb + kWaveRays < n_pad ? rb + kWaveRays : n_pad;
 	float *pp = part + (size_t)blk * kParams;
 	if (grp < kGroups4) {
-		const int t0 = kGradNT * grp, nt = kOutP / 16 - t0 < kGradNT ? kOutP / 16 - t0 : kGradNT;
-		grad_group<kHid / 16>(pp, red, in.dout, in.h3, in.ld, 16 * t0, nt, r0, active, oW4, kHid,
-		                      oB4, false, wave, lane);
-		return;
+		const int t0 = 2 * grp, nt = kOutP / 16 - t0 < 2 ? kOutP / 16 - t0 : 2;
+		grad_group<kHid / 16, 2>(pp, red, in.dout, in.h3, in.ld, 16 * t0, nt, r0, r1, oW4,
+		                         kHid, oB4, false, wave, lane);
+	} else if (grp < kGroups4 + 2) {
+		grad_group<kHid / 16, 2>(pp, red, in.dz3, in.h2, in.ld, 32 * (grp - kGroups4), 2, r0,
+		                         r1, oW3, kHid, oB3, false, wave, lane);
+	} else if (grp < kGroups4 + 4) {
+		grad_group<kHid / 16, 2>(pp, red, in.dz2, in.h1, in.ld, 32 * (grp - kGroups4 - 2), 2, r0,
+		                         r1, oW2, kHid, oB2, false, wave, lane);
+	} else {
+		grad_group<kInP / 16, 1>(pp, red, in.dz1, in.x, in.ld, 16 * (grp - kGroups4 - 4), 1, r0,
+		                         r1, oW1, kInP, oB1, true, wave, lane);
 	}
-	const int h = grp - kGroups4, layer = 3 - h / kGroupsH, n0 = 16 * kGradNT * (h % kGroupsH);
-	if (layer == 3)
-		grad_group<kHid / 16>(pp, red, in.dz3, in.h2, in.ld, n0, kGradNT, r0, active, oW3, kHid,
-		                      oB3, false, wave, lane);
-	else if (layer == 2)
-		grad_group<kHid / 16>(pp, red, in.dz2, in.h1, in.ld, n0, kGradNT, r0, active, oW2, kHid,
-		                      oB2, false, wave, lane);
-	else
-		grad_group<kInP / 16>(pp, red, in.dz1, in.x, in.ld, n0, kGradNT, r0, active, oW1, kInP,
-		                      oB1, true, wave, lane);
 }
 
 // gradient = sum of the n_part partials (in order), Adam with the step word, the bf16
@@ -1217,7 +1212,7 @@ struct State {
 	uint16_t *wpack = nullptr;   // forward MFMA pack of the weights
 	uint16_t *wpackt = nullptr;  // backward MFMA pack (W4, W3, W2 transposed)
 	TrainRows rows{};            // the fused training step's feature-major rows
-	float *part = nullptr;       // partial gradients, one per kGradWaves ray splits
+	float *part = nullptr;       // partial gradients, one per 2,048 rays
 	int cus = 0;                 // compute units of the device the state lives on
 	int n_fwd = 0;               // rows of the last forward pass
 	std::vector<void *> allocs;
@@ -1289,7 +1284,8 @@ int train_step(State *p, const float *o, const float *d, const int32_t *label,
 	                   reinterpret_cast<const v4u *>(p->wpackt), p->params, o, d, label, target, n,
 	                   n_sets, lo, sc, 1.0f / (float)n, p->rows, step);
 	RTA_LAUNCH_CHECK("train_rows_kernel");
-	const int splits = n_pad / kSplitRays, blks = (splits + kGradWaves - 1) / kGradWaves;
+	const int splits = n_pad / kSplitRays;
+	const int blks = (n_pad + kGradWaves * kWaveRays - 1) / (kGradWaves * kWaveRays);
 	hipLaunchKernelGGL(grad_partials_kernel, dim3(kGradGroups * blks), dim3(kGradWaves * kWave), 0,
 	                   st, p->part, p->rows, splits);
 	RTA_LAUNCH_CHECK("grad_partials_kernel");
@@ -1349,7 +1345,7 @@ int rta_pred_create(int max_batch, uint64_t seed, rta_pred *out) {
 	rc |= p->alloc(p->rows.dz2, ld * kHid);
 	rc |= p->alloc(p->rows.dz1, ld * kHid);
 	rc |= p->alloc(p->rows.loss_part, ld / 8);
-	rc |= p->alloc(p->part, (ld / kSplitRays + kGradWaves - 1) / kGradWaves * kParams);
+	rc |= p->alloc(p->part, (ld + kGradWaves * kWaveRays - 1) / (kGradWaves * kWaveRays) * kParams);
 	if (rc != RTA_OK) {
 		delete p;
 		set_error("rta_pred_create: device allocation failed");
