@@ -414,6 +414,9 @@ def cpu_baseline(tris_h, bo, bd, bt, n_bounce, w, max_rays):
 
 
 PRED_FLOPS_PER_RAY = 2 * (102 * 64 + 64 * 64 + 64 * 64 + 64 * 1025)   # 160,640
+# a training step per ray: the forward pass, the data gradients of layers
+# 4, 3, 2 and the weight gradients of all four (468,864)
+PRED_TRAIN_FLOPS_PER_RAY = 2 * PRED_FLOPS_PER_RAY + 2 * (64 * 1025 + 64 * 64 + 64 * 64)
 
 
 def predictor_leg(args, tris_h, tris, nodes, packed, ranges, prim, hits_p, R, tcfg, s):
@@ -499,8 +502,9 @@ def predictor_leg(args, tris_h, tris, nodes, packed, ranges, prim, hits_p, R, tc
     losses, train_s = pred.train(o_b, d_b, l_b, g_b, n_tr, epochs=args.predictor_epochs,
                                  batch=8192, lr=2e-3, stream=s)
     del o_b, d_b, l_b, g_b
-    log(f"predictor trained: {args.predictor_epochs} epochs in {train_s:.1f} s, loss "
-        f"{losses[0]:.3f} -> {losses[-1]:.3f}")
+    train_steps = args.predictor_epochs * ((n_tr + 8191) // 8192)
+    log(f"predictor trained: {args.predictor_epochs} epochs ({train_steps} steps) in "
+        f"{train_s:.2f} s, loss {losses[0]:.3f} -> {losses[-1]:.3f}")
 
     top, t_seed, maxt2 = DeviceBuffer(16 * m_max), DeviceBuffer(4 * m_max), DeviceBuffer(4 * m_max)
     ev = [Event() for _ in range(5)]
@@ -672,7 +676,13 @@ def predictor_leg(args, tris_h, tris, nodes, packed, ranges, prim, hits_p, R, tc
                      "triangles_min_median_max": [int(sizes.min()), int(np.median(sizes)),
                                                   int(sizes.max())]},
         "training": {"rays": n_tr, "epochs": args.predictor_epochs, "batch": 8192, "lr": 2e-3,
-                     "seconds": round(train_s, 2),
+                     "seconds": round(train_s, 3),
+                     "path": "fused step: rta_pred_train_batches, 3 launches per batch",
+                     "steps": train_steps,
+                     "us_per_step": round(train_s / train_steps * 1e6, 1),
+                     "flops_per_ray": PRED_TRAIN_FLOPS_PER_RAY,
+                     "tflop_s": round(PRED_TRAIN_FLOPS_PER_RAY * n_tr * args.predictor_epochs /
+                                      train_s / 1e12, 2),
                      "loss_first_last": [round(losses[0], 4), round(losses[-1], 4)],
                      "bounce_seeds": [scenes.BOUNCE_SEED + k
                                       for k in scenes.PREDICTOR_TRAIN_SEEDS]},
