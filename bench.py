@@ -113,8 +113,14 @@ def batch_leg(args) -> dict:
     cfg, tune_ms = b.autotune(rounds=2, reps=2)
     log(f"autotuned {cfg.as_tuple()} -> {tune_ms:.3f} ms")
     trace_ms = timed(lambda: b.trace(config=cfg), max(args.steps, 3))
+    ovf = b.overflow_count()
     default_ms = timed(lambda: b.trace(config=TraceConfig(64, 16, 32, 0, 10)), 3)
     b.check_errors()
+    # the whole frame (restore + build + pack + reset + trace) as one graph
+    g = b.capture(config=cfg)
+    graph_ms = timed(lambda: g.launch(s), max(args.steps, 3))
+    b.check_errors()
+    del g
 
     # the same meshes one build + one trace launch at a time
     per = []
@@ -148,6 +154,10 @@ def batch_leg(args) -> dict:
         r = q["nodes"].download(NODE_DTYPE, 1)[0]
         roots.append((int(r["type_left_elems"]), int(r["right_tris"])))
 
+    # per-mesh launches: the autotuned shape with the full stack (a short
+    # stack needs the batch's overflow list)
+    cfg_each = TraceConfig(*cfg.as_tuple()[:5], 32)
+
     def trace_each():
         b.reset_hits()
         for m, q in enumerate(per):
@@ -157,7 +167,7 @@ def batch_leg(args) -> dict:
             call("rta_trace_cis_persistent", ptr(q["tri4"]), ptr(q["pairs"]), ptr(q["flags"]),
                  roots[m][0], roots[m][1], C.c_void_p(b.ro.ptr + 12 * r0),
                  C.c_void_p(b.rd.ptr + 12 * r0), C.c_void_p(b.rt.ptr + 4 * r0), q["nr"], None,
-                 C.c_void_p(b.hits.ptr + 16 * r0), ptr(b.err), ptr(b.queue), C.byref(cfg), sp)
+                 C.c_void_p(b.hits.ptr + 16 * r0), ptr(b.err), ptr(b.queue), C.byref(cfg_each), sp)
 
     each_build_ms = timed(build_each, 3)
     each_trace_ms = timed(trace_each, 3)
@@ -177,6 +187,11 @@ def batch_leg(args) -> dict:
         "batch_build_mtri_s": round(N / (build_ms * 1e3), 3),
         "batch_trace_ms": round(trace_ms, 4),
         "autotuned_config": list(cfg.as_tuple()),
+        "autotuned_config_fields": ["block", "refill_min", "steps", "blocks_per_cu",
+                                    "leaf_min", "stack"],
+        "short_stack_rays_retraced": ovf,
+        "graph_frame_ms": round(graph_ms, 4),
+        "graph_frame": "restore + build + pack + reset + trace captured as one graph",
         "default_config_trace_ms": round(default_ms, 4),
         "per_mesh_launches_build_ms": round(each_build_ms, 4),
         "per_mesh_launches_trace_ms": round(each_trace_ms, 4),

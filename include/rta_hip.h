@@ -322,9 +322,12 @@ int rta_trace_cis_packed(const float *tri4, const rta_node_pair *pairs,
  * checks), blocks_per_cu (0 = as many as fit) and leaf_min (a wave tests
  * leaf triangles once at least ceil(leaf_min * active / 64) of its active
  * lanes wait at a leaf, or no lane is at an inner node; 1..64, 0 = default
- * 10).  None of these changes a result. */
+ * 10) and stack (LDS stack entries: 32, the reference's depth, or 16 or 24
+ * for a smaller LDS footprint and more resident waves, batch traversal
+ * only: a ray that would need more is traced again with 32 entries; 0 =
+ * 32).  None of these changes a result. */
 typedef struct rta_trace_config {
-	int block, refill_min, steps, blocks_per_cu, leaf_min;
+	int block, refill_min, steps, blocks_per_cu, leaf_min, stack;
 } rta_trace_config;
 
 /* Persistent form of rta_trace_cis_packed: a machine-filling grid of waves
@@ -401,9 +404,12 @@ int rta_batch_trace_persistent(rta_batch batch, const float *tri4, const rta_nod
                                const uint32_t *flags, const float *ray_orig, const float *ray_dir,
                                const float *max_t, rta_hit *hits, uint32_t *err, uint32_t *queue,
                                int any_hit, const rta_trace_config *cfg, rta_stream s);
+/* Rays the last batch traversal with stack 16 or 24 traced again because
+ * they needed more entries (synchronous; 0 after a 32-entry traversal). */
+int rta_batch_overflow_count(rta_batch batch, uint32_t *count, rta_stream s);
 /* Autotuner of the persistent launch shape on this batch: coordinate
- * descent over leaf_min, refill_min (the queue chunk a wave takes), steps
- * and block, `rounds` rounds, each candidate timed as the best of `reps`
+ * descent over leaf_min, refill_min (the queue chunk a wave takes), steps,
+ * block and stack, `rounds` rounds, each candidate timed as the best of `reps`
  * launches (synchronous).  *best: the starting shape (block 0 = defaults);
  * on return the fastest found and *best_ms its time.  The hits of the last
  * launch are valid results. */

@@ -58,12 +58,14 @@ class RtaTimes(C.Structure):
 
 
 class TraceConfig(C.Structure):
-    """rta_trace_config: launch shape of the persistent tracer."""
+    """rta_trace_config: launch shape of the persistent tracer (stack 0 =
+    the full 32 entries; 16 or 24 for batch traversal)."""
     _fields_ = [("block", C.c_int), ("refill_min", C.c_int), ("steps", C.c_int),
-                ("blocks_per_cu", C.c_int), ("leaf_min", C.c_int)]
+                ("blocks_per_cu", C.c_int), ("leaf_min", C.c_int), ("stack", C.c_int)]
 
     def as_tuple(self):
-        return (self.block, self.refill_min, self.steps, self.blocks_per_cu, self.leaf_min)
+        return (self.block, self.refill_min, self.steps, self.blocks_per_cu, self.leaf_min,
+                self.stack)
 
 
 class RtaError(RuntimeError):
@@ -150,6 +152,7 @@ HIP_SIGNATURES = {
     "rta_batch_pack": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "rta_batch_trace_persistent": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp,
                                    _vp],
+    "rta_batch_overflow_count": [_vp, _vp, _vp],
     "rta_batch_autotune": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _vp,
                            _pf, _vp],
     "rta_pred_param_count": [_pi],

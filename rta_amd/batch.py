@@ -99,6 +99,29 @@ class MeshBatch:
         self.config = best
         return best, ms.value
 
+    def capture(self, any_hit: bool = False, config: TraceConfig | None = None):
+        """The whole batch frame -- restore the triangles, build and pack
+        every mesh, reset the hits, trace all rays -- captured as one graph
+        (rta_hip_graph_*), replayed with .launch(self.stream)."""
+        from .frame import Graph
+
+        call("rta_hip_graph_begin", ptr(self.stream))
+        h = C.c_void_p()
+        try:
+            self.build()
+            self.trace(any_hit=any_hit, config=config)
+        finally:
+            # always end the capture so the stream is usable again
+            call("rta_hip_graph_end", ptr(self.stream), C.byref(h))
+        return Graph(h)
+
+    def overflow_count(self) -> int:
+        """Rays the last traversal with stack 16 or 24 traced again with the
+        full 32 entries."""
+        c = C.c_uint32(0)
+        call("rta_batch_overflow_count", self.handle, C.byref(c), ptr(self.stream))
+        return int(c.value)
+
     def check_errors(self) -> None:
         e = int(self.err.download(np.uint32, 1)[0])
         if e:

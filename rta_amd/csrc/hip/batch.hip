@@ -239,6 +239,8 @@ int rta_batch_create(int n_meshes, const int *n_tris, const int *n_rays, rta_bat
 	hipError_t e = hipMalloc(&p->d_meshes, sizeof(BatchMesh) * n_meshes);
 	if (e == hipSuccess) e = hipMalloc(&p->d_items, sizeof(BatchItem) * std::max<size_t>(1, items.size()));
 	if (e == hipSuccess) e = hipMalloc(&p->d_root_words, sizeof(uint32_t) * n_meshes);
+	if (e == hipSuccess) e = hipMalloc(&p->d_ovf, sizeof(uint32_t) * (size_t)(p->total_rays + 1));
+	if (e == hipSuccess) e = hipMemset(p->d_ovf, 0, sizeof(uint32_t));
 	if (e == hipSuccess)
 		e = hipMemcpy(p->d_meshes, p->meshes.data(), sizeof(BatchMesh) * n_meshes,
 		              hipMemcpyHostToDevice);
@@ -249,6 +251,7 @@ int rta_batch_create(int n_meshes, const int *n_tris, const int *n_rays, rta_bat
 		(void)hipFree(p->d_meshes);
 		(void)hipFree(p->d_items);
 		(void)hipFree(p->d_root_words);
+		(void)hipFree(p->d_ovf);
 		delete p;
 		return hip_status(e, "rta_batch_create");
 	}
@@ -260,10 +263,21 @@ int rta_batch_destroy(rta_batch b) {
 	if (!b) return RTA_OK;
 	BatchPlan *p = (BatchPlan *)b;
 	hipError_t e1 = hipFree(p->d_meshes), e2 = hipFree(p->d_items), e3 = hipFree(p->d_root_words);
+	hipError_t e4 = hipFree(p->d_ovf);
 	delete p;
 	if (e1 != hipSuccess) return hip_status(e1, "hipFree");
 	if (e2 != hipSuccess) return hip_status(e2, "hipFree");
 	if (e3 != hipSuccess) return hip_status(e3, "hipFree");
+	if (e4 != hipSuccess) return hip_status(e4, "hipFree");
+	return RTA_OK;
+}
+
+int rta_batch_overflow_count(rta_batch b, uint32_t *count, rta_stream s) {
+	if (!b || !count) return invalid("rta_batch_overflow_count: NULL argument");
+	const BatchPlan *p = (const BatchPlan *)b;
+	hipStream_t st = as_stream(s);
+	RTA_HIP_TRY(hipMemcpyAsync(count, p->d_ovf, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+	RTA_HIP_TRY(hipStreamSynchronize(st));
 	return RTA_OK;
 }
 

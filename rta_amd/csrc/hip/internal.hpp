@@ -50,6 +50,7 @@ struct BatchPlan {
 	size_t chunk_items_off = 0, chunk_items = 0;   // 1024 leaves per block
 	std::vector<size_t> merge_off, merge_len;      // per merge level
 	uint32_t *d_root_words = nullptr;              // per mesh, written by the pack
+	uint32_t *d_ovf = nullptr;    // short-stack overflow list: count, then up to total_rays rays
 };
 
 }  // namespace rta_hip

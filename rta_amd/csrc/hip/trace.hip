@@ -638,7 +638,13 @@ struct SeedArgs {
 	const int32_t *top = nullptr;
 };
 
-template <int BLOCK, bool ANY_HIT, bool BATCHED, bool SEEDED = false>
+// STACK: the LDS stack's entries (kStack, or fewer for a smaller LDS
+// footprint and so more resident waves).  With STACK < kStack a ray that
+// would push beyond it is abandoned and appended to the overflow list ovf
+// (ovf[0] the count, rays from ovf[1]); the caller traces those rays again
+// with kStack entries.  ray_list (optional): trace the rays ray_list[1 ..
+// ray_list[0]] instead of 0 .. n-1.
+template <int BLOCK, bool ANY_HIT, bool BATCHED, bool SEEDED = false, int STACK = kStack>
 __global__ __launch_bounds__(BLOCK) void trace_cis_persistent_kernel(
     const float4 *__restrict__ tri4, const float4 *__restrict__ pairs,
     const uint32_t *__restrict__ flags, uint32_t root_word, const float *__restrict__ ro,
@@ -646,14 +652,17 @@ __global__ __launch_bounds__(BLOCK) void trace_cis_persistent_kernel(
     const uint32_t *__restrict__ ray_count, rta_hit *__restrict__ hits,
     uint32_t *__restrict__ err, uint32_t *__restrict__ queue, int refill_min, int steps,
     int leaf_min, const BatchMesh *__restrict__ meshes, int n_meshes,
-    const uint32_t *__restrict__ root_words, SeedArgs seed) {
+    const uint32_t *__restrict__ root_words, uint32_t *__restrict__ ovf,
+    const uint32_t *__restrict__ ray_list, SeedArgs seed) {
 	static_assert(!SEEDED || (!ANY_HIT && !BATCHED), "seeding: closest hit on one mesh");
-	__shared__ uint32_t stack[kStack][BLOCK];
+	static_assert(STACK <= kStack, "at most kStack entries");
+	__shared__ uint32_t stack[STACK][BLOCK];
 	const int lt = threadIdx.x;
 	const int lane = __lane_id();
 	const uint64_t lower = lane == 0 ? 0ull : (~0ull >> (64 - lane));
 	uint32_t n = (uint32_t)n_rays;
 	if (ray_count) n = min(n, *ray_count);
+	if (ray_list) n = min(n, ray_list[0]);
 	const bool scene_fast = BATCHED ? false : (*flags & 1u) == 0u;
 	const uint32_t n_treelets = SEEDED ? *seed.count : 0u;
 	bool seeding = false;   // SEEDED: the lane is in its treelet phase
@@ -700,7 +709,7 @@ __global__ __launch_bounds__(BLOCK) void trace_cis_persistent_kernel(
 			if (!active) {
 				const uint32_t my = base + (uint32_t)__popcll(idle & lower);
 				if (my < n) {
-					ray = my;
+					ray = ray_list ? ray_list[1 + my] : my;
 					ox = ro[3 * (size_t)ray + 0]; oy = ro[3 * (size_t)ray + 1]; oz = ro[3 * (size_t)ray + 2];
 					dx = rd[3 * (size_t)ray + 0]; dy = rd[3 * (size_t)ray + 1]; dz = rd[3 * (size_t)ray + 2];
 					tmax = rmax[ray];
@@ -780,15 +789,16 @@ __global__ __launch_bounds__(BLOCK) void trace_cis_persistent_kernel(
 				// or finish on an empty stack.  The far child is written one
 				// slot above the top on every step (after the top is read);
 				// only a push makes that slot part of the stack.  On a full
-				// stack (kStack entries) there is no slot above the top: the
+				// stack (STACK entries) there is no slot above the top: the
 				// top entry is written back with its own value, so a step that
 				// keeps one child or none leaves the stack intact, and a push
-				// ends the ray with an error count.
+				// ends the ray with an error count (STACK = kStack) or on the
+				// overflow list (STACK < kStack).
 				const bool both = hl && hr, any = hl || hr;
 				const bool enter_left = both ? (dl <= dr) : hl;
-				const bool full = sp + 1 >= kStack;
+				const bool full = sp + 1 >= STACK;
 				const uint32_t popped = stack[sp >= 0 ? sp : 0][lt];
-				stack[full ? kStack - 1 : sp + 1][lt] = full ? popped : (enter_left ? wr : wl);
+				stack[full ? STACK - 1 : sp + 1][lt] = full ? popped : (enter_left ? wr : wl);
 				curr = any ? (enter_left ? wl : wr) : popped;
 				overflow = both && full;
 				done = overflow || (!any && sp < 0);
@@ -819,7 +829,10 @@ __global__ __launch_bounds__(BLOCK) void trace_cis_persistent_kernel(
 				--sp;
 			}
 			if (done) {
-				if (overflow) atomicAdd(err, 1u);
+				if (overflow) {
+					if (STACK < kStack) ovf[1 + atomicAdd(ovf, 1u)] = ray;   // traced again
+					else atomicAdd(err, 1u);
+				}
 				if (SEEDED && seeding) {
 					// the treelet phase ends: its closest t (a real hit, or
 					// FLT_MAX), widened by seed_bound's margin, bounds the
@@ -997,24 +1010,28 @@ namespace rta_hip {
 
 // Grid of the persistent kernel: as many blocks as fit on every CU (or
 // blocks_per_cu), never more than the rays need.  The CU count and the
-// occupancy are queried once per device and kernel instance.
-template <int BLOCK, bool ANY_HIT, bool BATCHED, bool SEEDED = false>
+// occupancy are queried once per device and kernel instance.  With STACK <
+// kStack the rays that overflow the stack (the list ovf, n_rays + 1 words)
+// are traced again by a kStack launch over that list.
+template <int BLOCK, bool ANY_HIT, bool BATCHED, bool SEEDED = false, int STACK = kStack>
 int launch_persistent(const float *tri4, const rta_node_pair *pairs, const uint32_t *flags,
                       uint32_t root_word, const float *ro, const float *rd, const float *rmax,
                       int n_rays, const uint32_t *ray_count, rta_hit *hits, uint32_t *err,
                       uint32_t *queue, int refill_min, int steps, int blocks_per_cu,
                       int leaf_min, const BatchMesh *meshes, int n_meshes,
-                      const uint32_t *root_words, hipStream_t st, SeedArgs seed = SeedArgs{}) {
+                      const uint32_t *root_words, hipStream_t st, SeedArgs seed = SeedArgs{},
+                      uint32_t *ovf = nullptr, const uint32_t *ray_list = nullptr) {
 	constexpr int kMaxDevices = 64;
 	static int cus[kMaxDevices] = {}, occ[kMaxDevices] = {};
 	int dev = 0;
 	RTA_HIP_TRY(hipGetDevice(&dev));
 	if (dev < 0 || dev >= kMaxDevices) return invalid("launch_persistent: device id out of range");
+	if (STACK < kStack && !ovf) return invalid("launch_persistent: a short stack needs an overflow list");
 	if (cus[dev] == 0) {
 		int c = 0, o = 0;
 		RTA_HIP_TRY(hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev));
 		RTA_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-		    &o, trace_cis_persistent_kernel<BLOCK, ANY_HIT, BATCHED, SEEDED>, BLOCK, 0));
+		    &o, trace_cis_persistent_kernel<BLOCK, ANY_HIT, BATCHED, SEEDED, STACK>, BLOCK, 0));
 		occ[dev] = o < 1 ? 1 : o;
 		cus[dev] = c;
 	}
@@ -1022,29 +1039,70 @@ int launch_persistent(const float *tri4, const rta_node_pair *pairs, const uint3
 	const long long want = (long long)cus[dev] * bpc;
 	const long long need = ((long long)n_rays + BLOCK - 1) / BLOCK;
 	const unsigned blocks = (unsigned)(want < need ? want : need);
+	if (STACK < kStack) {
+		const int rc = rta_hip_fill_u32(ovf, 0u, 1, reinterpret_cast<rta_stream>(st));
+		if (rc != RTA_OK) return rc;
+	}
 	hipLaunchKernelGGL(fill_queue_kernel, dim3(1), dim3(64), 0, st, queue);
 	RTA_LAUNCH_CHECK("fill_queue_kernel");
-	hipLaunchKernelGGL((trace_cis_persistent_kernel<BLOCK, ANY_HIT, BATCHED, SEEDED>),
+	hipLaunchKernelGGL((trace_cis_persistent_kernel<BLOCK, ANY_HIT, BATCHED, SEEDED, STACK>),
 	                   dim3(blocks), dim3(BLOCK), 0, st, (const float4 *)tri4,
 	                   (const float4 *)pairs, flags, root_word, ro, rd, rmax, n_rays, ray_count,
 	                   hits, err, queue, refill_min, steps, leaf_min, meshes, n_meshes,
-	                   root_words, seed);
+	                   root_words, STACK < kStack ? ovf : nullptr, ray_list, seed);
 	RTA_LAUNCH_CHECK("trace_cis_persistent_kernel");
+	if (STACK < kStack)
+		return launch_persistent<BLOCK, ANY_HIT, BATCHED, SEEDED, kStack>(
+		    tri4, pairs, flags, root_word, ro, rd, rmax, n_rays, nullptr, hits, err, queue,
+		    refill_min, steps, blocks_per_cu, leaf_min, meshes, n_meshes, root_words, st, seed,
+		    nullptr, ovf);
 	return RTA_OK;
 }
 
 // Validated launch shape with defaults filled in.
 static int resolve_config(const char *name, const rta_trace_config *cfg, rta_trace_config &c) {
-	c = {64, 16, 32, 0, 0};
+	c = {64, 16, 32, 0, 0, 0};
 	if (cfg) c = *cfg;
 	if (c.leaf_min == 0) c.leaf_min = 10;
+	if (c.stack == 0) c.stack = kStack;
 	if (c.refill_min < 1 || c.refill_min > 64 || c.steps < 1 || c.blocks_per_cu < 0 ||
-	    c.leaf_min < 1 || c.leaf_min > 64 || (c.block != 64 && c.block != 128 && c.block != 256)) {
+	    c.leaf_min < 1 || c.leaf_min > 64 || (c.block != 64 && c.block != 128 && c.block != 256) ||
+	    (c.stack != 16 && c.stack != 24 && c.stack != kStack)) {
 		set_error(std::string(name) + ": bad config (block 64/128/256, refill_min and leaf_min "
-		                              "1..64, steps >= 1, blocks_per_cu >= 0)");
+		                              "1..64, steps >= 1, blocks_per_cu >= 0, stack 16/24/32)");
 		return RTA_E_INVALID;
 	}
 	return RTA_OK;
+}
+
+// the single-mesh entry points hold no overflow list: full stack only
+static int require_full_stack(const char *name, const rta_trace_config &c) {
+	if (c.stack == kStack) return RTA_OK;
+	set_error(std::string(name) + ": stack 16 and 24 need the batch API's overflow list");
+	return RTA_E_INVALID;
+}
+
+template <int BLOCK, bool ANY_HIT, bool BATCHED>
+int launch_stack(const rta_trace_config &c, const float *tri4, const rta_node_pair *pairs,
+                 const uint32_t *flags, uint32_t root_word, const float *ro, const float *rd,
+                 const float *rmax, int n_rays, const uint32_t *ray_count, rta_hit *hits,
+                 uint32_t *err, uint32_t *queue, const BatchMesh *meshes, int n_meshes,
+                 const uint32_t *root_words, uint32_t *ovf, hipStream_t st) {
+	if constexpr (BATCHED) {
+		if (c.stack == 16)
+			return launch_persistent<BLOCK, ANY_HIT, BATCHED, false, 16>(
+			    tri4, pairs, flags, root_word, ro, rd, rmax, n_rays, ray_count, hits, err, queue,
+			    c.refill_min, c.steps, c.blocks_per_cu, c.leaf_min, meshes, n_meshes, root_words,
+			    st, SeedArgs{}, ovf);
+		if (c.stack == 24)
+			return launch_persistent<BLOCK, ANY_HIT, BATCHED, false, 24>(
+			    tri4, pairs, flags, root_word, ro, rd, rmax, n_rays, ray_count, hits, err, queue,
+			    c.refill_min, c.steps, c.blocks_per_cu, c.leaf_min, meshes, n_meshes, root_words,
+			    st, SeedArgs{}, ovf);
+	}
+	return launch_persistent<BLOCK, ANY_HIT, BATCHED>(
+	    tri4, pairs, flags, root_word, ro, rd, rmax, n_rays, ray_count, hits, err, queue,
+	    c.refill_min, c.steps, c.blocks_per_cu, c.leaf_min, meshes, n_meshes, root_words, st);
 }
 
 template <bool ANY_HIT, bool BATCHED>
@@ -1053,20 +1111,20 @@ int launch_persistent_cfg(const rta_trace_config &c, const float *tri4, const rt
                           const float *rd, const float *rmax, int n_rays,
                           const uint32_t *ray_count, rta_hit *hits, uint32_t *err,
                           uint32_t *queue, const BatchMesh *meshes, int n_meshes,
-                          const uint32_t *root_words, hipStream_t st) {
+                          const uint32_t *root_words, hipStream_t st, uint32_t *ovf = nullptr) {
 	switch (c.block) {
 	case 64:
-		return launch_persistent<64, ANY_HIT, BATCHED>(
-		    tri4, pairs, flags, root_word, ro, rd, rmax, n_rays, ray_count, hits, err, queue,
-		    c.refill_min, c.steps, c.blocks_per_cu, c.leaf_min, meshes, n_meshes, root_words, st);
+		return launch_stack<64, ANY_HIT, BATCHED>(c, tri4, pairs, flags, root_word, ro, rd, rmax,
+		                                          n_rays, ray_count, hits, err, queue, meshes,
+		                                          n_meshes, root_words, ovf, st);
 	case 128:
-		return launch_persistent<128, ANY_HIT, BATCHED>(
-		    tri4, pairs, flags, root_word, ro, rd, rmax, n_rays, ray_count, hits, err, queue,
-		    c.refill_min, c.steps, c.blocks_per_cu, c.leaf_min, meshes, n_meshes, root_words, st);
+		return launch_stack<128, ANY_HIT, BATCHED>(c, tri4, pairs, flags, root_word, ro, rd, rmax,
+		                                           n_rays, ray_count, hits, err, queue, meshes,
+		                                           n_meshes, root_words, ovf, st);
 	default:
-		return launch_persistent<256, ANY_HIT, BATCHED>(
-		    tri4, pairs, flags, root_word, ro, rd, rmax, n_rays, ray_count, hits, err, queue,
-		    c.refill_min, c.steps, c.blocks_per_cu, c.leaf_min, meshes, n_meshes, root_words, st);
+		return launch_stack<256, ANY_HIT, BATCHED>(c, tri4, pairs, flags, root_word, ro, rd, rmax,
+		                                           n_rays, ray_count, hits, err, queue, meshes,
+		                                           n_meshes, root_words, ovf, st);
 	}
 }
 
@@ -1104,7 +1162,8 @@ int trace_persistent(const char *name, const float *tri4, const rta_node_pair *p
 		return RTA_E_INVALID;
 	}
 	rta_trace_config c;
-	const int rc = resolve_config(name, cfg, c);
+	int rc = resolve_config(name, cfg, c);
+	if (rc == RTA_OK) rc = require_full_stack(name, c);
 	if (rc != RTA_OK) return rc;
 	const uint32_t root_word = link_inner(root_link) ? 1u : (root_index << 1);
 	return launch_persistent_cfg<ANY_HIT, false>(c, tri4, pairs, flags, root_word, ray_orig,
@@ -1238,7 +1297,8 @@ int rta_trace_cis_persistent_seeded(const float *tri4, const rta_node_pair *pair
 	    !queue || !table || !count || !top || n_rays <= 0)
 		return invalid("rta_trace_cis_persistent_seeded: bad arguments");
 	rta_trace_config c;
-	const int rc = resolve_config(name, cfg, c);
+	int rc = resolve_config(name, cfg, c);
+	if (rc == RTA_OK) rc = require_full_stack(name, c);
 	if (rc != RTA_OK) return rc;
 	const uint32_t root_word = link_inner(root_link) ? 1u : (root_index << 1);
 	SeedArgs seed;
@@ -1356,15 +1416,16 @@ int rta_batch_trace_persistent(rta_batch b, const float *tri4, const rta_node_pa
 	if (any_hit)
 		return launch_persistent_cfg<true, true>(c, tri4, pairs, flags, 0u, ray_orig, ray_dir,
 		                                         max_t, (int)p.total_rays, nullptr, hits, err,
-		                                         queue, p.d_meshes, p.n_meshes, p.d_root_words, st);
+		                                         queue, p.d_meshes, p.n_meshes, p.d_root_words, st,
+		                                         p.d_ovf);
 	return launch_persistent_cfg<false, true>(c, tri4, pairs, flags, 0u, ray_orig, ray_dir, max_t,
 	                                          (int)p.total_rays, nullptr, hits, err, queue,
-	                                          p.d_meshes, p.n_meshes, p.d_root_words, st);
+	                                          p.d_meshes, p.n_meshes, p.d_root_words, st, p.d_ovf);
 }
 
 // Coordinate descent over the launch shape: starting from `*best` (or the
 // defaults), each round tries every candidate of one parameter at a time
-// (leaf_min, refill_min, steps, block) with the others fixed and keeps the
+// (leaf_min, refill_min, steps, block, stack) with the others fixed and keeps the
 // fastest, timing each candidate as the best of `reps` launches.  Results do
 // not depend on the shape, so the hits of the last launch are valid.
 int rta_batch_autotune(rta_batch b, const float *tri4, const rta_node_pair *pairs,
@@ -1408,14 +1469,16 @@ int rta_batch_autotune(rta_batch b, const float *tri4, const rta_node_pair *pair
 	static const int refills[] = {4, 8, 16, 24, 32};
 	static const int stepss[] = {8, 16, 24, 32, 64};
 	static const int blocks[] = {64, 128};
+	static const int stacks[] = {16, 24, kStack};
 	for (int round = 0; round < rounds && rc == RTA_OK; ++round) {
-		for (int param = 0; param < 4 && rc == RTA_OK; ++param) {
-			const int *vals = param == 0 ? leafs : param == 1 ? refills : param == 2 ? stepss : blocks;
-			const int nv = param == 3 ? 2 : 5;
+		for (int param = 0; param < 5 && rc == RTA_OK; ++param) {
+			const int *vals = param == 0 ? leafs : param == 1 ? refills : param == 2 ? stepss
+			                  : param == 3 ? blocks : stacks;
+			const int nv = param == 3 ? 2 : (param == 4 ? 3 : 5);
 			for (int v = 0; v < nv && rc == RTA_OK; ++v) {
 				rta_trace_config c = cur;
 				int &field = param == 0 ? c.leaf_min : param == 1 ? c.refill_min
-				             : param == 2 ? c.steps : c.block;
+				             : param == 2 ? c.steps : param == 3 ? c.block : c.stack;
 				if (field == vals[v]) continue;
 				field = vals[v];
 				float ms = 0;

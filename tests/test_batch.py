@@ -11,7 +11,7 @@ import pytest
 import oracle
 import rta_amd
 from rta_amd import scenes
-from rta_amd.abi import TRI_DTYPE
+from rta_amd.abi import HIT_DTYPE, TRI_DTYPE
 from rta_amd.batch import MeshBatch
 
 F32MAX = np.float32(np.finfo(np.float32).max)
@@ -131,6 +131,118 @@ def test_batch_rebuild_and_autotune(gpu):
         assert np.array_equal(bits(g["nodes"]), bits(first[m]["nodes"]))
         if g["hits"] is not None:
             assert np.array_equal(bits(g["hits"]), bits(first[m]["hits"]))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("stack", [16, 24])
+def test_short_stack_traversal_equals_full_stack(gpu, stack):
+    """Batch traversal with a 16- or 24-entry LDS stack: rays that would
+    push beyond it are traced again with 32 entries, so every record equals
+    the 32-entry traversal's bit for bit.  Two hand-made chain meshes (the
+    trees of test_stack_depth: 20 and 28 pushes) force such rays; the
+    overflow count equals the rays whose oracle traversal holds more than
+    `stack` entries, and the chain records equal the oracle's."""
+    import ctypes as C
+    from rta_amd.abi import TraceConfig, call, ptr
+    from test_stack_depth import chain_rays, chain_tree
+
+    meshes, rays = _edge_batch()
+    chains = [chain_tree(20), chain_tree(28)]
+    for j, (tris_c, _) in enumerate(chains):
+        meshes.append(tris_c)
+        rays.append(chain_rays(n_random=97 + j, seed=11 + j))
+    b = MeshBatch(meshes, rays)
+    b.build()
+    b.stream.synchronize()              # the uploads below run on another stream
+    s = ptr(b.stream)
+    # the chain meshes get their hand-made trees instead of LBVH ones
+    for j, (tris_c, nodes_c) in enumerate(chains):
+        m = len(meshes) - 2 + j
+        call("rta_hip_memcpy_htod", C.c_void_p(b.tris.ptr + 104 * int(b.tri_off[m])),
+             ptr(tris_c), C.c_size_t(tris_c.nbytes))
+        call("rta_hip_memcpy_htod", C.c_void_p(b.nodes.ptr + 32 * int(b.node_off[m])),
+             ptr(nodes_c), C.c_size_t(nodes_c.nbytes))
+    call("rta_batch_pack", b.handle, ptr(b.pairs), ptr(b.tri4), ptr(b.flags), ptr(b.nodes),
+         ptr(b.tris), s)
+    b.trace(config=TraceConfig(64, 16, 32, 0, 10, 32))
+    b.stream.synchronize()
+    b.check_errors()
+    assert b.overflow_count() == 0
+    full = b.hits.download(np.uint8, 16 * b.total_rays).copy()
+    b.trace(config=TraceConfig(64, 16, 32, 0, 10, stack))
+    b.stream.synchronize()
+    b.check_errors()
+    n_ovf = b.overflow_count()
+    assert np.array_equal(b.hits.download(np.uint8, 16 * b.total_rays), full)
+    expected = 0
+    for m in range(len(meshes)):
+        o, d, t = rays[m]
+        if len(t) == 0:
+            continue
+        if m >= len(meshes) - 2:
+            tris_c, nodes_c = chains[m - (len(meshes) - 2)]
+            ref, deep, depth = oracle.trace_kernel(tris_c, nodes_c, o, d, t, depth=True)
+            g = b.mesh_results(m)["hits"]
+            for f in ("t", "beta", "gamma", "ref"):
+                assert np.array_equal(bits(g[f]), bits(ref[f])), (m, f)
+        else:
+            exp = oracle.lbvh_build(meshes[m])
+            _, deep, depth = oracle.trace_kernel(exp["tris"], exp["nodes"], o, d, t, depth=True)
+        assert deep == 0
+        expected += int(np.count_nonzero(depth > stack))
+    say(f"stack {stack}: {n_ovf} of {b.total_rays} rays traced again")
+    assert n_ovf == expected and n_ovf >= (len(rays[-1][2]) if stack == 24 else
+                                            len(rays[-1][2]) + len(rays[-2][2]))
+
+
+@pytest.mark.gpu
+def test_batch_frame_graph_replays_equal_eager(gpu):
+    """Build + pack + trace of the whole batch captured as one graph (with a
+    short stack, so the overflow pass is captured too): every replay, with
+    eager launches and synchronisations between replays, gives the eager
+    frame's trees and hits bit for bit."""
+    from rta_amd.abi import TraceConfig
+    meshes, rays = _edge_batch()
+    b = MeshBatch(meshes, rays)
+    cfg = TraceConfig(128, 16, 32, 0, 8, 16)
+    b.build()
+    b.trace(config=cfg)
+    b.stream.synchronize()
+    b.check_errors()
+    eager_nodes = b.nodes.download(np.uint8, 32 * b.total_nodes).copy()
+    eager = b.hits.download(HIT_DTYPE, b.total_rays).copy()
+    hit = eager["t"] != F32MAX
+    g = b.capture(config=cfg)
+    for rep in range(3):
+        # eager work between replays: scribble over the outputs
+        b.hits.fill_u32(0x7f7f7f7f, stream=b.stream)
+        b.nodes.fill_u32(0, stream=b.stream)
+        b.stream.synchronize()
+        g.launch(b.stream)
+        b.stream.synchronize()
+        b.check_errors()
+        assert np.array_equal(b.nodes.download(np.uint8, 32 * b.total_nodes), eager_nodes), rep
+        got = b.hits.download(HIT_DTYPE, b.total_rays)
+        # the reset defines t, beta and gamma of a miss; ref only for a hit
+        for f in ("t", "beta", "gamma"):
+            assert np.array_equal(bits(got[f]), bits(eager[f])), (rep, f)
+        assert np.array_equal(got["ref"][hit], eager["ref"][hit]), rep
+    say(f"3 graph replays equal the eager frame ({int(hit.sum())} hits)")
+
+
+def test_short_stack_rejected_outside_batches():
+    """The single-mesh tracers hold no overflow list: stack 16 or 24 is an
+    argument error there, and 20 anywhere (checked before any device work)."""
+    import ctypes as C
+    from rta_amd.abi import RTA_E_INVALID, TraceConfig
+    lib = rta_amd.hip()
+    fake = C.c_void_p(64)
+    for stack in (16, 24, 20):
+        cfg = TraceConfig(64, 16, 32, 0, 10, stack)
+        rc = lib.rta_trace_cis_persistent(fake, fake, fake, 1, 0, fake, fake, fake, 10, None,
+                                          fake, fake, fake, C.byref(cfg), None)
+        assert rc == RTA_E_INVALID
+        assert b"stack" in lib.rta_hip_last_error()
 
 
 def test_batch_rejects_bad_sizes():
