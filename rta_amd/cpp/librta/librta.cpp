@@ -56,7 +56,7 @@ void register_plugin_search_path(const std::string &path) { search_paths().push_
 
 bool load_plugin_functions(const std::string &plugin_name) {
 	if (plugin_name.empty()) {
-		std::cerr << "no plugin name specified!" << std::endl;
+		std::cerr << "plugin loader: empty plugin name" << std::endl;
 		return false;
 	}
 	const bool absolute = plugin_name[0] == '/';
@@ -68,10 +68,10 @@ bool load_plugin_functions(const std::string &plugin_name) {
 	}
 	if (!handle) handle = dlopen(plugin_name.c_str(), RTLD_LAZY);
 	if (!handle) {
-		std::cerr << "cannot find plugin '" << plugin_name << "'" << std::endl;
-		std::cerr << "\ti looked in" << std::endl;
-		for (const std::string &p : search_paths()) std::cerr << "\t\t- " << p << std::endl;
-		std::cerr << "error message for the last attempt: " << dlerror() << std::endl;
+		std::cerr << "plugin loader: no library for plugin '" << plugin_name << "'; searched:"
+		          << std::endl;
+		for (const std::string &p : search_paths()) std::cerr << "  " << p << std::endl;
+		std::cerr << "last dlopen error: " << dlerror() << std::endl;
 		return false;
 	}
 	if (!resolve("description", plugin_description, handle)) return false;

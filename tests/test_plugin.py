@@ -56,7 +56,7 @@ def test_plugin_option_parsing():
 def test_host_reports_missing_plugin():
     r = run_driver("no-such-plugin", timeout=60)
     assert r.returncode != 0
-    assert "cannot find plugin" in r.stderr
+    assert "no library for plugin 'no-such-plugin'" in r.stderr
 
 
 def test_host_reports_plugin_missing_a_symbol(tmp_path):
