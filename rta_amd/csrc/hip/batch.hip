@@ -70,7 +70,8 @@ __global__ __launch_bounds__(256) void batch_karras_kernel(
 	const int base = (int)it.block * kKarrasBlock;
 	const CodeWindow w = stage_codes(window, codes + m.tri_off, (int)m.n, base);
 	const int i = base + (int)threadIdx.x;
-	if (i >= 2 * (int)m.n - 1) return;
+	// inner nodes only: the bottom-up pass writes the whole leaf nodes
+	if (i >= (int)m.n - 1) return;
 	karras_node(nodes + m.node_off, parents + m.node_off, visit + m.tri_off, ranges + m.tri_off, w,
 	            (int)m.n, i);
 }

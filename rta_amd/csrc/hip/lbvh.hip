@@ -37,15 +37,17 @@ __global__ __launch_bounds__(256) void morton_kernel(
 	if (code == 0u && status) atomicOr(status, (uint32_t)RTA_LBVH_CODE_ZERO);
 }
 
+// nodes 0 .. emit-1: emit = 2n-1 writes the leaves' link words too, emit =
+// n-1 leaves them to the bottom-up pass, which writes whole leaf nodes
 __global__ __launch_bounds__(kKarrasBlock) void karras_kernel(
     rta_node *__restrict__ nodes, uint32_t *__restrict__ parents,
     uint32_t *__restrict__ visit, uint2 *__restrict__ ranges,
-    const uint32_t *__restrict__ codes, int n) {
+    const uint32_t *__restrict__ codes, int n, int emit) {
 	__shared__ uint32_t window[kKarrasWindow];
 	const int base = blockIdx.x * kKarrasBlock;
 	const CodeWindow w = stage_codes(window, codes, n, base);
 	const int i = base + threadIdx.x;
-	if (i >= 2 * n - 1) return;
+	if (i >= emit) return;
 	karras_node(nodes, parents, visit, ranges, w, n, i);
 }
 
@@ -172,6 +174,22 @@ static LbvhWorkspace carve(void *base, int n) {
 
 using namespace rta_hip;
 
+// The hierarchy stage; with leaves false only the n-1 inner nodes, as the
+// bottom-up stage that follows writes the whole leaf nodes.
+static int build_tree(rta_node *nodes, uint32_t *parents, void *work, const uint32_t *codes, int n,
+                      bool leaves, rta_stream s) {
+	BoxesWork w = carve_boxes(work, n);
+	// the bottom-up stage ORs RTA_LBVH_FRONTIER_OVERFLOW into this word
+	const int rc = rta_hip_fill_u32(w.status, 0u, 1, s);
+	if (rc != RTA_OK) return rc;
+	const long long emit = leaves ? 2LL * n - 1 : (long long)n - 1;
+	if (emit == 0) return RTA_OK;   // one triangle: a single leaf, no inner node
+	hipLaunchKernelGGL(karras_kernel, dim3(grid_for(emit, kKarrasBlock)), dim3(kKarrasBlock), 0,
+	                   as_stream(s), nodes, parents, w.visit, w.ranges, codes, n, (int)emit);
+	RTA_LAUNCH_CHECK("karras_kernel");
+	return RTA_OK;
+}
+
 extern "C" {
 
 int rta_lbvh_init_triangles(rta_triangle *scratch, const rta_triangle *src, rta_aabb *boxes,
@@ -216,14 +234,7 @@ int rta_lbvh_build_tree(rta_node *nodes, uint32_t *parents, void *work, const ui
                         int n, rta_stream s) {
 	if (n <= 0) return invalid("rta_lbvh_build_tree: n must be > 0");
 	if (!nodes || !parents || !codes || !work) return invalid("rta_lbvh_build_tree: NULL pointer");
-	BoxesWork w = carve_boxes(work, n);
-	// the bottom-up stage ORs RTA_LBVH_FRONTIER_OVERFLOW into this word
-	const int rc = rta_hip_fill_u32(w.status, 0u, 1, s);
-	if (rc != RTA_OK) return rc;
-	hipLaunchKernelGGL(karras_kernel, dim3(grid_for(2LL * n - 1, kKarrasBlock)), dim3(kKarrasBlock), 0,
-	                   as_stream(s), nodes, parents, w.visit, w.ranges, codes, n);
-	RTA_LAUNCH_CHECK("karras_kernel");
-	return RTA_OK;
+	return build_tree(nodes, parents, work, codes, n, true, s);
 }
 
 }  // extern "C"
@@ -348,7 +359,7 @@ static int build_impl(const char *name, rta_triangle *tris, int n, rta_node *nod
 	mark(3);
 	if (rc == RTA_OK) rc = rta_lbvh_sort(w.codes, indices, n, w.sort_work, w.sort_bytes, s);
 	mark(4);
-	if (rc == RTA_OK) rc = rta_lbvh_build_tree(nodes, w.parents, w.boxes_work, w.codes, n, s);
+	if (rc == RTA_OK) rc = build_tree(nodes, w.parents, w.boxes_work, w.codes, n, false, s);
 	mark(5);
 	if (packed) {
 		permute();

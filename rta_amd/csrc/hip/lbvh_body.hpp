@@ -348,6 +348,15 @@ __device__ inline void clear_leaf_end(float4 *tri4, uint32_t pos) {
 	tri4[3 * (size_t)pos].w = 0.0f;
 }
 
+// one whole node: its link words and the box held in LDS column q of
+// b[6][kChunk], as two 16-byte stores
+__device__ inline void store_node(rta_node *node, uint2 links, const float (*b)[kChunk], int q) {
+	uint4 *o = reinterpret_cast<uint4 *>(node);
+	o[0] = make_uint4(links.x, links.y, __float_as_uint(b[0][q]), __float_as_uint(b[1][q]));
+	o[1] = make_uint4(__float_as_uint(b[2][q]), __float_as_uint(b[3][q]), __float_as_uint(b[4][q]),
+	                  __float_as_uint(b[5][q]));
+}
+
 __device__ inline uint32_t rep_code(const uint32_t *codes, uint32_t node, uint32_t leaf0) {
 	return node >= leaf0 ? codes[node - leaf0] : codes[node];
 }
@@ -404,11 +413,13 @@ __device__ inline void bottom_up_chunk(rta_node *__restrict__ nodes,
 		lcode[t] = codes[k];
 	}
 	const bool inner = k < leaf0 && k < end;
+	uint2 own_links = make_uint2(0u, 0u);   // node k's links when its range is in the chunk
 	if (inner) {
 		const uint2 rg = ranges[k];
 		uint32_t kids = ~0u;
 		if (rg.x >= base && rg.y < end) {
 			const uint2 pl = *reinterpret_cast<const uint2 *>(&nodes[k]);
+			own_links = pl;
 			const uint32_t l = pl.x >> 1, r = pl.y;
 			const uint32_t ll = l >= leaf0 ? (l - leaf0 - base) | kLocalLeaf : l - base;
 			const uint32_t rl = r >= leaf0 ? (r - leaf0 - base) | kLocalLeaf : r - base;
@@ -485,19 +496,15 @@ __device__ inline void bottom_up_chunk(rta_node *__restrict__ nodes,
 	}
 	__syncthreads();
 
-	// write-out: leaf boxes, completed inner nodes, frontier
-	if (k < (uint32_t)n) {
-		float *f = reinterpret_cast<float *>(&nodes[leaf0 + k]) + 2;
-		f[0] = lb_leaf[0][t]; f[1] = lb_leaf[1][t]; f[2] = lb_leaf[2][t];
-		f[3] = lb_leaf[3][t]; f[4] = lb_leaf[4][t]; f[5] = lb_leaf[5][t];
-	}
+	// write-out: whole 32-byte leaf nodes (links: one triangle, the first
+	// at position k) and completed inner nodes, then the frontier
+	if (k < (uint32_t)n) store_node(&nodes[leaf0 + k], make_uint2(1u << 1, k), lb_leaf, t);
 	bool slow = false;
 	if (k < leaf0 && lelems[t] != kStateNone) {
-		float *f = reinterpret_cast<float *>(&nodes[k]) + 2;
-		f[0] = lb_int[0][t]; f[1] = lb_int[1][t]; f[2] = lb_int[2][t];
-		f[3] = lb_int[3][t]; f[4] = lb_int[4][t]; f[5] = lb_int[5][t];
-		if (lelems[t] != kStateInner) {
-			*reinterpret_cast<uint2 *>(&nodes[k]) = make_uint2(lelems[t] << 1, ltris[t]);
+		const bool collapsed = lelems[t] != kStateInner;
+		store_node(&nodes[k], collapsed ? make_uint2(lelems[t] << 1, ltris[t]) : own_links, lb_int,
+		           t);
+		if (collapsed) {
 			// the collapse merged two leaves that meet at the node's split,
 			// the left child's last position: base + its local reference
 			// (a leaf's position, or an inner left child's index, which is
