@@ -68,7 +68,8 @@ typedef struct __attribute__((aligned(64))) rta_node_pair {
 } rta_node_pair;
 
 /* Per-stage build times in milliseconds (hipEvent based), in the stage
- * order of bbvh-cuda/lbvh.h:173-198. */
+ * order of bbvh-cuda/lbvh.h:173-198.  The hierarchy is emitted by the
+ * bottom-up pass, so `tree` is (close to) zero and `boxes` covers both. */
 typedef struct rta_lbvh_times {
 	float init_triangles, voxel_bounds, morton, sort, tree, boxes, permute;
 	float total;
@@ -202,14 +203,16 @@ int rta_lbvh_sort(uint32_t *codes, uint32_t *indices, int n,
                   void *work, size_t work_bytes, rta_stream s);
 
 /* Scratch shared by the two calls below (visit counters, node ranges,
- * frontier lists): rta_lbvh_boxes_work_bytes(n) bytes.  The reference
- * allocates its counters inside the timed call and never frees them
- * (lbvh-kernels.cu:451-453); here the caller owns them. */
+ * first-arrival records, frontier lists): rta_lbvh_boxes_work_bytes(n)
+ * bytes.  The reference allocates its counters inside the timed call and
+ * never frees them (lbvh-kernels.cu:451-453); here the caller owns them. */
 int rta_lbvh_boxes_work_bytes(int n, size_t *bytes);
 
 /* buildTreeKarras + k::build_tree_karras, lbvh-kernels.cu:186-256,424-446:
- * inner nodes 0..n-2, leaves n-1..2n-2; writes links and parents[child];
- * also initialises `work` for rta_lbvh_build_boxes. */
+ * inner nodes 0..n-2, leaves n-1..2n-2; writes links and parents[child]
+ * (and the inner nodes' ranges into `work`).  rta_lbvh_build and
+ * rta_lbvh_build_packed do not call it: their bottom-up pass emits the
+ * same links itself. */
 int rta_lbvh_build_tree(rta_node *nodes, uint32_t *parents, void *work,
                         const uint32_t *codes, int n, rta_stream s);
 
@@ -217,7 +220,10 @@ int rta_lbvh_build_tree(rta_node *nodes, uint32_t *parents, void *work,
  * lbvh-kernels.cu:258-341,448-478: bottom-up box union with a visit
  * counter per inner node; a parent whose two children are leaves with
  * equal representative codes collapses into one multi-triangle leaf.
- * `work` must come from rta_lbvh_build_tree of the same codes. */
+ * The climb derives the hierarchy from the sorted codes itself (the radix
+ * tree over (code, position) is unique) and writes every node whole, with
+ * the same links rta_lbvh_build_tree writes, so `parents` is not read;
+ * `work`: rta_lbvh_boxes_work_bytes(n) bytes. */
 int rta_lbvh_build_boxes(rta_node *nodes, void *work,
                          const rta_aabb *boxes, const uint32_t *codes,
                          const uint32_t *indices, const uint32_t *parents,

@@ -60,32 +60,16 @@ __global__ __launch_bounds__(256) void batch_local_index_kernel(uint32_t *__rest
 	if (i < n) idx[i] &= 0xffffffu;
 }
 
-__global__ __launch_bounds__(256) void batch_karras_kernel(
-    const BatchItem *__restrict__ items, const BatchMesh *__restrict__ meshes,
-    rta_node *__restrict__ nodes, uint32_t *__restrict__ parents, uint32_t *__restrict__ visit,
-    uint2 *__restrict__ ranges, const uint32_t *__restrict__ codes) {
-	__shared__ uint32_t window[kKarrasWindow];
-	const BatchItem it = items[blockIdx.x];
-	const BatchMesh m = meshes[it.mesh];
-	const int base = (int)it.block * kKarrasBlock;
-	const CodeWindow w = stage_codes(window, codes + m.tri_off, (int)m.n, base);
-	const int i = base + (int)threadIdx.x;
-	// inner nodes only: the bottom-up pass writes the whole leaf nodes
-	if (i >= (int)m.n - 1) return;
-	karras_node(nodes + m.node_off, parents + m.node_off, visit + m.tri_off, ranges + m.tri_off, w,
-	            (int)m.n, i);
-}
-
 __global__ __launch_bounds__(kChunk) void batch_chunk_kernel(
     const BatchItem *__restrict__ items, const BatchMesh *__restrict__ meshes,
     rta_node *__restrict__ nodes, const rta_aabb *__restrict__ boxes,
     const uint32_t *__restrict__ codes, const uint32_t *__restrict__ indices,
-    const uint32_t *__restrict__ parents, const uint2 *__restrict__ ranges,
-    uint32_t *__restrict__ slots, uint32_t *__restrict__ counts, uint32_t *__restrict__ status) {
+    uint2 *__restrict__ ranges, uint32_t *__restrict__ visit, uint4 *__restrict__ slots,
+    uint32_t *__restrict__ counts, uint32_t *__restrict__ status) {
 	const BatchItem it = items[blockIdx.x];
 	const BatchMesh m = meshes[it.mesh];
 	bottom_up_chunk(nodes + m.node_off, boxes + m.tri_off, codes + m.tri_off, indices + m.tri_off,
-	                parents + m.node_off, ranges + m.tri_off,
+	                ranges + m.tri_off, visit + m.tri_off,
 	                slots + (size_t)(m.chunk_off + it.block) * kSlots, counts + m.chunk_off + it.block,
 	                status, (int)m.n, (int)it.block);
 }
@@ -94,10 +78,10 @@ __global__ __launch_bounds__(kChunk) void batch_chunk_kernel(
 // kChunk * kMergeFan^level leaves.
 __global__ __launch_bounds__(1024) void batch_merge_kernel(
     const BatchItem *__restrict__ items, const BatchMesh *__restrict__ meshes,
-    rta_node *__restrict__ nodes, uint32_t *__restrict__ visit, const uint32_t *__restrict__ codes,
-    const uint32_t *__restrict__ parents, const uint2 *__restrict__ ranges,
-    const uint32_t *__restrict__ in_slots, const uint32_t *__restrict__ in_counts,
-    uint32_t *__restrict__ out_slots, uint32_t *__restrict__ out_counts,
+    rta_node *__restrict__ nodes, uint32_t *__restrict__ visit, uint2 *__restrict__ stash,
+    const uint32_t *__restrict__ codes, uint2 *__restrict__ ranges,
+    const uint4 *__restrict__ in_slots, const uint32_t *__restrict__ in_counts,
+    uint4 *__restrict__ out_slots, uint32_t *__restrict__ out_counts,
     uint32_t *__restrict__ status, long long width) {
 	const BatchItem it = items[blockIdx.x];
 	const BatchMesh m = meshes[it.mesh];
@@ -106,7 +90,7 @@ __global__ __launch_bounds__(1024) void batch_merge_kernel(
 	const uint32_t lo = (uint32_t)(it.block * width);
 	const uint32_t hi = (uint32_t)min((long long)m.n, (long long)(it.block + 1) * width);
 	const bool last = it.blocks == 1;
-	bottom_up_merge(nodes + m.node_off, visit + m.tri_off, codes + m.tri_off, parents + m.node_off,
+	bottom_up_merge(nodes + m.node_off, visit + m.tri_off, stash + m.tri_off, codes + m.tri_off,
 	                ranges + m.tri_off, in_slots + (size_t)(m.chunk_off + g0) * kSlots,
 	                in_counts + m.chunk_off + g0, ng,
 	                last ? nullptr : out_slots + (size_t)(m.chunk_off + it.block) * kSlots,
@@ -130,9 +114,10 @@ struct BatchWork {
 	rta_triangle *scratch;
 	rta_aabb *boxes, *partials, *bounds;
 	rta_float3 *centers;
-	uint32_t *codes, *parents, *visit;
-	uint2 *ranges;
-	uint32_t *slots[2], *counts[2];
+	uint32_t *codes, *visit;
+	uint2 *ranges, *stash;
+	uint4 *slots[2];
+	uint32_t *counts[2];
 	uint32_t *bu_status;
 	void *sort_work;
 	size_t sort_bytes, total;
@@ -154,11 +139,11 @@ static BatchWork carve_batch(void *base, const BatchPlan &p) {
 	w.partials = (rta_aabb *)take((size_t)p.total_parts * sizeof(rta_aabb));
 	w.bounds = (rta_aabb *)take((size_t)p.n_meshes * sizeof(rta_aabb));
 	w.codes = (uint32_t *)take(N * 4);
-	w.parents = (uint32_t *)take((size_t)p.total_nodes * 4);
 	w.visit = (uint32_t *)take(N * 4);
 	w.ranges = (uint2 *)take(N * 8);
+	w.stash = (uint2 *)take(N * 8);
 	for (int i = 0; i < 2; ++i) {
-		w.slots[i] = (uint32_t *)take((size_t)p.total_chunks * kSlots * 4);
+		w.slots[i] = (uint4 *)take((size_t)p.total_chunks * kSlots * 16);
 		w.counts[i] = (uint32_t *)take((size_t)p.total_chunks * 4);
 	}
 	w.bu_status = (uint32_t *)take(4);
@@ -347,20 +332,17 @@ int rta_batch_build(rta_batch b, rta_triangle *tris, rta_node *nodes, uint32_t *
 	hipLaunchKernelGGL(batch_local_index_kernel, dim3(grid_for(N, 256)), dim3(256), 0, st,
 	                   indices, (long long)N);
 	RTA_LAUNCH_CHECK("batch_local_index_kernel");
-	hipLaunchKernelGGL(batch_karras_kernel, dim3((unsigned)p.node_blocks), dim3(kKarrasBlock), 0, st,
-	                   items + p.node_blocks_off, p.d_meshes, nodes, w.parents, w.visit, w.ranges,
-	                   w.codes);
-	RTA_LAUNCH_CHECK("batch_karras_kernel");
+	// the bottom-up pass emits the hierarchy too (lbvh_body.hpp)
 	hipLaunchKernelGGL(batch_chunk_kernel, dim3((unsigned)p.chunk_items), dim3(kChunk), 0, st,
 	                   items + p.chunk_items_off, p.d_meshes, nodes, w.boxes, w.codes, indices,
-	                   w.parents, w.ranges, w.slots[0], w.counts[0], bu_status);
+	                   w.ranges, w.visit, w.slots[0], w.counts[0], bu_status);
 	RTA_LAUNCH_CHECK("batch_chunk_kernel");
 	long long width = kChunk;
 	for (size_t L = 0; L < p.merge_off.size(); ++L) {
 		width *= kMergeFan;
 		const int in = (int)(L & 1), outb = 1 - in;
 		hipLaunchKernelGGL(batch_merge_kernel, dim3((unsigned)p.merge_len[L]), dim3(1024), 0, st,
-		                   items + p.merge_off[L], p.d_meshes, nodes, w.visit, w.codes, w.parents,
+		                   items + p.merge_off[L], p.d_meshes, nodes, w.visit, w.stash, w.codes,
 		                   w.ranges, w.slots[in], w.counts[in], w.slots[outb], w.counts[outb],
 		                   bu_status, width);
 		RTA_LAUNCH_CHECK("batch_merge_kernel");

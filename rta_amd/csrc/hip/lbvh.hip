@@ -37,27 +37,24 @@ __global__ __launch_bounds__(256) void morton_kernel(
 	if (code == 0u && status) atomicOr(status, (uint32_t)RTA_LBVH_CODE_ZERO);
 }
 
-// nodes 0 .. emit-1: emit = 2n-1 writes the leaves' link words too, emit =
-// n-1 leaves them to the bottom-up pass, which writes whole leaf nodes
 __global__ __launch_bounds__(kKarrasBlock) void karras_kernel(
     rta_node *__restrict__ nodes, uint32_t *__restrict__ parents,
     uint32_t *__restrict__ visit, uint2 *__restrict__ ranges,
-    const uint32_t *__restrict__ codes, int n, int emit) {
+    const uint32_t *__restrict__ codes, int n) {
 	__shared__ uint32_t window[kKarrasWindow];
 	const int base = blockIdx.x * kKarrasBlock;
 	const CodeWindow w = stage_codes(window, codes, n, base);
 	const int i = base + threadIdx.x;
-	if (i >= emit) return;
+	if (i >= 2 * n - 1) return;
 	karras_node(nodes, parents, visit, ranges, w, n, i);
 }
 
 __global__ __launch_bounds__(kChunk) void bottom_up_chunk_kernel(
     rta_node *__restrict__ nodes, const rta_aabb *__restrict__ boxes,
     const uint32_t *__restrict__ codes, const uint32_t *__restrict__ indices,
-    const uint32_t *__restrict__ parents, const uint2 *__restrict__ ranges,
-    uint32_t *__restrict__ slots, uint32_t *__restrict__ counts, uint32_t *__restrict__ status,
-    int n, PackOut pk) {
-	bottom_up_chunk(nodes, boxes, codes, indices, parents, ranges,
+    uint2 *__restrict__ ranges, uint32_t *__restrict__ visit, uint4 *__restrict__ slots,
+    uint32_t *__restrict__ counts, uint32_t *__restrict__ status, int n, PackOut pk) {
+	bottom_up_chunk(nodes, boxes, codes, indices, ranges, visit,
 	                slots + (size_t)blockIdx.x * kSlots, counts + blockIdx.x, status, n,
 	                (int)blockIdx.x, pk);
 }
@@ -65,33 +62,26 @@ __global__ __launch_bounds__(kChunk) void bottom_up_chunk_kernel(
 // One block merges the frontier slots of up to kMergeFan chunks of the
 // previous level; its own chunk is `chunk` leaves wide.
 __global__ __launch_bounds__(1024) void bottom_up_merge_kernel(
-    rta_node *__restrict__ nodes, uint32_t *__restrict__ visit,
-    const uint32_t *__restrict__ codes, const uint32_t *__restrict__ parents,
-    const uint2 *__restrict__ ranges, const uint32_t *__restrict__ in_slots,
-    const uint32_t *__restrict__ in_counts, int n_in, uint32_t *__restrict__ out_slots,
-    uint32_t *__restrict__ out_counts, uint32_t *__restrict__ status, long long chunk, int n,
-    PackOut pk) {
+    rta_node *__restrict__ nodes, uint32_t *__restrict__ visit, uint2 *__restrict__ stash,
+    const uint32_t *__restrict__ codes, uint2 *__restrict__ ranges,
+    const uint4 *__restrict__ in_slots, const uint32_t *__restrict__ in_counts, int n_in,
+    uint4 *__restrict__ out_slots, uint32_t *__restrict__ out_counts,
+    uint32_t *__restrict__ status, long long chunk, int n, PackOut pk) {
 	const int g0 = blockIdx.x * kMergeFan;
 	const int ng = min(kMergeFan, n_in - g0);
 	const uint32_t lo = (uint32_t)(blockIdx.x * chunk);
 	const uint32_t hi = (uint32_t)min((long long)n, (long long)(blockIdx.x + 1) * chunk);
-	bottom_up_merge(nodes, visit, codes, parents, ranges, in_slots + (size_t)g0 * kSlots,
+	bottom_up_merge(nodes, visit, stash, codes, ranges, in_slots + (size_t)g0 * kSlots,
 	                in_counts + g0, ng,
 	                out_slots ? out_slots + (size_t)blockIdx.x * kSlots : nullptr,
 	                out_counts ? out_counts + blockIdx.x : nullptr, status, lo, hi, n, pk);
 }
 
-__global__ __launch_bounds__(256) void pack_boundary_pairs_kernel(
-    rta_node_pair *__restrict__ pairs, uint32_t *__restrict__ flags,
-    const rta_node *__restrict__ nodes, const uint2 *__restrict__ ranges, int n_inner) {
-	const int x = blockIdx.x * blockDim.x + threadIdx.x;
-	if (x < n_inner) pack_boundary_pair(pairs, flags, nodes, ranges, (uint32_t)x);
-}
-
 struct BoxesWork {
-	uint32_t *visit;
-	uint2 *ranges;
-	uint32_t *slots[2];
+	uint32_t *visit;     // per split
+	uint2 *ranges;       // per inner node
+	uint2 *stash;        // per node index: a first arrival's far end and delta
+	uint4 *slots[2];     // frontier entries
 	uint32_t *counts[2];
 	uint32_t *status;
 	size_t total;
@@ -110,8 +100,9 @@ static BoxesWork carve_boxes(void *base, int n) {
 	const size_t chunks = (nb + kChunk - 1) / kChunk;
 	w.visit = (uint32_t *)take(nb * 4);
 	w.ranges = (uint2 *)take(nb * 8);
+	w.stash = (uint2 *)take(nb * 8);
 	for (int i = 0; i < 2; ++i) {
-		w.slots[i] = (uint32_t *)take(chunks * kSlots * 4);
+		w.slots[i] = (uint4 *)take(chunks * kSlots * 16);
 		w.counts[i] = (uint32_t *)take(chunks * 4);
 	}
 	w.status = (uint32_t *)take(4);
@@ -136,7 +127,6 @@ struct LbvhWorkspace {
 	rta_aabb *partials;
 	rta_aabb *bounds;
 	uint32_t *codes;
-	uint32_t *parents;
 	void *boxes_work;
 	void *sort_work;
 	size_t sort_bytes;
@@ -160,7 +150,6 @@ static LbvhWorkspace carve(void *base, int n) {
 	w.partials = (rta_aabb *)take(nparts * sizeof(rta_aabb));
 	w.bounds = (rta_aabb *)take(sizeof(rta_aabb));
 	w.codes = (uint32_t *)take(nb * 4);
-	w.parents = (uint32_t *)take((2 * nb - 1) * 4);
 	w.boxes_work = take(carve_boxes(nullptr, n).total);
 	size_t sb = 0;
 	rta_lbvh_sort_work_bytes(n, &sb);
@@ -173,22 +162,6 @@ static LbvhWorkspace carve(void *base, int n) {
 }  // namespace rta_hip
 
 using namespace rta_hip;
-
-// The hierarchy stage; with leaves false only the n-1 inner nodes, as the
-// bottom-up stage that follows writes the whole leaf nodes.
-static int build_tree(rta_node *nodes, uint32_t *parents, void *work, const uint32_t *codes, int n,
-                      bool leaves, rta_stream s) {
-	BoxesWork w = carve_boxes(work, n);
-	// the bottom-up stage ORs RTA_LBVH_FRONTIER_OVERFLOW into this word
-	const int rc = rta_hip_fill_u32(w.status, 0u, 1, s);
-	if (rc != RTA_OK) return rc;
-	const long long emit = leaves ? 2LL * n - 1 : (long long)n - 1;
-	if (emit == 0) return RTA_OK;   // one triangle: a single leaf, no inner node
-	hipLaunchKernelGGL(karras_kernel, dim3(grid_for(emit, kKarrasBlock)), dim3(kKarrasBlock), 0,
-	                   as_stream(s), nodes, parents, w.visit, w.ranges, codes, n, (int)emit);
-	RTA_LAUNCH_CHECK("karras_kernel");
-	return RTA_OK;
-}
 
 extern "C" {
 
@@ -234,22 +207,35 @@ int rta_lbvh_build_tree(rta_node *nodes, uint32_t *parents, void *work, const ui
                         int n, rta_stream s) {
 	if (n <= 0) return invalid("rta_lbvh_build_tree: n must be > 0");
 	if (!nodes || !parents || !codes || !work) return invalid("rta_lbvh_build_tree: NULL pointer");
-	return build_tree(nodes, parents, work, codes, n, true, s);
+	BoxesWork w = carve_boxes(work, n);
+	// the bottom-up stage ORs RTA_LBVH_FRONTIER_OVERFLOW into this word
+	const int rc = rta_hip_fill_u32(w.status, 0u, 1, s);
+	if (rc != RTA_OK) return rc;
+	hipLaunchKernelGGL(karras_kernel, dim3(grid_for(2LL * n - 1, kKarrasBlock)), dim3(kKarrasBlock), 0,
+	                   as_stream(s), nodes, parents, w.visit, w.ranges, codes, n);
+	RTA_LAUNCH_CHECK("karras_kernel");
+	return RTA_OK;
 }
 
 }  // extern "C"
 
-// The bottom-up stage; a frontier overflow sets RTA_LBVH_FRONTIER_OVERFLOW
-// in *status.
+// The bottom-up stage, which also emits the hierarchy (every node whole,
+// and the inner nodes' ranges); a frontier overflow sets
+// RTA_LBVH_FRONTIER_OVERFLOW in *status (the workspace's own word when
+// status is null).
 static int build_boxes(rta_node *nodes, void *work, const rta_aabb *boxes, const uint32_t *codes,
-                       const uint32_t *indices, const uint32_t *parents, int n, uint32_t *status,
-                       rta_stream s, PackOut pk = PackOut{}) {
+                       const uint32_t *indices, int n, uint32_t *status, rta_stream s,
+                       PackOut pk = PackOut{}) {
 	BoxesWork w = carve_boxes(work, n);
-	if (!status) status = w.status;
+	if (!status) {
+		status = w.status;
+		const int rc = rta_hip_fill_u32(status, 0u, 1, s);
+		if (rc != RTA_OK) return rc;
+	}
 	hipStream_t st = as_stream(s);
 	int groups = (int)grid_for(n, kChunk);
 	hipLaunchKernelGGL(bottom_up_chunk_kernel, dim3(groups), dim3(kChunk), 0, st, nodes, boxes,
-	                   codes, indices, parents, w.ranges, w.slots[0], w.counts[0], status, n, pk);
+	                   codes, indices, w.ranges, w.visit, w.slots[0], w.counts[0], status, n, pk);
 	RTA_LAUNCH_CHECK("bottom_up_chunk_kernel");
 	long long chunk = kChunk;
 	int in = 0;
@@ -257,17 +243,12 @@ static int build_boxes(rta_node *nodes, void *work, const rta_aabb *boxes, const
 		const int blocks = (groups + kMergeFan - 1) / kMergeFan;
 		chunk *= kMergeFan;
 		hipLaunchKernelGGL(bottom_up_merge_kernel, dim3(blocks), dim3(1024), 0, st, nodes,
-		                   w.visit, codes, parents, w.ranges, w.slots[in], w.counts[in], groups,
+		                   w.visit, w.stash, codes, w.ranges, w.slots[in], w.counts[in], groups,
 		                   blocks > 1 ? w.slots[1 - in] : nullptr,
 		                   blocks > 1 ? w.counts[1 - in] : nullptr, status, chunk, n, pk);
 		RTA_LAUNCH_CHECK("bottom_up_merge_kernel");
 		groups = blocks;
 		in = 1 - in;
-	}
-	if (pk.pairs && n > kChunk) {
-		hipLaunchKernelGGL(pack_boundary_pairs_kernel, dim3(grid_for(n - 1, 256)), dim3(256), 0, st,
-		                   pk.pairs, pk.flags, nodes, w.ranges, n - 1);
-		RTA_LAUNCH_CHECK("pack_boundary_pairs_kernel");
 	}
 	return RTA_OK;
 }
@@ -278,9 +259,10 @@ int rta_lbvh_build_boxes(rta_node *nodes, void *work, const rta_aabb *boxes,
                          const uint32_t *codes, const uint32_t *indices,
                          const uint32_t *parents, int n, rta_stream s) {
 	if (n <= 0) return invalid("rta_lbvh_build_boxes: n must be > 0");
-	if (!nodes || !boxes || !codes || !indices || !parents || !work)
+	(void)parents;   // the climb derives the hierarchy from the codes
+	if (!nodes || !boxes || !codes || !indices || !work)
 		return invalid("rta_lbvh_build_boxes: NULL pointer");
-	return build_boxes(nodes, work, boxes, codes, indices, parents, n, nullptr, s);
+	return build_boxes(nodes, work, boxes, codes, indices, n, nullptr, s);
 }
 
 int rta_lbvh_permute(rta_triangle *out, const rta_triangle *scratch, const uint32_t *indices,
@@ -359,19 +341,17 @@ static int build_impl(const char *name, rta_triangle *tris, int n, rta_node *nod
 	mark(3);
 	if (rc == RTA_OK) rc = rta_lbvh_sort(w.codes, indices, n, w.sort_work, w.sort_bytes, s);
 	mark(4);
-	if (rc == RTA_OK) rc = build_tree(nodes, w.parents, w.boxes_work, w.codes, n, false, s);
+	// the hierarchy is emitted by the bottom-up stage: no pass of its own
 	mark(5);
 	if (packed) {
 		permute();
 		mark(6);
 		if (rc == RTA_OK)
-			rc = build_boxes(nodes, w.boxes_work, w.boxes, w.codes, indices, w.parents, n, status,
-			                 s, pk);
+			rc = build_boxes(nodes, w.boxes_work, w.boxes, w.codes, indices, n, status, s, pk);
 		mark(7);
 	} else {
 		if (rc == RTA_OK)
-			rc = build_boxes(nodes, w.boxes_work, w.boxes, w.codes, indices, w.parents, n, status,
-			                 s);
+			rc = build_boxes(nodes, w.boxes_work, w.boxes, w.codes, indices, n, status, s);
 		mark(6);
 		permute();
 		mark(7);

@@ -15,8 +15,7 @@ constexpr int kBoundsBlock = 1024;
 constexpr int kChunk = 1024;
 constexpr int kSlots = 128;
 constexpr int kMergeFan = 64;
-constexpr uint32_t kStateNone = 0u;           // lelems: node not completed
-constexpr uint32_t kStateInner = 0xffffffffu; // lelems: completed inner node
+constexpr uint32_t kStateInner = 0xffffffffu; // leaf count of an inner node in the climb
 
 // ---------------------------------------------------------- wave helpers
 
@@ -266,31 +265,43 @@ __device__ inline void karras_node(rta_node *__restrict__ nodes, uint32_t *__res
 
 // ----------------------------------------- stage 6: bottom-up boxes
 
-// Bottom-up boxes (k::build_bounding_boxes_karras2, lbvh-kernels.cu:258-341):
-// every leaf climbs towards the root; at each inner node the first arriving
-// child stops and the second forms the union and continues.  Deterministic
-// form of the reference:
-//  * a separate visit counter decides first/second arrival (the reference
-//    sums representative codes into the flag, so a first arrival with code
-//    0 also stops the second thread, lbvh-kernels.cu:288-291);
-//  * the union is MIN(left, right) / MAX(left, right) with the reference's
-//    `(a<b)?a:b`, fixing the sign of zero that the reference leaves to
-//    arrival order;
-//  * merge rule: both children are leaves and their representative codes
-//    (code of the leaf, or codes[i] for inner node i, :261) are equal
-//    (`code == old` in the reference) -> the parent becomes one leaf with
-//    tris = min, elems = sum (lbvh-kernels.cu:302-330).
+// Bottom-up boxes (k::build_bounding_boxes_karras2, lbvh-kernels.cu:258-341),
+// with the hierarchy of k::build_tree_karras (:186-256) emitted by the same
+// climb.  The hierarchy is the binary radix tree of the augmented keys
+// (code, position), which is unique, so it can be grown from the leaves: a
+// completed node covering sorted positions [l, r] has its parent on the
+// side of the longer common prefix with its neighbour -- split r (the node
+// is the left child) when delta(r) > delta(l-1), else split l-1 (the right
+// child), with delta(i) the common prefix length of positions i and i+1
+// (cpl above) and delta(-1) = delta(n-1) = -1; two neighbours never tie.
+// Karras numbering follows: the children of split s are nodes s and s+1
+// (leaves when their range is one position), so a node's index is r as a
+// left child, l as a right child, and 0 for the root.
+//
+// At each split the first arriving child stops and the second forms the
+// parent: the union MIN(left, right) / MAX(left, right) with the
+// reference's `(a<b)?a:b`, the whole 32-byte node, its range and (packed
+// build) its pair record, and climbs on.  Merge rule: both children are
+// leaves (original or merged) and their representative codes are equal
+// (codes[s] == codes[s+1], i.e. delta(s) >= 32; `code == old` in the
+// reference) -> the parent becomes one leaf with tris = min, elems = sum
+// (lbvh-kernels.cu:302-330).  Deterministic form of the reference: a visit
+// counter per split decides first/second arrival (the reference sums
+// representative codes into the flag, so a first arrival with code 0 also
+// stops the second thread, :288-291), and the union order is fixed (the
+// reference leaves the sign of a zero to arrival order).
 //
 // Hierarchical schedule.  A subtree covers a contiguous range of sorted
 // positions, so all communication inside a chunk of consecutive leaves
 // stays in one workgroup:
-//  level 1: a 1024-thread block owns 1024 leaves; counters, boxes and leaf
-//           state live in LDS; a climb stops when the parent's range leaves
-//           the chunk and the child is recorded in the block's frontier
-//           slots (at most 2 per tree level, so <= 126 for depth <= 63);
+//  level 1: a 1024-thread block owns 1024 leaves; the deltas, the visit
+//           counters and the waiting children live in LDS; a climb stops
+//           when its parent's split lies outside the chunk, and the
+//           completed nodes whose parents stay incomplete form the chunk's
+//           frontier (at most 2 per tree level, so <= 126 for depth <= 63);
 //  level k: one block merges the frontiers of kMergeFan level-(k-1) chunks
-//           with workgroup-scope atomics on the global visit counters,
-//           until one block covers all leaves and reaches the root.
+//           with workgroup-scope atomics on global visit counters, until
+//           one block covers all leaves and reaches the root.
 // No step needs device-scope cache maintenance; results do not depend on
 // the schedule.
 
@@ -342,233 +353,275 @@ __device__ inline bool write_pair(rta_node_pair *pairs, uint32_t x, const rta_aa
 	return box_fast(lb) && box_fast(rb);
 }
 
-// a collapse merges two leaves: the left one's last triangle no longer ends
-// a leaf (the permute wrote every triangle as a leaf end)
+// a merge joins two leaves: the left one's last triangle no longer ends a
+// leaf (the permute wrote every triangle as a leaf end)
 __device__ inline void clear_leaf_end(float4 *tri4, uint32_t pos) {
 	tri4[3 * (size_t)pos].w = 0.0f;
 }
 
-// one whole node: its link words and the box held in LDS column q of
-// b[6][kChunk], as two 16-byte stores
-__device__ inline void store_node(rta_node *node, uint2 links, const float (*b)[kChunk], int q) {
+// delta(i): common prefix length of the augmented keys at sorted positions
+// i and i+1, -1 outside [0, n-1)
+__device__ inline int delta_at(const uint32_t *__restrict__ codes, long long i, int n) {
+	if (i < 0 || i >= (long long)n - 1) return -1;
+	const uint32_t a = codes[i], b = codes[i + 1];
+	if (a == b) return __clz((int)((uint32_t)i ^ (uint32_t)(i + 1))) + 32;
+	return __clz((int)(a ^ b));
+}
+
+// A completed node during the climb.
+struct ClimbNode {
+	uint32_t l, r;     // sorted range
+	int dl, dr;        // delta(l - 1), delta(r)
+	rta_aabb box;
+	uint32_t elems;    // kStateInner for an inner node, else the leaf's triangle count
+	uint32_t tris;     // the leaf's first triangle
+};
+
+// its parent's split, and its own index there (r as a left child, l as a
+// right child); the root (dl = dr = -1) has neither
+__device__ inline bool is_left_child(const ClimbNode &c) { return c.dr > c.dl; }
+__device__ inline uint32_t node_index(const ClimbNode &c, uint32_t leaf0) {
+	if (c.l == c.r) return leaf0 + c.l;
+	return is_left_child(c) ? c.r : c.l;
+}
+
+// a frontier entry: index, range, deltas (+1, a byte each)
+__device__ inline uint4 frontier_entry(const ClimbNode &c, uint32_t leaf0) {
+	return make_uint4(node_index(c, leaf0), c.l, c.r,
+	                  (uint32_t)(c.dl + 1) | ((uint32_t)(c.dr + 1) << 8));
+}
+
+// the whole node: link words and box, as two 16-byte stores
+__device__ inline void store_node(rta_node *node, uint2 links, const rta_aabb &b) {
 	uint4 *o = reinterpret_cast<uint4 *>(node);
-	o[0] = make_uint4(links.x, links.y, __float_as_uint(b[0][q]), __float_as_uint(b[1][q]));
-	o[1] = make_uint4(__float_as_uint(b[2][q]), __float_as_uint(b[3][q]), __float_as_uint(b[4][q]),
-	                  __float_as_uint(b[5][q]));
+	o[0] = make_uint4(links.x, links.y, __float_as_uint(b.min.x), __float_as_uint(b.min.y));
+	o[1] = make_uint4(__float_as_uint(b.min.z), __float_as_uint(b.max.x), __float_as_uint(b.max.y),
+	                  __float_as_uint(b.max.z));
 }
 
-__device__ inline uint32_t rep_code(const uint32_t *codes, uint32_t node, uint32_t leaf0) {
-	return node >= leaf0 ? codes[node - leaf0] : codes[node];
+// leaf state of a finished node from its link words
+__device__ inline void node_state(uint2 links, uint32_t &elems, uint32_t &tris) {
+	elems = link_inner(links.x) ? kStateInner : links.x >> 1;
+	tris = links.y;
 }
 
-// Level 1 for leaves [chunk*kChunk, +kChunk) of a mesh; the frontier goes
-// to slots[0..kSlots) and its size to *count.
-//
-// An inner node whose range lies inside the chunk has its index inside it
-// too (Karras: a node's index is one end of its range), and so do its
-// children.  The block therefore first loads, coalesced, everything the
-// climbs read: representative codes, each such node's children as 16-bit
-// local references (kLocalLeaf marks a leaf), and each node's parent as a
-// local index, kParentExit when the parent's range leaves the chunk (the
-// climb stops and records a frontier entry) or kParentRoot for the root.
-// The climbs then touch LDS only.
-constexpr uint16_t kLocalLeaf = 0x8000u;
-constexpr uint16_t kParentExit = 0xffffu;
-constexpr uint16_t kParentRoot = 0xfffeu;
+// The parent of lc = [l, s] and rc = [s+1, r]: its whole node, range and,
+// after a merge, the cleared leaf-end mark; returns it, its index in x.
+__device__ inline ClimbNode form_parent(const ClimbNode &lc, const ClimbNode &rc, uint32_t s,
+                                        uint32_t leaf0, rta_node *__restrict__ nodes,
+                                        uint2 *__restrict__ ranges, const PackOut &pk,
+                                        uint32_t &x) {
+	ClimbNode p;
+	p.l = lc.l;
+	p.r = rc.r;
+	p.dl = lc.dl;
+	p.dr = rc.dr;
+	p.box = union_lr(lc.box, rc.box);
+	const uint32_t lid = lc.l == lc.r ? leaf0 + s : s;
+	const uint32_t rid = rc.l == rc.r ? leaf0 + s + 1u : s + 1u;
+	const bool merge = lc.dr >= 32 && lc.elems != kStateInner && rc.elems != kStateInner;
+	p.elems = merge ? lc.elems + rc.elems : kStateInner;
+	p.tris = merge ? (lc.tris < rc.tris ? lc.tris : rc.tris) : 0u;
+	x = node_index(p, leaf0);
+	store_node(&nodes[x], merge ? make_uint2(p.elems << 1, p.tris) : make_uint2((lid << 1) | 1u, rid),
+	           p.box);
+	ranges[x] = make_uint2(p.l, p.r);
+	if (merge && pk.tri4) clear_leaf_end(pk.tri4, s);
+	return p;
+}
 
+// Level 1 for leaves [chunk*kChunk, +kChunk) of a mesh: every leaf's node,
+// every node completed inside the chunk, and the frontier in slots[0..kSlots)
+// with its size in *count.  Also clears the global visit counters of the
+// chunk's splits for the merge levels.  The climbs touch LDS only: a node
+// formed here keeps its box, state, split and the far end of its range
+// under its local index (an inner node's index is one end of its range, so
+// it lies in the chunk), and an arriving child leaves its local index per
+// split and side; the nodes, ranges and pair records are written
+// afterwards, one node per thread.
+constexpr uint16_t kLocalLeaf = 0x8000u;    // lchild: the child is a leaf
+constexpr uint16_t kNoChild = 0xffffu;
 __device__ inline void bottom_up_chunk(rta_node *__restrict__ nodes,
                                        const rta_aabb *__restrict__ boxes,
                                        const uint32_t *__restrict__ codes,
                                        const uint32_t *__restrict__ indices,
-                                       const uint32_t *__restrict__ parents,
-                                       const uint2 *__restrict__ ranges,
-                                       uint32_t *__restrict__ slots, uint32_t *__restrict__ count,
+                                       uint2 *__restrict__ ranges, uint32_t *__restrict__ visit,
+                                       uint4 *__restrict__ slots, uint32_t *__restrict__ count,
                                        uint32_t *__restrict__ status, int n, int chunk,
                                        PackOut pk = PackOut{}) {
-	__shared__ float lb_leaf[6][kChunk];
-	__shared__ float lb_int[6][kChunk];
-	__shared__ uint32_t lvisit[kChunk];
-	__shared__ uint32_t ltris[kChunk];
-	__shared__ uint32_t lelems[kChunk];
-	__shared__ uint32_t lcode[kChunk];
-	__shared__ uint32_t lkids[kChunk];      // left | right << 16, ~0u: range leaves the chunk
-	__shared__ uint16_t lpar_leaf[kChunk];
-	__shared__ uint16_t lpar_int[kChunk];
-	__shared__ uint32_t front[kSlots];
+	__shared__ float lb_leaf[6][kChunk];   // leaf boxes by position
+	__shared__ float lb_int[6][kChunk];    // boxes of the inner nodes formed here, by index
+	__shared__ uint32_t lstate[kChunk];    // 0: not formed, kStateInner, or a leaf's count
+	__shared__ uint16_t lother[kChunk];    // the end of the node's range that is not its index
+	__shared__ uint16_t lsplit[kChunk];
+	__shared__ uint16_t ltris[kChunk];     // a merged leaf's first position
+	__shared__ uint32_t lvisit[kChunk];    // per split base + j
+	__shared__ uint16_t lchild[2][kChunk]; // per split: the left / right child that arrived
+	__shared__ int8_t ldelta[kChunk + 1];  // delta(base + j - 1)
+	__shared__ uint4 front[kSlots];
 	__shared__ uint32_t nfront;
 
 	const int t = threadIdx.x;
 	const uint32_t base = (uint32_t)chunk * kChunk;
 	const uint32_t leaf0 = (uint32_t)(n - 1);
 	const uint32_t end = min(base + (uint32_t)kChunk, (uint32_t)n);
-	const uint32_t k = base + t;          // leaf k and inner node k
+	const uint32_t cnt = end - base;
+	const uint32_t k = base + t;
 	lvisit[t] = 0;
-	lelems[t] = kStateNone;
+	lstate[t] = 0;
+	lchild[0][t] = kNoChild;
+	lchild[1][t] = kNoChild;
 	if (t == 0) nfront = 0;
-	rta_aabb box;
-	if (k < (uint32_t)n) {
-		box = boxes[indices[k]];
-		lb_leaf[0][t] = box.min.x; lb_leaf[1][t] = box.min.y; lb_leaf[2][t] = box.min.z;
-		lb_leaf[3][t] = box.max.x; lb_leaf[4][t] = box.max.y; lb_leaf[5][t] = box.max.z;
-		lcode[t] = codes[k];
+	if ((uint32_t)t <= cnt) ldelta[t] = (int8_t)delta_at(codes, (long long)base + t - 1, n);
+	if (t == 0 && cnt == (uint32_t)kChunk) ldelta[kChunk] = (int8_t)delta_at(codes, (long long)end - 1, n);
+	if (k + 1u < (uint32_t)n) visit[k] = 0u;
+	const bool live = k < (uint32_t)n;
+	ClimbNode c;
+	if (live) {
+		c.l = c.r = k;
+		c.box = boxes[indices[k]];
+		c.elems = 1u;
+		c.tris = k;
+		lb_leaf[0][t] = c.box.min.x; lb_leaf[1][t] = c.box.min.y; lb_leaf[2][t] = c.box.min.z;
+		lb_leaf[3][t] = c.box.max.x; lb_leaf[4][t] = c.box.max.y; lb_leaf[5][t] = c.box.max.z;
+		store_node(&nodes[leaf0 + k], make_uint2(1u << 1, k), c.box);
 	}
-	const bool inner = k < leaf0 && k < end;
-	uint2 own_links = make_uint2(0u, 0u);   // node k's links when its range is in the chunk
-	if (inner) {
-		const uint2 rg = ranges[k];
-		uint32_t kids = ~0u;
-		if (rg.x >= base && rg.y < end) {
-			const uint2 pl = *reinterpret_cast<const uint2 *>(&nodes[k]);
-			own_links = pl;
-			const uint32_t l = pl.x >> 1, r = pl.y;
-			const uint32_t ll = l >= leaf0 ? (l - leaf0 - base) | kLocalLeaf : l - base;
-			const uint32_t rl = r >= leaf0 ? (r - leaf0 - base) | kLocalLeaf : r - base;
-			kids = ll | (rl << 16);
-		}
-		lkids[t] = kids;
-	}
-	// parents are loaded with the rest; only the decision needs lkids
-	const uint32_t leaf_parent = k < (uint32_t)n && leaf0 + k != 0u ? parents[leaf0 + k] : 0u;
-	const uint32_t inner_parent = inner && k != 0u ? parents[k] : 0u;
-	__syncthreads();
-	auto local_parent = [&](uint32_t node, uint32_t p) -> uint16_t {
-		if (node == 0u) return kParentRoot;
-		if (p < base || p >= end || p >= leaf0 || lkids[p - base] == ~0u) return kParentExit;
-		return (uint16_t)(p - base);
+	auto push = [&](const uint4 &e) {
+		const uint32_t slot = atomicAdd(&nfront, 1u);
+		if (slot < (uint32_t)kSlots) front[slot] = e;
+		else atomicOr(status, (uint32_t)RTA_LBVH_FRONTIER_OVERFLOW);
 	};
-	if (k < (uint32_t)n) lpar_leaf[t] = local_parent(leaf0 + k, leaf_parent);
-	if (inner) lpar_int[t] = local_parent(k, inner_parent);
+	// the node that arrived as child `side` of local split j
+	auto arrived = [&](uint32_t j, int side) -> ClimbNode {
+		const uint16_t id = lchild[side][j];
+		const uint32_t q = id & 0x3ffu;
+		ClimbNode a;
+		const float(*b)[kChunk] = (id & kLocalLeaf) ? lb_leaf : lb_int;
+		a.box.min = {b[0][q], b[1][q], b[2][q]};
+		a.box.max = {b[3][q], b[4][q], b[5][q]};
+		if (id & kLocalLeaf) {
+			a.l = a.r = base + q;
+			a.elems = 1u;
+			a.tris = base + q;
+		} else {
+			const uint32_t o = base + lother[q];
+			a.l = min(base + q, o);
+			a.r = max(base + q, o);
+			a.elems = lstate[q];
+			a.tris = base + ltris[q];
+		}
+		a.dl = ldelta[a.l - base];
+		a.dr = ldelta[a.r - base + 1];
+		return a;
+	};
 	__syncthreads();
 
-	if (k < (uint32_t)n) {
-		uint32_t cur = (uint32_t)t | kLocalLeaf;
-		uint32_t p = lpar_leaf[t];
+	if (live) {
+		c.dl = ldelta[t];
+		c.dr = ldelta[t + 1];
 		for (int level = 0; level < 128; ++level) {
-			if (p == kParentRoot) break;
-			if (p == kParentExit) {
-				const uint32_t g = (cur & kLocalLeaf) ? leaf0 + base + (cur & 0x3ffu) : base + cur;
-				const uint32_t slot = atomicAdd(&nfront, 1u);
-				if (slot < (uint32_t)kSlots) front[slot] = g;
-				else atomicOr(status, (uint32_t)RTA_LBVH_FRONTIER_OVERFLOW);
+			if (c.dl < 0 && c.dr < 0) break;   // the root of the mesh
+			const bool left = is_left_child(c);
+			const uint32_t s = left ? c.r : c.l - 1u;
+			if (s < base || s + 1u >= end) {
+				push(frontier_entry(c, leaf0));   // the sibling starts in another chunk
 				break;
 			}
+			const uint32_t j = s - base;
+			lchild[left ? 0 : 1][j] = c.l == c.r ? (uint16_t)((c.l - base) | kLocalLeaf)
+			                                     : (uint16_t)(node_index(c, leaf0) - base);
 			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-			const uint32_t old = atomicAdd(&lvisit[p], 1u);
+			const uint32_t old = atomicAdd(&lvisit[j], 1u);
 			if (old == 0u) break;
 			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-
-			const uint32_t kids = lkids[p];
-			const uint32_t left = kids & 0xffffu, right = kids >> 16;
-			const bool cur_is_left = (left == cur);
-			const uint32_t sib = cur_is_left ? right : left;
-			const uint32_t q = sib & 0x3ffu;
-			rta_aabb sbox;
-			if (sib & kLocalLeaf) {
-				sbox.min = {lb_leaf[0][q], lb_leaf[1][q], lb_leaf[2][q]};
-				sbox.max = {lb_leaf[3][q], lb_leaf[4][q], lb_leaf[5][q]};
-			} else {
-				sbox.min = {lb_int[0][q], lb_int[1][q], lb_int[2][q]};
-				sbox.max = {lb_int[3][q], lb_int[4][q], lb_int[5][q]};
-			}
-			box = cur_is_left ? union_lr(box, sbox) : union_lr(sbox, box);
-			lb_int[0][p] = box.min.x; lb_int[1][p] = box.min.y; lb_int[2][p] = box.min.z;
-			lb_int[3][p] = box.max.x; lb_int[4][p] = box.max.y; lb_int[5][p] = box.max.z;
-
-			// children state: original leaf, collapsed inner node, or inner.
-			// The representative code of leaf j and of inner node j are both
-			// codes[j] (lbvh-kernels.cu:261), i.e. lcode of the local index.
-			uint32_t state = kStateInner, ptris = 0;
-			if (lcode[cur & 0x3ffu] == lcode[q]) {
-				uint32_t le, lt, re, rt;
-				bool l_leaf, r_leaf;
-				const uint32_t li = left & 0x3ffu, ri = right & 0x3ffu;
-				if (left & kLocalLeaf) { l_leaf = true; le = 1; lt = base + li; }
-				else { le = lelems[li]; lt = ltris[li]; l_leaf = le != kStateInner; }
-				if (right & kLocalLeaf) { r_leaf = true; re = 1; rt = base + ri; }
-				else { re = lelems[ri]; rt = ltris[ri]; r_leaf = re != kStateInner; }
-				if (l_leaf && r_leaf) { state = re + le; ptris = lt < rt ? lt : rt; }
-			}
-			ltris[p] = ptris;
-			lelems[p] = state;
-			cur = p;
-			p = lpar_int[p];
+			const ClimbNode sib = arrived(j, left ? 1 : 0);
+			const ClimbNode &lc = left ? c : sib;
+			const ClimbNode &rc = left ? sib : c;
+			ClimbNode p;
+			p.l = lc.l; p.r = rc.r; p.dl = lc.dl; p.dr = rc.dr;
+			p.box = union_lr(lc.box, rc.box);
+			const bool merge = lc.dr >= 32 && lc.elems != kStateInner && rc.elems != kStateInner;
+			p.elems = merge ? lc.elems + rc.elems : kStateInner;
+			p.tris = merge ? (lc.tris < rc.tris ? lc.tris : rc.tris) : 0u;
+			const uint32_t x = node_index(p, leaf0) - base;
+			lb_int[0][x] = p.box.min.x; lb_int[1][x] = p.box.min.y; lb_int[2][x] = p.box.min.z;
+			lb_int[3][x] = p.box.max.x; lb_int[4][x] = p.box.max.y; lb_int[5][x] = p.box.max.z;
+			lstate[x] = p.elems;
+			ltris[x] = (uint16_t)(p.tris - (merge ? base : 0u));
+			lother[x] = (uint16_t)((x + base == p.l ? p.r : p.l) - base);
+			lsplit[x] = (uint16_t)j;
+			c = p;
 		}
 	}
 	__syncthreads();
 
-	// write-out: whole 32-byte leaf nodes (links: one triangle, the first
-	// at position k) and completed inner nodes, then the frontier
-	if (k < (uint32_t)n) store_node(&nodes[leaf0 + k], make_uint2(1u << 1, k), lb_leaf, t);
+	// write-out: the inner node with index k when it was formed here
 	bool slow = false;
-	if (k < leaf0 && lelems[t] != kStateNone) {
-		const bool collapsed = lelems[t] != kStateInner;
-		store_node(&nodes[k], collapsed ? make_uint2(lelems[t] << 1, ltris[t]) : own_links, lb_int,
-		           t);
-		if (collapsed) {
-			// the collapse merged two leaves that meet at the node's split,
-			// the left child's last position: base + its local reference
-			// (a leaf's position, or an inner left child's index, which is
-			// the end of its range)
-			if (pk.tri4) clear_leaf_end(pk.tri4, base + (lkids[t] & 0x3ffu));
+	if (live && k < leaf0 && lstate[t] != 0u) {
+		const uint32_t s = base + lsplit[t], o = base + lother[t];
+		const uint32_t l = min(k, o), r = max(k, o);
+		const uint32_t lid = l == s ? leaf0 + s : s;
+		const uint32_t rid = r == s + 1u ? leaf0 + s + 1u : s + 1u;
+		const bool merged = lstate[t] != kStateInner;
+		rta_aabb box;
+		box.min = {lb_int[0][t], lb_int[1][t], lb_int[2][t]};
+		box.max = {lb_int[3][t], lb_int[4][t], lb_int[5][t]};
+		store_node(&nodes[k], merged ? make_uint2(lstate[t] << 1, base + ltris[t])
+		                             : make_uint2((lid << 1) | 1u, rid), box);
+		ranges[k] = make_uint2(l, r);
+		if (merged) {
+			if (pk.tri4) clear_leaf_end(pk.tri4, s);
 		} else if (pk.pairs) {
-			// a completed inner node: its pair record from the children's
-			// final state (an original leaf, a collapsed node or an inner one)
-			const uint32_t kids = lkids[t];
 			rta_aabb cb[2];
-			uint32_t word[2], id[2];
-			for (int c = 0; c < 2; ++c) {
-				const uint32_t ref = c == 0 ? (kids & 0xffffu) : (kids >> 16);
-				const uint32_t q = ref & 0x3ffu;
-				const float(*src)[kChunk] = (ref & kLocalLeaf) ? lb_leaf : lb_int;
-				cb[c].min = {src[0][q], src[1][q], src[2][q]};
-				cb[c].max = {src[3][q], src[4][q], src[5][q]};
-				if (ref & kLocalLeaf) {
-					word[c] = (base + q) << 1;
-					id[c] = leaf0 + base + q;
-				} else {
-					word[c] = lelems[q] == kStateInner ? ((base + q) << 1) | 1u : ltris[q] << 1;
-					id[c] = base + q;
-				}
+			uint32_t word[2];
+			for (int side = 0; side < 2; ++side) {
+				const uint32_t q = s + side - base;
+				const bool leaf = side == 0 ? l == s : r == s + 1u;
+				const float(*b)[kChunk] = leaf ? lb_leaf : lb_int;
+				cb[side].min = {b[0][q], b[1][q], b[2][q]};
+				cb[side].max = {b[3][q], b[4][q], b[5][q]};
+				if (leaf) word[side] = (base + q) << 1;
+				else if (lstate[q] == kStateInner) word[side] = ((base + q) << 1) | 1u;
+				else word[side] = (base + ltris[q]) << 1;
 			}
-			slow = !write_pair(pk.pairs, k, cb[0], cb[1], word[0], word[1], id[0], id[1]);
+			slow = !write_pair(pk.pairs, k, cb[0], cb[1], word[0], word[1], lid, rid);
 		}
 	}
+	// a child still waiting at a split of the chunk has a sibling that
+	// reaches beyond it: a frontier entry
+	if ((uint32_t)t + 1u < cnt && lvisit[t] == 1u)
+		push(frontier_entry(arrived(t, lchild[0][t] != kNoChild ? 0 : 1), leaf0));
 	if (pk.flags && __syncthreads_or(slow) && t == 0) atomicOr(pk.flags, 1u);
+	__syncthreads();
 	const uint32_t nf = min(nfront, (uint32_t)kSlots);
 	if ((uint32_t)t < nf) slots[t] = front[t];
 	if (t == 0) *count = nf;
 }
 
-__device__ inline rta_aabb load_box(const rta_node *node) {
-	const float *f = reinterpret_cast<const float *>(node) + 2;
-	rta_aabb b;
-	b.min = {f[0], f[1], f[2]};
-	b.max = {f[3], f[4], f[5]};
-	return b;
-}
-__device__ inline void store_box(rta_node *node, const rta_aabb &b) {
-	float *f = reinterpret_cast<float *>(node) + 2;
-	f[0] = b.min.x; f[1] = b.min.y; f[2] = b.min.z;
-	f[3] = b.max.x; f[4] = b.max.y; f[5] = b.max.z;
-}
-
 // Level k >= 2 for one mesh: merges the frontier slots of the ng chunks of
-// the previous level starting at in_slots/in_counts (kSlots words per
+// the previous level starting at in_slots/in_counts (kSlots entries per
 // chunk), whose leaves span [lo, hi); the new frontier goes to out_slots /
-// *out_count when given.
+// *out_count when given.  A child arriving first at a split records its far
+// end and the delta there in stash[its index] before the visit counter; the
+// second finds its sibling's node from them.  A first arrival whose sibling
+// never comes (it reaches beyond [lo, hi)) joins the new frontier and its
+// visit is taken back, so that the next level counts its arrival afresh.
+constexpr int kMergeWaits = kMergeFan * kSlots;   // bounds the first arrivals of a block
 __device__ inline void bottom_up_merge(rta_node *__restrict__ nodes, uint32_t *__restrict__ visit,
+                                       uint2 *__restrict__ stash,
                                        const uint32_t *__restrict__ codes,
-                                       const uint32_t *__restrict__ parents,
-                                       const uint2 *__restrict__ ranges,
-                                       const uint32_t *__restrict__ in_slots,
+                                       uint2 *__restrict__ ranges,
+                                       const uint4 *__restrict__ in_slots,
                                        const uint32_t *__restrict__ in_counts, int ng,
-                                       uint32_t *__restrict__ out_slots,
+                                       uint4 *__restrict__ out_slots,
                                        uint32_t *__restrict__ out_count,
                                        uint32_t *__restrict__ status, uint32_t lo, uint32_t hi,
                                        int n, PackOut pk = PackOut{}) {
 	__shared__ uint32_t prefix[kMergeFan + 1];
-	__shared__ uint32_t front[kSlots];
-	__shared__ uint32_t nfront;
+	__shared__ uint4 front[kSlots];
+	__shared__ uint32_t waits[kMergeWaits];   // split | (right child) << 31
+	__shared__ uint32_t formed[kMergeWaits];  // the inner nodes formed here
+	__shared__ uint32_t nfront, nwaits, nformed;
 	const int t = threadIdx.x;
 	if (t == 0) {
 		uint32_t acc = 0;
@@ -578,106 +631,117 @@ __device__ inline void bottom_up_merge(rta_node *__restrict__ nodes, uint32_t *_
 			prefix[g + 1] = acc;
 		}
 		nfront = 0;
+		nwaits = 0;
+		nformed = 0;
 	}
 	__syncthreads();
 	const uint32_t leaf0 = (uint32_t)(n - 1);
+	auto push = [&](const uint4 &e) {
+		const uint32_t slot = atomicAdd(&nfront, 1u);
+		if (out_slots && slot < (uint32_t)kSlots) front[slot] = e;
+		else atomicOr(status, (uint32_t)RTA_LBVH_FRONTIER_OVERFLOW);
+	};
 	const uint32_t total = prefix[ng];
 	for (uint32_t f = t; f < total; f += blockDim.x) {
 		int g = 0;
 		while (prefix[g + 1] <= f) ++g;
-		uint32_t cur = in_slots[(size_t)g * kSlots + (f - prefix[g])];
-		if (cur == 0u) continue;
-		rta_aabb box = load_box(&nodes[cur]);
-		uint32_t code_cur = rep_code(codes, cur, leaf0);
-		uint32_t parent = parents[cur];
-		// Per level: (a) what earlier stages wrote and nothing changes
-		// before this thread arrives -- a node's range, links, code and
-		// parent; (b) the visit counter; (c) after its acquire, what the
-		// sibling subtree's thread wrote -- the sibling's box and the
-		// children's (possibly collapsed) links.  (a) for the grandparent is
-		// issued before (b), so a level costs two dependent round trips.
-		// The grandparent's links change only when a thread arrives there
-		// second; this thread reads them before it could arrive, and uses
-		// them only if it does.
-		uint2 rg = ranges[parent];
-		uint2 pl = *reinterpret_cast<const uint2 *>(&nodes[parent]);
-		uint32_t grand = parent != 0u ? parents[parent] : 0u;
-		uint32_t code_parent = codes[parent];
-		for (int level = 0; cur != 0u && level < 128; ++level) {
-			if (rg.x < lo || rg.y >= hi) {
-				const uint32_t slot = atomicAdd(&nfront, 1u);
-				if (slot < (uint32_t)kSlots) front[slot] = cur;
+		const uint4 e = in_slots[(size_t)g * kSlots + (f - prefix[g])];
+		ClimbNode c;
+		c.l = e.y;
+		c.r = e.z;
+		c.dl = (int)(e.w & 0xffu) - 1;
+		c.dr = (int)((e.w >> 8) & 0xffu) - 1;
+		{
+			const rta_node nd = nodes[e.x];
+			c.box = nd.box;
+			node_state(make_uint2(nd.type_left_elems, nd.right_tris), c.elems, c.tris);
+		}
+		for (int level = 0; level < 128; ++level) {
+			if (c.dl < 0 && c.dr < 0) break;   // the root of the mesh
+			const bool left = is_left_child(c);
+			const uint32_t s = left ? c.r : c.l - 1u;
+			if (s < lo || s + 1u >= hi) {
+				push(frontier_entry(c, leaf0));
+				break;
+			}
+			stash[left ? c.r : c.l] = make_uint2(left ? c.l : c.r, (uint32_t)(left ? c.dl : c.dr));
+			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+			const uint32_t old = __hip_atomic_fetch_add(&visit[s], 1u, __ATOMIC_RELAXED,
+			                                            __HIP_MEMORY_SCOPE_WORKGROUP);
+			if (old == 0u) {
+				const uint32_t w = atomicAdd(&nwaits, 1u);
+				if (w < (uint32_t)kMergeWaits) waits[w] = s | (left ? 0u : 0x80000000u);
 				else atomicOr(status, (uint32_t)RTA_LBVH_FRONTIER_OVERFLOW);
 				break;
 			}
-			uint2 rg_next = make_uint2(0u, 0u), pl_next = make_uint2(0u, 0u);
-			uint32_t grand_next = 0u, code_next = 0u;
-			if (parent != 0u) {
-				rg_next = ranges[grand];
-				pl_next = *reinterpret_cast<const uint2 *>(&nodes[grand]);
-				grand_next = grand != 0u ? parents[grand] : 0u;
-				code_next = codes[grand];
+			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+			const uint32_t sx = left ? s + 1u : s;   // the sibling's index
+			// its record and both nodes it may be (inner node sx, or the
+			// leaf at position sx: then its far end is sx) read together
+			const uint2 st = stash[sx];
+			const rta_node as_inner = nodes[sx], as_leaf = nodes[leaf0 + sx];
+			const int ds = left ? c.dr : c.dl;
+			ClimbNode sib;
+			if (left) { sib.l = s + 1u; sib.r = st.x; sib.dl = ds; sib.dr = (int)st.y; }
+			else { sib.l = st.x; sib.r = s; sib.dl = (int)st.y; sib.dr = ds; }
+			if (sib.l < lo || sib.r >= hi || sib.l > sib.r) {
+				// cannot happen for sorted codes; guards the writes below
+				atomicOr(status, (uint32_t)RTA_LBVH_FRONTIER_OVERFLOW);
+				break;
 			}
-			// acquire only: this thread's earlier box and link stores were
-			// released by the fence that ends the previous level (or by the
-			// launch boundary), so the prefetch above need not complete first
-			const uint32_t old = __hip_atomic_fetch_add(&visit[parent], 1u, __ATOMIC_ACQUIRE,
-			                                            __HIP_MEMORY_SCOPE_WORKGROUP);
-			if (old == 0u) break;
-			const uint32_t left = pl.x >> 1, right = pl.y;
-			const bool cur_is_left = (left == cur);
-			const uint32_t sib = cur_is_left ? right : left;
-			const rta_aabb sbox = load_box(&nodes[sib]);
-			const uint32_t code_sib = rep_code(codes, sib, leaf0);
-			const uint2 ll = *reinterpret_cast<const uint2 *>(&nodes[left]);
-			const uint2 rl = *reinterpret_cast<const uint2 *>(&nodes[right]);
-			const rta_aabb lbox = cur_is_left ? box : sbox, rbox = cur_is_left ? sbox : box;
-			box = union_lr(lbox, rbox);
-			store_box(&nodes[parent], box);
-			const bool collapse = code_cur == code_sib && !link_inner(ll.x) && !link_inner(rl.x);
-			if (collapse) {
-				const uint32_t tris = ll.y < rl.y ? ll.y : rl.y;
-				const uint32_t elems = (rl.x >> 1) + (ll.x >> 1);
-				*reinterpret_cast<uint2 *>(&nodes[parent]) = make_uint2(elems << 1, tris);
+			const rta_node &sn = sib.l == sib.r ? as_leaf : as_inner;
+			sib.box = sn.box;
+			node_state(make_uint2(sn.type_left_elems, sn.right_tris), sib.elems, sib.tris);
+			uint32_t x;
+			c = left ? form_parent(c, sib, s, leaf0, nodes, ranges, pk, x)
+			         : form_parent(sib, c, s, leaf0, nodes, ranges, pk, x);
+			if (pk.pairs && c.elems == kStateInner) {
+				const uint32_t i = atomicAdd(&nformed, 1u);
+				if (i < (uint32_t)kMergeWaits) formed[i] = x;
+				else atomicOr(status, (uint32_t)RTA_LBVH_FRONTIER_OVERFLOW);
 			}
-			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-			// the leaf-end mark is read only after the build: its store
-			// follows the fence, which waits for the box and links only.
-			// (Pair records of the nodes completed here are written by
-			// pack_boundary_pairs after the last level.)
-			if (collapse && pk.tri4) clear_leaf_end(pk.tri4, ll.y + (ll.x >> 1) - 1u);
-			cur = parent;
-			parent = grand;
-			code_cur = code_parent;
-			rg = rg_next;
-			pl = pl_next;
-			grand = grand_next;
-			code_parent = code_next;
 		}
+	}
+	__syncthreads();
+	// pair records of the inner nodes formed here, from the finished nodes
+	// (kept out of the climb: their stores would lengthen every wait on it)
+	bool slow = false;
+	const uint32_t nfm = min(nformed, (uint32_t)kMergeWaits);
+	for (uint32_t i = t; i < nfm; i += blockDim.x) {
+		const uint32_t x = formed[i];
+		const uint2 links = *reinterpret_cast<const uint2 *>(&nodes[x]);
+		const uint32_t L = links.x >> 1, R = links.y;
+		const rta_node nl = nodes[L], nr = nodes[R];
+		const uint32_t lw = link_inner(nl.type_left_elems) ? (L << 1) | 1u : nl.right_tris << 1;
+		const uint32_t rw = link_inner(nr.type_left_elems) ? (R << 1) | 1u : nr.right_tris << 1;
+		if (!write_pair(pk.pairs, x, nl.box, nr.box, lw, rw, L, R)) slow = true;
+	}
+	if (slow && pk.flags) atomicOr(pk.flags, 1u);
+	// (none at the last level: every sibling lies within it)
+	const uint32_t nw = out_slots ? min(nwaits, (uint32_t)kMergeWaits) : 0u;
+	for (uint32_t i = t; i < nw; i += blockDim.x) {
+		const uint32_t s = waits[i] & 0x7fffffffu;
+		const bool right = (waits[i] >> 31) != 0u;
+		// the counter (agent scope: where its atomics went), the waiting
+		// child's record and the delta at the split, read together
+		const uint32_t v = __hip_atomic_load(&visit[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		const uint2 st = stash[right ? s + 1u : s];
+		const int ds = delta_at(codes, s, n);
+		if (v != 1u) continue;   // formed
+		visit[s] = 0u;
+		ClimbNode w;
+		if (right) { w.l = s + 1u; w.r = st.x; w.dl = ds; w.dr = (int)st.y; }
+		else { w.l = st.x; w.r = s; w.dl = (int)st.y; w.dr = ds; }
+		if (w.l < lo || w.r >= hi || w.l > w.r) {
+			atomicOr(status, (uint32_t)RTA_LBVH_FRONTIER_OVERFLOW);
+			continue;
+		}
+		push(frontier_entry(w, leaf0));
 	}
 	__syncthreads();
 	const uint32_t nf = min(nfront, (uint32_t)kSlots);
 	if (out_slots && (uint32_t)t < nf) out_slots[t] = front[t];
 	if (out_count && t == 0) *out_count = nf;
-}
-
-// Pair records of the inner nodes the chunk level cannot complete: those
-// whose range [from, to] spans more than one kChunk-leaf chunk, finished by
-// the merge levels.  Node x is one of them iff from / kChunk != to / kChunk;
-// its record comes from the finished nodes, as rta_bvh_pack forms it.
-__device__ inline void pack_boundary_pair(rta_node_pair *__restrict__ pairs,
-                                          uint32_t *__restrict__ flags,
-                                          const rta_node *__restrict__ nodes,
-                                          const uint2 *__restrict__ ranges, uint32_t x) {
-	const uint2 rg = ranges[x];
-	if (rg.x / (uint32_t)kChunk == rg.y / (uint32_t)kChunk) return;
-	const uint2 links = *reinterpret_cast<const uint2 *>(&nodes[x]);
-	if (!link_inner(links.x)) return;   // collapsed into a leaf
-	const uint32_t L = links.x >> 1, R = links.y;
-	const rta_node nl = nodes[L], nr = nodes[R];
-	const uint32_t lw = link_inner(nl.type_left_elems) ? (L << 1) | 1u : nl.right_tris << 1;
-	const uint32_t rw = link_inner(nr.type_left_elems) ? (R << 1) | 1u : nr.right_tris << 1;
-	if (!write_pair(pairs, x, nl.box, nr.box, lw, rw, L, R)) atomicOr(flags, 1u);
 }
 
 // -------------------------------------------------- stage 7: permute
