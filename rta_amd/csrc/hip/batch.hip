@@ -64,22 +64,21 @@ __global__ __launch_bounds__(kChunk) void batch_chunk_kernel(
     const BatchItem *__restrict__ items, const BatchMesh *__restrict__ meshes,
     rta_node *__restrict__ nodes, const rta_aabb *__restrict__ boxes,
     const uint32_t *__restrict__ codes, const uint32_t *__restrict__ indices,
-    uint2 *__restrict__ ranges, uint32_t *__restrict__ visit, uint4 *__restrict__ slots,
-    uint32_t *__restrict__ counts, uint32_t *__restrict__ status) {
+    uint2 *__restrict__ ranges, uint4 *__restrict__ slots, uint32_t *__restrict__ counts,
+    uint32_t *__restrict__ status) {
 	const BatchItem it = items[blockIdx.x];
 	const BatchMesh m = meshes[it.mesh];
 	bottom_up_chunk(nodes + m.node_off, boxes + m.tri_off, codes + m.tri_off, indices + m.tri_off,
-	                ranges + m.tri_off, visit + m.tri_off,
+	                ranges + m.tri_off,
 	                slots + (size_t)(m.chunk_off + it.block) * kSlots, counts + m.chunk_off + it.block,
 	                status, (int)m.n, (int)it.block);
 }
 
 // Merge level `level` (>= 1): level-`level` groups of a mesh cover
 // kChunk * kMergeFan^level leaves.
-__global__ __launch_bounds__(1024) void batch_merge_kernel(
+__global__ __launch_bounds__(kMergeThreads) void batch_merge_kernel(
     const BatchItem *__restrict__ items, const BatchMesh *__restrict__ meshes,
-    rta_node *__restrict__ nodes, uint32_t *__restrict__ visit, uint2 *__restrict__ stash,
-    const uint32_t *__restrict__ codes, uint2 *__restrict__ ranges,
+    rta_node *__restrict__ nodes, uint2 *__restrict__ ranges,
     const uint4 *__restrict__ in_slots, const uint32_t *__restrict__ in_counts,
     uint4 *__restrict__ out_slots, uint32_t *__restrict__ out_counts,
     uint32_t *__restrict__ status, long long width) {
@@ -90,9 +89,8 @@ __global__ __launch_bounds__(1024) void batch_merge_kernel(
 	const uint32_t lo = (uint32_t)(it.block * width);
 	const uint32_t hi = (uint32_t)min((long long)m.n, (long long)(it.block + 1) * width);
 	const bool last = it.blocks == 1;
-	bottom_up_merge(nodes + m.node_off, visit + m.tri_off, stash + m.tri_off, codes + m.tri_off,
-	                ranges + m.tri_off, in_slots + (size_t)(m.chunk_off + g0) * kSlots,
-	                in_counts + m.chunk_off + g0, ng,
+	bottom_up_merge(nodes + m.node_off, ranges + m.tri_off,
+	                in_slots + (size_t)(m.chunk_off + g0) * kSlots, in_counts + m.chunk_off + g0, ng,
 	                last ? nullptr : out_slots + (size_t)(m.chunk_off + it.block) * kSlots,
 	                last ? nullptr : out_counts + m.chunk_off + it.block, status, lo, hi,
 	                (int)m.n);
@@ -114,8 +112,8 @@ struct BatchWork {
 	rta_triangle *scratch;
 	rta_aabb *boxes, *partials, *bounds;
 	rta_float3 *centers;
-	uint32_t *codes, *visit;
-	uint2 *ranges, *stash;
+	uint32_t *codes;
+	uint2 *ranges;
 	uint4 *slots[2];
 	uint32_t *counts[2];
 	uint32_t *bu_status;
@@ -139,9 +137,7 @@ static BatchWork carve_batch(void *base, const BatchPlan &p) {
 	w.partials = (rta_aabb *)take((size_t)p.total_parts * sizeof(rta_aabb));
 	w.bounds = (rta_aabb *)take((size_t)p.n_meshes * sizeof(rta_aabb));
 	w.codes = (uint32_t *)take(N * 4);
-	w.visit = (uint32_t *)take(N * 4);
 	w.ranges = (uint2 *)take(N * 8);
-	w.stash = (uint2 *)take(N * 8);
 	for (int i = 0; i < 2; ++i) {
 		w.slots[i] = (uint4 *)take((size_t)p.total_chunks * kSlots * 16);
 		w.counts[i] = (uint32_t *)take((size_t)p.total_chunks * 4);
@@ -335,14 +331,14 @@ int rta_batch_build(rta_batch b, rta_triangle *tris, rta_node *nodes, uint32_t *
 	// the bottom-up pass emits the hierarchy too (lbvh_body.hpp)
 	hipLaunchKernelGGL(batch_chunk_kernel, dim3((unsigned)p.chunk_items), dim3(kChunk), 0, st,
 	                   items + p.chunk_items_off, p.d_meshes, nodes, w.boxes, w.codes, indices,
-	                   w.ranges, w.visit, w.slots[0], w.counts[0], bu_status);
+	                   w.ranges, w.slots[0], w.counts[0], bu_status);
 	RTA_LAUNCH_CHECK("batch_chunk_kernel");
 	long long width = kChunk;
 	for (size_t L = 0; L < p.merge_off.size(); ++L) {
 		width *= kMergeFan;
 		const int in = (int)(L & 1), outb = 1 - in;
-		hipLaunchKernelGGL(batch_merge_kernel, dim3((unsigned)p.merge_len[L]), dim3(1024), 0, st,
-		                   items + p.merge_off[L], p.d_meshes, nodes, w.visit, w.stash, w.codes,
+		hipLaunchKernelGGL(batch_merge_kernel, dim3((unsigned)p.merge_len[L]), dim3(kMergeThreads), 0,
+		                   st, items + p.merge_off[L], p.d_meshes, nodes,
 		                   w.ranges, w.slots[in], w.counts[in], w.slots[outb], w.counts[outb],
 		                   bu_status, width);
 		RTA_LAUNCH_CHECK("batch_merge_kernel");

@@ -52,18 +52,17 @@ __global__ __launch_bounds__(kKarrasBlock) void karras_kernel(
 __global__ __launch_bounds__(kChunk) void bottom_up_chunk_kernel(
     rta_node *__restrict__ nodes, const rta_aabb *__restrict__ boxes,
     const uint32_t *__restrict__ codes, const uint32_t *__restrict__ indices,
-    uint2 *__restrict__ ranges, uint32_t *__restrict__ visit, uint4 *__restrict__ slots,
-    uint32_t *__restrict__ counts, uint32_t *__restrict__ status, int n, PackOut pk) {
-	bottom_up_chunk(nodes, boxes, codes, indices, ranges, visit,
+    uint2 *__restrict__ ranges, uint4 *__restrict__ slots, uint32_t *__restrict__ counts,
+    uint32_t *__restrict__ status, int n, PackOut pk) {
+	bottom_up_chunk(nodes, boxes, codes, indices, ranges,
 	                slots + (size_t)blockIdx.x * kSlots, counts + blockIdx.x, status, n,
 	                (int)blockIdx.x, pk);
 }
 
 // One block merges the frontier slots of up to kMergeFan chunks of the
 // previous level; its own chunk is `chunk` leaves wide.
-__global__ __launch_bounds__(1024) void bottom_up_merge_kernel(
-    rta_node *__restrict__ nodes, uint32_t *__restrict__ visit, uint2 *__restrict__ stash,
-    const uint32_t *__restrict__ codes, uint2 *__restrict__ ranges,
+__global__ __launch_bounds__(kMergeThreads) void bottom_up_merge_kernel(
+    rta_node *__restrict__ nodes, uint2 *__restrict__ ranges,
     const uint4 *__restrict__ in_slots, const uint32_t *__restrict__ in_counts, int n_in,
     uint4 *__restrict__ out_slots, uint32_t *__restrict__ out_counts,
     uint32_t *__restrict__ status, long long chunk, int n, PackOut pk) {
@@ -71,16 +70,15 @@ __global__ __launch_bounds__(1024) void bottom_up_merge_kernel(
 	const int ng = min(kMergeFan, n_in - g0);
 	const uint32_t lo = (uint32_t)(blockIdx.x * chunk);
 	const uint32_t hi = (uint32_t)min((long long)n, (long long)(blockIdx.x + 1) * chunk);
-	bottom_up_merge(nodes, visit, stash, codes, ranges, in_slots + (size_t)g0 * kSlots,
+	bottom_up_merge(nodes, ranges, in_slots + (size_t)g0 * kSlots,
 	                in_counts + g0, ng,
 	                out_slots ? out_slots + (size_t)blockIdx.x * kSlots : nullptr,
 	                out_counts ? out_counts + blockIdx.x : nullptr, status, lo, hi, n, pk);
 }
 
 struct BoxesWork {
-	uint32_t *visit;     // per split
+	uint32_t *visit;     // per inner node (rta_lbvh_build_tree clears them)
 	uint2 *ranges;       // per inner node
-	uint2 *stash;        // per node index: a first arrival's far end and delta
 	uint4 *slots[2];     // frontier entries
 	uint32_t *counts[2];
 	uint32_t *status;
@@ -100,7 +98,6 @@ static BoxesWork carve_boxes(void *base, int n) {
 	const size_t chunks = (nb + kChunk - 1) / kChunk;
 	w.visit = (uint32_t *)take(nb * 4);
 	w.ranges = (uint2 *)take(nb * 8);
-	w.stash = (uint2 *)take(nb * 8);
 	for (int i = 0; i < 2; ++i) {
 		w.slots[i] = (uint4 *)take(chunks * kSlots * 16);
 		w.counts[i] = (uint32_t *)take(chunks * 4);
@@ -235,15 +232,15 @@ static int build_boxes(rta_node *nodes, void *work, const rta_aabb *boxes, const
 	hipStream_t st = as_stream(s);
 	int groups = (int)grid_for(n, kChunk);
 	hipLaunchKernelGGL(bottom_up_chunk_kernel, dim3(groups), dim3(kChunk), 0, st, nodes, boxes,
-	                   codes, indices, w.ranges, w.visit, w.slots[0], w.counts[0], status, n, pk);
+	                   codes, indices, w.ranges, w.slots[0], w.counts[0], status, n, pk);
 	RTA_LAUNCH_CHECK("bottom_up_chunk_kernel");
 	long long chunk = kChunk;
 	int in = 0;
 	while (groups > 1) {
 		const int blocks = (groups + kMergeFan - 1) / kMergeFan;
 		chunk *= kMergeFan;
-		hipLaunchKernelGGL(bottom_up_merge_kernel, dim3(blocks), dim3(1024), 0, st, nodes,
-		                   w.visit, w.stash, codes, w.ranges, w.slots[in], w.counts[in], groups,
+		hipLaunchKernelGGL(bottom_up_merge_kernel, dim3(blocks), dim3(kMergeThreads), 0, st, nodes,
+		                   w.ranges, w.slots[in], w.counts[in], groups,
 		                   blocks > 1 ? w.slots[1 - in] : nullptr,
 		                   blocks > 1 ? w.counts[1 - in] : nullptr, status, chunk, n, pk);
 		RTA_LAUNCH_CHECK("bottom_up_merge_kernel");

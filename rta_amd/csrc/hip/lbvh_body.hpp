@@ -14,7 +14,7 @@ constexpr float kBoundsInit = 100000000000.0f;        // lbvh-kernels.cu:519
 constexpr int kBoundsBlock = 1024;
 constexpr int kChunk = 1024;
 constexpr int kSlots = 128;
-constexpr int kMergeFan = 64;
+constexpr int kMergeFan = 8;
 constexpr uint32_t kStateInner = 0xffffffffu; // leaf count of an inner node in the climb
 
 // ---------------------------------------------------------- wave helpers
@@ -299,9 +299,10 @@ __device__ inline void karras_node(rta_node *__restrict__ nodes, uint32_t *__res
 //           when its parent's split lies outside the chunk, and the
 //           completed nodes whose parents stay incomplete form the chunk's
 //           frontier (at most 2 per tree level, so <= 126 for depth <= 63);
-//  level k: one block merges the frontiers of kMergeFan level-(k-1) chunks
-//           with workgroup-scope atomics on global visit counters, until
-//           one block covers all leaves and reaches the root.
+//  level k: one block merges the frontiers of kMergeFan level-(k-1) chunks,
+//           its visit counters and waiting children in LDS again (a table
+//           keyed by split), until one block covers all leaves and reaches
+//           the root.
 // No step needs device-scope cache maintenance; results do not depend on
 // the schedule.
 
@@ -432,8 +433,7 @@ __device__ inline ClimbNode form_parent(const ClimbNode &lc, const ClimbNode &rc
 
 // Level 1 for leaves [chunk*kChunk, +kChunk) of a mesh: every leaf's node,
 // every node completed inside the chunk, and the frontier in slots[0..kSlots)
-// with its size in *count.  Also clears the global visit counters of the
-// chunk's splits for the merge levels.  The climbs touch LDS only: a node
+// with its size in *count.  The climbs touch LDS only: a node
 // formed here keeps its box, state, split and the far end of its range
 // under its local index (an inner node's index is one end of its range, so
 // it lies in the chunk), and an arriving child leaves its local index per
@@ -445,7 +445,7 @@ __device__ inline void bottom_up_chunk(rta_node *__restrict__ nodes,
                                        const rta_aabb *__restrict__ boxes,
                                        const uint32_t *__restrict__ codes,
                                        const uint32_t *__restrict__ indices,
-                                       uint2 *__restrict__ ranges, uint32_t *__restrict__ visit,
+                                       uint2 *__restrict__ ranges,
                                        uint4 *__restrict__ slots, uint32_t *__restrict__ count,
                                        uint32_t *__restrict__ status, int n, int chunk,
                                        PackOut pk = PackOut{}) {
@@ -474,7 +474,6 @@ __device__ inline void bottom_up_chunk(rta_node *__restrict__ nodes,
 	if (t == 0) nfront = 0;
 	if ((uint32_t)t <= cnt) ldelta[t] = (int8_t)delta_at(codes, (long long)base + t - 1, n);
 	if (t == 0 && cnt == (uint32_t)kChunk) ldelta[kChunk] = (int8_t)delta_at(codes, (long long)end - 1, n);
-	if (k + 1u < (uint32_t)n) visit[k] = 0u;
 	const bool live = k < (uint32_t)n;
 	ClimbNode c;
 	if (live) {
@@ -530,10 +529,10 @@ __device__ inline void bottom_up_chunk(rta_node *__restrict__ nodes,
 			const uint32_t j = s - base;
 			lchild[left ? 0 : 1][j] = c.l == c.r ? (uint16_t)((c.l - base) | kLocalLeaf)
 			                                     : (uint16_t)(node_index(c, leaf0) - base);
-			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
 			const uint32_t old = atomicAdd(&lvisit[j], 1u);
 			if (old == 0u) break;
-			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 			const ClimbNode sib = arrived(j, left ? 1 : 0);
 			const ClimbNode &lc = left ? c : sib;
 			const ClimbNode &rc = left ? sib : c;
@@ -600,17 +599,24 @@ __device__ inline void bottom_up_chunk(rta_node *__restrict__ nodes,
 
 // Level k >= 2 for one mesh: merges the frontier slots of the ng chunks of
 // the previous level starting at in_slots/in_counts (kSlots entries per
-// chunk), whose leaves span [lo, hi); the new frontier goes to out_slots /
-// *out_count when given.  A child arriving first at a split records its far
-// end and the delta there in stash[its index] before the visit counter; the
-// second finds its sibling's node from them.  A first arrival whose sibling
-// never comes (it reaches beyond [lo, hi)) joins the new frontier and its
-// visit is taken back, so that the next level counts its arrival afresh.
-constexpr int kMergeWaits = kMergeFan * kSlots;   // bounds the first arrivals of a block
-__device__ inline void bottom_up_merge(rta_node *__restrict__ nodes, uint32_t *__restrict__ visit,
-                                       uint2 *__restrict__ stash,
-                                       const uint32_t *__restrict__ codes,
-                                       uint2 *__restrict__ ranges,
+// chunk, at most kMergeThreads in all: one thread each), whose leaves span
+// [lo, hi); the new frontier goes to out_slots / *out_count when given.
+// The climbs touch LDS only: every arrival at a split appends its record
+// (far end, the deltas there and at the split, box, leaf state) and swaps
+// its number into the split's slot of an open-addressing table; the thread
+// that finds a number there is the second and forms the parent.  A block
+// sees at most kMergeThreads distinct splits and 2 kMergeThreads arrivals,
+// so the table (half full at most) and the records never run out.  A child whose
+// sibling lies beyond [lo, hi) -- its parent's split is outside, or its
+// slot holds a single arrival at the end -- joins the new frontier.
+constexpr int kMergeThreads = kMergeFan * kSlots;
+constexpr int kTableBits = 11;
+constexpr int kTable = 1 << kTableBits;              // >= 2 kMergeThreads
+constexpr int kRecs = 2 * kMergeThreads;
+constexpr uint32_t kEmpty = 0xffffffffu;             // key: free slot
+constexpr uint32_t kFormed = 0xffffffffu;            // link: both children arrived
+static_assert(kTable >= 2 * kMergeThreads, "merge table at most half full");
+__device__ inline void bottom_up_merge(rta_node *__restrict__ nodes, uint2 *__restrict__ ranges,
                                        const uint4 *__restrict__ in_slots,
                                        const uint32_t *__restrict__ in_counts, int ng,
                                        uint4 *__restrict__ out_slots,
@@ -618,20 +624,32 @@ __device__ inline void bottom_up_merge(rta_node *__restrict__ nodes, uint32_t *_
                                        uint32_t *__restrict__ status, uint32_t lo, uint32_t hi,
                                        int n, PackOut pk = PackOut{}) {
 	__shared__ uint32_t prefix[kMergeFan + 1];
+	__shared__ uint32_t key[kTable];        // the split of a slot
+	__shared__ uint32_t link[kTable];       // 0, the first arrival's record + 1, or kFormed
+	__shared__ float rbox[6][kRecs];
+	__shared__ uint32_t rfar[kRecs];        // far end of the child's range
+	__shared__ uint32_t relems[kRecs];
+	__shared__ uint32_t rtris[kRecs];
+	__shared__ int8_t rdfar[kRecs];         // delta at the far end
+	__shared__ int8_t rdnear[kRecs];        // delta at the split
+	__shared__ uint8_t rleft[kRecs];        // the child is the left one
+	__shared__ uint32_t formed[kMergeThreads];   // inner nodes formed here
 	__shared__ uint4 front[kSlots];
-	__shared__ uint32_t waits[kMergeWaits];   // split | (right child) << 31
-	__shared__ uint32_t formed[kMergeWaits];  // the inner nodes formed here
-	__shared__ uint32_t nfront, nwaits, nformed;
+	__shared__ uint32_t nfront, nrec, nformed;
 	const int t = threadIdx.x;
+	for (int i = t; i < kTable; i += blockDim.x) {
+		key[i] = kEmpty;
+		link[i] = 0u;
+	}
 	if (t == 0) {
 		uint32_t acc = 0;
 		prefix[0] = 0;
 		for (int g = 0; g < ng; ++g) {
-			acc += in_counts[g];
+			acc += min(in_counts[g], (uint32_t)kSlots);
 			prefix[g + 1] = acc;
 		}
 		nfront = 0;
-		nwaits = 0;
+		nrec = 0;
 		nformed = 0;
 	}
 	__syncthreads();
@@ -642,10 +660,10 @@ __device__ inline void bottom_up_merge(rta_node *__restrict__ nodes, uint32_t *_
 		else atomicOr(status, (uint32_t)RTA_LBVH_FRONTIER_OVERFLOW);
 	};
 	const uint32_t total = prefix[ng];
-	for (uint32_t f = t; f < total; f += blockDim.x) {
+	if ((uint32_t)t < total) {
 		int g = 0;
-		while (prefix[g + 1] <= f) ++g;
-		const uint4 e = in_slots[(size_t)g * kSlots + (f - prefix[g])];
+		while (prefix[g + 1] <= (uint32_t)t) ++g;
+		const uint4 e = in_slots[(size_t)g * kSlots + (t - prefix[g])];
 		ClimbNode c;
 		c.l = e.y;
 		c.r = e.z;
@@ -664,49 +682,65 @@ __device__ inline void bottom_up_merge(rta_node *__restrict__ nodes, uint32_t *_
 				push(frontier_entry(c, leaf0));
 				break;
 			}
-			stash[left ? c.r : c.l] = make_uint2(left ? c.l : c.r, (uint32_t)(left ? c.dl : c.dr));
-			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-			const uint32_t old = __hip_atomic_fetch_add(&visit[s], 1u, __ATOMIC_RELAXED,
-			                                            __HIP_MEMORY_SCOPE_WORKGROUP);
-			if (old == 0u) {
-				const uint32_t w = atomicAdd(&nwaits, 1u);
-				if (w < (uint32_t)kMergeWaits) waits[w] = s | (left ? 0u : 0x80000000u);
-				else atomicOr(status, (uint32_t)RTA_LBVH_FRONTIER_OVERFLOW);
-				break;
+			const uint32_t r = atomicAdd(&nrec, 1u);
+			rbox[0][r] = c.box.min.x; rbox[1][r] = c.box.min.y; rbox[2][r] = c.box.min.z;
+			rbox[3][r] = c.box.max.x; rbox[4][r] = c.box.max.y; rbox[5][r] = c.box.max.z;
+			rfar[r] = left ? c.l : c.r;
+			rdfar[r] = (int8_t)(left ? c.dl : c.dr);
+			rdnear[r] = (int8_t)(left ? c.dr : c.dl);
+			relems[r] = c.elems;
+			rtris[r] = c.tris;
+			rleft[r] = left ? 1u : 0u;
+			uint32_t h = (s * 2654435761u) >> (32 - kTableBits);
+			while (true) {
+				const uint32_t k = atomicCAS(&key[h], kEmpty, s);
+				if (k == kEmpty || k == s) break;
+				h = (h + 1u) & (kTable - 1);
 			}
-			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-			const uint32_t sx = left ? s + 1u : s;   // the sibling's index
-			// its record and both nodes it may be (inner node sx, or the
-			// leaf at position sx: then its far end is sx) read together
-			const uint2 st = stash[sx];
-			const rta_node as_inner = nodes[sx], as_leaf = nodes[leaf0 + sx];
-			const int ds = left ? c.dr : c.dl;
+			// LDS only (the node stores need not wait): the record before
+			// the swap, the sibling's record after
+			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+			const uint32_t prev = atomicExch(&link[h], r + 1u);
+			if (prev == 0u) break;   // the sibling forms the parent
+			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+			link[h] = kFormed;
+			const uint32_t q = prev - 1u;
+			const int ds = left ? c.dr : c.dl;   // delta(s)
 			ClimbNode sib;
-			if (left) { sib.l = s + 1u; sib.r = st.x; sib.dl = ds; sib.dr = (int)st.y; }
-			else { sib.l = st.x; sib.r = s; sib.dl = (int)st.y; sib.dr = ds; }
+			sib.box.min = {rbox[0][q], rbox[1][q], rbox[2][q]};
+			sib.box.max = {rbox[3][q], rbox[4][q], rbox[5][q]};
+			sib.elems = relems[q];
+			sib.tris = rtris[q];
+			if (left) { sib.l = s + 1u; sib.r = rfar[q]; sib.dl = ds; sib.dr = rdfar[q]; }
+			else { sib.l = rfar[q]; sib.r = s; sib.dl = rdfar[q]; sib.dr = ds; }
 			if (sib.l < lo || sib.r >= hi || sib.l > sib.r) {
 				// cannot happen for sorted codes; guards the writes below
 				atomicOr(status, (uint32_t)RTA_LBVH_FRONTIER_OVERFLOW);
 				break;
 			}
-			const rta_node &sn = sib.l == sib.r ? as_leaf : as_inner;
-			sib.box = sn.box;
-			node_state(make_uint2(sn.type_left_elems, sn.right_tris), sib.elems, sib.tris);
 			uint32_t x;
 			c = left ? form_parent(c, sib, s, leaf0, nodes, ranges, pk, x)
 			         : form_parent(sib, c, s, leaf0, nodes, ranges, pk, x);
-			if (pk.pairs && c.elems == kStateInner) {
-				const uint32_t i = atomicAdd(&nformed, 1u);
-				if (i < (uint32_t)kMergeWaits) formed[i] = x;
-				else atomicOr(status, (uint32_t)RTA_LBVH_FRONTIER_OVERFLOW);
-			}
+			if (pk.pairs && c.elems == kStateInner) formed[atomicAdd(&nformed, 1u)] = x;
 		}
 	}
 	__syncthreads();
+	// a single arrival at a split: the sibling lies beyond [lo, hi)
+	for (int i = t; i < kTable; i += blockDim.x) {
+		const uint32_t v = link[i];
+		if (v == 0u || v == kFormed) continue;
+		const uint32_t q = v - 1u, s = key[i];
+		const int ds = rdnear[q];
+		ClimbNode w;
+		if (rleft[q]) { w.l = rfar[q]; w.r = s; w.dl = rdfar[q]; w.dr = ds; }
+		else { w.l = s + 1u; w.r = rfar[q]; w.dl = ds; w.dr = rdfar[q]; }
+		push(frontier_entry(w, leaf0));
+	}
 	// pair records of the inner nodes formed here, from the finished nodes
-	// (kept out of the climb: their stores would lengthen every wait on it)
+	// (written by this block's threads before the barrier); inside the climb
+	// their stores held it up
 	bool slow = false;
-	const uint32_t nfm = min(nformed, (uint32_t)kMergeWaits);
+	const uint32_t nfm = min(nformed, (uint32_t)kMergeThreads);
 	for (uint32_t i = t; i < nfm; i += blockDim.x) {
 		const uint32_t x = formed[i];
 		const uint2 links = *reinterpret_cast<const uint2 *>(&nodes[x]);
@@ -717,27 +751,6 @@ __device__ inline void bottom_up_merge(rta_node *__restrict__ nodes, uint32_t *_
 		if (!write_pair(pk.pairs, x, nl.box, nr.box, lw, rw, L, R)) slow = true;
 	}
 	if (slow && pk.flags) atomicOr(pk.flags, 1u);
-	// (none at the last level: every sibling lies within it)
-	const uint32_t nw = out_slots ? min(nwaits, (uint32_t)kMergeWaits) : 0u;
-	for (uint32_t i = t; i < nw; i += blockDim.x) {
-		const uint32_t s = waits[i] & 0x7fffffffu;
-		const bool right = (waits[i] >> 31) != 0u;
-		// the counter (agent scope: where its atomics went), the waiting
-		// child's record and the delta at the split, read together
-		const uint32_t v = __hip_atomic_load(&visit[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-		const uint2 st = stash[right ? s + 1u : s];
-		const int ds = delta_at(codes, s, n);
-		if (v != 1u) continue;   // formed
-		visit[s] = 0u;
-		ClimbNode w;
-		if (right) { w.l = s + 1u; w.r = st.x; w.dl = ds; w.dr = (int)st.y; }
-		else { w.l = st.x; w.r = s; w.dl = (int)st.y; w.dr = ds; }
-		if (w.l < lo || w.r >= hi || w.l > w.r) {
-			atomicOr(status, (uint32_t)RTA_LBVH_FRONTIER_OVERFLOW);
-			continue;
-		}
-		push(frontier_entry(w, leaf0));
-	}
 	__syncthreads();
 	const uint32_t nf = min(nfront, (uint32_t)kSlots);
 	if (out_slots && (uint32_t)t < nf) out_slots[t] = front[t];

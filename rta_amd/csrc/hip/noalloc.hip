@@ -16,7 +16,7 @@
 namespace rta_hip {
 
 // Sized for RTA_LBVH_NOALLOC_MAX_TRIS triangles: rta_lbvh_workspace_bytes
-// gives 46,296,576 bytes at 262,144 triangles (176.6 bytes per triangle);
+// gives 44,199,424 bytes at 262,144 triangles (168.6 bytes per triangle);
 // the first call checks it against the array.
 constexpr size_t kNoAllocScratchBytes = (size_t)46 << 20;
 static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
