@@ -56,6 +56,10 @@ def parse():
     p.add_argument("--refill", type=int, default=16)
     p.add_argument("--steps-per-refill", dest="steps_per_refill", type=int, default=32)
     p.add_argument("--leaf-min", dest="leaf_min", type=int, default=10)
+    p.add_argument("--primary-stack", dest="primary_stack", type=int, default=16,
+                   choices=(16, 24, 32),
+                   help="LDS stack entries of the primary trace (rays that need more are "
+                        "traced again with 32); the bounce trace keeps 32")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--no-legs", dest="no_legs", action="store_true",
                    help="skip the many-mesh and dynamic-frame legs of the default run")
@@ -289,6 +293,7 @@ def dynamic_leg(args) -> dict:
                      "2 diffuse bounces and 1 shadow ray per hit, frames replayed from one "
                      "captured graph"),
         "n_tris": n, "rays_per_frame": counts_last,
+        "primary_stack": fr.primary_config.stack,
         "frame_mrays_s": round(rays / (frame_ms * 1e3), 3),
         "steps": args.steps, "warmup": n_eager,
         "frame_wall_ms": round(wall_ms, 4),
@@ -837,11 +842,21 @@ def main():
     queue = DeviceBuffer(4)
     tcfg = TraceConfig(args.block, args.refill, args.steps_per_refill, 0, args.leaf_min)
 
-    def trace(rays, ray_count, hits, root):
+    # the coherent primary rays gain from a short stack's extra resident
+    # waves; the incoherent bounce rays lose L2 hits with them
+    pcfg = TraceConfig(args.block, args.refill, args.steps_per_refill, 0, args.leaf_min,
+                       args.primary_stack)
+    povf = DeviceBuffer(4 * (R + 1))
+
+    def trace(rays, ray_count, hits, root, primary=False):
         if args.trav == "packed":
             call("rta_trace_cis_packed", ptr(tri4), ptr(pairs), ptr(pflags), root[0], root[1],
                  ptr(rays.gpu_origin), ptr(rays.gpu_direction), ptr(rays.gpu_maxt), R,
                  ptr(ray_count), ptr(hits), ptr(err), sp)
+        elif primary:
+            call("rta_trace_cis_persistent_ovf", ptr(tri4), ptr(pairs), ptr(pflags), root[0],
+                 root[1], ptr(rays.gpu_origin), ptr(rays.gpu_direction), ptr(rays.gpu_maxt), R,
+                 ptr(ray_count), ptr(hits), ptr(err), ptr(queue), C.byref(pcfg), ptr(povf), sp)
         else:
             call("rta_trace_cis_persistent", ptr(tri4), ptr(pairs), ptr(pflags), root[0], root[1],
                  ptr(rays.gpu_origin), ptr(rays.gpu_direction), ptr(rays.gpu_maxt), R,
@@ -853,7 +868,7 @@ def main():
              ptr(prim.gpu_maxt), cam.fovy, cam.aspect, cam.w, cam.h, vd, pos, up, fmax, sp)
         call("rta_reset_intersections", ptr(hits_p), R, 1, sp)
         ev[3].record(s)
-        trace(prim, None, hits_p, root)
+        trace(prim, None, hits_p, root, primary=True)
         ev[4].record(s)
         call("rta_diffuse_bounce", ptr(bnc.gpu_origin), ptr(bnc.gpu_direction), ptr(bnc.gpu_maxt),
              ptr(src), ptr(count), ptr(hits_p), ptr(tris), ptr(prim.gpu_origin),
@@ -1013,7 +1028,9 @@ def main():
             "n_tris": n, "primary_rays": R, "bounce_rays": n_bounce,
             "traversal": (f"child-is, packed 64-B child-pair nodes, LDS stack, {args.trav}"
                           + (f" (block {args.block}, refill {args.refill}, "
-                             f"steps {args.steps_per_refill}, leaf_min {args.leaf_min})"
+                             f"steps {args.steps_per_refill}, leaf_min {args.leaf_min}; "
+                             f"stack 32 for the bounce rays, {args.primary_stack} for the "
+                             f"primary rays)"
                              if args.trav == "persistent" else "")),
         },
         "primary_mrays_s": round(primary_mrays, 3),

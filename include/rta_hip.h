@@ -112,7 +112,8 @@ const char *rta_hip_last_error(void);
  * `status` (non-zero) before doing any work, with the message "<entry>:
  * injected fault"; one shot.  Entry points with this hook: rta_hip_malloc,
  * rta_lbvh_build, rta_lbvh_build_packed, rta_trace_cis_persistent,
- * rta_pred_infer, rta_pred_train_step, rta_pred_train_batches. */
+ * rta_trace_cis_persistent_ovf, rta_pred_infer, rta_pred_train_step,
+ * rta_pred_train_batches. */
 int rta_hip_inject_fault(const char *entry, int status);
 /* Named host-side ranges (roctx) for a profiler's timeline; the library
  * marks rta_lbvh_build and the persistent traversals with them. */
@@ -329,9 +330,10 @@ int rta_trace_cis_packed(const float *tri4, const rta_node_pair *pairs,
  * leaf triangles once at least ceil(leaf_min * active / 64) of its active
  * lanes wait at a leaf, or no lane is at an inner node; 1..64, 0 = default
  * 10) and stack (LDS stack entries: 32, the reference's depth, or 16 or 24
- * for a smaller LDS footprint and more resident waves, batch traversal
- * only: a ray that would need more is traced again with 32 entries; 0 =
- * 32).  None of these changes a result. */
+ * for a smaller LDS footprint and more resident waves, with an overflow
+ * list (the _ovf entry points and the batch API): a ray that would need
+ * more is traced again with 32 entries; 0 = 32).  None of these changes a
+ * result. */
 typedef struct rta_trace_config {
 	int block, refill_min, steps, blocks_per_cu, leaf_min, stack;
 } rta_trace_config;
@@ -370,6 +372,37 @@ int rta_trace_shadow_cis_persistent(const float *tri4,
                                     rta_hit *hits, uint32_t *err,
                                     uint32_t *queue,
                                     const rta_trace_config *cfg, rta_stream s);
+
+/* The two persistent traversals with a short LDS stack (no counterpart in
+ * the reference, whose stack depth is fixed at 32, bbvh-cuda/kernels.cu:
+ * 787): with cfg->stack 16 or 24 a ray that would push beyond it is
+ * abandoned onto `overflow` (device words: a count, then up to n_rays ray
+ * ids; cleared by the call with a kernel) and traced again with 32 entries
+ * by a second launch over that list, so every record equals the 32-entry
+ * traversal's.  Fewer entries mean more resident waves: coherent rays gain,
+ * incoherent ones can lose L2 hits.  With cfg->stack 32 (or cfg NULL) the
+ * list is not used; overflow must not be NULL either way. */
+int rta_trace_cis_persistent_ovf(const float *tri4, const rta_node_pair *pairs,
+                                 const uint32_t *flags, uint32_t root_link,
+                                 uint32_t root_index, const float *ray_orig,
+                                 const float *ray_dir, const float *max_t,
+                                 int n_rays, const uint32_t *ray_count,
+                                 rta_hit *hits, uint32_t *err, uint32_t *queue,
+                                 const rta_trace_config *cfg,
+                                 uint32_t *overflow, rta_stream s);
+int rta_trace_shadow_cis_persistent_ovf(const float *tri4,
+                                        const rta_node_pair *pairs,
+                                        const uint32_t *flags,
+                                        uint32_t root_link,
+                                        uint32_t root_index,
+                                        const float *ray_orig,
+                                        const float *ray_dir,
+                                        const float *max_t, int n_rays,
+                                        const uint32_t *ray_count,
+                                        rta_hit *hits, uint32_t *err,
+                                        uint32_t *queue,
+                                        const rta_trace_config *cfg,
+                                        uint32_t *overflow, rta_stream s);
 
 /* ------------------------------------------------ batches of many meshes */
 

@@ -139,6 +139,10 @@ HIP_SIGNATURES = {
                                     _vp],
     "rta_trace_shadow_cis_persistent": [_vp, _vp, _vp, _u32, _u32, _vp, _vp, _vp, _i, _vp, _vp,
                                         _vp, _vp, _vp, _vp],
+    "rta_trace_cis_persistent_ovf": [_vp, _vp, _vp, _u32, _u32, _vp, _vp, _vp, _i, _vp, _vp,
+                                     _vp, _vp, _vp, _vp, _vp],
+    "rta_trace_shadow_cis_persistent_ovf": [_vp, _vp, _vp, _u32, _u32, _vp, _vp, _vp, _i, _vp,
+                                            _vp, _vp, _vp, _vp, _vp, _vp],
     "rta_bounce_work_bytes": [_i, _psz],
     "rta_diffuse_bounce": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _u64, _vp, _sz, _vp],
     "rta_shadow_rays": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp, _sz, _vp],

@@ -1075,10 +1075,12 @@ static int resolve_config(const char *name, const rta_trace_config *cfg, rta_tra
 	return RTA_OK;
 }
 
-// the single-mesh entry points hold no overflow list: full stack only
-static int require_full_stack(const char *name, const rta_trace_config &c) {
-	if (c.stack == kStack) return RTA_OK;
-	set_error(std::string(name) + ": stack 16 and 24 need the batch API's overflow list");
+// stack 16 and 24 need an overflow list (n_rays + 1 words)
+static int require_full_stack(const char *name, const rta_trace_config &c,
+                              const uint32_t *ovf = nullptr) {
+	if (c.stack == kStack || ovf) return RTA_OK;
+	set_error(std::string(name) + ": stack 16 and 24 need an overflow list (the _ovf entry "
+	                              "points, or the batch API)");
 	return RTA_E_INVALID;
 }
 
@@ -1088,18 +1090,16 @@ int launch_stack(const rta_trace_config &c, const float *tri4, const rta_node_pa
                  const float *rmax, int n_rays, const uint32_t *ray_count, rta_hit *hits,
                  uint32_t *err, uint32_t *queue, const BatchMesh *meshes, int n_meshes,
                  const uint32_t *root_words, uint32_t *ovf, hipStream_t st) {
-	if constexpr (BATCHED) {
-		if (c.stack == 16)
-			return launch_persistent<BLOCK, ANY_HIT, BATCHED, false, 16>(
-			    tri4, pairs, flags, root_word, ro, rd, rmax, n_rays, ray_count, hits, err, queue,
-			    c.refill_min, c.steps, c.blocks_per_cu, c.leaf_min, meshes, n_meshes, root_words,
-			    st, SeedArgs{}, ovf);
-		if (c.stack == 24)
-			return launch_persistent<BLOCK, ANY_HIT, BATCHED, false, 24>(
-			    tri4, pairs, flags, root_word, ro, rd, rmax, n_rays, ray_count, hits, err, queue,
-			    c.refill_min, c.steps, c.blocks_per_cu, c.leaf_min, meshes, n_meshes, root_words,
-			    st, SeedArgs{}, ovf);
-	}
+	if (c.stack == 16)
+		return launch_persistent<BLOCK, ANY_HIT, BATCHED, false, 16>(
+		    tri4, pairs, flags, root_word, ro, rd, rmax, n_rays, ray_count, hits, err, queue,
+		    c.refill_min, c.steps, c.blocks_per_cu, c.leaf_min, meshes, n_meshes, root_words, st,
+		    SeedArgs{}, ovf);
+	if (c.stack == 24)
+		return launch_persistent<BLOCK, ANY_HIT, BATCHED, false, 24>(
+		    tri4, pairs, flags, root_word, ro, rd, rmax, n_rays, ray_count, hits, err, queue,
+		    c.refill_min, c.steps, c.blocks_per_cu, c.leaf_min, meshes, n_meshes, root_words, st,
+		    SeedArgs{}, ovf);
 	return launch_persistent<BLOCK, ANY_HIT, BATCHED>(
 	    tri4, pairs, flags, root_word, ro, rd, rmax, n_rays, ray_count, hits, err, queue,
 	    c.refill_min, c.steps, c.blocks_per_cu, c.leaf_min, meshes, n_meshes, root_words, st);
@@ -1154,7 +1154,7 @@ int trace_persistent(const char *name, const float *tri4, const rta_node_pair *p
                      const uint32_t *flags, uint32_t root_link, uint32_t root_index,
                      const float *ray_orig, const float *ray_dir, const float *max_t, int n_rays,
                      const uint32_t *ray_count, rta_hit *hits, uint32_t *err, uint32_t *queue,
-                     const rta_trace_config *cfg, hipStream_t st) {
+                     const rta_trace_config *cfg, hipStream_t st, uint32_t *ovf = nullptr) {
 	const ScopedRange range(name);
 	if (!tri4 || !pairs || !flags || !ray_orig || !ray_dir || !max_t || !hits || !err ||
 	    !queue || n_rays <= 0) {
@@ -1163,12 +1163,13 @@ int trace_persistent(const char *name, const float *tri4, const rta_node_pair *p
 	}
 	rta_trace_config c;
 	int rc = resolve_config(name, cfg, c);
-	if (rc == RTA_OK) rc = require_full_stack(name, c);
+	if (rc == RTA_OK) rc = require_full_stack(name, c, ovf);
 	if (rc != RTA_OK) return rc;
 	const uint32_t root_word = link_inner(root_link) ? 1u : (root_index << 1);
 	return launch_persistent_cfg<ANY_HIT, false>(c, tri4, pairs, flags, root_word, ray_orig,
 	                                             ray_dir, max_t, n_rays, ray_count, hits, err,
-	                                             queue, nullptr, 0, nullptr, st);
+	                                             queue, nullptr, 0, nullptr, st,
+	                                             c.stack < kStack ? ovf : nullptr);
 }
 
 }  // namespace rta_hip
@@ -1282,6 +1283,32 @@ int rta_trace_cis_persistent(const float *tri4, const rta_node_pair *pairs,
 	return trace_persistent<false>("rta_trace_cis_persistent", tri4, pairs, flags, root_link,
 	                               root_index, ray_orig, ray_dir, max_t, n_rays, ray_count, hits,
 	                               err, queue, cfg, as_stream(s));
+}
+
+int rta_trace_cis_persistent_ovf(const float *tri4, const rta_node_pair *pairs,
+                                 const uint32_t *flags, uint32_t root_link, uint32_t root_index,
+                                 const float *ray_orig, const float *ray_dir, const float *max_t,
+                                 int n_rays, const uint32_t *ray_count, rta_hit *hits,
+                                 uint32_t *err, uint32_t *queue, const rta_trace_config *cfg,
+                                 uint32_t *overflow, rta_stream s) {
+	RTA_FAULT_POINT("rta_trace_cis_persistent_ovf");
+	if (!overflow) return invalid("rta_trace_cis_persistent_ovf: NULL overflow list");
+	return trace_persistent<false>("rta_trace_cis_persistent_ovf", tri4, pairs, flags, root_link,
+	                               root_index, ray_orig, ray_dir, max_t, n_rays, ray_count, hits,
+	                               err, queue, cfg, as_stream(s), overflow);
+}
+
+int rta_trace_shadow_cis_persistent_ovf(const float *tri4, const rta_node_pair *pairs,
+                                        const uint32_t *flags, uint32_t root_link,
+                                        uint32_t root_index, const float *ray_orig,
+                                        const float *ray_dir, const float *max_t, int n_rays,
+                                        const uint32_t *ray_count, rta_hit *hits, uint32_t *err,
+                                        uint32_t *queue, const rta_trace_config *cfg,
+                                        uint32_t *overflow, rta_stream s) {
+	if (!overflow) return invalid("rta_trace_shadow_cis_persistent_ovf: NULL overflow list");
+	return trace_persistent<true>("rta_trace_shadow_cis_persistent_ovf", tri4, pairs, flags,
+	                              root_link, root_index, ray_orig, ray_dir, max_t, n_rays,
+	                              ray_count, hits, err, queue, cfg, as_stream(s), overflow);
 }
 
 int rta_trace_cis_persistent_seeded(const float *tri4, const rta_node_pair *pairs,

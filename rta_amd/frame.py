@@ -118,17 +118,24 @@ class OnlineFineTune:
 
 
 class DynamicFrame:
-    """Buffers and launch sequence of one frame of the dynamic scene."""
+    """Buffers and launch sequence of one frame of the dynamic scene.
+    primary_stack: LDS stack entries of the primary trace (16, 24 or 32;
+    rays that need more are traced again with 32 entries, so the records do
+    not change; coherent rays can gain from a short stack's extra resident
+    waves, the dynamic scene's measured none); the other traces use
+    config's stack."""
 
     def __init__(self, verts: np.ndarray, idx: np.ndarray, camera, light, sigma: float,
                  seed: int, bounces: int = 2, shadows: bool = True,
-                 config: TraceConfig | None = None, stream: Stream | None = None):
+                 config: TraceConfig | None = None, stream: Stream | None = None,
+                 primary_stack: int = 32):
         self.n = len(idx) // 3
         if self.n < 2:
             raise ValueError("the dynamic frame needs at least two triangles")
         self.cam, self.light = camera, tuple(float(x) for x in light)
         self.sigma, self.seed, self.bounces, self.shadows = sigma, seed, bounces, shadows
         self.config = config if config is not None else TraceConfig(64, 16, 32, 0, 10)
+        self.primary_config = TraceConfig(*self.config.as_tuple()[:5], primary_stack)
         self.stream = stream if stream is not None else Stream()
         n, R = self.n, camera.w * camera.h
         self.R = R
@@ -157,6 +164,7 @@ class DynamicFrame:
         self.err = DeviceBuffer(4)
         self.err.fill_u32(0)
         self.queue = DeviceBuffer(4)
+        self.overflow = DeviceBuffer(4 * (R + 1))   # short-stack rays traced again
         self.frame_word = DeviceBuffer(4)
         self.frame_word.fill_u32(0)
         self.counts[0].fill_u32(R)
@@ -170,13 +178,15 @@ class DynamicFrame:
     # ------------------------------------------------------------ stages
 
     def _trace(self, rays: RayBuffers, count: DeviceBuffer, hits: DeviceBuffer,
-               shadow: bool) -> None:
+               shadow: bool, config: TraceConfig | None = None) -> None:
         s = ptr(self.stream)
         call("rta_reset_intersections", ptr(hits), self.R, 1, s)
-        entry = "rta_trace_shadow_cis_persistent" if shadow else "rta_trace_cis_persistent"
+        entry = "rta_trace_shadow_cis_persistent_ovf" if shadow else "rta_trace_cis_persistent_ovf"
+        cfg = config if config is not None else self.config
         call(entry, ptr(self.tri4), ptr(self.pairs), ptr(self.flags), self._root[0], self._root[1],
              ptr(rays.gpu_origin), ptr(rays.gpu_direction), ptr(rays.gpu_maxt), self.R,
-             ptr(count), ptr(hits), ptr(self.err), ptr(self.queue), C.byref(self.config), s)
+             ptr(count), ptr(hits), ptr(self.err), ptr(self.queue), C.byref(cfg),
+             ptr(self.overflow), s)
 
     def _mark(self, timed: bool) -> None:
         if timed:
@@ -207,7 +217,8 @@ class DynamicFrame:
         call("rta_setup_shirley_rays", ptr(p.gpu_direction), ptr(p.gpu_origin), ptr(p.gpu_maxt),
              cam.fovy, cam.aspect, cam.w, cam.h, f3(cam.view_dir), f3(cam.pos), f3(cam.up),
              float(FLT_MAX), s)
-        self._trace(self.rays[0], self.counts[0], self.hits[0], shadow=False)
+        self._trace(self.rays[0], self.counts[0], self.hits[0], shadow=False,
+                    config=self.primary_config)
         self._mark(timed)
         for k in range(self.bounces + 1):
             ft = self.finetune

@@ -231,8 +231,9 @@ def test_batch_frame_graph_replays_equal_eager(gpu):
 
 
 def test_short_stack_rejected_outside_batches():
-    """The single-mesh tracers hold no overflow list: stack 16 or 24 is an
-    argument error there, and 20 anywhere (checked before any device work)."""
+    """rta_trace_cis_persistent holds no overflow list (its _ovf form does):
+    stack 16 or 24 is an argument error there, and 20 anywhere (checked
+    before any device work)."""
     import ctypes as C
     from rta_amd.abi import RTA_E_INVALID, TraceConfig
     lib = rta_amd.hip()
