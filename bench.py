@@ -146,10 +146,9 @@ def batch_leg(args) -> dict:
         for m, q in enumerate(per):
             call("rta_hip_memcpy_dtod", ptr(q["src"]), C.c_void_p(b.pristine.ptr + 104 * int(b.tri_off[m])),
                  C.c_size_t(104 * q["n"]), sp)
-            call("rta_lbvh_build", ptr(q["src"]), q["n"], ptr(q["nodes"]), ptr(q["idx"]), ptr(ws),
-                 C.c_size_t(sz.value), None, None, sp)
-            call("rta_bvh_pack", ptr(q["pairs"]), ptr(q["tri4"]), ptr(q["flags"]), ptr(q["nodes"]),
-                 2 * q["n"] - 1, ptr(q["src"]), q["n"], sp)
+            call("rta_lbvh_build_packed", ptr(q["src"]), q["n"], ptr(q["nodes"]), ptr(q["idx"]),
+                 ptr(q["pairs"]), ptr(q["tri4"]), ptr(q["flags"]), ptr(ws), C.c_size_t(sz.value),
+                 None, None, sp)
 
     build_each()
     s.synchronize()
@@ -195,7 +194,7 @@ def batch_leg(args) -> dict:
                                     "leaf_min", "stack"],
         "short_stack_rays_retraced": ovf,
         "graph_frame_ms": round(graph_ms, 4),
-        "graph_frame": "restore + build + pack + reset + trace captured as one graph",
+        "graph_frame": "restore + build with the traversal layout + reset + trace captured as one graph",
         "default_config_trace_ms": round(default_ms, 4),
         "per_mesh_launches_build_ms": round(each_build_ms, 4),
         "per_mesh_launches_trace_ms": round(each_trace_ms, 4),

@@ -436,6 +436,14 @@ int rta_batch_build(rta_batch batch, rta_triangle *tris, rta_node *nodes, uint32
 /* rta_bvh_pack of every mesh; also records each mesh's root word. */
 int rta_batch_pack(rta_batch batch, rta_node_pair *pairs, float *tri4, uint32_t *flags,
                    const rta_node *nodes, const rta_triangle *tris, rta_stream s);
+/* rta_batch_build with the traversal layout written by the build's own
+ * passes (as rta_lbvh_build_packed does for one mesh): the same nodes,
+ * indices, triangles and status, and the same tri4, flags, root words and
+ * inner nodes' pair records as rta_batch_pack after rta_batch_build. */
+int rta_batch_build_packed(rta_batch batch, rta_triangle *tris, rta_node *nodes,
+                           uint32_t *indices, rta_node_pair *pairs, float *tri4,
+                           uint32_t *flags, void *workspace, size_t workspace_bytes,
+                           uint32_t *status, rta_stream s);
 /* Closest-hit (any_hit = 0) or shadow (any_hit = 1) traversal of all rays
  * of all meshes in one persistent launch; results as the single-mesh
  * persistent forms.  Reset the hits first. */

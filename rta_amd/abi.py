@@ -154,6 +154,7 @@ HIP_SIGNATURES = {
     "rta_batch_workspace_bytes": [_vp, _psz],
     "rta_batch_build": [_vp, _vp, _vp, _vp, _vp, _sz, _vp, _vp],
     "rta_batch_pack": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "rta_batch_build_packed": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp, _vp],
     "rta_batch_trace_persistent": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp,
                                    _vp],
     "rta_batch_overflow_count": [_vp, _vp, _vp],

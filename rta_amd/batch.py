@@ -66,11 +66,18 @@ class MeshBatch:
                 pass
             self.handle = None
 
-    def build(self) -> None:
-        """Restore the input triangles, build every mesh and pack them."""
+    def build(self, packed: bool = True) -> None:
+        """Restore the input triangles and build every mesh with its
+        traversal layout: in the build's own passes (rta_batch_build_packed),
+        or with packed=False as a build followed by rta_batch_pack."""
         s = ptr(self.stream)
         call("rta_hip_memcpy_dtod", ptr(self.tris), ptr(self.pristine),
              C.c_size_t(104 * self.total_tris), s)
+        if packed:
+            call("rta_batch_build_packed", self.handle, ptr(self.tris), ptr(self.nodes),
+                 ptr(self.indices), ptr(self.pairs), ptr(self.tri4), ptr(self.flags),
+                 ptr(self.ws), C.c_size_t(self.ws_bytes), ptr(self.status), s)
+            return
         call("rta_batch_build", self.handle, ptr(self.tris), ptr(self.nodes), ptr(self.indices),
              ptr(self.ws), C.c_size_t(self.ws_bytes), ptr(self.status), s)
         call("rta_batch_pack", self.handle, ptr(self.pairs), ptr(self.tri4), ptr(self.flags),
@@ -100,8 +107,9 @@ class MeshBatch:
         return best, ms.value
 
     def capture(self, any_hit: bool = False, config: TraceConfig | None = None):
-        """The whole batch frame -- restore the triangles, build and pack
-        every mesh, reset the hits, trace all rays -- captured as one graph
+        """The whole batch frame -- restore the triangles, build every mesh
+        with its traversal layout, reset the hits, trace all rays -- captured
+        as one graph
         (rta_hip_graph_*), replayed with .launch(self.stream)."""
         from .frame import Graph
 

@@ -60,18 +60,31 @@ __global__ __launch_bounds__(256) void batch_local_index_kernel(uint32_t *__rest
 	if (i < n) idx[i] &= 0xffffffu;
 }
 
+// The traversal layout of one mesh of the batch (pairs, tri4 and flags
+// are the batch's arrays; none when pairs is null).
+__device__ inline PackOut mesh_pack(const BatchMesh &m, uint32_t mesh, rta_node_pair *pairs,
+                                    float4 *tri4, uint32_t *flags) {
+	PackOut pk;
+	if (pairs) {
+		pk.pairs = pairs + m.node_off;
+		pk.tri4 = tri4 + 3 * (size_t)m.tri_off;
+		pk.flags = flags + mesh;
+	}
+	return pk;
+}
+
 __global__ __launch_bounds__(kChunk) void batch_chunk_kernel(
     const BatchItem *__restrict__ items, const BatchMesh *__restrict__ meshes,
     rta_node *__restrict__ nodes, const rta_aabb *__restrict__ boxes,
     const uint32_t *__restrict__ codes, const uint32_t *__restrict__ indices,
     uint2 *__restrict__ ranges, uint4 *__restrict__ slots, uint32_t *__restrict__ counts,
-    uint32_t *__restrict__ status) {
+    uint32_t *__restrict__ status, rta_node_pair *pairs, float4 *tri4, uint32_t *flags) {
 	const BatchItem it = items[blockIdx.x];
 	const BatchMesh m = meshes[it.mesh];
 	bottom_up_chunk(nodes + m.node_off, boxes + m.tri_off, codes + m.tri_off, indices + m.tri_off,
 	                ranges + m.tri_off,
 	                slots + (size_t)(m.chunk_off + it.block) * kSlots, counts + m.chunk_off + it.block,
-	                status, (int)m.n, (int)it.block);
+	                status, (int)m.n, (int)it.block, mesh_pack(m, it.mesh, pairs, tri4, flags));
 }
 
 // Merge level `level` (>= 1): level-`level` groups of a mesh cover
@@ -81,7 +94,8 @@ __global__ __launch_bounds__(kMergeThreads) void batch_merge_kernel(
     rta_node *__restrict__ nodes, uint2 *__restrict__ ranges,
     const uint4 *__restrict__ in_slots, const uint32_t *__restrict__ in_counts,
     uint4 *__restrict__ out_slots, uint32_t *__restrict__ out_counts,
-    uint32_t *__restrict__ status, long long width) {
+    uint32_t *__restrict__ status, long long width, rta_node_pair *pairs, float4 *tri4,
+    uint32_t *flags) {
 	const BatchItem it = items[blockIdx.x];
 	const BatchMesh m = meshes[it.mesh];
 	const uint32_t g0 = it.block * kMergeFan;
@@ -93,17 +107,27 @@ __global__ __launch_bounds__(kMergeThreads) void batch_merge_kernel(
 	                in_slots + (size_t)(m.chunk_off + g0) * kSlots, in_counts + m.chunk_off + g0, ng,
 	                last ? nullptr : out_slots + (size_t)(m.chunk_off + it.block) * kSlots,
 	                last ? nullptr : out_counts + m.chunk_off + it.block, status, lo, hi,
-	                (int)m.n);
+	                (int)m.n, mesh_pack(m, it.mesh, pairs, tri4, flags));
 }
 
 __global__ __launch_bounds__(kPermuteBlock) void batch_permute_kernel(
     const BatchItem *__restrict__ items, const BatchMesh *__restrict__ meshes,
     rta_triangle *__restrict__ out, const rta_triangle *__restrict__ scratch,
-    const uint32_t *__restrict__ indices) {
+    const uint32_t *__restrict__ indices, float4 *__restrict__ tri4) {
 	const BatchItem it = items[blockIdx.x];
 	const BatchMesh m = meshes[it.mesh];
 	permute_block(out + m.tri_off, scratch + m.tri_off, indices + m.tri_off, (int)m.n,
-	              (int)it.block);
+	              (int)it.block, tri4 ? tri4 + 3 * (size_t)m.tri_off : nullptr);
+}
+
+// each mesh's root as its parent's pair record would describe it
+__global__ void batch_root_words_kernel(const BatchMesh *__restrict__ meshes, int n_meshes,
+                                        const rta_node *__restrict__ nodes,
+                                        uint32_t *__restrict__ root_words) {
+	const int m = blockIdx.x * blockDim.x + threadIdx.x;
+	if (m >= n_meshes) return;
+	const rta_node r = nodes[meshes[m].node_off];
+	root_words[m] = link_inner(r.type_left_elems) ? 1u : (r.right_tris << 1);
 }
 
 static inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
@@ -290,14 +314,20 @@ int rta_batch_workspace_bytes(rta_batch b, size_t *bytes) {
 	return RTA_OK;
 }
 
-int rta_batch_build(rta_batch b, rta_triangle *tris, rta_node *nodes, uint32_t *indices,
-                    void *workspace, size_t workspace_bytes, uint32_t *status, rta_stream s) {
+}  // extern "C"
+
+// The stages of rta_batch_build; with pairs set, the traversal layout too
+// (the permutation before the bottom-up pass writes tri4, the bottom-up
+// pass the pair records and leaf-end marks, then each mesh's root word).
+static int batch_build(const char *name, rta_batch b, rta_triangle *tris, rta_node *nodes,
+                       uint32_t *indices, rta_node_pair *pairs, float4 *tri4, uint32_t *flags,
+                       void *workspace, size_t workspace_bytes, uint32_t *status, rta_stream s) {
 	if (!b || !tris || !nodes || !indices || !workspace)
-		return invalid("rta_batch_build: NULL argument");
-	const BatchPlan &p = *(const BatchPlan *)b;
+		return invalid((std::string(name) + ": NULL argument").c_str());
+	BatchPlan &p = *(BatchPlan *)b;
 	BatchWork w = carve_batch(workspace, p);
 	if (workspace_bytes < w.total) {
-		set_error("rta_batch_build: workspace too small");
+		set_error(std::string(name) + ": workspace too small");
 		return RTA_E_WORKSPACE;
 	}
 	hipStream_t st = as_stream(s);
@@ -307,7 +337,8 @@ int rta_batch_build(rta_batch b, rta_triangle *tris, rta_node *nodes, uint32_t *
 	// word, or into the workspace's own when there is none
 	uint32_t *bu_status = status ? status : w.bu_status;
 	{
-		const int r = rta_hip_fill_u32(bu_status, 0u, 1, s);
+		int r = rta_hip_fill_u32(bu_status, 0u, 1, s);
+		if (r == RTA_OK && pairs) r = rta_hip_fill_u32(flags, 0u, (size_t)p.n_meshes, s);
 		if (r != RTA_OK) return r;
 	}
 	hipLaunchKernelGGL(batch_init_kernel, dim3((unsigned)p.tri_blocks), dim3(kInitBlock), 0, st,
@@ -328,10 +359,20 @@ int rta_batch_build(rta_batch b, rta_triangle *tris, rta_node *nodes, uint32_t *
 	hipLaunchKernelGGL(batch_local_index_kernel, dim3(grid_for(N, 256)), dim3(256), 0, st,
 	                   indices, (long long)N);
 	RTA_LAUNCH_CHECK("batch_local_index_kernel");
+	auto permute = [&]() -> int {
+		hipLaunchKernelGGL(batch_permute_kernel, dim3((unsigned)p.tri_blocks), dim3(256), 0, st,
+		                   items + p.tri_blocks_off, p.d_meshes, tris, w.scratch, indices, tri4);
+		RTA_LAUNCH_CHECK("batch_permute_kernel");
+		return RTA_OK;
+	};
+	if (pairs) {
+		rc = permute();
+		if (rc != RTA_OK) return rc;
+	}
 	// the bottom-up pass emits the hierarchy too (lbvh_body.hpp)
 	hipLaunchKernelGGL(batch_chunk_kernel, dim3((unsigned)p.chunk_items), dim3(kChunk), 0, st,
 	                   items + p.chunk_items_off, p.d_meshes, nodes, w.boxes, w.codes, indices,
-	                   w.ranges, w.slots[0], w.counts[0], bu_status);
+	                   w.ranges, w.slots[0], w.counts[0], bu_status, pairs, tri4, flags);
 	RTA_LAUNCH_CHECK("batch_chunk_kernel");
 	long long width = kChunk;
 	for (size_t L = 0; L < p.merge_off.size(); ++L) {
@@ -340,13 +381,33 @@ int rta_batch_build(rta_batch b, rta_triangle *tris, rta_node *nodes, uint32_t *
 		hipLaunchKernelGGL(batch_merge_kernel, dim3((unsigned)p.merge_len[L]), dim3(kMergeThreads), 0,
 		                   st, items + p.merge_off[L], p.d_meshes, nodes,
 		                   w.ranges, w.slots[in], w.counts[in], w.slots[outb], w.counts[outb],
-		                   bu_status, width);
+		                   bu_status, width, pairs, tri4, flags);
 		RTA_LAUNCH_CHECK("batch_merge_kernel");
 	}
-	hipLaunchKernelGGL(batch_permute_kernel, dim3((unsigned)p.tri_blocks), dim3(256), 0, st,
-	                   items + p.tri_blocks_off, p.d_meshes, tris, w.scratch, indices);
-	RTA_LAUNCH_CHECK("batch_permute_kernel");
-	return RTA_OK;
+	if (pairs) {
+		hipLaunchKernelGGL(batch_root_words_kernel, dim3(grid_for(p.n_meshes, 64)), dim3(64), 0, st,
+		                   p.d_meshes, p.n_meshes, nodes, p.d_root_words);
+		RTA_LAUNCH_CHECK("batch_root_words_kernel");
+		return RTA_OK;
+	}
+	return permute();
+}
+
+extern "C" {
+
+int rta_batch_build(rta_batch b, rta_triangle *tris, rta_node *nodes, uint32_t *indices,
+                    void *workspace, size_t workspace_bytes, uint32_t *status, rta_stream s) {
+	return batch_build("rta_batch_build", b, tris, nodes, indices, nullptr, nullptr, nullptr,
+	                   workspace, workspace_bytes, status, s);
+}
+
+int rta_batch_build_packed(rta_batch b, rta_triangle *tris, rta_node *nodes, uint32_t *indices,
+                           rta_node_pair *pairs, float *tri4, uint32_t *flags, void *workspace,
+                           size_t workspace_bytes, uint32_t *status, rta_stream s) {
+	if (!pairs || !tri4 || !flags) return invalid("rta_batch_build_packed: NULL layout pointer");
+	return batch_build("rta_batch_build_packed", b, tris, nodes, indices, pairs,
+	                   reinterpret_cast<float4 *>(tri4), flags, workspace, workspace_bytes, status,
+	                   s);
 }
 
 }  // extern "C"

@@ -90,6 +90,45 @@ def test_batch_build_and_trace_bit_exact(gpu):
 
 
 @pytest.mark.gpu
+def test_batch_build_packed_equals_build_then_pack(gpu):
+    """rta_batch_build_packed writes the traversal layout in the build's own
+    passes: nodes, indices, reordered triangles and status equal
+    rta_batch_build's; tri4, the per-mesh flags and every inner node's pair
+    record equal rta_batch_pack's; and the hits (which also read the
+    per-mesh root words) are identical."""
+    from rta_amd.abi import NODE_DTYPE
+    meshes, rays = _edge_batch()
+    out = []
+    for packed in (False, True):
+        b = MeshBatch(meshes, rays)
+        b.tri4.fill_u32(0x7fc00000)           # every tri4 word must be written
+        b.flags.fill_u32(0xffffffff)          # and every flag cleared
+        b.stream.synchronize()
+        b.build(packed=packed)
+        b.trace()
+        b.stream.synchronize()
+        b.check_errors()
+        out.append({
+            "nodes": b.nodes.download(NODE_DTYPE, b.total_nodes),
+            "indices": b.indices.download(np.uint32, b.total_tris),
+            "tris": b.tris.download(np.uint8, 104 * b.total_tris),
+            "status": int(b.status.download(np.uint32, 1)[0]),
+            "tri4": b.tri4.download(np.uint8, 48 * b.total_tris),
+            "flags": b.flags.download(np.uint32, b.M),
+            "pairs": b.pairs.download(np.uint8, 64 * b.total_nodes).reshape(-1, 64),
+            "hits": b.hits.download(np.uint8, 16 * b.total_rays),
+        })
+    a, c = out
+    assert a["nodes"].tobytes() == c["nodes"].tobytes()
+    for k in ("indices", "tris", "tri4", "flags", "hits"):
+        assert np.array_equal(a[k], c[k]), k
+    assert a["status"] == c["status"]
+    inner = (a["nodes"]["type_left_elems"] & 1) == 1
+    assert np.array_equal(a["pairs"][inner], c["pairs"][inner])
+    say(f"packed batch build equals build + pack ({int(inner.sum())} pair records)")
+
+
+@pytest.mark.gpu
 def test_batch_shadow_trace_bit_exact(gpu):
     meshes, rays = _edge_batch()
     b = MeshBatch(meshes, rays)
