@@ -60,6 +60,9 @@ def parse():
                    choices=(16, 24, 32),
                    help="LDS stack entries of the primary trace (rays that need more are "
                         "traced again with 32); the bounce trace keeps 32")
+    p.add_argument("--serial-shadows", dest="serial_shadows", action="store_true",
+                   help="dynamic frame: trace each level's shadow rays on the frame's stream "
+                        "instead of a second stream beside the next bounce")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--no-legs", dest="no_legs", action="store_true",
                    help="skip the many-mesh and dynamic-frame legs of the default run")
@@ -247,9 +250,10 @@ def dynamic_leg(args) -> dict:
         f"({time.perf_counter() - t0:.1f} s)")
     cfg = TraceConfig(args.block, args.refill, args.steps_per_refill, 0, args.leaf_min)
     fr = DynamicFrame(v, i, cam, scenes.dynamic_light(args.instances), scenes.DYNAMIC_SIGMA,
-                      scenes.DYNAMIC_SEED, bounces=2, shadows=True, config=cfg)
+                      scenes.DYNAMIC_SEED, bounces=2, shadows=True, config=cfg,
+                      concurrent_shadows=not args.serial_shadows)
     fr.prime()
-    # eager frames with stage events: the per-stage breakdown
+    # eager frames with stage events (stages in sequence): the per-stage breakdown
     stage_names = ["jitter", "build+pack", "primary", "shadow0", "bounce1", "shadow1", "bounce2",
                    "shadow2"]
     stage_ms = {k: 0.0 for k in stage_names}
@@ -293,6 +297,7 @@ def dynamic_leg(args) -> dict:
                      "captured graph"),
         "n_tris": n, "rays_per_frame": counts_last,
         "primary_stack": fr.primary_config.stack,
+        "concurrent_shadows": fr.concurrent,
         "frame_mrays_s": round(rays / (frame_ms * 1e3), 3),
         "steps": args.steps, "warmup": n_eager,
         "frame_wall_ms": round(wall_ms, 4),

@@ -138,6 +138,9 @@ int rta_hip_device_synchronize(void);
 int rta_hip_event_create(rta_event *e);
 int rta_hip_event_destroy(rta_event e);
 int rta_hip_event_record(rta_event e, rta_stream s);
+/* Work enqueued on s after this call waits for the event's last record
+ * (hipStreamWaitEvent); inside a capture it joins s to the capture. */
+int rta_hip_stream_wait_event(rta_stream s, rta_event e);
 int rta_hip_event_synchronize(rta_event e);
 int rta_hip_event_elapsed_ms(float *ms, rta_event start, rta_event stop);
 /* Stream capture into a replayable graph (hipStreamBeginCapture, thread-local

@@ -101,6 +101,7 @@ HIP_SIGNATURES = {
     "rta_hip_event_create": [C.POINTER(_vp)],
     "rta_hip_event_destroy": [_vp],
     "rta_hip_event_record": [_vp, _vp],
+    "rta_hip_stream_wait_event": [_vp, _vp],
     "rta_hip_event_synchronize": [_vp],
     "rta_hip_event_elapsed_ms": [_pf, _vp, _vp],
     "rta_hip_copy_kernel": [_vp, _vp, _sz, _vp],

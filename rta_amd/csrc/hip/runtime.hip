@@ -220,6 +220,12 @@ int rta_hip_event_record(rta_event e, rta_stream s) {
 	return RTA_OK;
 }
 
+int rta_hip_stream_wait_event(rta_stream s, rta_event e) {
+	if (!e) return invalid("rta_hip_stream_wait_event: NULL event");
+	RTA_HIP_TRY(hipStreamWaitEvent(as_stream(s), (hipEvent_t)e, 0));
+	return RTA_OK;
+}
+
 int rta_hip_event_synchronize(rta_event e) {
 	if (!e) return invalid("rta_hip_event_synchronize: NULL event");
 	RTA_HIP_TRY(hipEventSynchronize((hipEvent_t)e));

@@ -86,6 +86,10 @@ class Stream:
     def synchronize(self) -> None:
         call("rta_hip_stream_synchronize", C.c_void_p(self.ptr))
 
+    def wait(self, event: "Event") -> None:
+        """Work enqueued after this waits for the event's last record."""
+        call("rta_hip_stream_wait_event", C.c_void_p(self.ptr), C.c_void_p(event.ptr))
+
     def __del__(self):
         try:
             if self.ptr:

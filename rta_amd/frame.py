@@ -123,12 +123,15 @@ class DynamicFrame:
     rays that need more are traced again with 32 entries, so the records do
     not change; coherent rays can gain from a short stack's extra resident
     waves, the dynamic scene's measured none); the other traces use
-    config's stack."""
+    config's stack.  concurrent_shadows: the shadow rays of each level are
+    generated and traced on a second stream, beside the next bounce (both
+    only read that level's hits), except in timed frames, whose per-stage
+    events need the stages in sequence."""
 
     def __init__(self, verts: np.ndarray, idx: np.ndarray, camera, light, sigma: float,
                  seed: int, bounces: int = 2, shadows: bool = True,
                  config: TraceConfig | None = None, stream: Stream | None = None,
-                 primary_stack: int = 32):
+                 primary_stack: int = 32, concurrent_shadows: bool = True):
         self.n = len(idx) // 3
         if self.n < 2:
             raise ValueError("the dynamic frame needs at least two triangles")
@@ -167,6 +170,16 @@ class DynamicFrame:
         self.overflow = DeviceBuffer(4 * (R + 1))   # short-stack rays traced again
         self.frame_word = DeviceBuffer(4)
         self.frame_word.fill_u32(0)
+        # the second stream of the shadow traces, with its own scratch
+        self.concurrent = bool(concurrent_shadows and shadows)
+        if self.concurrent:
+            self.side = Stream()
+            self.side_queue = DeviceBuffer(4)
+            self.side_overflow = DeviceBuffer(4 * (R + 1))
+            self.side_src = DeviceBuffer(4 * R)
+            self.side_bwork = DeviceBuffer(self.bwork_bytes)
+            self.ev_ready = [Event() for _ in range(bounces + 1)]
+            self.ev_side_done = Event()
         self.counts[0].fill_u32(R)
         self.events: list[Event] = []
         self._root = None
@@ -178,15 +191,26 @@ class DynamicFrame:
     # ------------------------------------------------------------ stages
 
     def _trace(self, rays: RayBuffers, count: DeviceBuffer, hits: DeviceBuffer,
-               shadow: bool, config: TraceConfig | None = None) -> None:
-        s = ptr(self.stream)
+               shadow: bool, config: TraceConfig | None = None, side: bool = False) -> None:
+        s = ptr(self.side if side else self.stream)
         call("rta_reset_intersections", ptr(hits), self.R, 1, s)
         entry = "rta_trace_shadow_cis_persistent_ovf" if shadow else "rta_trace_cis_persistent_ovf"
         cfg = config if config is not None else self.config
+        queue, overflow = ((self.side_queue, self.side_overflow) if side
+                           else (self.queue, self.overflow))
         call(entry, ptr(self.tri4), ptr(self.pairs), ptr(self.flags), self._root[0], self._root[1],
              ptr(rays.gpu_origin), ptr(rays.gpu_direction), ptr(rays.gpu_maxt), self.R,
-             ptr(count), ptr(hits), ptr(self.err), ptr(self.queue), C.byref(cfg),
-             ptr(self.overflow), s)
+             ptr(count), ptr(hits), ptr(self.err), ptr(queue), C.byref(cfg), ptr(overflow), s)
+
+    def _shadows(self, k: int, side: bool) -> None:
+        """Shadow rays of level k from its hits, and their trace."""
+        s = ptr(self.side if side else self.stream)
+        src, bwork = (self.side_src, self.side_bwork) if side else (self.src, self.bwork)
+        call("rta_shadow_rays", ptr(self.srays[k].gpu_origin), ptr(self.srays[k].gpu_direction),
+             ptr(self.srays[k].gpu_maxt), ptr(src), ptr(self.scounts[k]), ptr(self.hits[k]),
+             ptr(self.tris), ptr(self.rays[k].gpu_origin), ptr(self.rays[k].gpu_direction),
+             self.R, (C.c_float * 3)(*self.light), ptr(bwork), C.c_size_t(self.bwork_bytes), s)
+        self._trace(self.srays[k], self.scounts[k], self.shits[k], shadow=True, side=side)
 
     def _mark(self, timed: bool) -> None:
         if timed:
@@ -220,6 +244,7 @@ class DynamicFrame:
         self._trace(self.rays[0], self.counts[0], self.hits[0], shadow=False,
                     config=self.primary_config)
         self._mark(timed)
+        side = self.concurrent and not timed
         for k in range(self.bounces + 1):
             ft = self.finetune
             if ft is not None and k == ft.level:
@@ -228,13 +253,12 @@ class DynamicFrame:
                 ft.enqueue(self, self.stream, self.frame_word if frame is None else None)
                 self._mark(timed)
             if self.shadows:
-                lp = f3(self.light)
-                call("rta_shadow_rays", ptr(self.srays[k].gpu_origin),
-                     ptr(self.srays[k].gpu_direction), ptr(self.srays[k].gpu_maxt),
-                     ptr(self.src), ptr(self.scounts[k]), ptr(self.hits[k]), ptr(self.tris),
-                     ptr(self.rays[k].gpu_origin), ptr(self.rays[k].gpu_direction), self.R, lp,
-                     ptr(self.bwork), C.c_size_t(self.bwork_bytes), s)
-                self._trace(self.srays[k], self.scounts[k], self.shits[k], shadow=True)
+                if side:
+                    # level k's hits are ready: its shadows on the second
+                    # stream, beside the next bounce
+                    self.ev_ready[k].record(self.stream)
+                    self.side.wait(self.ev_ready[k])
+                self._shadows(k, side)
                 self._mark(timed)
             if k < self.bounces:
                 nxt = self.rays[k + 1]
@@ -246,6 +270,10 @@ class DynamicFrame:
                      C.c_size_t(self.bwork_bytes), s)
                 self._trace(nxt, self.counts[k + 1], self.hits[k + 1], shadow=False)
                 self._mark(timed)
+        if side:
+            # the frame ends when both streams have
+            self.ev_side_done.record(self.side)
+            self.stream.wait(self.ev_side_done)
         if frame is None:
             call("rta_hip_add_u32", ptr(self.frame_word), 1, s)
 

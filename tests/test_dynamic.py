@@ -35,12 +35,12 @@ def _records_equal(got, exp):
     assert np.array_equal(got["ref"], exp["ref"])
 
 
-def _small_frame(bounces=2, shadows=True):
+def _small_frame(bounces=2, shadows=True, concurrent=True):
     v, i = scenes.instances(16, 32, seed=4)
     cam = scenes.dynamic_camera(256, 144, n_inst=16)
     light = scenes.dynamic_light(16)
     fr = DynamicFrame(v, i, cam, light, scenes.DYNAMIC_SIGMA, SEED, bounces=bounces,
-                      shadows=shadows)
+                      shadows=shadows, concurrent_shadows=concurrent)
     fr.prime()
     return fr, v, i, cam, light
 
@@ -107,9 +107,12 @@ def _compare(res, exp):
         _records_equal(g[:n][hit], e[hit])
 
 
-def test_dynamic_frame_eager_bit_exact(gpu):
-    say("eager frame 3")
-    fr, v, i, cam, light = _small_frame()
+@pytest.mark.parametrize("concurrent", [True, False])
+def test_dynamic_frame_eager_bit_exact(gpu, concurrent):
+    """Shadow rays on a second stream beside the next bounce, or in
+    sequence on the frame's stream: the same records."""
+    say(f"eager frame 3, concurrent shadows {concurrent}")
+    fr, v, i, cam, light = _small_frame(concurrent=concurrent)
     fr.enqueue(frame=3)
     fr.stream.synchronize()
     fr.check_errors()
