@@ -601,20 +601,25 @@ __device__ inline void bottom_up_chunk(rta_node *__restrict__ nodes,
 // the previous level starting at in_slots/in_counts (kSlots entries per
 // chunk, at most kMergeThreads in all: one thread each), whose leaves span
 // [lo, hi); the new frontier goes to out_slots / *out_count when given.
-// The climbs touch LDS only: every arrival at a split appends its record
-// (far end, the deltas there and at the split, box, leaf state) and swaps
-// its number into the split's slot of an open-addressing table; the thread
-// that finds a number there is the second and forms the parent.  A block
-// sees at most kMergeThreads distinct splits and 2 kMergeThreads arrivals,
-// so the table (half full at most) and the records never run out.  A child whose
-// sibling lies beyond [lo, hi) -- its parent's split is outside, or its
-// slot holds a single arrival at the end -- joins the new frontier.
+// The climbs touch LDS only: every arrival at a split writes its record
+// (far end, the deltas there and at the split, box, leaf state) into the
+// thread's own record slot and then claims the split's entry of an
+// open-addressing table with one 64-bit compare-and-swap of (split,
+// record + 1); a thread that finds the split already there is the second
+// and forms the parent from the first's record.  Only the first arrival
+// stops, so a thread's record is read only after its last write.  A block
+// sees at most kMergeThreads distinct splits, so the table (half full at
+// most) never runs out.  A child whose sibling lies beyond [lo, hi) -- its
+// parent's split is outside, or its entry holds a single arrival at the end
+// -- joins the new frontier.
 constexpr int kMergeThreads = kMergeFan * kSlots;
 constexpr int kTableBits = 11;
 constexpr int kTable = 1 << kTableBits;              // >= 2 kMergeThreads
-constexpr int kRecs = 2 * kMergeThreads;
-constexpr uint32_t kEmpty = 0xffffffffu;             // key: free slot
-constexpr uint32_t kFormed = 0xffffffffu;            // link: both children arrived
+constexpr uint64_t kEmpty = ~0ull;                   // free table entry
+// record word once both children arrived: kFormed, or kFormedInner | x for
+// an inner parent x that needs its pair record
+constexpr uint32_t kFormed = 0xffffffffu;
+constexpr uint32_t kFormedInner = 0x80000000u;
 static_assert(kTable >= 2 * kMergeThreads, "merge table at most half full");
 __device__ inline void bottom_up_merge(rta_node *__restrict__ nodes, uint2 *__restrict__ ranges,
                                        const uint4 *__restrict__ in_slots,
@@ -623,35 +628,25 @@ __device__ inline void bottom_up_merge(rta_node *__restrict__ nodes, uint2 *__re
                                        uint32_t *__restrict__ out_count,
                                        uint32_t *__restrict__ status, uint32_t lo, uint32_t hi,
                                        int n, PackOut pk = PackOut{}) {
-	__shared__ uint32_t prefix[kMergeFan + 1];
-	__shared__ uint32_t key[kTable];        // the split of a slot
-	__shared__ uint32_t link[kTable];       // 0, the first arrival's record + 1, or kFormed
-	__shared__ float rbox[6][kRecs];
-	__shared__ uint32_t rfar[kRecs];        // far end of the child's range
-	__shared__ uint32_t relems[kRecs];
-	__shared__ uint32_t rtris[kRecs];
-	__shared__ int8_t rdfar[kRecs];         // delta at the far end
-	__shared__ int8_t rdnear[kRecs];        // delta at the split
-	__shared__ uint8_t rleft[kRecs];        // the child is the left one
-	__shared__ uint32_t formed[kMergeThreads];   // inner nodes formed here
+	__shared__ uint32_t gcount[kMergeFan];
+	// kEmpty, or (split << 32) | (the first arrival's record + 1, or the
+	// formed mark)
+	__shared__ uint64_t table[kTable];
+	// records, one per thread
+	__shared__ float rbox[6][kMergeThreads];
+	__shared__ uint32_t rfar[kMergeThreads];     // far end of the child's range
+	__shared__ uint32_t relems[kMergeThreads];
+	__shared__ uint32_t rtris[kMergeThreads];
+	__shared__ int8_t rdfar[kMergeThreads];      // delta at the far end
+	__shared__ int8_t rdnear[kMergeThreads];     // delta at the split
+	__shared__ uint8_t rleft[kMergeThreads];     // the child is the left one
 	__shared__ uint4 front[kSlots];
-	__shared__ uint32_t nfront, nrec, nformed;
+	__shared__ uint32_t nfront;
 	const int t = threadIdx.x;
-	for (int i = t; i < kTable; i += blockDim.x) {
-		key[i] = kEmpty;
-		link[i] = 0u;
-	}
-	if (t == 0) {
-		uint32_t acc = 0;
-		prefix[0] = 0;
-		for (int g = 0; g < ng; ++g) {
-			acc += min(in_counts[g], (uint32_t)kSlots);
-			prefix[g + 1] = acc;
-		}
-		nfront = 0;
-		nrec = 0;
-		nformed = 0;
-	}
+	for (int i = t; i < kTable; i += blockDim.x) table[i] = kEmpty;
+	// the groups' entry counts, one load per thread
+	if (t < ng) gcount[t] = min(in_counts[t], (uint32_t)kSlots);
+	if (t == 0) nfront = 0;
 	__syncthreads();
 	const uint32_t leaf0 = (uint32_t)(n - 1);
 	auto push = [&](const uint4 &e) {
@@ -659,11 +654,19 @@ __device__ inline void bottom_up_merge(rta_node *__restrict__ nodes, uint2 *__re
 		if (out_slots && slot < (uint32_t)kSlots) front[slot] = e;
 		else atomicOr(status, (uint32_t)RTA_LBVH_FRONTIER_OVERFLOW);
 	};
-	const uint32_t total = prefix[ng];
-	if ((uint32_t)t < total) {
-		int g = 0;
-		while (prefix[g + 1] <= (uint32_t)t) ++g;
-		const uint4 e = in_slots[(size_t)g * kSlots + (t - prefix[g])];
+	// this thread's entry: group g, position t - before
+	int g = -1;
+	uint32_t before = 0, acc = 0;
+	for (int k = 0; k < ng; ++k) {
+		const uint32_t c = gcount[k];
+		if (g < 0 && (uint32_t)t < acc + c) {
+			g = k;
+			before = acc;
+		}
+		acc += c;
+	}
+	if (g >= 0) {
+		const uint4 e = in_slots[(size_t)g * kSlots + (t - before)];
 		ClimbNode c;
 		c.l = e.y;
 		c.r = e.z;
@@ -682,7 +685,7 @@ __device__ inline void bottom_up_merge(rta_node *__restrict__ nodes, uint2 *__re
 				push(frontier_entry(c, leaf0));
 				break;
 			}
-			const uint32_t r = atomicAdd(&nrec, 1u);
+			const uint32_t r = (uint32_t)t;
 			rbox[0][r] = c.box.min.x; rbox[1][r] = c.box.min.y; rbox[2][r] = c.box.min.z;
 			rbox[3][r] = c.box.max.x; rbox[4][r] = c.box.max.y; rbox[5][r] = c.box.max.z;
 			rfar[r] = left ? c.l : c.r;
@@ -691,20 +694,20 @@ __device__ inline void bottom_up_merge(rta_node *__restrict__ nodes, uint2 *__re
 			relems[r] = c.elems;
 			rtris[r] = c.tris;
 			rleft[r] = left ? 1u : 0u;
+			// LDS only (the node stores need not wait): the record before
+			// the claim, the sibling's record after
+			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+			const uint64_t mine = ((uint64_t)s << 32) | (r + 1u);
 			uint32_t h = (s * 2654435761u) >> (32 - kTableBits);
+			uint64_t prev;
 			while (true) {
-				const uint32_t k = atomicCAS(&key[h], kEmpty, s);
-				if (k == kEmpty || k == s) break;
+				prev = atomicCAS(&table[h], kEmpty, mine);
+				if (prev == kEmpty || (uint32_t)(prev >> 32) == s) break;
 				h = (h + 1u) & (kTable - 1);
 			}
-			// LDS only (the node stores need not wait): the record before
-			// the swap, the sibling's record after
-			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-			const uint32_t prev = atomicExch(&link[h], r + 1u);
-			if (prev == 0u) break;   // the sibling forms the parent
+			if (prev == kEmpty) break;   // the sibling forms the parent
 			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-			link[h] = kFormed;
-			const uint32_t q = prev - 1u;
+			const uint32_t q = (uint32_t)prev - 1u;
 			const int ds = left ? c.dr : c.dl;   // delta(s)
 			ClimbNode sib;
 			sib.box.min = {rbox[0][q], rbox[1][q], rbox[2][q]};
@@ -716,39 +719,60 @@ __device__ inline void bottom_up_merge(rta_node *__restrict__ nodes, uint2 *__re
 			if (sib.l < lo || sib.r >= hi || sib.l > sib.r) {
 				// cannot happen for sorted codes; guards the writes below
 				atomicOr(status, (uint32_t)RTA_LBVH_FRONTIER_OVERFLOW);
+				table[h] = ((uint64_t)s << 32) | kFormed;
 				break;
 			}
 			uint32_t x;
 			c = left ? form_parent(c, sib, s, leaf0, nodes, ranges, pk, x)
 			         : form_parent(sib, c, s, leaf0, nodes, ranges, pk, x);
-			if (pk.pairs && c.elems == kStateInner) formed[atomicAdd(&nformed, 1u)] = x;
+			table[h] = ((uint64_t)s << 32) |
+			           (pk.pairs && c.elems == kStateInner ? kFormedInner | x : kFormed);
 		}
 	}
 	__syncthreads();
-	// a single arrival at a split: the sibling lies beyond [lo, hi)
-	for (int i = t; i < kTable; i += blockDim.x) {
-		const uint32_t v = link[i];
-		if (v == 0u || v == kFormed) continue;
-		const uint32_t q = v - 1u, s = key[i];
+	// Per table entry: a single arrival at a split (the sibling lies beyond
+	// [lo, hi)) joins the frontier; an inner node formed here gets its pair
+	// record from the finished nodes (written by this block's threads before
+	// the barrier; inside the climb their stores held it up).
+	// The thread's kPer entries are taken together, so that the node loads
+	// of its pair records (links, then both children) are in flight at once.
+	constexpr int kPer = kTable / kMergeThreads;
+	static_assert(kTable % kMergeThreads == 0, "whole table entries per thread");
+	uint32_t xs[kPer];
+	bool inner[kPer];
+	#pragma unroll
+	for (int k = 0; k < kPer; ++k) {
+		const uint64_t e = table[t + k * kMergeThreads];
+		const uint32_t word = (uint32_t)e;
+		inner[k] = e != kEmpty && word != kFormed && (word & kFormedInner);
+		xs[k] = word & ~kFormedInner;
+		if (e == kEmpty || word == kFormed || (word & kFormedInner)) continue;
+		const uint32_t q = word - 1u, s = (uint32_t)(e >> 32);
 		const int ds = rdnear[q];
 		ClimbNode w;
 		if (rleft[q]) { w.l = rfar[q]; w.r = s; w.dl = rdfar[q]; w.dr = ds; }
 		else { w.l = s + 1u; w.r = rfar[q]; w.dl = ds; w.dr = rdfar[q]; }
 		push(frontier_entry(w, leaf0));
 	}
-	// pair records of the inner nodes formed here, from the finished nodes
-	// (written by this block's threads before the barrier); inside the climb
-	// their stores held it up
+	uint2 links[kPer];
+	#pragma unroll
+	for (int k = 0; k < kPer; ++k)
+		if (inner[k]) links[k] = *reinterpret_cast<const uint2 *>(&nodes[xs[k]]);
+	rta_node nl[kPer], nr[kPer];
+	#pragma unroll
+	for (int k = 0; k < kPer; ++k)
+		if (inner[k]) {
+			nl[k] = nodes[links[k].x >> 1];
+			nr[k] = nodes[links[k].y];
+		}
 	bool slow = false;
-	const uint32_t nfm = min(nformed, (uint32_t)kMergeThreads);
-	for (uint32_t i = t; i < nfm; i += blockDim.x) {
-		const uint32_t x = formed[i];
-		const uint2 links = *reinterpret_cast<const uint2 *>(&nodes[x]);
-		const uint32_t L = links.x >> 1, R = links.y;
-		const rta_node nl = nodes[L], nr = nodes[R];
-		const uint32_t lw = link_inner(nl.type_left_elems) ? (L << 1) | 1u : nl.right_tris << 1;
-		const uint32_t rw = link_inner(nr.type_left_elems) ? (R << 1) | 1u : nr.right_tris << 1;
-		if (!write_pair(pk.pairs, x, nl.box, nr.box, lw, rw, L, R)) slow = true;
+	#pragma unroll
+	for (int k = 0; k < kPer; ++k) {
+		if (!inner[k]) continue;
+		const uint32_t L = links[k].x >> 1, R = links[k].y;
+		const uint32_t lw = link_inner(nl[k].type_left_elems) ? (L << 1) | 1u : nl[k].right_tris << 1;
+		const uint32_t rw = link_inner(nr[k].type_left_elems) ? (R << 1) | 1u : nr[k].right_tris << 1;
+		if (!write_pair(pk.pairs, xs[k], nl[k].box, nr[k].box, lw, rw, L, R)) slow = true;
 	}
 	if (slow && pk.flags) atomicOr(pk.flags, 1u);
 	__syncthreads();
