@@ -433,14 +433,15 @@ __device__ inline ClimbNode form_parent(const ClimbNode &lc, const ClimbNode &rc
 
 // Level 1 for leaves [chunk*kChunk, +kChunk) of a mesh: every leaf's node,
 // every node completed inside the chunk, and the frontier in slots[0..kSlots)
-// with its size in *count.  The climbs touch LDS only: a node
-// formed here keeps its box, state, split and the far end of its range
-// under its local index (an inner node's index is one end of its range, so
-// it lies in the chunk), and an arriving child leaves its local index per
-// split and side; the nodes, ranges and pair records are written
-// afterwards, one node per thread.
-constexpr uint16_t kLocalLeaf = 0x8000u;    // lchild: the child is a leaf
-constexpr uint16_t kNoChild = 0xffffu;
+// with its size in *count.  The climbs touch LDS only: every child arriving
+// at a split of the chunk writes its record (box, far end, the delta there,
+// leaf state) under that split and its side, then counts its arrival; the
+// second arrival reads its sibling's record and climbs on with the parent.
+// Afterwards each split whose two children arrived writes the parent's
+// node, range and pair record from the two records, one split per thread,
+// and a split with one arrival gives a frontier entry.
+constexpr uint16_t kInner16 = 0xffffu;      // record: an inner node
+constexpr uint32_t kArriveRight = 0x10000u; // visit count of a right child (a left one adds 1)
 __device__ inline void bottom_up_chunk(rta_node *__restrict__ nodes,
                                        const rta_aabb *__restrict__ boxes,
                                        const uint32_t *__restrict__ codes,
@@ -449,15 +450,14 @@ __device__ inline void bottom_up_chunk(rta_node *__restrict__ nodes,
                                        uint4 *__restrict__ slots, uint32_t *__restrict__ count,
                                        uint32_t *__restrict__ status, int n, int chunk,
                                        PackOut pk = PackOut{}) {
-	__shared__ float lb_leaf[6][kChunk];   // leaf boxes by position
-	__shared__ float lb_int[6][kChunk];    // boxes of the inner nodes formed here, by index
-	__shared__ uint32_t lstate[kChunk];    // 0: not formed, kStateInner, or a leaf's count
-	__shared__ uint16_t lother[kChunk];    // the end of the node's range that is not its index
-	__shared__ uint16_t lsplit[kChunk];
-	__shared__ uint16_t ltris[kChunk];     // a merged leaf's first position
-	__shared__ uint32_t lvisit[kChunk];    // per split base + j
-	__shared__ uint16_t lchild[2][kChunk]; // per split: the left / right child that arrived
-	__shared__ int8_t ldelta[kChunk + 1];  // delta(base + j - 1)
+	// records per split base + j and side (0: the left child, 1: the right)
+	__shared__ float rbox[2][6][kChunk];
+	__shared__ uint16_t rfar[2][kChunk];     // the far end of the child's range, - base
+	__shared__ uint16_t relems[2][kChunk];   // a leaf's triangle count, or kInner16
+	__shared__ uint16_t rtris[2][kChunk];    // a leaf's first position, - base
+	__shared__ int8_t rdfar[2][kChunk];      // delta at the far end
+	__shared__ uint32_t lvisit[kChunk];      // arrivals per split
+	__shared__ int8_t ldelta[kChunk + 1];    // delta(base + j - 1)
 	__shared__ uint4 front[kSlots];
 	__shared__ uint32_t nfront;
 
@@ -468,9 +468,6 @@ __device__ inline void bottom_up_chunk(rta_node *__restrict__ nodes,
 	const uint32_t cnt = end - base;
 	const uint32_t k = base + t;
 	lvisit[t] = 0;
-	lstate[t] = 0;
-	lchild[0][t] = kNoChild;
-	lchild[1][t] = kNoChild;
 	if (t == 0) nfront = 0;
 	if ((uint32_t)t <= cnt) ldelta[t] = (int8_t)delta_at(codes, (long long)base + t - 1, n);
 	if (t == 0 && cnt == (uint32_t)kChunk) ldelta[kChunk] = (int8_t)delta_at(codes, (long long)end - 1, n);
@@ -481,8 +478,6 @@ __device__ inline void bottom_up_chunk(rta_node *__restrict__ nodes,
 		c.box = boxes[indices[k]];
 		c.elems = 1u;
 		c.tris = k;
-		lb_leaf[0][t] = c.box.min.x; lb_leaf[1][t] = c.box.min.y; lb_leaf[2][t] = c.box.min.z;
-		lb_leaf[3][t] = c.box.max.x; lb_leaf[4][t] = c.box.max.y; lb_leaf[5][t] = c.box.max.z;
 		store_node(&nodes[leaf0 + k], make_uint2(1u << 1, k), c.box);
 	}
 	auto push = [&](const uint4 &e) {
@@ -490,27 +485,18 @@ __device__ inline void bottom_up_chunk(rta_node *__restrict__ nodes,
 		if (slot < (uint32_t)kSlots) front[slot] = e;
 		else atomicOr(status, (uint32_t)RTA_LBVH_FRONTIER_OVERFLOW);
 	};
-	// the node that arrived as child `side` of local split j
-	auto arrived = [&](uint32_t j, int side) -> ClimbNode {
-		const uint16_t id = lchild[side][j];
-		const uint32_t q = id & 0x3ffu;
+	// the child that arrived at split base + j on `side`
+	auto record = [&](uint32_t j, int side) -> ClimbNode {
 		ClimbNode a;
-		const float(*b)[kChunk] = (id & kLocalLeaf) ? lb_leaf : lb_int;
-		a.box.min = {b[0][q], b[1][q], b[2][q]};
-		a.box.max = {b[3][q], b[4][q], b[5][q]};
-		if (id & kLocalLeaf) {
-			a.l = a.r = base + q;
-			a.elems = 1u;
-			a.tris = base + q;
-		} else {
-			const uint32_t o = base + lother[q];
-			a.l = min(base + q, o);
-			a.r = max(base + q, o);
-			a.elems = lstate[q];
-			a.tris = base + ltris[q];
-		}
-		a.dl = ldelta[a.l - base];
-		a.dr = ldelta[a.r - base + 1];
+		a.box.min = {rbox[side][0][j], rbox[side][1][j], rbox[side][2][j]};
+		a.box.max = {rbox[side][3][j], rbox[side][4][j], rbox[side][5][j]};
+		const uint32_t far = base + rfar[side][j];
+		const int dfar = rdfar[side][j], ds = ldelta[j + 1];
+		if (side == 0) { a.l = far; a.r = base + j; a.dl = dfar; a.dr = ds; }
+		else { a.l = base + j + 1u; a.r = far; a.dl = ds; a.dr = dfar; }
+		const uint16_t e = relems[side][j];
+		a.elems = e == kInner16 ? kStateInner : (uint32_t)e;
+		a.tris = base + rtris[side][j];
 		return a;
 	};
 	__syncthreads();
@@ -527,13 +513,19 @@ __device__ inline void bottom_up_chunk(rta_node *__restrict__ nodes,
 				break;
 			}
 			const uint32_t j = s - base;
-			lchild[left ? 0 : 1][j] = c.l == c.r ? (uint16_t)((c.l - base) | kLocalLeaf)
-			                                     : (uint16_t)(node_index(c, leaf0) - base);
+			const int side = left ? 0 : 1;
+			rbox[side][0][j] = c.box.min.x; rbox[side][1][j] = c.box.min.y; rbox[side][2][j] = c.box.min.z;
+			rbox[side][3][j] = c.box.max.x; rbox[side][4][j] = c.box.max.y; rbox[side][5][j] = c.box.max.z;
+			rfar[side][j] = (uint16_t)((left ? c.l : c.r) - base);
+			rdfar[side][j] = (int8_t)(left ? c.dl : c.dr);
+			const bool inner = c.elems == kStateInner;
+			relems[side][j] = inner ? kInner16 : (uint16_t)c.elems;
+			rtris[side][j] = inner ? (uint16_t)0 : (uint16_t)(c.tris - base);
 			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-			const uint32_t old = atomicAdd(&lvisit[j], 1u);
+			const uint32_t old = atomicAdd(&lvisit[j], left ? 1u : kArriveRight);
 			if (old == 0u) break;
 			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-			const ClimbNode sib = arrived(j, left ? 1 : 0);
+			const ClimbNode sib = record(j, left ? 1 : 0);
 			const ClimbNode &lc = left ? c : sib;
 			const ClimbNode &rc = left ? sib : c;
 			ClimbNode p;
@@ -542,54 +534,34 @@ __device__ inline void bottom_up_chunk(rta_node *__restrict__ nodes,
 			const bool merge = lc.dr >= 32 && lc.elems != kStateInner && rc.elems != kStateInner;
 			p.elems = merge ? lc.elems + rc.elems : kStateInner;
 			p.tris = merge ? (lc.tris < rc.tris ? lc.tris : rc.tris) : 0u;
-			const uint32_t x = node_index(p, leaf0) - base;
-			lb_int[0][x] = p.box.min.x; lb_int[1][x] = p.box.min.y; lb_int[2][x] = p.box.min.z;
-			lb_int[3][x] = p.box.max.x; lb_int[4][x] = p.box.max.y; lb_int[5][x] = p.box.max.z;
-			lstate[x] = p.elems;
-			ltris[x] = (uint16_t)(p.tris - (merge ? base : 0u));
-			lother[x] = (uint16_t)((x + base == p.l ? p.r : p.l) - base);
-			lsplit[x] = (uint16_t)j;
 			c = p;
 		}
 	}
 	__syncthreads();
 
-	// write-out: the inner node with index k when it was formed here
+	// write-out, per split s = base + t of the chunk: the parent when both
+	// children arrived, a frontier entry when one did (its sibling reaches
+	// beyond the chunk)
 	bool slow = false;
-	if (live && k < leaf0 && lstate[t] != 0u) {
-		const uint32_t s = base + lsplit[t], o = base + lother[t];
-		const uint32_t l = min(k, o), r = max(k, o);
-		const uint32_t lid = l == s ? leaf0 + s : s;
-		const uint32_t rid = r == s + 1u ? leaf0 + s + 1u : s + 1u;
-		const bool merged = lstate[t] != kStateInner;
-		rta_aabb box;
-		box.min = {lb_int[0][t], lb_int[1][t], lb_int[2][t]};
-		box.max = {lb_int[3][t], lb_int[4][t], lb_int[5][t]};
-		store_node(&nodes[k], merged ? make_uint2(lstate[t] << 1, base + ltris[t])
-		                             : make_uint2((lid << 1) | 1u, rid), box);
-		ranges[k] = make_uint2(l, r);
-		if (merged) {
-			if (pk.tri4) clear_leaf_end(pk.tri4, s);
-		} else if (pk.pairs) {
-			rta_aabb cb[2];
-			uint32_t word[2];
-			for (int side = 0; side < 2; ++side) {
-				const uint32_t q = s + side - base;
-				const bool leaf = side == 0 ? l == s : r == s + 1u;
-				const float(*b)[kChunk] = leaf ? lb_leaf : lb_int;
-				cb[side].min = {b[0][q], b[1][q], b[2][q]};
-				cb[side].max = {b[3][q], b[4][q], b[5][q]};
-				if (leaf) word[side] = (base + q) << 1;
-				else if (lstate[q] == kStateInner) word[side] = ((base + q) << 1) | 1u;
-				else word[side] = (base + ltris[q]) << 1;
+	if ((uint32_t)t + 1u < cnt) {
+		const uint32_t v = lvisit[t];
+		if (v == 1u + kArriveRight) {
+			const ClimbNode lc = record(t, 0), rc = record(t, 1);
+			const uint32_t s = base + t;
+			uint32_t x;
+			form_parent(lc, rc, s, leaf0, nodes, ranges, pk, x);
+			const bool merged = lc.dr >= 32 && lc.elems != kStateInner && rc.elems != kStateInner;
+			if (!merged && pk.pairs) {
+				const uint32_t lid = lc.l == lc.r ? leaf0 + s : s;
+				const uint32_t rid = rc.l == rc.r ? leaf0 + s + 1u : s + 1u;
+				const uint32_t lw = lc.elems == kStateInner ? (s << 1) | 1u : lc.tris << 1;
+				const uint32_t rw = rc.elems == kStateInner ? ((s + 1u) << 1) | 1u : rc.tris << 1;
+				slow = !write_pair(pk.pairs, x, lc.box, rc.box, lw, rw, lid, rid);
 			}
-			slow = !write_pair(pk.pairs, k, cb[0], cb[1], word[0], word[1], lid, rid);
+		} else if (v != 0u) {
+			push(frontier_entry(record(t, v == 1u ? 0 : 1), leaf0));
 		}
 	}
-	// a child still waiting at a split of the chunk has a sibling that
-	// reaches beyond it: a frontier entry
-	if ((uint32_t)t + 1u < cnt && lvisit[t] == 1u)
-		push(frontier_entry(arrived(t, lchild[0][t] != kNoChild ? 0 : 1), leaf0));
 	if (pk.flags && __syncthreads_or(slow) && t == 0) atomicOr(pk.flags, 1u);
 	__syncthreads();
 	const uint32_t nf = min(nfront, (uint32_t)kSlots);
