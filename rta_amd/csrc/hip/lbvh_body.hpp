@@ -14,7 +14,7 @@ constexpr float kBoundsInit = 100000000000.0f;        // lbvh-kernels.cu:519
 constexpr int kBoundsBlock = 1024;
 constexpr int kChunk = 1024;
 constexpr int kSlots = 128;
-constexpr int kMergeFan = 8;
+constexpr int kMergeFan = 16;
 constexpr uint32_t kStateInner = 0xffffffffu; // leaf count of an inner node in the climb
 
 // ---------------------------------------------------------- wave helpers
@@ -571,28 +571,30 @@ __device__ inline void bottom_up_chunk(rta_node *__restrict__ nodes,
 
 // Level k >= 2 for one mesh: merges the frontier slots of the ng chunks of
 // the previous level starting at in_slots/in_counts (kSlots entries per
-// chunk, at most kMergeThreads in all: one thread each), whose leaves span
-// [lo, hi); the new frontier goes to out_slots / *out_count when given.
-// The climbs touch LDS only: every arrival at a split writes its record
-// (far end, the deltas there and at the split, box, leaf state) into the
-// thread's own record slot and then claims the split's entry of an
-// open-addressing table with one 64-bit compare-and-swap of (split,
-// record + 1); a thread that finds the split already there is the second
-// and forms the parent from the first's record.  Only the first arrival
-// stops, so a thread's record is read only after its last write.  A block
-// sees at most kMergeThreads distinct splits, so the table (half full at
-// most) never runs out.  A child whose sibling lies beyond [lo, hi) -- its
-// parent's split is outside, or its entry holds a single arrival at the end
-// -- joins the new frontier.
-constexpr int kMergeThreads = kMergeFan * kSlots;
-constexpr int kTableBits = 11;
-constexpr int kTable = 1 << kTableBits;              // >= 2 kMergeThreads
+// chunk, at most kMergeEntries in all), whose leaves span [lo, hi); the new
+// frontier goes to out_slots / *out_count when given.  Thread t climbs from
+// entries t, t + kMergeThreads, ... in turn (a climb never waits, so one
+// thread may hold both children of a split).  The climbs touch LDS only:
+// every arrival at a split writes its record (far end, the deltas there and
+// at the split, box, leaf state) into its entry's record slot and then
+// claims the split's entry of an open-addressing table with one 64-bit
+// compare-and-swap of (split, record + 1); a climb that finds the split
+// already there is the second and forms the parent from the first's
+// record.  Only the first arrival stops, so a record is read only after its
+// last write.  A block sees at most kMergeEntries distinct splits, so the
+// table (half full at most) never runs out.  A child whose sibling lies
+// beyond [lo, hi) -- its parent's split is outside, or its entry holds a
+// single arrival at the end -- joins the new frontier.
+constexpr int kMergeThreads = 1024;
+constexpr int kMergeEntries = kMergeFan * kSlots;
+constexpr int kTableBits = 12;
+constexpr int kTable = 1 << kTableBits;              // >= 2 kMergeEntries
 constexpr uint64_t kEmpty = ~0ull;                   // free table entry
 // record word once both children arrived: kFormed, or kFormedInner | x for
 // an inner parent x that needs its pair record
 constexpr uint32_t kFormed = 0xffffffffu;
 constexpr uint32_t kFormedInner = 0x80000000u;
-static_assert(kTable >= 2 * kMergeThreads, "merge table at most half full");
+static_assert(kTable >= 2 * kMergeEntries, "merge table at most half full");
 __device__ inline void bottom_up_merge(rta_node *__restrict__ nodes, uint2 *__restrict__ ranges,
                                        const uint4 *__restrict__ in_slots,
                                        const uint32_t *__restrict__ in_counts, int ng,
@@ -604,14 +606,14 @@ __device__ inline void bottom_up_merge(rta_node *__restrict__ nodes, uint2 *__re
 	// kEmpty, or (split << 32) | (the first arrival's record + 1, or the
 	// formed mark)
 	__shared__ uint64_t table[kTable];
-	// records, one per thread
-	__shared__ float rbox[6][kMergeThreads];
-	__shared__ uint32_t rfar[kMergeThreads];     // far end of the child's range
-	__shared__ uint32_t relems[kMergeThreads];
-	__shared__ uint32_t rtris[kMergeThreads];
-	__shared__ int8_t rdfar[kMergeThreads];      // delta at the far end
-	__shared__ int8_t rdnear[kMergeThreads];     // delta at the split
-	__shared__ uint8_t rleft[kMergeThreads];     // the child is the left one
+	// records, one per entry
+	__shared__ float rbox[6][kMergeEntries];
+	__shared__ uint32_t rfar[kMergeEntries];     // far end of the child's range
+	__shared__ uint32_t relems[kMergeEntries];
+	__shared__ uint32_t rtris[kMergeEntries];
+	__shared__ int8_t rdfar[kMergeEntries];      // delta at the far end
+	__shared__ int8_t rdnear[kMergeEntries];     // delta at the split
+	__shared__ uint8_t rleft[kMergeEntries];     // the child is the left one
 	__shared__ uint4 front[kSlots];
 	__shared__ uint32_t nfront;
 	const int t = threadIdx.x;
@@ -626,19 +628,14 @@ __device__ inline void bottom_up_merge(rta_node *__restrict__ nodes, uint2 *__re
 		if (out_slots && slot < (uint32_t)kSlots) front[slot] = e;
 		else atomicOr(status, (uint32_t)RTA_LBVH_FRONTIER_OVERFLOW);
 	};
-	// this thread's entry: group g, position t - before
-	int g = -1;
-	uint32_t before = 0, acc = 0;
-	for (int k = 0; k < ng; ++k) {
-		const uint32_t c = gcount[k];
-		if (g < 0 && (uint32_t)t < acc + c) {
-			g = k;
-			before = acc;
-		}
-		acc += c;
-	}
-	if (g >= 0) {
-		const uint4 e = in_slots[(size_t)g * kSlots + (t - before)];
+	uint32_t total = 0;
+	for (int k = 0; k < ng; ++k) total += gcount[k];
+	for (uint32_t ei = (uint32_t)t; ei < total; ei += kMergeThreads) {
+		// entry ei: group g, position ei - before
+		int g = 0;
+		uint32_t before = 0;
+		while (ei >= before + gcount[g]) before += gcount[g++];
+		const uint4 e = in_slots[(size_t)g * kSlots + (ei - before)];
 		ClimbNode c;
 		c.l = e.y;
 		c.r = e.z;
@@ -657,7 +654,7 @@ __device__ inline void bottom_up_merge(rta_node *__restrict__ nodes, uint2 *__re
 				push(frontier_entry(c, leaf0));
 				break;
 			}
-			const uint32_t r = (uint32_t)t;
+			const uint32_t r = ei;
 			rbox[0][r] = c.box.min.x; rbox[1][r] = c.box.min.y; rbox[2][r] = c.box.min.z;
 			rbox[3][r] = c.box.max.x; rbox[4][r] = c.box.max.y; rbox[5][r] = c.box.max.z;
 			rfar[r] = left ? c.l : c.r;

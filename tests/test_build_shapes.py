@@ -3,8 +3,8 @@ hierarchy: every triangle identical (the whole tree collapses into one
 leaf at the root), a few distinct triangles repeated (thousands of merged
 leaves), positions crowded toward one corner at geometric spacing (long
 common prefixes, deep spines across chunk boundaries), and sizes at the
-chunk and merge-group boundaries (1,024 leaves per chunk, 8 chunks per
-merge block).  Both full builds (rta_lbvh_build and rta_lbvh_build_packed)
+chunk and merge-group boundaries (1,024 leaves per chunk, 16 chunks per
+merge block, so 16,384 and 262,144 leaves fill one and two merge levels).  Both full builds (rta_lbvh_build and rta_lbvh_build_packed)
 against the oracle (lbvh-kernels.cu:186-341 restated): nodes, sorted
 permutation, reordered triangles and status; the packed layout against
 rta_bvh_pack of the plain build."""
@@ -60,6 +60,10 @@ SHAPES = {
     "chunk_8193": lambda: scenes.random_triangles(8193, seed=8),
     "merge_65536": lambda: scenes.random_triangles(65_536, seed=9),
     "merge_65537": lambda: scenes.random_triangles(65_537, seed=10),
+    "merge_16384": lambda: scenes.random_triangles(16_384, seed=12),
+    "merge_16385": lambda: scenes.random_triangles(16_385, seed=13),
+    "merge_262144": lambda: scenes.random_triangles(262_144, seed=14),
+    "merge_262145": lambda: scenes.random_triangles(262_145, seed=15),
 }
 
 
