@@ -663,6 +663,9 @@ __global__ __launch_bounds__(BLOCK) void trace_cis_persistent_kernel(
 	uint32_t n = (uint32_t)n_rays;
 	if (ray_count) n = min(n, *ray_count);
 	if (ray_list) n = min(n, ray_list[0]);
+	// nothing to trace (an empty overflow list, most often): leave before
+	// the queue atomics of every wave in the grid
+	if (n == 0u) return;
 	const bool scene_fast = BATCHED ? false : (*flags & 1u) == 0u;
 	const uint32_t n_treelets = SEEDED ? *seed.count : 0u;
 	bool seeding = false;   // SEEDED: the lane is in its treelet phase
