@@ -24,30 +24,40 @@ __device__ __host__ inline uint64_t frame_key_of(uint64_t seed, uint32_t frame) 
 }
 
 // frame_word (device, may be NULL) overrides `frame`, so a captured graph
-// can advance the frame number itself (rta_hip_add_u32).
-__global__ __launch_bounds__(256) void jitter_triangles_kernel(
+// can advance the frame number itself (rta_hip_add_u32).  The block's
+// records go through LDS, so that each store instruction writes one
+// contiguous run of the output.
+constexpr int kJitterBlock = 256;
+constexpr int kTriWords32 = sizeof(rta_triangle) / 4;   // 26
+__global__ __launch_bounds__(kJitterBlock) void jitter_triangles_kernel(
     rta_triangle *__restrict__ out, const float *__restrict__ verts,
     const uint32_t *__restrict__ idx, int n_tris, float sigma, uint64_t seed, uint32_t frame,
     const uint32_t *__restrict__ frame_word) {
-	const int i = blockIdx.x * 256 + threadIdx.x;
-	if (i >= n_tris) return;
-	const uint64_t frame_key = frame_key_of(seed, frame_word ? *frame_word : frame);
-	float p[9];
-	for (int c = 0; c < 3; ++c) {
-		const uint32_t v = idx[3 * (size_t)i + c];
-		for (int a = 0; a < 3; ++a)
-			p[3 * c + a] = verts[3 * (size_t)v + a] + vertex_offset(frame_key, v, a, sigma);
+	__shared__ uint32_t tile[kJitterBlock * kTriWords32];
+	const int first = blockIdx.x * kJitterBlock;
+	const int i = first + threadIdx.x;
+	if (i < n_tris) {
+		const uint64_t frame_key = frame_key_of(seed, frame_word ? *frame_word : frame);
+		float p[9];
+		for (int c = 0; c < 3; ++c) {
+			const uint32_t v = idx[3 * (size_t)i + c];
+			for (int a = 0; a < 3; ++a)
+				p[3 * c + a] = verts[3 * (size_t)v + a] + vertex_offset(frame_key, v, a, sigma);
+		}
+		// the 104-byte record as 26 words: a, b, c; zero normals and
+		// texcoords; material -1; the trailing alignment word 0 as well
+		uint32_t *w = tile + threadIdx.x * kTriWords32;
+		for (int k = 0; k < 9; ++k) w[k] = __float_as_uint(p[k]);
+		for (int k = 9; k < 24; ++k) w[k] = 0u;
+		w[24] = 0xffffffffu;
+		w[25] = 0u;
 	}
-	// the 104-byte record as 26 words: a, b, c; zero normals and texcoords;
-	// material -1; the trailing alignment word 0 as well
-	uint32_t w[26];
-	for (int k = 0; k < 9; ++k) w[k] = __float_as_uint(p[k]);
-	for (int k = 9; k < 24; ++k) w[k] = 0u;
-	w[24] = 0xffffffffu;
-	w[25] = 0u;
-	uint2 *dst = reinterpret_cast<uint2 *>(out + i);
-#pragma unroll
-	for (int k = 0; k < 13; ++k) dst[k] = make_uint2(w[2 * k], w[2 * k + 1]);
+	__syncthreads();
+	// the block's records are one run of count * 13 8-byte words
+	const int count = min(kJitterBlock, n_tris - first);
+	uint2 *dst = reinterpret_cast<uint2 *>(out + first);
+	for (int k = threadIdx.x; k < count * (kTriWords32 / 2); k += kJitterBlock)
+		dst[k] = make_uint2(tile[2 * k], tile[2 * k + 1]);
 }
 
 }  // namespace rta_hip
@@ -59,7 +69,8 @@ extern "C" int rta_jitter_triangles(rta_triangle *out, const float *verts, const
                                     const uint32_t *frame_word, rta_stream s) {
 	if (!out || !verts || !idx || n_tris <= 0 || !(sigma >= 0.0f))
 		return invalid("rta_jitter_triangles: bad arguments");
-	hipLaunchKernelGGL(jitter_triangles_kernel, dim3(grid_for(n_tris, 256)), dim3(256), 0,
+	hipLaunchKernelGGL(jitter_triangles_kernel, dim3(grid_for(n_tris, kJitterBlock)),
+	                   dim3(kJitterBlock), 0,
 	                   as_stream(s), out, verts, idx, n_tris, sigma, seed, frame, frame_word);
 	RTA_LAUNCH_CHECK("jitter_triangles_kernel");
 	return RTA_OK;
