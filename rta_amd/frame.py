@@ -125,8 +125,10 @@ class DynamicFrame:
     waves, the dynamic scene's measured none); the other traces use
     config's stack.  concurrent_shadows: the shadow rays of each level are
     generated and traced on a second stream, beside the next bounce (both
-    only read that level's hits), except in timed frames, whose per-stage
-    events need the stages in sequence."""
+    only read that level's hits), and an attached fine-tune stage runs on a
+    third (it reads its level's rays and hits, the pair records and the
+    build's ranges, which nothing later in the frame writes); timed frames
+    keep every stage in sequence for their per-stage events."""
 
     def __init__(self, verts: np.ndarray, idx: np.ndarray, camera, light, sigma: float,
                  seed: int, bounces: int = 2, shadows: bool = True,
@@ -180,6 +182,8 @@ class DynamicFrame:
             self.side_bwork = DeviceBuffer(self.bwork_bytes)
             self.ev_ready = [Event() for _ in range(bounces + 1)]
             self.ev_side_done = Event()
+            self.ft_stream = Stream()
+            self.ev_ft_ready, self.ev_ft_done = Event(), Event()
         self.counts[0].fill_u32(R)
         self.events: list[Event] = []
         self._root = None
@@ -245,12 +249,23 @@ class DynamicFrame:
                     config=self.primary_config)
         self._mark(timed)
         side = self.concurrent and not timed
+        ft_forked = False
         for k in range(self.bounces + 1):
             ft = self.finetune
             if ft is not None and k == ft.level:
                 if frame is not None:
                     call("rta_hip_fill_u32", ptr(ft.frame_buf), frame, 1, s)
-                ft.enqueue(self, self.stream, self.frame_word if frame is None else None)
+                fw = self.frame_word if frame is None else None
+                if side:
+                    # level k's rays and hits are ready: the fine-tune on
+                    # its own stream, beside the rest of the frame
+                    self.ev_ft_ready.record(self.stream)
+                    self.ft_stream.wait(self.ev_ft_ready)
+                    ft.enqueue(self, self.ft_stream, fw)
+                    self.ev_ft_done.record(self.ft_stream)
+                    ft_forked = True
+                else:
+                    ft.enqueue(self, self.stream, fw)
                 self._mark(timed)
             if self.shadows:
                 if side:
@@ -271,9 +286,11 @@ class DynamicFrame:
                 self._trace(nxt, self.counts[k + 1], self.hits[k + 1], shadow=False)
                 self._mark(timed)
         if side:
-            # the frame ends when both streams have
+            # the frame ends when every stream has
             self.ev_side_done.record(self.side)
             self.stream.wait(self.ev_side_done)
+        if ft_forked:
+            self.stream.wait(self.ev_ft_done)
         if frame is None:
             call("rta_hip_add_u32", ptr(self.frame_word), 1, s)
 
