@@ -988,6 +988,14 @@ def main():
             limiter = {k: rec[k] for k in ("l2_hit_pct", "valu_issue_share", "lanes_per_valu",
                                            "wait_share") if k in rec}
 
+    # the build's HBM traffic per launch from the committed rocprofv3 --pmc
+    # measurement of this workload's packed build (same scaling)
+    build_traffic = None
+    bpmc_file = os.path.join(ROOT, "profiles", "lbvh_build_heightfield_2M_pmc.json")
+    if default_workload and os.path.exists(bpmc_file):
+        with open(bpmc_file) as f:
+            build_traffic = int(json.load(f)["total_bytes"])
+
     hit_predicted = None
     if not args.no_predictor and default_workload:
         from rta_amd.bvh import PackedBvh
@@ -1046,6 +1054,10 @@ def main():
             "bound": "hbm", "bytes_per_tri": 276, "unit": "GB/s", "peak": HBM_PEAK_GBS,
             "achieved": round(276 * n / (avg["build"] * 1e-3) / 1e9, 2),
             "frac": round(276 * n / (avg["build"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            # measured bytes between L2 and the fabric per build (all stages)
+            "traffic": build_traffic,
+            "traffic_frac": (round(build_traffic / (avg["build"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                             if build_traffic else None),
         },
         "build_ms": round(avg["build"], 4),
         "build_includes_traversal_layout": True,
